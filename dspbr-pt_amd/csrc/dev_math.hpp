@@ -1,0 +1,95 @@
+// Device math for the gfx950 path tracer.
+//
+// All HIP code here is compiled with -ffp-contract=off and correctly rounded
+// float division and sqrt (hipcc defaults), so these helpers give the same
+// bits as the CPU oracle for +,-,*,/,sqrt. Definitions that GLSL leaves to
+// the implementation are pinned the same way as in the oracle:
+//   normalize(v) = v * (1/sqrt(dot(v,v))), dot = (x*x'+y*y')+z*z',
+//   min/max = the GLSL select forms (NaN-sensitive, unlike fminf/fmaxf),
+//   mix(x,y,a) = x*(1-a) + y*a.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define DI __device__ __forceinline__
+
+struct f3 {
+  float x, y, z;
+};
+DI f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+DI f3 sp3(float s) { return f3{s, s, s}; }
+DI f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+DI f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+DI f3 operator*(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+DI f3 operator/(f3 a, f3 b) { return f3{a.x / b.x, a.y / b.y, a.z / b.z}; }
+DI f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+DI f3 operator*(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
+DI f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+DI f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+
+// GLSL min/max/clamp (select forms)
+DI float gmin(float x, float y) { return (y < x) ? y : x; }
+DI float gmax(float x, float y) { return (x < y) ? y : x; }
+DI float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
+DI float sat(float x) { return gclamp(x, 0.0f, 1.0f); }
+DI float gmix(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+DI f3 vmin(f3 a, f3 b) { return f3{gmin(a.x, b.x), gmin(a.y, b.y), gmin(a.z, b.z)}; }
+DI f3 vmax(f3 a, f3 b) { return f3{gmax(a.x, b.x), gmax(a.y, b.y), gmax(a.z, b.z)}; }
+DI f3 vmaxs(f3 a, float s) { return f3{gmax(a.x, s), gmax(a.y, s), gmax(a.z, s)}; }
+DI f3 vclamp01(f3 v) { return vmin(vmax(v, sp3(0.0f)), sp3(1.0f)); }
+DI f3 vmix(f3 x, f3 y, float a) { return f3{gmix(x.x, y.x, a), gmix(x.y, y.y, a), gmix(x.z, y.z, a)}; }
+DI float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+DI f3 cross3(f3 a, f3 b) { return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+DI float len3(f3 v) { return sqrtf(dot3(v, v)); }
+DI f3 nrm3(f3 v) { return v * (1.0f / sqrtf(dot3(v, v))); }
+DI f3 reflect3(f3 i, f3 n) { return i - (2.0f * dot3(n, i)) * n; }
+DI float max3c(f3 v) { return gmax(v.x, gmax(v.y, v.z)); }
+DI f3 vsqrt(f3 v) { return f3{sqrtf(v.x), sqrtf(v.y), sqrtf(v.z)}; }
+DI f3 vexp(f3 v) { return f3{expf(v.x), expf(v.y), expf(v.z)}; }
+DI f3 vlog(f3 v) { return f3{logf(v.x), logf(v.y), logf(v.z)}; }
+DI f3 vcos(f3 v) { return f3{cosf(v.x), cosf(v.y), cosf(v.z)}; }
+DI f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
+DI f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
+
+// SH/constants.glsl
+constexpr float kPI = 3.14159265358979323f;
+constexpr float kTWO_PI = 6.28318530717958648f;
+constexpr float kEPS = 1e-8f;
+constexpr float kEPS_COS = 0.001f;
+constexpr float kEPS_PDF = 0.001f;
+constexpr float kTFAR_MAX = 100000.0f;
+constexpr float kBVH_FAR = 1e30f;
+
+// Dual 16-bit multiply-with-carry generator, SH/rng.glsl:5-29.
+struct Rng {
+  uint32_t x, y;
+  DI uint32_t next() {
+    x = 36969u * (x & 65535u) + (x >> 16u);
+    y = 18000u * (y & 65535u) + (y >> 16u);
+    return (x << 16u) + y;
+  }
+  DI float uniform() { return float(next()) / 4294967296.0f; }
+};
+DI Rng rng_seed(int seed, int px, int py) {
+  float off = float(seed * 17);
+  float fx = float(px) + 0.5f, fy = float(py) + 0.5f;
+  Rng r;
+  r.x = (uint32_t)(397.6432f * (fx + off)) ^ (uint32_t)(32.9875f * (fy + off));
+  r.y = (uint32_t)(397.6432f * (fy + 0.0f)) ^ (uint32_t)(32.9875f * (fx + 0.0f));
+  return r;
+}
+
+// Wave-level (64-lane) reduction of a 64-bit counter, then one atomic.
+// Must be reached by all 64 lanes of the wave.
+DI void wave_add_u64(unsigned long long* dst, unsigned long long v) {
+  unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned l2 = __shfl_xor(lo, o, 64), h2 = __shfl_xor(hi, o, 64);
+    unsigned s = lo + l2;
+    hi = hi + h2 + (s < lo ? 1u : 0u);
+    lo = s;
+  }
+  unsigned long long tot = ((unsigned long long)hi << 32) | lo;
+  if ((threadIdx.x & 63) == 0 && tot) atomicAdd(dst, tot);
+}

@@ -1,0 +1,230 @@
+// Device-side scene/IBL views and the traversal, IBL and shading-point
+// routines shared by the wavefront kernels.
+#pragma once
+#include "dev_bsdf.hpp"
+
+// Device view of one uploaded scene + IBL. Layouts in HBM:
+//   nodes  float4[4*N]  TinyBVH bvh_gpu nodes, uploaded unchanged
+//   tri    float4[3*M]  per BVH index slot: (p0, tri id as int bits | -1), (p1), (p2)
+//   verts  float4[5*V]  20-float vertex stream, unchanged
+//   tris   u32[3*T], mats float4[17*K]
+//   env    float4[w*h], pdf/cdf f32[w*h], ypdf/ycdf f32[h]
+struct SceneView {
+  const float4* __restrict__ nodes;
+  const float4* __restrict__ tri;
+  const float4* __restrict__ verts;
+  const uint32_t* __restrict__ tris;
+  const float4* __restrict__ mats;
+  int32_t N, M, T, K;
+  const float4* __restrict__ env;
+  const float* __restrict__ pdf;
+  const float* __restrict__ cdf;
+  const float* __restrict__ ypdf;
+  const float* __restrict__ ycdf;
+  int32_t ew, eh;
+  int32_t has_ibl;  // 0: no IBL uploaded; radiance and pdf lookups return 0
+  int32_t full_profile;
+};
+
+// The 52-float uniform block (renderer.frag:10-31).
+struct FrameParams {
+  float view[16];
+  float bg[4];
+  float cam[4];
+  float inv_res[2];
+  float ibl_res[2];
+  float frame_count, debug_mode, pad1, use_ibl;
+  float ibl_rotation, bg_from_ibl, max_bounces, focal_length;
+  float ibl_total_sum, ray_eps, clamp_threshold, pad0;
+  float scene_counts[4];
+  float light_pos[4];
+  float light_emission[4];
+};
+
+constexpr int kStackSize = 64;       // BVH_STACK_SIZE, SH/bvh.glsl:8
+constexpr int kMaxSteps = 4096;      // MAX_BVH_TRAVERSAL_STEPS
+constexpr int kMaxLeafTris = 64;     // MAX_BVH_LEAF_TRIANGLES
+constexpr int kLdsStack = 16;        // stack entries held in LDS per lane
+constexpr int kTraceBlock = 256;
+
+struct RayHit {
+  int tri;
+  float t, u, v;
+};
+
+// Per-lane traversal stack: the first kLdsStack entries live in LDS
+// (lane-interleaved, conflict-free), the rest in a global spill area that
+// is private to the lane.
+struct LaneStack {
+  int* lds;        // &smem[threadIdx.x], stride kTraceBlock
+  int* spill;      // per-lane global area of kStackSize - kLdsStack ints
+  DI void put(int i, int v) {
+    if (i < kLdsStack) lds[i * kTraceBlock] = v;
+    else spill[i - kLdsStack] = v;
+  }
+  DI int get(int i) const { return i < kLdsStack ? lds[i * kTraceBlock] : spill[i - kLdsStack]; }
+};
+
+// Slab test, SH/bvh.glsl:31-41 (GLSL min/max select forms, NaN guarded).
+DI float slab(f3 bmin, f3 bmax, f3 org, f3 inv, float tfar) {
+  f3 t0 = (bmin - org) * inv;
+  f3 t1 = (bmax - org) * inv;
+  f3 lo = vmin(t0, t1), hi = vmax(t0, t1);
+  float tmin = gmax(gmax(lo.x, lo.y), lo.z);
+  float tmax = gmin(gmin(hi.x, hi.y), hi.z);
+  if (isnan(tmin) || isnan(tmax) || tmin > tmax || tmax < 0.0f || tmin >= tfar) return kBVH_FAR;
+  return tmin;
+}
+
+// Moeller-Trumbore, SH/bvh.glsl:44-62.
+DI bool tri_test(f3 org, f3 dir, f3 p0, f3 p1, f3 p2, float tfar, float& t, float& u, float& v) {
+  f3 e0 = p1 - p0, e1 = p2 - p0;
+  f3 pv = cross3(dir, e1);
+  float det = dot3(e0, pv);
+  if (fabsf(det) < kEPS) return false;
+  float f = 1.0f / det;
+  f3 s = org - p0;
+  float uu = f * dot3(s, pv);
+  if (uu < 0.0f || uu > 1.0f) return false;
+  f3 q = cross3(s, e0);
+  float vv = f * dot3(dir, q);
+  if (vv < 0.0f || uu + vv > 1.0f) return false;
+  float tt = f * dot3(e1, q);
+  if (tt < kEPS || tt >= tfar) return false;
+  t = tt; u = uu; v = vv;
+  return true;
+}
+
+// Stack traversal in the exact visit order of SH/bvh.glsl:149-212. With
+// ANY_HIT it returns at the first accepted triangle; the occlusion boolean
+// equals the reference's closest-hit based isOccluded() because both visit
+// the same nodes with the same tfar until the first hit.
+template <bool ANY_HIT>
+DI RayHit traverse(const SceneView& S, f3 org, f3 dir, float tfar, LaneStack& st, unsigned& nv, unsigned& nt) {
+  f3 inv = f3{1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
+  RayHit h{-1, tfar, 0.0f, 0.0f};
+  int top = 1;
+  st.put(0, 0);
+  for (int step = 0; step < kMaxSteps && top > 0; step++) {
+    int ni = st.get(--top);
+    if (ni < 0 || ni >= S.N) continue;
+    nv++;
+    const float4* nd = S.nodes + (size_t)ni * 4;
+    float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
+    int cnt = (int)c.w;
+    int first = (int)d.w;
+    if (cnt > 0) {
+      int end = min(first + min(cnt, kMaxLeafTris), S.M);
+      for (int i = first; i < end; i++) {
+        if (i < 0) continue;
+        const float4* tp = S.tri + (size_t)i * 3;
+        float4 q0 = tp[0];
+        int id = __float_as_int(q0.w);
+        if (id < 0) continue;
+        float4 q1 = tp[1], q2 = tp[2];
+        nt++;
+        float t, u, v;
+        if (tri_test(org, dir, xyz(q0), xyz(q1), xyz(q2), h.t, t, u, v)) {
+          h.t = t; h.tri = id; h.u = u; h.v = v;
+          if (ANY_HIT) return h;
+        }
+      }
+    } else {
+      int left = (int)a.w, right = (int)b.w;
+      float d0 = slab(xyz(a), xyz(b), org, inv, h.t);
+      float d1 = slab(xyz(c), xyz(d), org, inv, h.t);
+      auto push = [&](int child) {
+        if (top < kStackSize && child >= 0 && child < S.N && child != ni) st.put(top++, child);
+      };
+      if (d0 < kBVH_FAR && d1 < kBVH_FAR) {
+        if (d0 < d1) { push(right); push(left); }
+        else { push(left); push(right); }
+      } else if (d0 < kBVH_FAR) {
+        push(left);
+      } else if (d1 < kBVH_FAR) {
+        push(right);
+      }
+    }
+  }
+  return h;
+}
+
+// ---------------------------------------------------------------- IBL
+// lower_bound, SH/utils.glsl:196-217
+DI int lower_bound_f(const float* __restrict__ row, int size, float value) {
+  int count = size, first = 0;
+  while (count > 0) {
+    int step = count / 2;
+    int idx = first + step;
+    if (row[idx] < value) { first = idx + 1; count -= step + 1; }
+    else count = step;
+  }
+  return first;
+}
+DI f3 rotate_phi(f3 d, float rot, bool inverse) {
+  float a = inverse ? -rot : rot;
+  float c = cosf(a), s = sinf(a);
+  return f3{c * d.x + 0.0f * d.y + (-s) * d.z, 0.0f * d.x + 1.0f * d.y + 0.0f * d.z, s * d.x + 0.0f * d.y + c * d.z};
+}
+DI float sin_pdf(float theta) {
+  return 1.0f / (2.0f * kPI * kPI * gmax(kEPS_COS, sinf(theta)));
+}
+// dir_to_uv, SH/utils.glsl:141-147
+DI float2 dir_uv(f3 d, float& theta) {
+  theta = acosf(gmax(-1.0f, gmin(1.0f, d.y)));
+  float ph = atan2f(d.z, d.x);
+  if (ph < 0.0f) ph = kTWO_PI + ph;
+  return make_float2(ph / kTWO_PI, theta / kPI);
+}
+// texture(u_sampler_env_map, uv): GL LINEAR + REPEAT on RGBA32F
+DI f3 env_lookup(const SceneView& S, float2 uv) {
+  int w = S.ew, h = S.eh;
+  float x = uv.x * float(w) - 0.5f, y = uv.y * float(h) - 0.5f;
+  float fx = floorf(x), fy = floorf(y);
+  float a = x - fx, b = y - fy;
+  int ix = (int)fx, iy = (int)fy;
+  int x0 = ix % w; if (x0 < 0) x0 += w;
+  int x1 = (ix + 1) % w; if (x1 < 0) x1 += w;
+  int y0 = iy % h; if (y0 < 0) y0 += h;
+  int y1 = (iy + 1) % h; if (y1 < 0) y1 += h;
+  f3 t00 = xyz(S.env[(size_t)y0 * w + x0]), t10 = xyz(S.env[(size_t)y0 * w + x1]);
+  f3 t01 = xyz(S.env[(size_t)y1 * w + x0]), t11 = xyz(S.env[(size_t)y1 * w + x1]);
+  f3 top = t00 * (1.0f - a) + t10 * a;
+  f3 bot = t01 * (1.0f - a) + t11 * a;
+  return top * (1.0f - b) + bot * b;
+}
+DI f3 ibl_eval(const SceneView& S, const FrameParams& F, f3 d) {
+  if (!S.has_ibl) return sp3(0.0f);
+  float th;
+  float2 uv = dir_uv(rotate_phi(d, F.ibl_rotation, false), th);
+  return env_lookup(S, uv);
+}
+DI float ibl_pdf(const SceneView& S, const FrameParams& F, f3 d) {
+  if (!S.has_ibl || F.ibl_res[0] <= 0.0f || F.ibl_res[1] <= 0.0f) return 0.0f;
+  float th;
+  float2 uv = dir_uv(rotate_phi(d, F.ibl_rotation, false), th);
+  float pw = sin_pdf(th);
+  float w = F.ibl_res[0], h = F.ibl_res[1];
+  float x = gmin(uv.x * w, w - 1.0f);
+  float y = gmin(uv.y * h, h - 1.0f);
+  int ix = min(max((int)x, 0), S.ew - 1), iy = min(max((int)y, 0), S.eh - 1);  // memory guard only
+  return w * h * S.ypdf[iy] * S.pdf[(size_t)iy * S.ew + ix] * pw;
+}
+DI f3 ibl_sample(const SceneView& S, const FrameParams& F, float r0, float r1, float& pdf) {
+  int w = (int)F.ibl_res[0], h = (int)F.ibl_res[1];
+  if (!S.has_ibl || w <= 0 || h <= 0) { pdf = 0.0f; return f3{0.0f, 1.0f, 0.0f}; }
+  int y = lower_bound_f(S.ycdf, h, r1);
+  float py = y < h ? S.ypdf[y] : 0.0f;
+  y = y >= h ? h - 1 : y;
+  const float* crow = S.cdf + (size_t)y * w;
+  int x = lower_bound_f(crow, w, r0);
+  float px = x < w ? S.pdf[(size_t)y * w + x] : 0.0f;
+  x = x >= w ? w - 1 : x;
+  float spdf = F.ibl_res[0] * F.ibl_res[1] * py * px;
+  float u = float(x) / F.ibl_res[0], v = float(y) / F.ibl_res[1];
+  float theta = v * kPI, phi = u * kTWO_PI;
+  float pw = sin_pdf(theta);
+  f3 d = f3{sinf(theta) * cosf(phi), cosf(theta), sinf(theta) * sinf(phi)};
+  pdf = spdf * pw;
+  return rotate_phi(d, F.ibl_rotation, true);
+}

@@ -1,0 +1,513 @@
+// Wavefront path-tracing kernels for gfx950 (wave64, VALU only).
+//
+// One frame batch = `nslots` consecutive frames of one image, one path per
+// pixel per frame (path id p = slot * npix + pixel). Per bounce:
+//   trace_ext   closest hit of the extension rays in the active queue
+//   shade       render state, material, RR, emission, BSDF sample, NEE sample
+//               + BSDF eval/pdf; appends extension rays and shadow rays
+//   trace_shadow any-hit test of the NEE shadow rays, adds the deferred
+//               NEE radiance in the reference's summation order
+// then accumulate folds the finished samples into the running mean frame by
+// frame (renderer.frag:153-156), so the result does not depend on the batch
+// size.
+#include "dspt_kernels.hpp"
+
+// --------------------------------------------------------------------------
+// Scene preparation: per BVH index slot, the triangle's three positions in
+// leaf order (bvh.glsl:72-99 resolved once at upload).
+__global__ void k_prep_tris(const int32_t* __restrict__ bvh_idx, const uint32_t* __restrict__ tris,
+                            const float4* __restrict__ verts, int32_t M, int32_t T, float4* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  int id = bvh_idx[i];
+  float4 p[3] = {make_float4(0, 0, 0, 0), make_float4(0, 0, 0, 0), make_float4(0, 0, 0, 0)};
+  if (id >= 0 && id < T) {
+    for (int k = 0; k < 3; k++) p[k] = verts[(size_t)tris[(size_t)id * 3 + k] * 5];
+  } else {
+    id = -1;
+  }
+  out[i * 3 + 0] = make_float4(p[0].x, p[0].y, p[0].z, __int_as_float(id));
+  out[i * 3 + 1] = make_float4(p[1].x, p[1].y, p[1].z, 0.0f);
+  out[i * 3 + 2] = make_float4(p[2].x, p[2].y, p[2].z, 0.0f);
+}
+
+// --------------------------------------------------------------------------
+// IBL importance-sampling tables, renderer.ts:777-838, bit-identical: the
+// luminance and the per-row sums are double (JS numbers), pdf is rounded to
+// f32, the cdf is a sequential f32 scan. One lane per row (rows are
+// independent; the scan order inside a row is the reference's).
+__global__ void k_ibl_luminance(const float4* __restrict__ env, int64_t n, float* __restrict__ f) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 c = env[i];
+  f[i] = (float)((double)c.x * 0.299 + (double)c.y * 0.587 + (double)c.z * 0.114);
+}
+__device__ double cdf_row(const float* __restrict__ f, float* __restrict__ pdf, float* __restrict__ cdf, int num) {
+  double sum = 0.0;
+  for (int i = 0; i < num; i++) sum += (double)f[i];
+  if (sum == 0.0) sum = 1.0;
+  float acc = 0.0f;
+  for (int i = 0; i < num; i++) {
+    float p = (float)((double)f[i] / sum);
+    pdf[i] = p;
+    acc = (i == 0) ? p : acc + p;
+    cdf[i] = acc;
+  }
+  cdf[num - 1] = 1.0f;
+  return sum;
+}
+__global__ void k_ibl_rows(const float* __restrict__ f, int w, int h, float* __restrict__ pdf, float* __restrict__ cdf,
+                           float* __restrict__ rowsum) {
+  int y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= h) return;
+  size_t o = (size_t)y * w;
+  rowsum[y] = (float)cdf_row(f + o, pdf + o, cdf + o, w);
+}
+__global__ void k_ibl_marginal(const float* __restrict__ rowsum, int h, float* __restrict__ ypdf, float* __restrict__ ycdf,
+                               double* __restrict__ total) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *total = cdf_row(rowsum, ypdf, ycdf, h);
+}
+
+// --------------------------------------------------------------------------
+__global__ void k_fill_i32(int32_t* __restrict__ p, int n, int32_t v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_f4(float4* __restrict__ p, int64_t n, float4 v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Camera rays, renderer.frag:129-142 + rng_init (rng.glsl:23-29).
+__global__ void k_raygen(PathState ps, FrameParams F, int W, int H, int npix, int first_frame, int64_t P,
+                         int32_t* __restrict__ counts, unsigned long long* __restrict__ cnt) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) { counts[0] = (int32_t)P; }
+  if (p >= P) return;
+  int slot = (int)(p / npix);
+  int pix = (int)(p - (int64_t)slot * npix);
+  int px = pix % W, py = pix / W;
+  int frame = first_frame + slot;
+  Rng rng = rng_seed(frame * (int)F.max_bounces, px, py);
+  float r0 = rng.uniform(), r1 = rng.uniform();
+  float vux = (2.0f * (float(px) + 0.5f)) / float(W) - 1.0f;
+  float vuy = (2.0f * (float(py) + 0.5f)) / float(H) - 1.0f;
+  float offx = r0 * F.inv_res[0], offy = r1 * F.inv_res[1];
+  float aspect = F.inv_res[1] / F.inv_res[0];
+  float ux = (vux * aspect + offx) * F.cam[3];
+  float uy = (vuy * 1.0f + offy) * F.cam[3];
+  f3 d = nrm3(f3{ux, uy, -F.focal_length});
+  const float* m = F.view;
+  f3 wd = f3{m[0] * d.x + m[4] * d.y + m[8] * d.z, m[1] * d.x + m[5] * d.y + m[9] * d.z, m[2] * d.x + m[6] * d.y + m[10] * d.z};
+  ps.org[p] = make_float4(F.cam[0], F.cam[1], F.cam[2], 0.0f);
+  ps.dir[p] = make_float4(wd.x, wd.y, wd.z, 0.0f);
+  ps.pw[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+  ps.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  ps.rng[p] = make_uint2(rng.x, rng.y);
+  if (ps.sig) ps.sig[p] = 2166136261u;
+}
+
+// XCD-aware block order: consecutive queue ranges go to the same XCD (the
+// hardware deals workgroups round-robin over the 8 XCDs), so coherent rays
+// share an L2.
+DI int64_t xcd_block(int64_t b, int64_t nb) {
+  const int64_t X = 8;
+  if (nb < X) return b;
+  int64_t per = nb / X;
+  int64_t rem = nb - per * X;
+  int64_t x = b % X, k = b / X;
+  if (k >= per) return per * X + x;  // tail blocks beyond a full round
+  (void)rem;
+  return x * per + k;
+}
+
+__global__ __launch_bounds__(kTraceBlock) void k_trace_ext(SceneView S, PathState ps, const int32_t* __restrict__ queue,
+                                                           const int32_t* __restrict__ count_ptr, int* __restrict__ spill,
+                                                           unsigned long long* __restrict__ cnt) {
+  __shared__ int smem[kLdsStack * kTraceBlock];
+  const int count = *count_ptr;
+  LaneStack st;
+  st.lds = smem + threadIdx.x;
+  int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  st.spill = spill + gtid * (kStackSize - kLdsStack);
+  unsigned nv = 0, nt = 0, nr = 0;
+  int64_t nb = gridDim.x;
+  int64_t lb = xcd_block(blockIdx.x, nb);
+  for (int64_t q = lb * blockDim.x + threadIdx.x; q < count; q += nb * blockDim.x) {
+    int p = queue ? queue[q] : (int)q;
+    float4 o = ps.org[p], d = ps.dir[p];
+    RayHit h = traverse<false>(S, xyz(o), xyz(d), kTFAR_MAX, st, nv, nt);
+    ps.hit[p] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
+    nr++;
+  }
+  wave_add_u64(&cnt[4], nv);
+  wave_add_u64(&cnt[5], nt);
+  wave_add_u64(&cnt[9], nr);
+}
+
+__global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, PathState ps, const int32_t* __restrict__ queue,
+                                                              const int32_t* __restrict__ count_ptr, int* __restrict__ spill,
+                                                              unsigned long long* __restrict__ cnt) {
+  __shared__ int smem[kLdsStack * kTraceBlock];
+  const int count = *count_ptr;
+  LaneStack st;
+  st.lds = smem + threadIdx.x;
+  int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  st.spill = spill + gtid * (kStackSize - kLdsStack);
+  unsigned nv = 0, nt = 0, nr = 0;
+  int64_t nb = gridDim.x;
+  int64_t lb = xcd_block(blockIdx.x, nb);
+  for (int64_t q = lb * blockDim.x + threadIdx.x; q < count; q += nb * blockDim.x) {
+    int p = queue[q];
+    float4 so = ps.sh_org[p];
+    int flags = __float_as_int(so.w);
+    f3 org = xyz(so);
+    f3 contrib = sp3(0.0f);
+    if (flags & 1) {
+      float4 sd = ps.sh_dir[p];
+      RayHit h = traverse<true>(S, org, xyz(sd), sd.w, st, nv, nt);
+      nr++;
+      if (h.tri < 0) contrib = xyz(ps.sh_c[p]);
+    }
+    if (flags & 2) {
+      float4 pd = ps.sh_pdir[p];
+      RayHit h = traverse<true>(S, org, xyz(pd), pd.w, st, nv, nt);
+      nr++;
+      if (h.tri < 0) contrib = contrib + xyz(ps.sh_pc[p]);
+    }
+    // L += (L_ibl + L_point) * path_weight   (misptdl.glsl:72-74)
+    float4 L = ps.L[p];
+    f3 w = xyz(ps.sh_pw[p]);
+    f3 nl = xyz(L) + contrib * w;
+    ps.L[p] = make_float4(nl.x, nl.y, nl.z, L.w);
+  }
+  wave_add_u64(&cnt[10], nv);
+  wave_add_u64(&cnt[11], nt);
+  wave_add_u64(&cnt[3], nr);
+}
+
+DI void sig_mix(uint32_t& h, uint32_t v) { h = (h ^ v) * 16777619u; }
+
+// Finished sample: the reference returns early (no clamp) on a primary miss
+// and on an absorbed BSDF sample; every other exit clamps (misptdl.glsl:92).
+DI float4 finish(f3 L, float thr, bool do_clamp) {
+  if (do_clamp && thr > 0.0f) L = vmin(vmax(L, sp3(0.0f)), sp3(thr));
+  return make_float4(L.x, L.y, L.z, 1.0f);
+}
+
+struct Hitpoint {
+  f3 pos, ng_raw, n, t;
+  float tw;
+  f3 vc;
+  float vcw;
+  uint32_t mat;
+};
+
+// fillRenderState (renderer.frag:72-91) minus the material.
+DI Hitpoint fetch_hitpoint(const SceneView& S, f3 org, f3 dir, int tri, float t, float u, float v) {
+  Hitpoint hp;
+  hp.pos = org + dir * t;
+  uint32_t vi[3];
+  for (int k = 0; k < 3; k++) vi[k] = S.tris[(size_t)tri * 3 + k];
+  const float4* v0 = S.verts + (size_t)vi[0] * 5;
+  const float4* v1 = S.verts + (size_t)vi[1] * 5;
+  const float4* v2 = S.verts + (size_t)vi[2] * 5;
+  float4 p0 = v0[0], p1 = v1[0], p2 = v2[0];
+  hp.ng_raw = nrm3(cross3(xyz(p1) - xyz(p0), xyz(p2) - xyz(p0)));
+  float w0 = 1.0f - u - v;
+  hp.n = nrm3(w0 * xyz(v0[1]) + u * xyz(v1[1]) + v * xyz(v2[1]));
+  float4 t0 = v0[3], t1 = v1[3], t2 = v2[3];
+  float hand = (t0.w == t1.w && t0.w == t2.w) ? t0.w : 0.0f;
+  f3 tn = nrm3(w0 * xyz(t0) + u * xyz(t1) + v * xyz(t2));
+  if (len3(tn) > 0.99f && fabsf(hand) > 0.99f) {
+    hp.t = tn; hp.tw = -hand;
+  } else {
+    // get_onb(n)[0], utils.glsl:104-109
+    hp.t = nrm3(cross3(hp.n, f3{-hp.n.z, hp.n.x, -hp.n.y}));
+    hp.tw = 1.0f;
+  }
+  float4 c0 = v0[4], c1 = v1[4], c2 = v2[4];
+  hp.vc = w0 * xyz(c0) + u * xyz(c1) + v * xyz(c2);
+  hp.vcw = w0 * c0.w + u * c1.w + v * c2.w;
+  hp.mat = (uint32_t)p0.w;
+  return hp;
+}
+
+struct Shading {
+  bsdf::GltfInputs in;
+  f3 n, ng;          // flipped toward wi (fix_normals)
+  f3 tanA;           // anisotropy tangent
+  bool backside, thin;
+  float cutout;
+  f3 emissive, attColor;
+  float attDist;
+};
+
+// configure_gltf_material (material.glsl:5-110) for untextured materials.
+DI Shading configure(const SceneView& S, const Hitpoint& hp, f3 wi) {
+  const float4* m = S.mats + (size_t)hp.mat * 17;
+  float4 m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4], m5 = m[5], m6 = m[6], m7 = m[7], m8 = m[8], m9 = m[9],
+         m10 = m[10], m15 = m[15], m16 = m[16];
+  Shading sh;
+  bsdf::GltfInputs& g = sh.in;
+  g.baseColor = xyz(m0);
+  float opacity = 1.0f;
+  float l4 = sqrtf(hp.vc.x * hp.vc.x + hp.vc.y * hp.vc.y + hp.vc.z * hp.vc.z + hp.vcw * hp.vcw);
+  if (l4 > 0.0f) { g.baseColor = g.baseColor * hp.vc; opacity = opacity * hp.vcw; }
+  sh.cutout = m2.x * opacity;
+  if (m6.w > 0.0f) sh.cutout = (sh.cutout < m6.w) ? 0.0f : 1.0f;
+  if (m6.w == 1.0f) sh.cutout = 1.0f;
+  g.metallic = m0.w * 1.0f;
+  g.roughness = m1.x * 1.0f;
+  g.anisotropy = m1.y * 1.0f;
+  g.transmission = m1.w * 1.0f;
+  g.ior = m2.w;
+  g.specularColor = xyz(m3);
+  g.specular = m3.w * 1.0f;
+  g.sheenColor = xyz(m4);
+  g.sheenRoughness = m4.w * 1.0f;
+  g.clearcoat = m6.x * 1.0f;
+  g.clearcoatRoughness = m6.y * 1.0f;
+  g.diffuseTransmission = m6.z * 1.0f;
+  g.diffuseTransmissionColor = xyz(m15);
+  g.iridescence = m10.x * 1.0f;
+  g.iridescenceIor = m10.y;
+  g.iridescenceThickness = gmix(m10.z, m10.w, 1.0f);
+  g.dispersion = m16.x;
+  sh.thin = m8.w > 0.5f;
+  g.thin = sh.thin;
+  sh.emissive = xyz(m5);
+  sh.attColor = xyz(m7);
+  sh.attDist = m7.w;
+  // fix_normals (utils.glsl:88-98)
+  f3 n = hp.n, ng = hp.ng_raw;
+  sh.backside = false;
+  if (dot3(wi, ng) < 0.0f) { ng = -ng; sh.backside = true; }
+  if (dot3(ng, n) < 0.0f) n = -n;
+  sh.n = n; sh.ng = ng;
+  // anisotropy direction -> tangent rotation by atan2 + pi (material.glsl:74-83)
+  float rot = atan2f(m9.y, m9.x);
+  float angle = rot + kPI;
+  f3 t = hp.t;
+  if (angle > 0.0f) {
+    f3 b = cross3(n, hp.t) * hp.tw;
+    t = hp.t * cosf(angle) + b * sinf(angle);
+  }
+  sh.tanA = t;
+  return sh;
+}
+
+__global__ __launch_bounds__(256) void k_shade(SceneView S, FrameParams F, PathState ps, int iter, int npix,
+                                               const int32_t* __restrict__ queue_in, const int32_t* __restrict__ count_in,
+                                               int32_t* __restrict__ queue_out, int32_t* __restrict__ count_out,
+                                               int32_t* __restrict__ shq, int32_t* __restrict__ count_sh,
+                                               unsigned long long* __restrict__ cnt) {
+  const int count = *count_in;
+  const int maxB = (int)F.max_bounces;
+  unsigned n_paths = 0, n_hits = 0, n_miss = 0, n_nee = 0, n_ext = 0;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < count; q += (int64_t)gridDim.x * blockDim.x) {
+    int p = queue_in ? queue_in[q] : (int)q;
+    float4 hit = ps.hit[p];
+    int tri = __float_as_int(hit.x);
+    float4 o4 = ps.org[p], d4 = ps.dir[p];
+    f3 org = xyz(o4), dir = xyz(d4);
+    float4 pw4 = ps.pw[p];
+    f3 pw = xyz(pw4);
+    float last_pdf = pw4.w;
+    float4 L4 = ps.L[p];
+    f3 L = xyz(L4);
+    uint32_t sig = ps.sig ? ps.sig[p] : 0u;
+    const bool rec = ps.sig != nullptr;
+
+    if (tri < 0) {
+      n_paths++;
+      if (rec) sig_mix(sig, 0xFFFFFFFFu);
+      if (iter == 0) {
+        // primary miss: background, returned unclamped (misptdl.glsl:41-46)
+        float4 out;
+        if (F.bg_from_ibl > 0.0f) { f3 e = ibl_eval(S, F, dir) * pw; out = make_float4(e.x, e.y, e.z, 1.0f); }
+        else out = make_float4(powf(F.bg[0], 2.2f), powf(F.bg[1], 2.2f), powf(F.bg[2], 2.2f), F.bg[3]);
+        ps.L[p] = out;
+        if (rec) ps.prim[p] = -1;
+      } else {
+        n_miss++;
+        float ip = ibl_pdf(S, F, dir);
+        float mis = last_pdf / (last_pdf + ip);
+        L = L + ibl_eval(S, F, dir) * pw * mis * F.use_ibl;
+        ps.L[p] = finish(L, F.clamp_threshold, true);
+      }
+      if (rec) ps.sig[p] = sig;
+      continue;
+    }
+    if (rec) { sig_mix(sig, (uint32_t)tri); if (iter == 0) ps.prim[p] = tri; }
+    if (iter >= maxB) {
+      n_paths++;
+      ps.L[p] = finish(L, F.clamp_threshold, true);
+      if (rec) ps.sig[p] = sig;
+      continue;
+    }
+    uint2 rs = ps.rng[p];
+    Rng rng{rs.x, rs.y};
+    // Russian roulette (renderer.frag:119-123)
+    pw = pw * (1.0f / (1.0f - 0.1f));
+    if (iter > 2 && rng.uniform() <= 0.1f) {
+      n_paths++;
+      if (rec) { sig_mix(sig, 1u); ps.sig[p] = sig; }
+      ps.L[p] = finish(L, F.clamp_threshold, true);
+      continue;
+    }
+    n_hits++;
+    Hitpoint hp = fetch_hitpoint(S, org, dir, tri, hit.y, hit.z, hit.w);
+    f3 wi = -dir;
+    Shading sh = configure(S, hp, wi);
+    // Beer-Lambert absorption inside thick media (misptdl.glsl:60-63)
+    if (sh.backside && !sh.thin) {
+      f3 sigma = -vlog(sh.attColor) / sh.attDist;
+      pw = pw * vexp(-sigma * hit.y);
+    }
+    L = L + sh.emissive * 1.0f * pw;
+
+    // BSDF bounce (renderer.frag:94-116)
+    f3 wo, bw = sp3(1.0f);
+    float pdf = 1.0f;
+    bsdf::Closure cl;
+    bsdf::Frame3 fr;
+    bool have_closure = false;
+    if (rng.uniform() > sh.cutout) {
+      wo = -wi;
+      if (rec) sig_mix(sig, 2u);
+    } else {
+      float lx = rng.uniform(), ly = rng.uniform(), bz = rng.uniform();
+      float comp = rng.uniform();
+      cl = bsdf::build_closure(sh.in, sh.tanA, S.full_profile != 0);
+      fr = bsdf::aniso_frame(sh.n, cl.tanA);
+      have_closure = true;
+      bsdf::Sample bs = bsdf::sample(cl, fr, sh.n, wi, comp, lx, ly, bz);
+      pdf = bs.pdf;
+      bw = bs.w;
+      wo = bs.dir;
+      if (rec) sig_mix(sig, 3u + (uint32_t)(int)bs.specular);
+      if (pdf < kEPS_PDF) {
+        n_paths++;
+        if (rec) { sig_mix(sig, 7u); ps.sig[p] = sig; }
+        ps.L[p] = make_float4(L.x, L.y, L.z, 1.0f);
+        continue;
+      }
+    }
+    last_pdf = pdf;
+    f3 gn = dot3(hp.ng_raw, wo) < 0.0f ? -hp.ng_raw : hp.ng_raw;
+    f3 pos = hp.pos + gn * F.ray_eps;
+
+    // next-event estimation (misptdl.glsl:2-32); radiance added after the
+    // shadow test
+    float n0 = rng.uniform(), n1 = rng.uniform();
+    int flags = 0;
+    f3 c_ibl = sp3(0.0f), c_pt = sp3(0.0f);
+    f3 ldir = sp3(0.0f);
+    float4 pdir = make_float4(0, 0, 0, 0);
+    f3 nn = sh.backside ? -sh.n : sh.n;
+    if (F.use_ibl > 0.0f) {
+      float ipdf;
+      f3 d = ibl_sample(S, F, n0, n1, ipdf);
+      float cosNL = sat(dot3(d, nn));
+      if (cosNL > kEPS_COS && ipdf > kEPS_PDF) {
+        if (!have_closure) {
+          cl = bsdf::build_closure(sh.in, sh.tanA, S.full_profile != 0);
+          fr = bsdf::aniso_frame(sh.n, cl.tanA);
+          have_closure = true;
+        }
+        f3 e = bsdf::eval(cl, fr, sh.n, wi, d) / gmax(fabsf(dot3(sh.n, d)), kEPS);
+        float bp = bsdf::pdf(cl, fr, sh.n, wi, d);
+        f3 Li = ibl_eval(S, F, d) * e * cosNL / ipdf;
+        if (bp > kEPS_PDF) Li = Li * (ipdf / (ipdf + bp));
+        c_ibl = Li;
+        ldir = nrm3(d);
+        flags |= 1;
+        n_nee++;
+      }
+    }
+    if (F.light_pos[3] > 0.5f) {
+      f3 lp = f3{F.light_pos[0], F.light_pos[1], F.light_pos[2]};
+      f3 ld = lp - pos;
+      float dist2 = dot3(ld, ld);
+      ld = nrm3(ld);
+      float cosNL = sat(dot3(ld, nn));
+      f3 dd = lp - pos;
+      pdir = make_float4(0, 0, 0, len3(dd));
+      f3 dn = nrm3(dd);
+      pdir.x = dn.x; pdir.y = dn.y; pdir.z = dn.z;
+      flags |= 2;
+      if (cosNL > kEPS_COS) {
+        if (!have_closure) {
+          cl = bsdf::build_closure(sh.in, sh.tanA, S.full_profile != 0);
+          fr = bsdf::aniso_frame(sh.n, cl.tanA);
+          have_closure = true;
+        }
+        f3 e = bsdf::eval(cl, fr, sh.n, wi, ld) / gmax(fabsf(dot3(sh.n, ld)), kEPS);
+        c_pt = e * (f3{F.light_emission[0], F.light_emission[1], F.light_emission[2]} / dist2) * cosNL;
+      }
+    }
+    if (flags) {
+      int slot = atomicAdd(count_sh, 1);
+      shq[slot] = p;
+      ps.sh_org[p] = make_float4(pos.x, pos.y, pos.z, __int_as_float(flags));
+      ps.sh_dir[p] = make_float4(ldir.x, ldir.y, ldir.z, kTFAR_MAX);
+      ps.sh_c[p] = make_float4(c_ibl.x, c_ibl.y, c_ibl.z, 0.0f);
+      ps.sh_pw[p] = make_float4(pw.x, pw.y, pw.z, 0.0f);
+      if (flags & 2) { ps.sh_pdir[p] = pdir; ps.sh_pc[p] = make_float4(c_pt.x, c_pt.y, c_pt.z, 0.0f); }
+    }
+    pw = pw * bw;
+    int slot = atomicAdd(count_out, 1);
+    queue_out[slot] = p;
+    n_ext++;
+    ps.org[p] = make_float4(pos.x, pos.y, pos.z, 0.0f);
+    ps.dir[p] = make_float4(wo.x, wo.y, wo.z, 0.0f);
+    ps.pw[p] = make_float4(pw.x, pw.y, pw.z, last_pdf);
+    ps.L[p] = make_float4(L.x, L.y, L.z, L4.w);
+    ps.rng[p] = make_uint2(rng.x, rng.y);
+    if (rec) ps.sig[p] = sig;
+  }
+  wave_add_u64(&cnt[0], n_paths);
+  wave_add_u64(&cnt[6], n_hits);
+  wave_add_u64(&cnt[7], n_miss);
+  wave_add_u64(&cnt[8], n_nee);
+  wave_add_u64(&cnt[2], n_ext);
+}
+
+// Running mean (renderer.frag:153-156), frame by frame in frame order.
+__global__ void k_accumulate(const float4* __restrict__ samples, float4* __restrict__ accum, int npix, int nslots,
+                             int first_frame) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  float4 a = accum[i];
+  for (int s = 0; s < nslots; s++) {
+    float4 c = samples[(size_t)s * npix + i];
+    float N = float(first_frame + s);
+    a.x = (a.x * (N - 1.0f) + c.x) / N;
+    a.y = (a.y * (N - 1.0f) + c.y) / N;
+    a.z = (a.z * (N - 1.0f) + c.z) / N;
+    a.w = (a.w * (N - 1.0f) + c.w) / N;
+  }
+  accum[i] = a;
+}
+
+// Standalone traversal on caller rays {org.xyz, tfar, dir.xyz, pad}.
+__global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneView S, const float4* __restrict__ rays, int64_t n, int any,
+                                                            int32_t* __restrict__ tri, float* __restrict__ t, float* __restrict__ u,
+                                                            float* __restrict__ v, int* __restrict__ spill,
+                                                            unsigned long long* __restrict__ cnt) {
+  __shared__ int smem[kLdsStack * kTraceBlock];
+  LaneStack st;
+  st.lds = smem + threadIdx.x;
+  int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  st.spill = spill + gtid * (kStackSize - kLdsStack);
+  unsigned nv = 0, nt = 0;
+  for (int64_t i = gtid; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float4 a = rays[i * 2], b = rays[i * 2 + 1];
+    RayHit h = any ? traverse<true>(S, xyz(a), xyz(b), a.w, st, nv, nt) : traverse<false>(S, xyz(a), xyz(b), a.w, st, nv, nt);
+    tri[i] = h.tri;
+    if (t) { t[i] = h.t; u[i] = h.u; v[i] = h.v; }
+  }
+  wave_add_u64(&cnt[4], nv);
+  wave_add_u64(&cnt[5], nt);
+}

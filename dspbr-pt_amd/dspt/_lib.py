@@ -1,0 +1,237 @@
+"""ctypes binding of the engine's C ABI (include/dspt.h).
+
+The product path always goes through libdspt.so; there is no CPU fallback.
+If the library is missing, loading raises immediately.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libdspt.so")
+
+FRAME_FLOATS = 52
+PROFILE_LEAN = 0
+PROFILE_FULL = 1
+CNT_NAMES = [
+    "paths", "rays_primary", "rays_extension", "rays_shadow", "node_visits", "tri_tests",
+    "shade_hits", "env_misses", "nee_samples",
+]
+
+# Every exported symbol of include/dspt.h with (restype, argtypes).
+_c_int = ctypes.c_int
+_c_i32 = ctypes.c_int32
+_c_i64 = ctypes.c_int64
+_vp = ctypes.c_void_p
+_fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+SIGNATURES = {
+    "dspt_last_error": (ctypes.c_char_p, []),
+    "dspt_version": (ctypes.c_char_p, []),
+    "dspt_create": (_c_int, [_c_int, _c_int, ctypes.POINTER(_vp)]),
+    "dspt_destroy": (_c_int, [_vp]),
+    "dspt_upload_scene": (_c_int, [_vp, _fp, _c_i64, _u32p, _c_i64, _fp, _c_i64, _i32p, _c_i64, _fp, _c_i32]),
+    "dspt_upload_ibl": (_c_int, [_vp, _fp, _c_i32, _c_i32, _dp]),
+    "dspt_read_ibl_tables": (_c_int, [_vp, _fp, _fp, _fp, _fp]),
+    "dspt_resize": (_c_int, [_vp, _c_i32, _c_i32]),
+    "dspt_set_frame": (_c_int, [_vp, _fp]),
+    "dspt_render": (_c_int, [_vp, _c_i32, _c_i32]),
+    "dspt_synchronize": (_c_int, [_vp]),
+    "dspt_read_accum": (_c_int, [_vp, _fp]),
+    "dspt_set_record_samples": (_c_int, [_vp, _c_int]),
+    "dspt_read_samples": (_c_int, [_vp, _fp, _u32p, _i32p]),
+    "dspt_get_counters": (_c_int, [_vp, _i64p]),
+    "dspt_reset_counters": (_c_int, [_vp]),
+    "dspt_trace_closest": (_c_int, [_vp, _fp, _c_i64, _i32p, _fp, _fp, _fp]),
+    "dspt_trace_occluded": (_c_int, [_vp, _fp, _c_i64, _i32p]),
+    "dspt_build_bvh": (_c_int, [_fp, _c_i64, _u32p, _c_i64, _c_i32, _fp, _i64p, _i32p, _i64p]),
+    "dspt_memory_usage": (_c_int, [_vp, _i64p]),
+    "dspt_set_kernel_timing": (_c_int, [_vp, _c_int]),
+    "dspt_get_kernel_stats": (_c_int, [_vp, _dp]),
+}
+KERNEL_CLASSES = ["raygen", "trace_ext", "shade", "trace_shadow", "accumulate"]
+
+
+class DsptError(RuntimeError):
+    """Raised when an engine call returns a non-zero status (message from dspt_last_error)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"dspt error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Loads libdspt.so (once). Raises if the library has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"engine library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != 0:
+        msg = lib().dspt_last_error()
+        raise DsptError(code, msg.decode() if msg else "")
+    return code
+
+
+def ptr(a, ctype=ctypes.c_float):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays passed to the engine must be C-contiguous"
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def build_bvh(positions, indices, max_leaf=4):
+    """Engine-side builder of the TinyBVH bvh_gpu layout (nodes f32[16N], indices i32[M])."""
+    pos = f32(positions).reshape(-1)
+    ind = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
+    nn = ctypes.c_int64(0)
+    nm = ctypes.c_int64(0)
+    L = lib()
+    check(L.dspt_build_bvh(ptr(pos), pos.size // 3, ptr(ind, ctypes.c_uint32), ind.size // 3, max_leaf, None,
+                           ctypes.byref(nn), None, ctypes.byref(nm)))
+    nodes = np.zeros(nn.value * 16, np.float32)
+    bidx = np.zeros(nm.value, np.int32)
+    check(L.dspt_build_bvh(ptr(pos), pos.size // 3, ptr(ind, ctypes.c_uint32), ind.size // 3, max_leaf, ptr(nodes),
+                           ctypes.byref(nn), ptr(bidx, ctypes.c_int32), ctypes.byref(nm)))
+    return nodes, bidx
+
+
+class Engine:
+    """Thin RAII wrapper over one dspt_ctx (one device, one stream)."""
+
+    def __init__(self, profile=PROFILE_LEAN, device=0):
+        self._L = lib()
+        h = ctypes.c_void_p()
+        check(self._L.dspt_create(int(profile), int(device), ctypes.byref(h)))
+        self.h = h
+        self.W = self.H = 0
+        self.ibl_size = (0, 0)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.dspt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_scene(self, verts, tris, nodes, bvh_idx, mats):
+        verts, nodes, mats = f32(verts), f32(nodes), f32(mats)
+        tris = np.ascontiguousarray(tris, dtype=np.uint32)
+        bvh_idx = np.ascontiguousarray(bvh_idx, dtype=np.int32)
+        check(self._L.dspt_upload_scene(self.h, ptr(verts), verts.size // 20, ptr(tris, ctypes.c_uint32), tris.size // 3,
+                                        ptr(nodes), nodes.size // 16, ptr(bvh_idx, ctypes.c_int32), bvh_idx.size,
+                                        ptr(mats), mats.size // 68))
+
+    def upload_ibl(self, rgba, w, h):
+        rgba = f32(rgba)
+        assert rgba.size == 4 * w * h
+        tot = ctypes.c_double(0)
+        check(self._L.dspt_upload_ibl(self.h, ptr(rgba), int(w), int(h), ctypes.byref(tot)))
+        self.ibl_size = (w, h)
+        return tot.value
+
+    def read_ibl_tables(self):
+        w, h = self.ibl_size
+        pdf = np.zeros(w * h, np.float32)
+        cdf = np.zeros(w * h, np.float32)
+        ypdf = np.zeros(h, np.float32)
+        ycdf = np.zeros(h, np.float32)
+        check(self._L.dspt_read_ibl_tables(self.h, ptr(pdf), ptr(cdf), ptr(ypdf), ptr(ycdf)))
+        return pdf, cdf, ypdf, ycdf
+
+    def resize(self, w, h):
+        check(self._L.dspt_resize(self.h, int(w), int(h)))
+        self.W, self.H = int(w), int(h)
+
+    def set_frame(self, block):
+        block = f32(block)
+        assert block.size == FRAME_FLOATS
+        check(self._L.dspt_set_frame(self.h, ptr(block)))
+
+    def render(self, first_frame, num_frames):
+        check(self._L.dspt_render(self.h, int(first_frame), int(num_frames)))
+
+    def synchronize(self):
+        check(self._L.dspt_synchronize(self.h))
+
+    def read_accum(self):
+        out = np.zeros(self.W * self.H * 4, np.float32)
+        check(self._L.dspt_read_accum(self.h, ptr(out)))
+        return out.reshape(self.H, self.W, 4)
+
+    def set_record_samples(self, on):
+        check(self._L.dspt_set_record_samples(self.h, 1 if on else 0))
+
+    def read_samples(self, num_frames):
+        n = num_frames * self.W * self.H
+        rgba = np.zeros(n * 4, np.float32)
+        sig = np.zeros(n, np.uint32)
+        prim = np.zeros(n, np.int32)
+        check(self._L.dspt_read_samples(self.h, ptr(rgba), ptr(sig, ctypes.c_uint32), ptr(prim, ctypes.c_int32)))
+        return rgba.reshape(num_frames, self.H, self.W, 4), sig.reshape(num_frames, self.H, self.W), prim.reshape(
+            num_frames, self.H, self.W)
+
+    def counters(self):
+        out = np.zeros(16, np.int64)
+        check(self._L.dspt_get_counters(self.h, ptr(out, ctypes.c_int64)))
+        return {k: int(out[i]) for i, k in enumerate(CNT_NAMES)}
+
+    def reset_counters(self):
+        check(self._L.dspt_reset_counters(self.h))
+
+    def trace_closest(self, rays):
+        rays = f32(rays).reshape(-1, 8)
+        n = rays.shape[0]
+        tri = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        check(self._L.dspt_trace_closest(self.h, ptr(rays), n, ptr(tri, ctypes.c_int32), ptr(t), ptr(u), ptr(v)))
+        return tri, t, u, v
+
+    def trace_occluded(self, rays):
+        rays = f32(rays).reshape(-1, 8)
+        n = rays.shape[0]
+        occ = np.zeros(n, np.int32)
+        check(self._L.dspt_trace_occluded(self.h, ptr(rays), n, ptr(occ, ctypes.c_int32)))
+        return occ
+
+    def set_kernel_timing(self, on):
+        check(self._L.dspt_set_kernel_timing(self.h, 1 if on else 0))
+
+    def kernel_stats(self):
+        out = np.zeros(8 * 4, np.float64)
+        check(self._L.dspt_get_kernel_stats(self.h, ptr(out, ctypes.c_double)))
+        out = out.reshape(8, 4)
+        return {k: {"launches": int(out[i, 0]), "total_ms": float(out[i, 1]), "node_visits": int(out[i, 2]),
+                    "tri_tests": int(out[i, 3])} for i, k in enumerate(KERNEL_CLASSES)}
+
+    def memory_usage(self):
+        b = ctypes.c_int64(0)
+        check(self._L.dspt_memory_usage(self.h, ctypes.byref(b)))
+        return b.value

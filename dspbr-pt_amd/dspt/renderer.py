@@ -1,0 +1,327 @@
+"""Host-side mirror of the reference's renderer surface over the HIP engine.
+
+PathtracingRenderer   packages/lib/renderer.ts:116-1109 (the object the
+                      engine replaces): setScene, setIBL, resize, render,
+                      stopRendering, resetAccumulation, setInteractionMode and
+                      the settings properties; the GL render loop is replaced
+                      by dspt_render() on the MI355X.
+PathtracingViewport   packages/lib/pathtracing_viewport.ts:100-316 (public
+                      embedding API: setScene/setIBL/setLookAtCamera/start/
+                      stop/on/off/setSettings).
+Post-processing (tonemap, FXAA, display) and the debug AOV integrator are not
+part of the engine; the accumulation buffer is returned in linear radiance.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+from .scene import PathtracingSceneData, ViewportCamera
+
+
+def pack_frame_block(camera, width, height, *, frame_count=1, max_bounces=8, use_ibl=True, ibl_rotation_rad=0.0,
+                     show_background=True, background_color=(0, 0, 0, 1), ibl_size=(0, 0), ray_eps=1e-4,
+                     clamp_threshold=3.0, ibl_total_sum=0.0, scene_counts=(0, 0, 0, 0), light=None, debug_mode=0):
+    """The 52-float uniform block of renderer.ts:376-438 (JS doubles rounded into a Float32Array)."""
+    film_height = math.tan(camera.fov * 0.5 * math.pi / 180.0) * camera.near
+    vals = []
+    vals += [float(x) for x in np.asarray(camera.matrixWorld, np.float64).reshape(16)]
+    vals += [float(x) for x in background_color]
+    vals += [camera.position[0], camera.position[1], camera.position[2], film_height]
+    vals += [1.0 / width, 1.0 / height]
+    vals += [float(ibl_size[0]), float(ibl_size[1])]
+    vals += [float(frame_count), float(debug_mode), 0.0, 1.0 if use_ibl else 0.0]
+    vals += [float(ibl_rotation_rad), 1.0 if show_background else 0.0, float(max_bounces), camera.near]
+    vals += [float(ibl_total_sum), float(ray_eps), float(clamp_threshold), 0.0]
+    vals += [float(x) for x in scene_counts]
+    if light is not None:
+        vals += [light.position[0], light.position[1], light.position[2], 1.0]
+        vals += [light.emission[0], light.emission[1], light.emission[2], 1.0]
+    else:
+        vals += [0.0] * 8
+    block = np.array(vals, dtype=np.float64).astype(np.float32)
+    assert block.size == _lib.FRAME_FLOATS
+    return block
+
+
+class PathtracingRenderer:
+    """MI355X replacement of the reference PathtracingRenderer (renderer.ts:116)."""
+
+    debugModes = ["None"]
+    tonemappingModes = ["None", "Reinhard", "Cineon", "AcesFilm"]
+
+    def __init__(self, materialProfile="webgl-lean", device=0, width=1, height=1):
+        if materialProfile not in ("webgl-lean", "webgl-full"):
+            raise ValueError(f"unknown material profile {materialProfile!r}")
+        self.materialProfile = materialProfile
+        self._engine = _lib.Engine(_lib.PROFILE_FULL if materialProfile == "webgl-full" else _lib.PROFILE_LEAN, device)
+        # settings with the reference defaults (renderer.ts:172-306)
+        self._exposure = 1.0
+        self._tonemapping = "None"
+        self._maxBounces = 8
+        self._useIBL = True
+        self._showBackground = True
+        self._enableGamma = True
+        self._iblRotation = 0.0  # radians
+        self._pixelRatio = 1.0
+        self._backgroundColor = [0.0, 0.0, 0.0, 1.0]
+        self._rayEps = 0.0001
+        self._enableFxaa = False
+        self._tileRes = 4
+        self._clampThreshold = 3.0
+        self._debugMode = "None"
+        self._frameCount = 1
+        self._isRendering = False
+        self._scene = None
+        self._bvh_counts = (0, 0)
+        self._ibl = None
+        self._ibl_total = 0.0
+        self.displayRes = [0, 0]
+        self.renderRes = [0, 0]
+        self.diagnostics = {"backend": "hip-gfx950", "sampleCount": 0, "mode": "stopped", "profiling": {}}
+        self.resize(width, height)
+
+    # ---- settings (renderer.ts:172-306); every physics setter resets accumulation
+    def _setting(name, reset=True, conv=None):
+        def get(self):
+            return getattr(self, "_" + name)
+
+        def set_(self, v):
+            setattr(self, "_" + name, conv(v) if conv else v)
+            if reset:
+                self.resetAccumulation()
+        return property(get, set_)
+
+    exposure = _setting("exposure")
+    tonemapping = _setting("tonemapping", reset=False)
+    maxBounces = _setting("maxBounces", conv=int)
+    useIBL = _setting("useIBL", conv=bool)
+    showBackground = _setting("showBackground", conv=bool)
+    enableGamma = _setting("enableGamma")
+    backgroundColor = _setting("backgroundColor", conv=list)
+    rayEps = _setting("rayEps", conv=float)
+    enableFxaa = _setting("enableFxaa", reset=False)
+    clampThreshold = _setting("clampThreshold", conv=float)
+    del _setting
+
+    @property
+    def iblRotation(self):
+        return self._iblRotation / math.pi * 180.0
+
+    @iblRotation.setter
+    def iblRotation(self, deg):
+        self._iblRotation = deg / 180.0 * math.pi
+        self.resetAccumulation()
+
+    @property
+    def pixelRatio(self):
+        return self._pixelRatio
+
+    @pixelRatio.setter
+    def pixelRatio(self, v):
+        self._pixelRatio = v
+        self.resize(*self.displayRes)
+
+    @property
+    def tileRes(self):
+        return self._tileRes
+
+    @tileRes.setter
+    def tileRes(self, v):
+        self._tileRes = max(2, int(v) or 2)
+
+    @property
+    def debugMode(self):
+        return self._debugMode
+
+    @debugMode.setter
+    def debugMode(self, v):
+        if v != "None":
+            raise NotImplementedError("debug AOV integrator is not part of the engine")
+        self._debugMode = v
+
+    # ---- scene / IBL / resolution
+    def setScene(self, scene: PathtracingSceneData, progress=None):
+        t0 = time.perf_counter()
+        self.stopRendering()
+        if scene.bvhNodes is not None:
+            nodes, bidx = scene.bvhNodes, scene.bvhTriangleIndices
+        else:
+            pos = scene.bvhPositionBuffer if scene.bvhPositionBuffer is not None else scene.getPositionBuffer()
+            ind = scene.bvhIndexBuffer if scene.bvhIndexBuffer is not None else scene.triangleIndexBuffer
+            t1 = time.perf_counter()
+            nodes, bidx = _lib.build_bvh(pos, ind)
+            self.diagnostics["profiling"]["bvhBuildMs"] = (time.perf_counter() - t1) * 1e3
+        self._engine.upload_scene(scene.vertexBuffer, scene.triangleIndexBuffer, nodes, bidx,
+                                  scene.getFlatMaterialBuffer())
+        self._scene = scene
+        self._bvh_counts = (len(nodes) // 16, len(bidx))
+        self.diagnostics["scene"] = {"triangles": scene.num_triangles, "materials": scene.num_materials,
+                                     "lights": len(scene.lights)}
+        self.diagnostics["bvh"] = {"nodes": self._bvh_counts[0], "indices": self._bvh_counts[1]}
+        self.diagnostics["profiling"]["rendererSetupMs"] = (time.perf_counter() - t0) * 1e3
+        self.resetAccumulation()
+
+    def setIBL(self, texture):
+        """texture: {'image': {'data': f32[4wh], 'width': w, 'height': h}} (ibl_loader.ts:4-14)."""
+        img = texture["image"] if isinstance(texture, dict) else texture.image
+        data, w, h = img["data"], int(img["width"]), int(img["height"])
+        self._ibl_total = self._engine.upload_ibl(np.asarray(data, np.float32), w, h)
+        self._ibl = (w, h)
+        self.resetAccumulation()
+
+    def resize(self, width, height):
+        self.displayRes = [int(width), int(height)]
+        self.renderRes = [int(math.ceil(width * self._pixelRatio)), int(math.ceil(height * self._pixelRatio))]
+        self._engine.resize(*self.renderRes)
+        self.resetAccumulation()
+
+    # ---- render loop
+    def resetAccumulation(self):
+        self._frameCount = 1
+        self.diagnostics["sampleCount"] = 0
+
+    def stopRendering(self):
+        self._isRendering = False
+        self.diagnostics["mode"] = "stopped"
+
+    def setInteractionMode(self, flag):
+        self.diagnostics["mode"] = "interactive" if flag else ("settling" if self._isRendering else "stopped")
+        self.resetAccumulation()
+
+    def frame_block(self, camera, frame_count=None):
+        light = self._scene.lights[0] if (self._scene is not None and self._scene.lights) else None
+        return pack_frame_block(
+            camera, self.renderRes[0], self.renderRes[1],
+            frame_count=self._frameCount if frame_count is None else frame_count,
+            max_bounces=self._maxBounces, use_ibl=self._useIBL, ibl_rotation_rad=self._iblRotation,
+            show_background=self._showBackground, background_color=self._backgroundColor,
+            ibl_size=self._ibl or (0, 0), ray_eps=self._rayEps, clamp_threshold=self._clampThreshold,
+            ibl_total_sum=self._ibl_total,
+            scene_counts=(self._scene.num_triangles if self._scene else 0, self._bvh_counts[0], self._bvh_counts[1], 0),
+            light=light)
+
+    def render(self, camera, num_samples, tileFinishedCB=None, frameFinishedCB=None, renderingFinishedCB=None,
+               frames_per_call=None):
+        """renderer.ts:512-565: resets accumulation and renders frames while
+        frameCount < num_samples, i.e. frames 1 .. num_samples-1 (the
+        reference's loop stops once the counter reaches num_samples).
+        num_samples == -1 is not supported (no event loop here)."""
+        if num_samples < 0:
+            raise ValueError("continuous rendering (num_samples = -1) needs an event loop; pass a sample count")
+        self.resetAccumulation()
+        self._isRendering = True
+        self.diagnostics["mode"] = "accumulating"
+        self._engine.set_frame(self.frame_block(camera, 1))
+        total = max(0, num_samples - 1)
+        step = total if frames_per_call is None else max(1, frames_per_call)
+        while self._isRendering and self._frameCount < num_samples:
+            n = min(step, num_samples - self._frameCount)
+            self._engine.render(self._frameCount, n)
+            self._frameCount += n
+            self.diagnostics["sampleCount"] = self._frameCount - 1
+            if tileFinishedCB:
+                tileFinishedCB()
+            if frameFinishedCB:
+                self._engine.synchronize()
+                frameFinishedCB(self._frameCount)
+        self._engine.synchronize()
+        self.stopRendering()
+        if renderingFinishedCB:
+            renderingFinishedCB()
+
+    def readAccumulation(self):
+        """Linear RGBA f32 accumulation buffer, shape (H, W, 4), row 0 = bottom."""
+        return self._engine.read_accum()
+
+    @property
+    def engine(self):
+        return self._engine
+
+
+class PathtracingViewport:
+    """Mirror of pathtracing_viewport.ts:100-316 without a canvas."""
+
+    def __init__(self, width=1, height=1, materialProfile="webgl-lean", settings=None, device=0):
+        self.renderer = PathtracingRenderer(materialProfile=materialProfile, device=device, width=width, height=height)
+        self.camera = ViewportCamera()
+        self._listeners = {}
+        self.running = False
+        self.sampleLimit = -1
+        if settings:
+            self.setSettings(settings)
+
+    @property
+    def diagnostics(self):
+        return self.renderer.diagnostics
+
+    def on(self, event, listener):
+        self._listeners.setdefault(event, []).append(listener)
+        return lambda: self.off(event, listener)
+
+    def off(self, event, listener):
+        if listener in self._listeners.get(event, []):
+            self._listeners[event].remove(listener)
+
+    def _emit(self, event, payload):
+        for l in list(self._listeners.get(event, [])):
+            l(payload)
+
+    _SETTINGS = {
+        "exposure": "exposure", "toneMapping": "tonemapping", "pixelRatio": "pixelRatio", "maxBounces": "maxBounces",
+        "useIBL": "useIBL", "showBackground": "showBackground", "enableGamma": "enableGamma",
+        "enableFxaa": "enableFxaa", "iblRotationDegrees": "iblRotation", "backgroundColor": "backgroundColor",
+        "rayEps": "rayEps", "clampThreshold": "clampThreshold", "tileResolution": "tileRes", "debugMode": "debugMode",
+    }
+
+    def setSettings(self, settings):
+        for k, v in settings.items():
+            if k in self._SETTINGS:
+                setattr(self.renderer, self._SETTINGS[k], v)
+        self._emit("diagnosticsChanged", {"diagnostics": self.diagnostics})
+
+    def setScene(self, scene):
+        self.renderer.setScene(scene)
+        self._emit("sceneLoaded", {"scene": scene})
+
+    def setIBL(self, ibl):
+        self.renderer.setIBL(ibl)
+        self._emit("iblLoaded", {"ibl": ibl})
+
+    def setCamera(self, camera):
+        self.camera = camera
+        self.renderer.resetAccumulation()
+
+    def setLookAtCamera(self, position, target, up=(0, 1, 0), fov=None, near=None):
+        if fov is not None:
+            self.camera.fov = fov
+        if near is not None:
+            self.camera.near = near
+        self.camera.setLookAt(position, target, up)
+        self.setCamera(self.camera)
+
+    def resize(self, width, height):
+        self.renderer.resize(max(1, int(width)), max(1, int(height)))
+
+    def start(self, sampleLimit=-1):
+        self.running = True
+        self.sampleLimit = sampleLimit
+        self.renderer.render(
+            self.camera, sampleLimit, None,
+            lambda n: self._emit("frameRendered", {"frameCount": n}),
+            lambda: self._emit("renderFinished", {"frameCount": self.diagnostics["sampleCount"]}))
+        self.running = False
+
+    def stop(self):
+        self.running = False
+        self.renderer.stopRendering()
+
+    def resetAccumulation(self):
+        self.renderer.resetAccumulation()
+
+    def setInteractionMode(self, active):
+        self.renderer.setInteractionMode(active)
+
+    def readAccumulation(self):
+        return self.renderer.readAccumulation()

@@ -1,0 +1,365 @@
+"""Seeded procedural workloads (synthetic inputs of the benchmark configurations).
+
+Every generator returns a PathtracingSceneData whose buffers follow the
+reference's layouts (20-float vertex stream, u32 indices, 68-float
+materials), plus an IBL dict {'image': {'data', 'width', 'height'}} and a
+camera. Draws use numpy's PCG64 with fixed seeds; the draw order is part of
+the workload definition.
+
+  sphere_field   64 UV spheres (32x16) over a 20x20 ground quad, 61,442 tris,
+                 baseColor/metallic/roughness materials, sky + sun IBL (256^2, 16 spp, depth 8)
+  full_materials 1024x1024 Perlin terrain + 256 displaced icospheres (subdiv 5),
+                 7,335,938 tris, materials drawn from the clearcoat / sheen /
+                 anisotropy / specular / transmission+volume / iridescence /
+                 dispersion families, log-normal IBL + sun (1920x1080, 64 spp, depth 12)
+  batch_scene    one scene of the uneven batched mix (spheres / terrain / soup)
+"""
+import math
+
+import numpy as np
+
+from .scene import MaterialData, PathtracingSceneData, ViewportCamera, VERTEX_STRIDE
+
+
+# ----------------------------------------------------------------- geometry
+def uv_sphere(radius, width_segments=32, height_segments=16):
+    """Sphere with (ws+1)(hs+1) vertices and ws*(hs-1)*2 triangles (pole rows emit one triangle per quad)."""
+    ws, hs = width_segments, height_segments
+    u = np.arange(ws + 1) / ws
+    v = np.arange(hs + 1) / hs
+    uu, vv = np.meshgrid(u, v)
+    phi = uu * 2.0 * math.pi
+    theta = vv * math.pi
+    nx = -np.cos(phi) * np.sin(theta)
+    ny = np.cos(theta)
+    nz = np.sin(phi) * np.sin(theta)
+    n = np.stack([nx, ny, nz], -1).reshape(-1, 3)
+    p = n * radius
+    uv = np.stack([uu, 1.0 - vv], -1).reshape(-1, 2)
+    idx = []
+    grid = np.arange((hs + 1) * (ws + 1)).reshape(hs + 1, ws + 1)
+    for iy in range(hs):
+        for ix in range(ws):
+            a, b = grid[iy, ix + 1], grid[iy, ix]
+            c, d = grid[iy + 1, ix], grid[iy + 1, ix + 1]
+            if iy != 0:
+                idx.append((a, b, d))
+            if iy != hs - 1:
+                idx.append((b, c, d))
+    return p, n, uv, np.array(idx, np.int64)
+
+
+def icosphere(subdiv):
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+             (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+             (8, 6, 7), (9, 8, 1)]
+    v = np.array(verts, np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array(faces, np.int64)
+    for _ in range(subdiv):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        es = np.sort(e, axis=1)
+        uniq, inv = np.unique(es, axis=0, return_inverse=True)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = len(v)
+        v = np.concatenate([v, mid])
+        inv = inv.reshape(3, -1) + base
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        ab, bc, ca = inv[0], inv[1], inv[2]
+        f = np.concatenate([np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1), np.stack([c, ca, bc], 1),
+                            np.stack([ab, bc, ca], 1)])
+    return v, f
+
+
+def vertex_normals(p, f):
+    fn = np.cross(p[f[:, 1]] - p[f[:, 0]], p[f[:, 2]] - p[f[:, 0]])
+    n = np.zeros_like(p)
+    for k in range(3):
+        np.add.at(n, f[:, k], fn)
+    ln = np.linalg.norm(n, axis=1, keepdims=True)
+    return n / np.maximum(ln, 1e-20)
+
+
+def perlin2d(shape, scale, seed, octaves=4):
+    """Fractal gradient noise on a regular grid (vectorised)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    h, w = shape
+    out = np.zeros(shape, np.float64)
+    amp, freq = 1.0, 1.0
+    for _ in range(octaves):
+        gsz = int(max(2, scale * freq)) + 2
+        ang = rng.uniform(0, 2 * math.pi, (gsz, gsz))
+        gx, gy = np.cos(ang), np.sin(ang)
+        ys = np.linspace(0, scale * freq, h, endpoint=False)
+        xs = np.linspace(0, scale * freq, w, endpoint=False)
+        X, Y = np.meshgrid(xs, ys)
+        x0, y0 = np.floor(X).astype(int), np.floor(Y).astype(int)
+        fx, fy = X - x0, Y - y0
+
+        def dotg(ix, iy, dx, dy):
+            return gx[iy, ix] * dx + gy[iy, ix] * dy
+
+        n00 = dotg(x0, y0, fx, fy)
+        n10 = dotg(x0 + 1, y0, fx - 1, fy)
+        n01 = dotg(x0, y0 + 1, fx, fy - 1)
+        n11 = dotg(x0 + 1, y0 + 1, fx - 1, fy - 1)
+        sx = fx * fx * fx * (fx * (fx * 6 - 15) + 10)
+        sy = fy * fy * fy * (fy * (fy * 6 - 15) + 10)
+        nx0 = n00 + sx * (n10 - n00)
+        nx1 = n01 + sx * (n11 - n01)
+        out += amp * (nx0 + sy * (nx1 - nx0))
+        amp *= 0.5
+        freq *= 2.0
+    return out
+
+
+class MeshBuilder:
+    """Accumulates objects into the flat reference buffers."""
+
+    def __init__(self):
+        self.pos, self.nrm, self.uv, self.idx, self.mat = [], [], [], [], []
+        self.nverts = 0
+
+    def add(self, p, n, uv, f, mat_index):
+        self.pos.append(np.asarray(p, np.float64))
+        self.nrm.append(np.asarray(n, np.float64))
+        self.uv.append(np.asarray(uv, np.float64) if uv is not None else np.zeros((len(p), 2)))
+        self.idx.append(np.asarray(f, np.int64) + self.nverts)
+        self.mat.append(np.full(len(p), mat_index, np.float64))
+        self.nverts += len(p)
+
+    def finish(self, scene):
+        p = np.concatenate(self.pos)
+        n = np.concatenate(self.nrm)
+        uv = np.concatenate(self.uv)
+        m = np.concatenate(self.mat)
+        V = len(p)
+        vb = np.zeros((V, VERTEX_STRIDE), np.float32)
+        vb[:, 0:3] = p
+        vb[:, 3] = m
+        vb[:, 4:7] = n
+        vb[:, 8:10] = uv
+        vb[:, 16:20] = 1.0  # vertex colour default (scene_loader.ts:340-343)
+        scene.vertexBuffer = vb.reshape(-1)
+        tri = np.concatenate(self.idx).astype(np.uint32).reshape(-1)
+        scene.triangleIndexBuffer = tri
+        scene.bvhPositionBuffer = np.ascontiguousarray(vb[:, 0:3]).reshape(-1)
+        scene.bvhIndexBuffer = tri
+        return scene
+
+
+# ----------------------------------------------------------------- IBLs
+def equirect_dirs(w, h):
+    x = (np.arange(w) + 0.5) / w
+    y = (np.arange(h) + 0.5) / h
+    phi = x[None, :] * 2 * math.pi
+    theta = y[:, None] * math.pi
+    return np.stack(np.broadcast_arrays(np.sin(theta) * np.cos(phi), np.cos(theta) + 0 * phi, np.sin(theta) * np.sin(phi)), -1)
+
+
+def add_sun(img, dirs, elevation_deg=45.0, azimuth_deg=0.0, diameter_deg=2.0, radiance=5e4):
+    el, az = math.radians(elevation_deg), math.radians(azimuth_deg)
+    sun = np.array([math.cos(el) * math.cos(az), math.sin(el), math.cos(el) * math.sin(az)])
+    cosang = dirs @ sun
+    mask = cosang >= math.cos(math.radians(diameter_deg * 0.5))
+    img[mask, :3] = radiance
+    return img
+
+
+def sky_ibl(w=512, h=256):
+    d = equirect_dirs(w, h)
+    up = np.clip(d[..., 1], -1, 1)
+    t = np.clip(up, 0, 1)[..., None]
+    zen, hor, gnd = np.array([0.25, 0.45, 1.0]), np.array([0.9, 0.9, 1.0]), np.array([0.3, 0.27, 0.22])
+    sky = hor * (1 - t) + zen * t
+    col = np.where(up[..., None] >= 0, sky, gnd * (0.5 + 0.5 * (1 + up[..., None])))
+    img = np.concatenate([col, np.ones((h, w, 1))], -1)
+    add_sun(img, d)
+    return {"image": {"data": img.astype(np.float32).reshape(-1), "width": w, "height": h}}
+
+
+def lognormal_ibl(w, h, seed, mu=0.0, sigma=2.0):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lum = np.exp(rng.normal(mu, sigma, (h, w)))
+    tint = rng.uniform(0.6, 1.0, (h, w, 3))
+    img = np.concatenate([lum[..., None] * tint, np.ones((h, w, 1))], -1)
+    add_sun(img, equirect_dirs(w, h))
+    return {"image": {"data": img.astype(np.float32).reshape(-1), "width": w, "height": h}}
+
+
+# ----------------------------------------------------------------- materials
+def basic_material(rng):
+    """baseColor U[0,1]^3, metallic Bernoulli(0.3), roughness U[0.05,1] (in that draw order)."""
+    m = MaterialData()
+    m.baseColorFactor = rng.uniform(0, 1, 3)
+    m.metallicFactor = 1.0 if rng.uniform() < 0.3 else 0.0
+    m.roughnessFactor = rng.uniform(0.05, 1.0)
+    return m
+
+
+FAMILIES = ["clearcoat", "sheen", "anisotropy", "specular", "transmission", "iridescence", "dispersion"]
+
+
+def family_material(rng, family=None):
+    """Material from one lobe family with seeded parameters (full profile)."""
+    if family is None:
+        family = FAMILIES[int(rng.integers(0, len(FAMILIES)))]
+    m = basic_material(rng)
+    if family == "clearcoat":
+        m.clearcoatFactor = rng.uniform(0.5, 1.0)
+        m.clearcoatRoughnessFactor = rng.uniform(0.0, 0.3)
+    elif family == "sheen":
+        m.metallicFactor = 0.0
+        m.sheenColorFactor = rng.uniform(0, 1, 3)
+        m.sheenRoughnessFactor = rng.uniform(0.2, 1.0)
+    elif family == "anisotropy":
+        m.metallicFactor = 1.0
+        m.roughnessFactor = rng.uniform(0.1, 0.6)
+        m.anisotropy = rng.uniform(0.3, 1.0)
+        a = rng.uniform(0, 2 * math.pi)
+        m.anisotropyDirection = [math.cos(a), math.sin(a), 0.0]
+    elif family == "specular":
+        m.metallicFactor = 0.0
+        m.specularFactor = rng.uniform(0.0, 1.0)
+        m.specularColorFactor = rng.uniform(0, 1, 3)
+        m.ior = rng.uniform(1.3, 2.0)
+    elif family == "transmission":
+        m.metallicFactor = 0.0
+        m.transmissionFactor = 1.0
+        m.thinWalled = 0
+        m.roughnessFactor = rng.uniform(0.0, 0.3)
+        m.ior = rng.uniform(1.3, 1.8)
+        m.attenuationDistance = rng.uniform(0.1, 2.0)
+        m.attenuationColor = rng.uniform(0.3, 1.0, 3)
+    elif family == "iridescence":
+        m.iridescenceFactor = rng.uniform(0.5, 1.0)
+        m.iridescenceIor = rng.uniform(1.2, 1.8)
+        m.iridescenceThicknessMinimum = rng.uniform(100, 400)
+        m.iridescenceThicknessMaximum = rng.uniform(400, 1200)
+    elif family == "dispersion":
+        m.metallicFactor = 0.0
+        m.transmissionFactor = 1.0
+        m.thinWalled = 0
+        m.roughnessFactor = rng.uniform(0.0, 0.2)
+        m.ior = rng.uniform(1.4, 1.8)
+        m.dispersion = rng.uniform(0.5, 5.0)
+        m.attenuationDistance = rng.uniform(0.1, 2.0)
+        m.attenuationColor = rng.uniform(0.3, 1.0, 3)
+    else:
+        raise ValueError(family)
+    return m
+
+
+# ----------------------------------------------------------------- workloads
+class Workload:
+    def __init__(self, name, scene, ibl, camera, width, height, spp, max_bounces, profile):
+        self.name, self.scene, self.ibl, self.camera = name, scene, ibl, camera
+        self.width, self.height, self.spp, self.max_bounces, self.profile = width, height, spp, max_bounces, profile
+
+
+def sphere_field(seed=1, n_spheres=64, width=256, height=256, spp=16, max_bounces=8, ibl_size=(512, 256)):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    scene = PathtracingSceneData()
+    mb = MeshBuilder()
+    gm = scene.addMaterial(basic_material(rng))
+    gp = np.array([[-10, 0, -10], [10, 0, -10], [10, 0, 10], [-10, 0, 10]], np.float64)
+    mb.add(gp, np.tile([0, 1, 0], (4, 1)), [[0, 0], [1, 0], [1, 1], [0, 1]], [[0, 2, 1], [0, 3, 2]], gm)
+    sp, sn, suv, sf = uv_sphere(1.0)
+    for _ in range(n_spheres):
+        c = np.array([rng.uniform(-5, 5), rng.uniform(0, 3), rng.uniform(-5, 5)])
+        r = rng.uniform(0.2, 0.6)
+        mi = scene.addMaterial(basic_material(rng))
+        mb.add(sp * r + c, sn, suv, sf, mi)
+    mb.finish(scene)
+    cam = ViewportCamera(position=(0, 3, 12), target=(0, 1, 0), fov=45, near=0.01)
+    return Workload("sphere_field", scene, sky_ibl(*ibl_size), cam, width, height, spp, max_bounces, "webgl-lean")
+
+
+def full_materials(seed=2, grid=1024, n_objects=256, subdiv=5, width=1920, height=1080, spp=64, max_bounces=12,
+                   ibl_size=(2048, 1024)):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    scene = PathtracingSceneData()
+    mb = MeshBuilder()
+    extent = 40.0
+    hgt = perlin2d((grid, grid), 6.0, seed + 1000) * 3.0
+    xs = np.linspace(-extent, extent, grid)
+    X, Z = np.meshgrid(xs, xs)
+    P = np.stack([X, hgt, Z], -1)
+    dx = np.gradient(hgt, xs, axis=1)
+    dz = np.gradient(hgt, xs, axis=0)
+    Nrm = np.stack([-dx, np.ones_like(dx), -dz], -1)
+    Nrm /= np.linalg.norm(Nrm, axis=-1, keepdims=True)
+    ii = np.arange(grid * grid).reshape(grid, grid)
+    a, b, c, d = ii[:-1, :-1].ravel(), ii[:-1, 1:].ravel(), ii[1:, 1:].ravel(), ii[1:, :-1].ravel()
+    tf = np.concatenate([np.stack([a, d, b], 1), np.stack([b, d, c], 1)])
+    tm = scene.addMaterial(family_material(rng))
+    mb.add(P.reshape(-1, 3), Nrm.reshape(-1, 3), None, tf, tm)
+    iv, iff = icosphere(subdiv)
+    for k in range(n_objects):
+        cx, cz = rng.uniform(-30, 30), rng.uniform(-30, 30)
+        r = rng.uniform(0.5, 2.5)
+        gx = int(np.clip((cx + extent) / (2 * extent) * (grid - 1), 0, grid - 1))
+        gz = int(np.clip((cz + extent) / (2 * extent) * (grid - 1), 0, grid - 1))
+        cy = hgt[gz, gx] + r * rng.uniform(0.6, 1.4)
+        amp = rng.uniform(0.02, 0.12) * r
+        ph = rng.uniform(0, 2 * math.pi, 3)
+        disp = 1.0 + amp / r * np.sin(5 * iv[:, 0] + ph[0]) * np.sin(5 * iv[:, 1] + ph[1]) * np.sin(5 * iv[:, 2] + ph[2])
+        p = iv * (r * disp)[:, None] + np.array([cx, cy, cz])
+        n = vertex_normals(p, iff)
+        mi = scene.addMaterial(family_material(rng))
+        mb.add(p, n, None, iff, mi)
+    mb.finish(scene)
+    cam = ViewportCamera(position=(0, 18, 45), target=(0, 0, 0), fov=45, near=0.01)
+    return Workload("full_materials", scene, lognormal_ibl(*ibl_size, seed=seed + 2000), cam, width, height, spp,
+                    max_bounces, "webgl-full")
+
+
+def triangle_soup(rng, n, extent=5.0):
+    c = rng.uniform(-extent, extent, (n, 3)) + np.array([0, extent, 0])
+    e = rng.uniform(0.01, 0.5, n)
+    d1 = rng.normal(size=(n, 3))
+    d2 = rng.normal(size=(n, 3))
+    d1 /= np.linalg.norm(d1, axis=1, keepdims=True)
+    d2 /= np.linalg.norm(d2, axis=1, keepdims=True)
+    p = np.stack([c, c + d1 * e[:, None], c + d2 * e[:, None]], 1).reshape(-1, 3)
+    f = np.arange(3 * n).reshape(n, 3)
+    nn = np.cross(d1, d2)
+    nn /= np.maximum(np.linalg.norm(nn, axis=1, keepdims=True), 1e-12)
+    return p, np.repeat(nn, 3, axis=0), f
+
+
+def batch_scene(seed, n_tris, kind):
+    """One scene of the uneven batched mix: kind in {'spheres', 'terrain', 'soup'}, about n_tris triangles."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    scene = PathtracingSceneData()
+    mb = MeshBuilder()
+    gm = scene.addMaterial(family_material(rng))
+    gp = np.array([[-20, 0, -20], [20, 0, -20], [20, 0, 20], [-20, 0, 20]], np.float64)
+    mb.add(gp, np.tile([0, 1, 0], (4, 1)), None, [[0, 2, 1], [0, 3, 2]], gm)
+    if kind == "spheres":
+        sp, sn, suv, sf = uv_sphere(1.0)
+        k = max(1, n_tris // len(sf))
+        for _ in range(k):
+            c = np.array([rng.uniform(-6, 6), rng.uniform(0, 4), rng.uniform(-6, 6)])
+            r = rng.uniform(0.2, 0.8)
+            mb.add(sp * r + c, sn, suv, sf, scene.addMaterial(family_material(rng)))
+    elif kind == "terrain":
+        g = max(8, int(math.sqrt(n_tris / 2)) + 1)
+        hgt = perlin2d((g, g), 4.0, seed + 7) * 1.5
+        xs = np.linspace(-8, 8, g)
+        X, Z = np.meshgrid(xs, xs)
+        P = np.stack([X, hgt, Z], -1).reshape(-1, 3)
+        ii = np.arange(g * g).reshape(g, g)
+        a, b, c, d = ii[:-1, :-1].ravel(), ii[:-1, 1:].ravel(), ii[1:, 1:].ravel(), ii[1:, :-1].ravel()
+        tf = np.concatenate([np.stack([a, d, b], 1), np.stack([b, d, c], 1)])
+        mb.add(P, vertex_normals(P, tf), None, tf, scene.addMaterial(family_material(rng)))
+    elif kind == "soup":
+        p, n, f = triangle_soup(rng, n_tris)
+        mb.add(p, n, None, f, scene.addMaterial(family_material(rng)))
+    else:
+        raise ValueError(kind)
+    return mb.finish(scene)

@@ -1,0 +1,152 @@
+/*
+ * dspt.h — C ABI of the MI355X path-tracing engine (gfx950, HIP).
+ *
+ * The engine replaces the GPU render loop behind the reference's
+ * PathtracingRenderer (packages/lib/renderer.ts:116-1109, "RT" below) and
+ * keeps its data formats: the 20-float vertex stream, u32 triangle indices,
+ * the TinyBVH "bvh_gpu" node array (16 floats per node) with its i32 index
+ * array, 68-float materials, an RGBA32F equirect IBL and the 52-float
+ * uniform block of packages/lib/shader/renderer.frag:10-31.
+ *
+ * Conventions: every call returns 0 on success and a negative code on
+ * failure; dspt_last_error() then gives a thread-local message (mirrors the
+ * int-status + last-error channel of packages/lib/bvh/tinybvh_builder.ts:86-97).
+ * Buffers are caller-owned host memory unless stated; the engine copies what
+ * it keeps into device memory (HBM) it owns. Images are RGBA f32, row 0 =
+ * bottom row (GL convention), pixel (x, y) at index y*W + x.
+ */
+#ifndef DSPT_H
+#define DSPT_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dspt_ctx dspt_ctx;
+
+enum {
+  DSPT_OK = 0,
+  DSPT_ERR_ARG = -1,       /* invalid argument / malformed input */
+  DSPT_ERR_DEVICE = -2,    /* HIP runtime error or no GPU */
+  DSPT_ERR_STATE = -3,     /* call order (e.g. render before upload) */
+  DSPT_ERR_MEMORY = -4,    /* device allocation or admission budget exceeded */
+  DSPT_ERR_UNSUPPORTED = -5
+};
+
+enum { DSPT_PROFILE_LEAN = 0, DSPT_PROFILE_FULL = 1 };
+
+/* Number of floats in the frame parameter block (std140 uniform block of
+ * renderer.frag:10-31, packed in the key order of RT:376-438). */
+#define DSPT_FRAME_FLOATS 52
+
+/* Counter slots returned by dspt_get_counters(). */
+enum {
+  DSPT_CNT_PATHS = 0,        /* pixel samples completed */
+  DSPT_CNT_RAYS_PRIMARY = 1,
+  DSPT_CNT_RAYS_EXTENSION = 2,
+  DSPT_CNT_RAYS_SHADOW = 3,
+  DSPT_CNT_NODE_VISITS = 4,  /* closest-hit + any-hit traversals */
+  DSPT_CNT_TRI_TESTS = 5,
+  DSPT_CNT_SHADE_HITS = 6,
+  DSPT_CNT_ENV_MISSES = 7,
+  DSPT_CNT_NEE_SAMPLES = 8,
+  DSPT_CNT_COUNT = 16
+};
+
+const char* dspt_last_error(void);
+const char* dspt_version(void);
+
+/* Replaces `new PathtracingRenderer({materialProfile})` (RT:309-325). */
+int dspt_create(int profile, int device, dspt_ctx** out);
+int dspt_destroy(dspt_ctx* ctx);
+
+/* Replaces RT.setScene (RT:914-958) after the BVH build of RT.initBvh
+ * (RT:872-912): vertex stream f32[20*V] (packages/lib/scene_data.ts:21),
+ * triangle indices u32[3*T], BVH nodes f32[16*N] and BVH indices i32[M]
+ * (packages/lib/bvh/tinybvh_builder.ts:1-5), materials f32[68*K]
+ * (packages/lib/material.ts:2-481). Material textures are not supported:
+ * every texture id must be < 0. */
+int dspt_upload_scene(dspt_ctx* ctx, const float* vertices, int64_t num_vertices, const uint32_t* tri_indices,
+                      int64_t num_triangles, const float* bvh_nodes, int64_t num_nodes, const int32_t* bvh_indices,
+                      int64_t num_bvh_indices, const float* materials, int32_t num_materials);
+
+/* Replaces RT.setIBL (RT:840-865): uploads the RGBA32F equirect map and
+ * builds the importance-sampling pdf/cdf tables on the device, bit-identical
+ * to precomputeIBLImportanceSamplingData (RT:777-838). If total_sum is not
+ * NULL it receives the marginal sum (u_ibl_pdf_total_sum). */
+int dspt_upload_ibl(dspt_ctx* ctx, const float* rgba, int32_t width, int32_t height, double* total_sum);
+/* Reads the device CDF tables back (pdf, cdf f32[w*h]; ypdf, ycdf f32[h]). */
+int dspt_read_ibl_tables(dspt_ctx* ctx, float* pdf, float* cdf, float* ypdf, float* ycdf);
+
+/* Replaces RT.resize (RT:365-372): render resolution; clears accumulation. */
+int dspt_resize(dspt_ctx* ctx, int32_t width, int32_t height);
+
+/* Replaces the uniform upload of RT.updatePathracingUniforms (RT:376-438).
+ * u_frame_count in the block is ignored: dspt_render() sets it per frame. */
+int dspt_set_frame(dspt_ctx* ctx, const float block[DSPT_FRAME_FLOATS]);
+
+/* Replaces RT.render(camera, num_samples, ...) + the renderFrame loop
+ * (RT:512-714): renders frames first_frame .. first_frame+num_frames-1
+ * (u_frame_count is 1-based) into the accumulation buffer with the running
+ * mean of renderer.frag:153-156. first_frame == 1 resets the accumulation
+ * (RT.resetAccumulation, RT:327-336). Asynchronous on the context stream;
+ * dspt_synchronize() or dspt_read_accum() waits. */
+int dspt_render(dspt_ctx* ctx, int32_t first_frame, int32_t num_frames);
+int dspt_synchronize(dspt_ctx* ctx);
+
+/* Accumulation readback: f32[4*W*H]. No counterpart in the reference lib
+ * (the app reads the canvas, packages/app/demo_viewer.ts:246-251). */
+int dspt_read_accum(dspt_ctx* ctx, float* rgba);
+
+/* Per-sample radiance of the last dspt_render() call, for parity checks:
+ * f32[4 * num_frames * W * H] ordered [frame][y][x][rgba]. Only valid when
+ * dspt_set_record_samples(ctx, 1) was called before rendering. sigs receives
+ * the per-sample path signature (branch/hit-id hash), prim the primary hit
+ * triangle id (-1 on miss); either may be NULL. */
+int dspt_set_record_samples(dspt_ctx* ctx, int enable);
+int dspt_read_samples(dspt_ctx* ctx, float* rgba, uint32_t* sigs, int32_t* prim);
+
+/* Work counters accumulated since the last reset (int64[DSPT_CNT_COUNT]). */
+int dspt_get_counters(dspt_ctx* ctx, int64_t* out);
+int dspt_reset_counters(dspt_ctx* ctx);
+
+/* Per-kernel timing with HIP events recorded on the engine stream around
+ * every launch of dspt_render(). Kernel classes: */
+enum {
+  DSPT_K_RAYGEN = 0,
+  DSPT_K_TRACE_EXT = 1,    /* closest-hit traversal of camera + extension rays */
+  DSPT_K_SHADE = 2,
+  DSPT_K_TRACE_SHADOW = 3, /* any-hit traversal of NEE shadow rays */
+  DSPT_K_ACCUM = 4,
+  DSPT_K_COUNT = 8
+};
+/* Stats per class, double[DSPT_K_COUNT * 4]: {launches, total_ms,
+ * node_visits, tri_tests} (visits/tests only for the two traversal classes).
+ * Reset together with dspt_reset_counters(). */
+int dspt_set_kernel_timing(dspt_ctx* ctx, int enable);
+int dspt_get_kernel_stats(dspt_ctx* ctx, double* out);
+
+/* Traversal entry points on caller rays {org.xyz, tfar, dir.xyz, pad}
+ * (f32[8*n]): closest hit (SH/bvh.glsl:214-216) -> tri id (-1 = miss), t,
+ * barycentric u, v; and occlusion (SH/bvh.glsl:225-229) -> 1 if any
+ * triangle lies in [1e-8, tfar). Synchronous. */
+int dspt_trace_closest(dspt_ctx* ctx, const float* rays, int64_t n, int32_t* tri, float* t, float* u, float* v);
+int dspt_trace_occluded(dspt_ctx* ctx, const float* rays, int64_t n, int32_t* occluded);
+
+/* Engine-side BVH builder emitting the TinyBVH bvh_gpu layout consumed by
+ * SH/bvh.glsl:1-6 (replaces the WASM call of RT:880-882 /
+ * tinybvh_builder.ts:142-215). positions f32[3*V], indices u32[3*T].
+ * Two calls: with nodes == NULL it returns the sizes in *num_nodes and
+ * *num_bvh_indices; then fill caller buffers of those sizes. */
+int dspt_build_bvh(const float* positions, int64_t num_vertices, const uint32_t* indices, int64_t num_triangles,
+                   int32_t max_leaf, float* nodes, int64_t* num_nodes, int32_t* bvh_indices, int64_t* num_bvh_indices);
+
+/* Device memory currently held by the context, bytes. */
+int dspt_memory_usage(dspt_ctx* ctx, int64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSPT_H */

@@ -1,0 +1,180 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) — test infrastructure.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
+and only as the checker / CPU baseline.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_f = ctypes.POINTER(ctypes.c_float)
+_u32 = ctypes.POINTER(ctypes.c_uint32)
+_i32 = ctypes.POINTER(ctypes.c_int32)
+_i64 = ctypes.POINTER(ctypes.c_int64)
+_d = ctypes.POINTER(ctypes.c_double)
+
+
+class CScene(ctypes.Structure):
+    _fields_ = [
+        ("verts", _f), ("V", ctypes.c_int), ("tris", _u32), ("T", ctypes.c_int), ("nodes", _f), ("N", ctypes.c_int),
+        ("bvh_idx", _i32), ("M", ctypes.c_int), ("mats", _f), ("K", ctypes.c_int), ("env", _f), ("ew", ctypes.c_int),
+        ("eh", ctypes.c_int), ("pdf", _f), ("cdf", _f), ("ypdf", _f), ("ycdf", _f), ("profile", ctypes.c_int),
+    ]
+
+
+def build_oracle():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_LIB):
+            build_oracle()
+        L = ctypes.CDLL(ORACLE_LIB)
+        L.oracle_ibl_cdf.restype = ctypes.c_double
+        L.oracle_ibl_cdf.argtypes = [_f, ctypes.c_int, ctypes.c_int, _f, _f, _f, _f]
+        L.oracle_rng_seq.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _u32, _u32, _f, _u32]
+        L.oracle_rng_step.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _u32]
+        L.oracle_trace.argtypes = [ctypes.POINTER(CScene), _f, ctypes.c_int, _i32, _f, _f, _f, _i64, ctypes.c_int]
+        L.oracle_render.argtypes = [ctypes.POINTER(CScene), _f, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f, _f, _u32, _i32, _i64,
+                                    ctypes.c_int, ctypes.c_int]
+        L.oracle_bsdf_eval.argtypes = [_f, ctypes.c_int, _f, _f, ctypes.c_int, _d]
+        L.oracle_bsdf_sample.argtypes = [_f, ctypes.c_int, _f, _f, ctypes.c_int, ctypes.c_int, _d]
+        L.oracle_ibl_sample.argtypes = [ctypes.POINTER(CScene), _f, _f, ctypes.c_int, _f]
+        L.oracle_ibl_lookup.argtypes = [ctypes.POINTER(CScene), _f, _f, ctypes.c_int, _f]
+        assert L.oracle_scene_struct_size() == ctypes.sizeof(CScene)
+        assert L.oracle_frame_struct_size() == 52 * 4
+        _lib = L
+    return _lib
+
+
+def _p(a, t=ctypes.c_float):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(t))
+
+
+class OracleScene:
+    """Holds the numpy buffers alive for the C struct."""
+
+    def __init__(self, verts, tris, nodes, bvh_idx, mats, ibl=None, profile=0):
+        self.verts = np.ascontiguousarray(verts, np.float32).reshape(-1)
+        self.tris = np.ascontiguousarray(tris, np.uint32).reshape(-1)
+        self.nodes = np.ascontiguousarray(nodes, np.float32).reshape(-1)
+        self.bvh_idx = np.ascontiguousarray(bvh_idx, np.int32).reshape(-1)
+        self.mats = np.ascontiguousarray(mats, np.float32).reshape(-1)
+        self.c = CScene()
+        self.c.verts, self.c.V = _p(self.verts), self.verts.size // 20
+        self.c.tris, self.c.T = _p(self.tris, ctypes.c_uint32), self.tris.size // 3
+        self.c.nodes, self.c.N = _p(self.nodes), self.nodes.size // 16
+        self.c.bvh_idx, self.c.M = _p(self.bvh_idx, ctypes.c_int32), self.bvh_idx.size
+        self.c.mats, self.c.K = _p(self.mats), self.mats.size // 68
+        self.c.profile = profile
+        self.total = None
+        if ibl is not None:
+            self.set_ibl(ibl)
+
+    def set_ibl(self, ibl):
+        img = ibl["image"]
+        w, h = int(img["width"]), int(img["height"])
+        self.env = np.ascontiguousarray(img["data"], np.float32).reshape(-1)
+        self.pdf, self.cdf, self.ypdf, self.ycdf, self.total = ibl_cdf(self.env, w, h)
+        self.c.env, self.c.ew, self.c.eh = _p(self.env), w, h
+        self.c.pdf, self.c.cdf, self.c.ypdf, self.c.ycdf = _p(self.pdf), _p(self.cdf), _p(self.ypdf), _p(self.ycdf)
+
+    def trace(self, rays, threads=8):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        tri = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        cnt = np.zeros(2, np.int64)
+        lib().oracle_trace(ctypes.byref(self.c), _p(rays), n, _p(tri, ctypes.c_int32), _p(t), _p(u), _p(v),
+                           _p(cnt, ctypes.c_int64), threads)
+        return tri, t, u, v, cnt
+
+    def render(self, block, W, H, first_frame, nframes, rect=None, accum=None, record=True, threads=8, use_double=False):
+        x0, y0, x1, y1 = rect if rect is not None else (0, 0, W, H)
+        rw, rh = x1 - x0, y1 - y0
+        block = np.ascontiguousarray(block, np.float32)
+        if accum is None:
+            accum = np.zeros(W * H * 4, np.float32)
+        samples = np.zeros(nframes * rw * rh * 4, np.float32) if record else None
+        sigs = np.zeros(nframes * rw * rh, np.uint32) if record else None
+        prim = np.zeros(nframes * rw * rh, np.int32) if record else None
+        cnt = np.zeros(3, np.int64)
+        lib().oracle_render(ctypes.byref(self.c), _p(block), W, H, x0, y0, x1, y1, first_frame, nframes, _p(accum),
+                            _p(samples), _p(sigs, ctypes.c_uint32), _p(prim, ctypes.c_int32), _p(cnt, ctypes.c_int64),
+                            threads, 1 if use_double else 0)
+        out = {"accum": accum.reshape(H, W, 4), "counters": cnt}
+        if record:
+            out["samples"] = samples.reshape(nframes, rh, rw, 4)
+            out["sigs"] = sigs.reshape(nframes, rh, rw)
+            out["prim"] = prim.reshape(nframes, rh, rw)
+        return out
+
+    def ibl_sample(self, block, r01):
+        r01 = np.ascontiguousarray(r01, np.float32).reshape(-1, 2)
+        out = np.zeros((r01.shape[0], 4), np.float32)
+        lib().oracle_ibl_sample(ctypes.byref(self.c), _p(np.ascontiguousarray(block, np.float32)), _p(r01),
+                                r01.shape[0], _p(out))
+        return out
+
+    def ibl_lookup(self, block, dirs):
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros((dirs.shape[0], 4), np.float32)
+        lib().oracle_ibl_lookup(ctypes.byref(self.c), _p(np.ascontiguousarray(block, np.float32)), _p(dirs),
+                                dirs.shape[0], _p(out))
+        return out
+
+
+def ibl_cdf(rgba, w, h):
+    rgba = np.ascontiguousarray(rgba, np.float32).reshape(-1)
+    pdf = np.zeros(w * h, np.float32)
+    cdf = np.zeros(w * h, np.float32)
+    ypdf = np.zeros(h, np.float32)
+    ycdf = np.zeros(h, np.float32)
+    tot = lib().oracle_ibl_cdf(_p(rgba), w, h, _p(pdf), _p(cdf), _p(ypdf), _p(ycdf))
+    return pdf, cdf, ypdf, ycdf, tot
+
+
+def rng_seq(seed, px, py, n):
+    """-> initial state (x, y), states after each step [n, 2], u32 outputs, floats."""
+    states = np.zeros(2 * n, np.uint32)
+    u = np.zeros(n, np.uint32)
+    f = np.zeros(n, np.float32)
+    st = np.zeros(2, np.uint32)
+    lib().oracle_rng_seq(seed, px, py, n, _p(states, ctypes.c_uint32), _p(u, ctypes.c_uint32), _p(f),
+                         _p(st, ctypes.c_uint32))
+    return st, states.reshape(n, 2), u, f
+
+
+def rng_step(x, y, n):
+    states = np.zeros(2 * n, np.uint32)
+    lib().oracle_rng_step(int(x), int(y), n, _p(states, ctypes.c_uint32))
+    return states.reshape(n, 2)
+
+
+def bsdf_eval(mat, geo, wo, profile=0, use_double=True):
+    out = np.zeros(8, np.float64)
+    lib().oracle_bsdf_eval(_p(np.ascontiguousarray(mat, np.float32)), profile, _p(np.ascontiguousarray(geo, np.float32)),
+                           _p(np.ascontiguousarray(wo, np.float32)), 1 if use_double else 0, _p(out, ctypes.c_double))
+    return out
+
+
+def bsdf_sample(mat, geo, rnd, profile=0, use_double=True):
+    rnd = np.ascontiguousarray(rnd, np.float32).reshape(-1, 4)
+    out = np.zeros((rnd.shape[0], 8), np.float64)
+    lib().oracle_bsdf_sample(_p(np.ascontiguousarray(mat, np.float32)), profile, _p(np.ascontiguousarray(geo, np.float32)),
+                             _p(rnd), rnd.shape[0], 1 if use_double else 0, _p(out, ctypes.c_double))
+    return out

@@ -1,0 +1,610 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+The reference ships no tests, golden images or known-answer vectors for this
+path (its CI only compiles the shaders), so every function is pinned here by
+what the mathematics fixes, chosen so that a plausible slip (dropped term,
+wrong sign/index, swapped operand) fails:
+
+  rng (SH/rng.glsl)             MWC closed form: state_n == b^-n * state_0 mod (a*b-1)
+                                for both 16-bit lanes; output/float mapping; seeding
+                                re-evaluated with numpy IEEE f32 (parity by re-evaluation)
+  Moeller-Trumbore (bvh.glsl)   exact rational arithmetic (fractions) on random
+                                triangles/rays away from the decision boundaries
+  traversal (bvh.glsl)          brute force over all triangles (float64) gives the same
+                                closest triangle; malformed trees terminate (cycles,
+                                self-links, out-of-range children, step cap)
+  IBL CDF (renderer.ts)         float64 closed form; monotone, ends at 1
+  IBL sampling (lighting.glsl)  histogram of sampled pixels follows pdf (chi^2);
+                                sample pdf == ibl_pdf at the sampled texel
+  BSDF (material_kernel.glsl)   Lambert closed form; clearcoat == textbook GGX
+                                (F0 = 0.04, height-correlated Smith); sample weight
+                                == eval/pdf; pdf integrates to <= 1 (~1 when diffuse);
+                                sampled-direction histogram follows pdf (chi^2);
+                                full profile with zero iridescence/dispersion == lean
+  integrator (misptdl.glsl)     emissive furnace closed form including the
+                                (1/0.9)^k Russian-roulette factor; constant-environment
+                                NEE+MIS furnace; background; running mean
+Parity unpinned (no check above distinguishes a plausible slip): the thin-film
+Airy sum (iridescence) and the dispersion channel IORs; they are checked only for
+invariants (finite, in range, ordering) and GPU-vs-oracle agreement.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle_ref as O
+
+
+# --------------------------------------------------------------------- RNG
+def test_rng_mwc_closed_form():
+    # each 16-bit lane is a multiply-with-carry generator t' = a*(t & 0xffff) + (t >> 16);
+    # then b*t' = (a*b - 1)*(t & 0xffff) + t, i.e. t' == b^-1 * t (mod a*b - 1)
+    rng = np.random.default_rng(0)
+    b = 1 << 16
+    for x0, y0 in rng.integers(0, 2**32, (20, 2)):
+        st = O.rng_step(int(x0), int(y0), 50)
+        for lane, (a, s0) in enumerate(((36969, int(x0)), (18000, int(y0)))):
+            m = a * b - 1
+            binv = pow(b, -1, m)
+            for n in (1, 2, 7, 50):
+                assert int(st[n - 1, lane]) % m == (pow(binv, n, m) * s0) % m
+
+
+def test_rng_outputs_and_floats():
+    st0, states, u, f = O.rng_seq(24, 10, 7, 200)
+    x, y = states[:, 0].astype(np.uint64), states[:, 1].astype(np.uint64)
+    np.testing.assert_array_equal(u, ((x << np.uint64(16)) + y).astype(np.uint64) & np.uint64(0xFFFFFFFF))
+    assert ((f >= 0) & (f <= 1)).all()
+    np.testing.assert_array_equal(f, (u.astype(np.float32) / np.float32(4294967296.0)).astype(np.float32))
+
+
+def test_rng_seeding_reevaluated_in_f32():
+    for seed, px, py in ((0, 0, 0), (8, 12, 200), (64 * 12, 1919, 1079), (17, 5, 3)):
+        st0, _, _, _ = O.rng_seq(seed, px, py, 1)
+        off = np.float32(seed * 17)
+        fx, fy = np.float32(px) + np.float32(0.5), np.float32(py) + np.float32(0.5)
+        k1, k2 = np.float32(397.6432), np.float32(32.9875)
+        x = np.uint32(k1 * (fx + off)) ^ np.uint32(k2 * (fy + off))
+        y = np.uint32(k1 * (fy + np.float32(0))) ^ np.uint32(k2 * (fx + np.float32(0)))
+        assert (st0[0], st0[1]) == (x, y)
+
+
+# --------------------------------------------------------------------- geometry helpers
+def _scene_from_tris(P, mats=None):
+    """Oracle scene over explicit triangles (P: [T,3,3]) with a 1-leaf-per-tri BVH from the engine builder."""
+    from dspt import build_bvh
+    T = len(P)
+    verts = np.zeros((3 * T, 20), np.float32)
+    verts[:, 0:3] = P.reshape(-1, 3)
+    verts[:, 16:20] = 1
+    tris = np.arange(3 * T, dtype=np.uint32)
+    nodes, bidx = build_bvh(P.reshape(-1, 3).astype(np.float32), tris, max_leaf=2)
+    if mats is None:
+        from dspt import MaterialData
+        mats = MaterialData().data
+    return O.OracleScene(verts, tris, nodes, bidx, mats)
+
+
+def _rays(org, d, tfar=1e5):
+    r = np.zeros((len(org), 8), np.float32)
+    r[:, 0:3] = org
+    r[:, 3] = tfar
+    r[:, 4:7] = d
+    return r
+
+
+def test_moller_trumbore_exact_rational():
+    rng = np.random.default_rng(5)
+    P = rng.integers(-8, 9, (1, 3, 3)).astype(np.float32) / np.float32(4)
+    while abs(np.linalg.det(np.stack([P[0, 1] - P[0, 0], P[0, 2] - P[0, 0], [0.3, 0.7, 0.2]]))) < 0.5:
+        P = rng.integers(-8, 9, (1, 3, 3)).astype(np.float32) / np.float32(4)
+    sc = _scene_from_tris(P)
+    n = 3000
+    org = (rng.integers(-40, 41, (n, 3)) / 8).astype(np.float32)
+    tgt = (P[0].T @ rng.dirichlet([1, 1, 1], n).T).T + rng.normal(0, 0.3, (n, 3))
+    d = (tgt - org).astype(np.float32)
+    tri, t, u, v, _ = sc.trace(_rays(org, d))
+    checked = 0
+    p0, p1, p2 = [[Fraction(float(c)) for c in P[0, k]] for k in range(3)]
+    for i in range(n):
+        o = [Fraction(float(c)) for c in org[i]]
+        dd = [Fraction(float(c)) for c in d[i]]
+        e1 = [p1[k] - p0[k] for k in range(3)]
+        e2 = [p2[k] - p0[k] for k in range(3)]
+        s = [o[k] - p0[k] for k in range(3)]
+        # Cramer's rule: o + t d = p0 + u e1 + v e2
+        M = [[-dd[k], e1[k], e2[k]] for k in range(3)]
+
+        def det3(A):
+            return (A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0])
+                    + A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]))
+        D = det3(M)
+        if abs(D) < Fraction(1, 10):
+            continue
+        cols = []
+        for c in range(3):
+            A = [row[:] for row in M]
+            for k in range(3):
+                A[k][c] = s[k]
+            cols.append(det3(A) / D)
+        te, ue, ve = cols
+        margin = Fraction(1, 1000)
+        inside = ue > margin and ve > margin and ue + ve < 1 - margin and te > margin
+        outside = ue < -margin or ve < -margin or ue + ve > 1 + margin or te < -margin
+        if inside:
+            assert tri[i] == 0, i
+            assert abs(float(te) - t[i]) <= 1e-5 * max(1.0, abs(float(te)))
+            assert abs(float(ue) - u[i]) <= 1e-4 and abs(float(ve) - v[i]) <= 1e-4
+            checked += 1
+        elif outside:
+            assert tri[i] == -1, i
+            checked += 1
+    assert checked > 1000
+
+
+def test_traversal_matches_brute_force(sphere_field_small):
+    wl = sphere_field_small
+    s = wl.scene
+    sc = O.OracleScene(s.vertexBuffer, s.triangleIndexBuffer, s.bvhNodes, s.bvhTriangleIndices, s.getFlatMaterialBuffer())
+    rng = np.random.default_rng(7)
+    n = 400
+    org = np.tile(np.float32([0, 3, 12]), (n, 1))
+    d = rng.uniform([-6, -0.5, -6], [6, 3.5, 6], (n, 3)) - org
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    tri, t, _, _, _ = sc.trace(_rays(org, d))
+    V = np.asarray(s.vertexBuffer, np.float64).reshape(-1, 20)[:, :3]
+    F = np.asarray(s.triangleIndexBuffer, np.int64).reshape(-1, 3)
+    p0, e1, e2 = V[F[:, 0]], V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]]
+    for i in range(n):
+        o, dd = org[i].astype(np.float64), d[i].astype(np.float64)
+        pv = np.cross(dd, e2)
+        det = np.einsum("ij,ij->i", e1, pv)
+        ok = np.abs(det) > 1e-12
+        inv = np.where(ok, 1.0 / np.where(ok, det, 1), 0)
+        sv = o - p0
+        uu = np.einsum("ij,ij->i", sv, pv) * inv
+        qv = np.cross(sv, e1)
+        vv = (qv @ dd) * inv
+        tt = np.einsum("ij,ij->i", e2, qv) * inv
+        hit = ok & (uu >= 0) & (vv >= 0) & (uu + vv <= 1) & (tt > 1e-8)
+        if not hit.any():
+            assert tri[i] == -1
+            continue
+        best = np.argmin(np.where(hit, tt, np.inf))
+        tb = tt[best]
+        if tri[i] != best:
+            # only a numerical near-tie may pick another triangle
+            assert tri[i] >= 0 and hit[tri[i]] and abs(tt[tri[i]] - tb) <= 1e-5 * tb, (i, tri[i], best)
+        assert abs(t[i] - tb) <= 1e-5 * tb
+
+
+def test_traversal_terminates_on_malformed_trees():
+    P = np.array([[[0, 0, 0], [1, 0, 0], [0, 1, 0]]], np.float32)
+    sc = _scene_from_tris(P)
+    bad = np.zeros((3, 16), np.float32)
+    # node 0: both children = node 1 / node 0 (self-link skipped), node 1: child back to 0 (cycle)
+    big = [-10, -10, -10]
+    big2 = [10, 10, 10]
+    for k in range(2):
+        bad[k, 0:3], bad[k, 4:7], bad[k, 8:11], bad[k, 12:15] = big, big2, big, big2
+    bad[0, 3], bad[0, 7] = 1, 0
+    bad[1, 3], bad[1, 7] = 0, 99  # cycle + out of range
+    sc.nodes = bad.reshape(-1).copy()
+    sc.c.nodes = O._p(sc.nodes)
+    sc.c.N = 2
+    r = _rays(np.float32([[0.2, 0.2, 1]]), np.float32([[0, 0, -1]]))
+    tri, t, _, _, cnt = sc.trace(r)
+    assert tri[0] == -1
+    assert cnt[0] <= 4096  # step cap
+
+
+# --------------------------------------------------------------------- IBL
+def _toy_ibl(w, h, seed):
+    rng = np.random.default_rng(seed)
+    img = np.ones((h, w, 4), np.float32)
+    img[..., :3] = rng.lognormal(0, 1.0, (h, w, 3)).astype(np.float32)
+    return {"image": {"data": img.reshape(-1), "width": w, "height": h}}
+
+
+def test_ibl_cdf_closed_form():
+    w, h = 37, 19
+    ibl = _toy_ibl(w, h, 1)
+    rgba = ibl["image"]["data"].reshape(h, w, 4).astype(np.float64)
+    pdf, cdf, ypdf, ycdf, tot = O.ibl_cdf(ibl["image"]["data"], w, h)
+    # luminance and row sums are stored in Float32Arrays (renderer.ts:808-827)
+    lum = (rgba[..., 0] * 0.299 + rgba[..., 1] * 0.587 + rgba[..., 2] * 0.114).astype(np.float32).astype(np.float64)
+    rs = lum.sum(1).astype(np.float32).astype(np.float64)
+    np.testing.assert_allclose(pdf.reshape(h, w), lum / rs[:, None], rtol=1e-6)
+    np.testing.assert_allclose(cdf.reshape(h, w), np.cumsum(lum / rs[:, None], 1), rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(ypdf, rs / rs.sum(), rtol=1e-6)
+    assert abs(tot - rs.sum()) <= 1e-12 * tot
+    assert (np.diff(cdf.reshape(h, w), axis=1) >= 0).all() and (cdf.reshape(h, w)[:, -1] == 1).all()
+    assert (np.diff(ycdf) >= 0).all() and ycdf[-1] == 1
+    # a black row: sum == 0 -> divided by 1
+    ibl["image"]["data"].reshape(h, w, 4)[3, :, :3] = 0
+    pdf, cdf, _, _, _ = O.ibl_cdf(ibl["image"]["data"], w, h)
+    assert (pdf.reshape(h, w)[3] == 0).all() and cdf.reshape(h, w)[3, -1] == 1
+
+
+def _ibl_block(w, h, rot=0.0):
+    from dspt import ViewportCamera, pack_frame_block
+    return pack_frame_block(ViewportCamera(), 8, 8, ibl_size=(w, h), ibl_rotation_rad=rot)
+
+
+def test_ibl_sampling_follows_pdf():
+    w, h = 16, 8
+    ibl = _toy_ibl(w, h, 2)
+    sc = O.OracleScene(np.zeros(60, np.float32), np.arange(3, dtype=np.uint32), np.zeros(16, np.float32),
+                       np.zeros(1, np.int32), np.zeros(68, np.float32), ibl)
+    blk = _ibl_block(w, h)
+    rng = np.random.default_rng(3)
+    n = 200000
+    out = sc.ibl_sample(blk, rng.uniform(0, 1, (n, 2)).astype(np.float32))
+    d = out[:, :3].astype(np.float64)
+    theta = np.arccos(np.clip(d[:, 1], -1, 1))
+    phi = np.mod(np.arctan2(d[:, 2], d[:, 0]), 2 * np.pi)
+    # directions come from texel corners: recover the texel index
+    ix = np.clip(np.round(phi / (2 * np.pi) * w).astype(int) % w, 0, w - 1)
+    iy = np.clip(np.round(theta / np.pi * h).astype(int), 0, h - 1)
+    counts = np.bincount(iy * w + ix, minlength=w * h).astype(np.float64)
+    expect = (sc.ypdf[:, None].astype(np.float64) * sc.pdf.reshape(h, w)).reshape(-1) * n
+    # texel row 0 maps to the pole (theta = 0), where every column gives the same direction
+    expect[0] = expect[:w].sum()
+    expect[1:w] = 0
+    m = expect > 5
+    chi2 = ((counts[m] - expect[m]) ** 2 / expect[m]).sum()
+    dof = m.sum() - 1
+    assert chi2 < dof + 6 * math.sqrt(2 * dof), (chi2, dof)
+    # the returned pdf is w*h*ypdf*pdf over the sin(theta) Jacobian of the corner direction
+    pw = 1.0 / (2 * np.pi * np.pi * np.maximum(1e-3, np.sin(iy * np.pi / h)))
+    keep = iy > 0
+    np.testing.assert_allclose(out[keep, 3], (w * h * sc.ypdf[iy] * sc.pdf[iy * w + ix] * pw)[keep], rtol=2e-5)
+
+
+def test_ibl_lookup_constant_and_pdf_normalised():
+    w, h = 32, 16
+    img = np.ones((h, w, 4), np.float32) * np.float32(0.75)
+    ibl = {"image": {"data": img.reshape(-1), "width": w, "height": h}}
+    sc = O.OracleScene(np.zeros(60, np.float32), np.arange(3, dtype=np.uint32), np.zeros(16, np.float32),
+                       np.zeros(1, np.int32), np.zeros(68, np.float32), ibl)
+    rng = np.random.default_rng(4)
+    dirs = rng.normal(size=(20000, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    out = sc.ibl_lookup(_ibl_block(w, h, rot=0.3), dirs.astype(np.float32))
+    np.testing.assert_allclose(out[:, :3], 0.75, rtol=1e-6)  # bilinear of a constant
+    # E_uniform[pdf / (1/4pi)] == 1 for a normalised directional pdf
+    assert abs(out[:, 3].mean() * 4 * np.pi - 1.0) < 0.03
+
+
+# --------------------------------------------------------------------- BSDF
+def _mat(**kw):
+    from dspt import MaterialData
+    return MaterialData(**kw).data
+
+
+def _geo(n=(0, 0, 1), wi=(0.3, -0.2, 0.9)):
+    n = np.asarray(n, np.float64)
+    n /= np.linalg.norm(n)
+    wi = np.asarray(wi, np.float64)
+    wi /= np.linalg.norm(wi)
+    t = np.cross(n, [-n[2], n[0], -n[1]])
+    t /= np.linalg.norm(t)
+    return np.concatenate([n, n, t, [1.0], wi]).astype(np.float32), wi
+
+
+def _hemi_dirs(n, rng):
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d[:, 2] = np.abs(d[:, 2])
+    return d
+
+
+def test_bsdf_lambert_closed_form():
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        albedo = rng.uniform(0, 1, 3)
+        mat = _mat(baseColorFactor=albedo, specularFactor=0.0, roughnessFactor=rng.uniform(0, 1))
+        geo, wi = _geo(wi=rng.normal(size=3) * [1, 1, 0] + [0, 0, 2])
+        for l in _hemi_dirs(20, rng):
+            e = O.bsdf_eval(mat, geo, l.astype(np.float32))
+            np.testing.assert_allclose(e[:3], albedo.astype(np.float32).astype(np.float64) / np.pi * float(np.float32(l[2])),
+                                       rtol=1e-6)
+
+
+def _ggx_textbook(f0, alpha, n, v, l):
+    h = (v + l) / np.linalg.norm(v + l)
+    nv, nl, nh, vh = n @ v, n @ l, n @ h, v @ h
+    D = alpha ** 2 / (np.pi * (nh * nh * (alpha ** 2 - 1) + 1) ** 2)
+
+    def lam(c):
+        t2 = (1 - c * c) / (c * c)
+        return 0.5 * (-1 + math.sqrt(1 + alpha ** 2 * t2))
+    G = 1.0 / (1 + lam(nv) + lam(nl))
+    F = f0 + (1 - f0) * (1 - vh) ** 5
+    return F * D * G / (4 * nv * nl)
+
+
+def test_bsdf_clearcoat_is_textbook_ggx():
+    rng = np.random.default_rng(2)
+    for cr in (0.2, 0.5, 0.9):
+        mat = _mat(baseColorFactor=[0, 0, 0], specularFactor=0.0, clearcoatFactor=1.0, clearcoatRoughnessFactor=cr)
+        geo, wi = _geo(wi=(0.2, 0.1, 0.95))
+        n = np.array([0, 0, 1.0])
+        for l in _hemi_dirs(30, rng):
+            l32 = l.astype(np.float32).astype(np.float64)
+            if l32[2] < 0.05:
+                continue
+            e = O.bsdf_eval(mat, geo, l32.astype(np.float32))
+            ref = _ggx_textbook(0.04, cr * cr, n, wi.astype(np.float32).astype(np.float64), l32) * l32[2]
+            np.testing.assert_allclose(e[0], ref, rtol=2e-5)
+
+
+MATS = {
+    "diffuse_spec": dict(baseColorFactor=[0.8, 0.5, 0.3], roughnessFactor=0.4),
+    "metal": dict(baseColorFactor=[0.9, 0.8, 0.7], metallicFactor=1.0, roughnessFactor=0.3),
+    "metal_aniso": dict(baseColorFactor=[0.9, 0.9, 0.9], metallicFactor=1.0, roughnessFactor=0.4, anisotropy=0.7,
+                        anisotropyDirection=[0.6, 0.8, 0]),
+    "clearcoat": dict(baseColorFactor=[0.2, 0.5, 0.8], roughnessFactor=0.7, clearcoatFactor=1.0,
+                      clearcoatRoughnessFactor=0.2),
+    "sheen": dict(baseColorFactor=[0.3, 0.3, 0.3], roughnessFactor=0.5, sheenColorFactor=[0.9, 0.6, 0.6],
+                  sheenRoughnessFactor=0.5),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MATS))
+def test_bsdf_sample_consistent_with_eval_and_pdf(name):
+    mat = _mat(**MATS[name])
+    geo, wi = _geo(wi=(0.3, -0.2, 0.9))
+    rng = np.random.default_rng(11)
+    rnd = rng.uniform(0, 1, (400, 4)).astype(np.float32)
+    s = O.bsdf_sample(mat, geo, rnd)
+    ok = 0
+    for k in range(len(s)):
+        d, pdf, w = s[k, :3], s[k, 3], s[k, 4:7]
+        if d[2] <= 1e-3 or pdf < 1e-3:
+            continue
+        e = O.bsdf_eval(mat, geo, d.astype(np.float32))
+        # eval/pdf evaluated independently at the float32 direction
+        np.testing.assert_allclose(pdf, e[3], rtol=2e-4)
+        np.testing.assert_allclose(w, e[:3] / e[3], rtol=2e-4, atol=1e-7)
+        ok += 1
+    assert ok > 300
+
+
+def _pdf_integral(mat, geo, n=20000, seed=12):
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    p = np.array([O.bsdf_eval(mat, geo, x.astype(np.float32))[3] for x in d])
+    return p.mean() * 4 * np.pi, p.std() * 4 * np.pi / math.sqrt(n)
+
+
+@pytest.mark.parametrize("name", ["diffuse_spec", "clearcoat", "sheen"])
+def test_bsdf_pdf_integrates_to_one_when_diffuse_dominated(name):
+    geo, wi = _geo(wi=(0.1, 0.0, 1.0))
+    integral, err = _pdf_integral(_mat(**MATS[name]), geo)
+    assert abs(integral - 1.0) < 4 * err + 0.01, (integral, err)
+
+
+def test_bsdf_metal_pdf_integral_equals_visible_lobe_fraction():
+    # a pure GGX reflection lobe integrates to the probability that the mirror
+    # direction of a half-vector drawn from D(h)(n.h) stays above the horizon
+    rng = np.random.default_rng(21)
+    for rough, wi_in in ((0.3, (0.1, 0.0, 1.0)), (0.6, (0.8, 0.0, 0.6)), (0.9, (0.5, 0.3, 0.4))):
+        geo, wi = _geo(wi=wi_in)
+        integral, err = _pdf_integral(_mat(baseColorFactor=[1, 1, 1], metallicFactor=1.0, roughnessFactor=rough), geo)
+        a = rough * rough
+        u1, u2 = rng.uniform(size=200000), rng.uniform(size=200000)
+        th = np.arctan(a * np.sqrt(u1 / (1 - u1)))
+        ph = 2 * np.pi * u2
+        h = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], -1)
+        l = -wi + 2 * (h @ wi)[:, None] * h
+        frac = (l[:, 2] > 0).mean()
+        assert abs(integral - frac) < 4 * err + 0.01, (rough, integral, frac)
+
+
+def test_bsdf_pdf_never_exceeds_one_anisotropic():
+    geo, wi = _geo(wi=(0.1, 0.0, 1.0))
+    integral, err = _pdf_integral(_mat(**MATS["metal_aniso"]), geo)
+    assert 0.5 < integral < 1.0 + 4 * err + 0.01
+
+
+# Anisotropic GGX is excluded on purpose: the reference draws the half-vector
+# azimuth uniformly (KF:2806-2826 sampleGGXHalfVector, phi = 2*pi*u1) while its
+# pdf is the anisotropic D(h)(n.h), so for anisotropy > 0 the sampled density and
+# the pdf used for the weights differ in the reference itself; the oracle keeps
+# that behaviour (see test_bsdf_anisotropic_sampling_keeps_uniform_azimuth).
+@pytest.mark.parametrize("name", ["diffuse_spec", "metal", "clearcoat", "sheen"])
+def test_bsdf_sampling_histogram_follows_pdf(name):
+    mat = _mat(**MATS[name])
+    geo, wi = _geo(wi=(0.3, -0.2, 0.9))
+    rng = np.random.default_rng(13)
+    n = 60000
+    s = O.bsdf_sample(mat, geo, rng.uniform(0, 1, (n, 4)).astype(np.float32))
+    d = s[:, :3]
+    up = d[:, 2] > 0
+    # bin on (cos theta, phi) over the upper hemisphere: equal solid angle bins
+    nb_c, nb_p = 8, 16
+    ct = np.clip(d[up, 2], 0, 1 - 1e-12)
+    ph = np.mod(np.arctan2(d[up, 1], d[up, 0]), 2 * np.pi)
+    bi = np.minimum((ct * nb_c).astype(int), nb_c - 1) * nb_p + np.minimum((ph / (2 * np.pi) * nb_p).astype(int), nb_p - 1)
+    counts = np.bincount(bi, minlength=nb_c * nb_p).astype(np.float64)
+    # expected mass per bin by stratified quadrature of the pdf
+    exp = np.zeros(nb_c * nb_p)
+    q = 6
+    for b in range(nb_c * nb_p):
+        ic, ip = divmod(b, nb_p)
+        cs = (ic + (np.arange(q) + 0.5) / q) / nb_c
+        ps = (ip + (np.arange(q) + 0.5) / q) / nb_p * 2 * np.pi
+        C, Pp = np.meshgrid(cs, ps)
+        S = np.sqrt(1 - C ** 2)
+        dirs = np.stack([S * np.cos(Pp), S * np.sin(Pp), C], -1).reshape(-1, 3)
+        pv = np.array([O.bsdf_eval(mat, geo, x.astype(np.float32))[3] for x in dirs])
+        exp[b] = pv.mean() * (2 * np.pi / (nb_c * nb_p)) * n
+    m = exp > 20
+    chi2 = ((counts[m] - exp[m]) ** 2 / exp[m]).sum()
+    dof = m.sum()
+    assert chi2 < dof + 8 * math.sqrt(2 * dof), (chi2, dof)
+
+
+def _aniso_mirror_azimuths(alpha_x, alpha_y, n, rng, textbook):
+    """Azimuths of mirror directions about half-vectors drawn (view = normal) by
+    the reference's sampler (uniform phi) or by the textbook anisotropic GGX
+    sampler (phi = atan(ay/ax tan(2 pi u)))."""
+    u0, u1 = rng.uniform(size=n), rng.uniform(size=n)
+    if textbook:
+        ph = np.arctan(alpha_y / alpha_x * np.tan(2 * np.pi * u1))
+        ph = np.where((u1 > 0.25) & (u1 <= 0.75), ph + np.pi, ph)
+    else:
+        ph = 2 * np.pi * u1
+    A = np.cos(ph) ** 2 / alpha_x ** 2 + np.sin(ph) ** 2 / alpha_y ** 2
+    ct = 1 / np.sqrt(1 + u0 / (1 - u0) / A)
+    st = np.sqrt(1 - ct * ct)
+    # tangent frame of the probe: t = (0, 1, 0), b = (-1, 0, 0), n = (0, 0, 1)
+    h = np.stack([-st * np.sin(ph), st * np.cos(ph), ct], -1)
+    l = -np.array([0, 0, 1.0]) + 2 * h[:, 2:3] * h
+    l = l[l[:, 2] > 0.05]
+    return np.mod(np.arctan2(l[:, 1], l[:, 0]), 2 * np.pi)
+
+
+def test_bsdf_anisotropic_sampling_uses_uniform_azimuth():
+    # The reference draws the half-vector azimuth uniformly (KF:2806-2826); the
+    # oracle's sampled directions follow that sampler and not the textbook one.
+    geo, wi = _geo(wi=(0.0, 0.0, 1.0))
+    rough, aniso = 0.5, 0.9
+    mat = _mat(baseColorFactor=[1, 1, 1], metallicFactor=1.0, roughnessFactor=rough, anisotropy=aniso)
+    rng = np.random.default_rng(22)
+    s = O.bsdf_sample(mat, geo, rng.uniform(0, 1, (40000, 4)).astype(np.float32))
+    d = s[(s[:, 2] > 0.05) & (s[:, 4] > 0), :3]
+    ph = np.mod(np.arctan2(d[:, 1], d[:, 0]), 2 * np.pi)
+    a = rough * rough
+    ax, ay = a * (1 - aniso * aniso) + aniso * aniso, a
+
+    def hist(x):
+        c = np.bincount((x / (2 * np.pi) * 16).astype(int) % 16, minlength=16).astype(np.float64)
+        return c / c.sum()
+    ho = hist(ph)
+    for textbook, ok in ((False, True), (True, False)):
+        hr = hist(_aniso_mirror_azimuths(ax, ay, 400000, rng, textbook))
+        chi2 = (len(ph) * (ho - hr) ** 2 / hr).sum()
+        assert (chi2 < 15 + 8 * math.sqrt(30)) == ok, (textbook, chi2)
+
+
+def test_bsdf_white_furnace():
+    rng = np.random.default_rng(14)
+    geo, wi = _geo(wi=(0.2, 0.3, 0.9))
+    # white Lambertian: weight == 1 for every sample
+    s = O.bsdf_sample(_mat(baseColorFactor=[1, 1, 1], specularFactor=0.0), geo, rng.uniform(0, 1, (2000, 4)).astype(np.float32))
+    np.testing.assert_allclose(s[:, 4:7], 1.0, rtol=1e-6)
+    # white rough metal: the sampling estimator E[w] equals the directional albedo
+    # integral of eval (uniform hemisphere quadrature). The analytic energy
+    # compensation of the Enterprise model (KF:5-91) overshoots at this roughness
+    # (albedo > 1), which the reference has too; only the bound is asserted.
+    mat = _mat(baseColorFactor=[1, 1, 1], metallicFactor=1.0, roughnessFactor=0.6)
+    s = O.bsdf_sample(mat, geo, rng.uniform(0, 1, (20000, 4)).astype(np.float32))
+    e_samp, e_err = s[:, 4].mean(), s[:, 4].std() / math.sqrt(len(s))
+    dirs = _hemi_dirs(20000, rng)
+    ev = np.array([O.bsdf_eval(mat, geo, x.astype(np.float32))[0] for x in dirs]) * 2 * np.pi
+    e_quad, q_err = ev.mean(), ev.std() / math.sqrt(len(ev))
+    assert abs(e_samp - e_quad) < 4 * math.hypot(e_err, q_err), (e_samp, e_quad)
+    assert 0.9 < e_quad < 1.2
+
+
+def test_full_profile_equals_lean_without_film_and_dispersion():
+    rng = np.random.default_rng(15)
+    geo, wi = _geo(wi=(0.4, 0.1, 0.8))
+    for kw in MATS.values():
+        mat = _mat(**kw, transmissionFactor=0.5)
+        for l in np.concatenate([_hemi_dirs(10, rng), -_hemi_dirs(5, rng)]):
+            a = O.bsdf_eval(mat, geo, l.astype(np.float32), profile=0)
+            b = O.bsdf_eval(mat, geo, l.astype(np.float32), profile=1)
+            np.testing.assert_array_equal(a, b)
+
+
+def test_film_and_dispersion_invariants():
+    rng = np.random.default_rng(16)
+    geo, wi = _geo(wi=(0.4, 0.1, 0.8))
+    irid = _mat(baseColorFactor=[0.5, 0.5, 0.5], iridescenceFactor=1.0, iridescenceIor=1.5,
+                iridescenceThicknessMinimum=300, iridescenceThicknessMaximum=500)
+    for l in _hemi_dirs(20, rng):
+        e = O.bsdf_eval(irid, geo, l.astype(np.float32), profile=1)
+        assert np.isfinite(e).all() and (e[:3] >= 0).all()
+    disp = _mat(transmissionFactor=1.0, thinWalled=0, dispersion=3.0, ior=1.5, roughnessFactor=0.1)
+    s = O.bsdf_sample(disp, geo, rng.uniform(0, 1, (4000, 4)).astype(np.float32), profile=1)
+    refr = s[:, 2] < 0
+    assert refr.any()
+    w = s[refr, 4:7]
+    # a refracted dispersion sample carries exactly one colour channel
+    assert ((w > 0).sum(1) <= 1).all()
+
+
+# --------------------------------------------------------------------- integrator
+def _furnace_sphere(mat, n_seg=48):
+    from dspt import PathtracingSceneData, build_bvh, scenes
+    p, nrm, uv, f = scenes.uv_sphere(10.0, n_seg, n_seg // 2)
+    s = PathtracingSceneData()
+    mb = scenes.MeshBuilder()
+    s.addMaterial(mat)
+    mb.add(p, -nrm, uv, f, 0)
+    mb.finish(s)
+    nodes, bidx = build_bvh(s.bvhPositionBuffer, s.bvhIndexBuffer)
+    return s, nodes, bidx
+
+
+def test_integrator_emissive_furnace_with_rr_factor():
+    from dspt import MaterialData, ViewportCamera, pack_frame_block
+    rho, e, B = 0.5, 1.0, 6
+    m = MaterialData(baseColorFactor=[rho, rho, rho], specularFactor=0.0, emissiveFactor=[e, e, e])
+    s, nodes, bidx = _furnace_sphere(m)
+    sc = O.OracleScene(s.vertexBuffer, s.triangleIndexBuffer, nodes, bidx, s.getFlatMaterialBuffer())
+    W = H = 32
+    cam = ViewportCamera(position=(0, 0, 0), target=(0, 0, -1))
+    blk = pack_frame_block(cam, W, H, max_bounces=B, use_ibl=False, show_background=False, clamp_threshold=0.0)
+    out = sc.render(blk, W, H, 1, 24)
+    L = out["samples"][..., 0].astype(np.float64)
+    # E[L] = e * sum_b rho^b (1/0.9)^(b+1) P(survive RR through b), P = 0.9^max(0, b-2)
+    expect = e * sum(rho ** b * (1 / 0.9) ** (b + 1) * 0.9 ** max(0, b - 2) for b in range(B))
+    assert abs(L.mean() - expect) < 4 * L.std() / math.sqrt(L.size) + 1e-6, (L.mean(), expect)
+    # the running mean over frames is the arithmetic mean of the frame samples
+    np.testing.assert_allclose(out["accum"][..., 0], L.mean(0), rtol=2e-6, atol=1e-7)
+
+
+def test_integrator_constant_environment_nee_mis():
+    from dspt import MaterialData, ViewportCamera, pack_frame_block, scenes
+    rho, Le = 0.6, 1.0
+    w, h = 64, 32
+    img = np.full((h, w, 4), Le, np.float32)
+    ibl = {"image": {"data": img.reshape(-1), "width": w, "height": h}}
+    m = MaterialData(baseColorFactor=[rho, rho, rho], specularFactor=0.0)
+    from dspt import PathtracingSceneData, build_bvh
+    s = PathtracingSceneData()
+    mb = scenes.MeshBuilder()
+    s.addMaterial(m)
+    P = np.array([[-1e3, 0, -1e3], [1e3, 0, -1e3], [1e3, 0, 1e3], [-1e3, 0, 1e3]], np.float64)
+    mb.add(P, np.tile([0, 1, 0], (4, 1)), None, [[0, 2, 1], [0, 3, 2]], 0)
+    mb.finish(s)
+    nodes, bidx = build_bvh(s.bvhPositionBuffer, s.bvhIndexBuffer)
+    sc = O.OracleScene(s.vertexBuffer, s.triangleIndexBuffer, nodes, bidx, s.getFlatMaterialBuffer(), ibl)
+    cam = ViewportCamera(position=(0, 1, 0), target=(0, 0, -0.2))
+    W = H = 32
+    blk = pack_frame_block(cam, W, H, max_bounces=1, ibl_size=(w, h), clamp_threshold=0.0)
+    out = sc.render(blk, W, H, 1, 32)
+    hit = out["prim"] >= 0
+    L = out["samples"][..., 0][hit].astype(np.float64)
+    # one bounce: (NEE + BSDF-sampled env, balance heuristic) estimates rho*Le, times the RR factor 1/0.9
+    expect = rho * Le / 0.9
+    assert abs(L.mean() - expect) < 4 * L.std() / math.sqrt(L.size) + 2e-3, (L.mean(), expect)
+    # primary misses return the environment unclamped
+    np.testing.assert_allclose(out["samples"][..., 0][~hit], Le, rtol=1e-6)
+
+
+def test_integrator_background_colour():
+    from dspt import MaterialData, ViewportCamera, pack_frame_block
+    P = np.array([[[100, 100, 100], [101, 100, 100], [100, 101, 100]]], np.float32)
+    sc = _scene_from_tris(P, MaterialData().data)
+    blk = pack_frame_block(ViewportCamera(), 8, 8, use_ibl=False, show_background=False,
+                           background_color=(0.5, 0.25, 1.0, 0.75))
+    out = sc.render(blk, 8, 8, 1, 1)
+    np.testing.assert_allclose(out["samples"][0, 0, 0], [0.5 ** 2.2, 0.25 ** 2.2, 1.0, 0.75], rtol=1e-6)
