@@ -1,0 +1,36 @@
+// Types of the Node drop-in (mirrors packages/lib/renderer.ts PathtracingRenderer).
+export interface PathtracingCamera {
+  fov: number;
+  near: number;
+  matrixWorld: { elements: ArrayLike<number> };
+  position: { x: number; y: number; z: number };
+}
+export interface SceneBuffers {
+  vertexBuffer: Float32Array;          // 20 f32 per vertex
+  triangleIndexBuffer: Uint32Array;
+  bvhPositionBuffer: Float32Array;
+  bvhIndexBuffer: Uint32Array;
+  materials: { data: Float32Array }[]; // 68 f32 each
+  lights?: { position: number[]; emission: number[] }[];
+  bvhNodes?: Float32Array;             // optional prebuilt bvh_gpu nodes
+  bvhTriangleIndices?: Int32Array;
+}
+export declare class PathtracingRenderer {
+  constructor(parameters?: { materialProfile?: 'webgl-lean' | 'webgl-full'; device?: number; width?: number; height?: number });
+  maxBounces: number; useIBL: boolean; showBackground: boolean; iblRotation: number;
+  backgroundColor: number[]; rayEps: number; clampThreshold: number; exposure: number;
+  tonemapping: string; enableGamma: boolean; enableFxaa: boolean; pixelRatio: number; tileRes: number;
+  readonly diagnostics: { backend: string; sampleCount: number; mode: string; profiling: Record<string, number> };
+  setScene(scene: SceneBuffers, progress?: (e: { phase: string }) => void): Promise<void>;
+  setIBL(texture: { image: { data: Float32Array; width: number; height: number } }): void;
+  resize(width: number, height: number): void;
+  resetAccumulation(): void;
+  stopRendering(): void;
+  setInteractionMode(flag: boolean): void;
+  render(camera: PathtracingCamera, numSamples: number, tileFinishedCB?: () => void,
+         frameFinishedCB?: (frameCount: number) => void, renderingFinishedCB?: () => void,
+         framesPerCall?: number): Promise<void>;
+  readAccumulation(): Float32Array;
+  counters(): Float64Array;
+}
+export declare function packFrameBlock(r: PathtracingRenderer, camera: PathtracingCamera, frameCount: number): Float32Array;

@@ -1,0 +1,138 @@
+'use strict';
+// Node-side drop-in for the reference's PathtracingRenderer
+// (packages/lib/renderer.ts:116-1109) backed by the MI355X engine through the
+// N-API addon. Same method names, argument meaning and error behaviour:
+// setScene(scene) / setIBL(texture) / resize(w, h) / render(camera, numSamples,
+// tileCB, frameCB, finishedCB) / stopRendering() / resetAccumulation() and the
+// settings properties. The GL post chain (tonemap, FXAA, display) is not part
+// of the engine; readAccumulation() returns the linear RGBA32F buffer.
+const path = require('path');
+const native = require(path.join(__dirname, 'build', 'dspt.node'));
+
+const FRAME_FLOATS = 52;
+
+function packFrameBlock(r, camera, frameCount) {
+  // renderer.ts:376-438: JS numbers rounded into a Float32Array in key order
+  const filmHeight = Math.tan(camera.fov * 0.5 * Math.PI / 180.0) * camera.near;
+  const light = r._scene && r._scene.lights && r._scene.lights.length ? r._scene.lights[0] : null;
+  const v = [];
+  v.push(...Array.from(camera.matrixWorld.elements));
+  v.push(...r._backgroundColor);
+  v.push(camera.position.x, camera.position.y, camera.position.z, filmHeight);
+  v.push(1.0 / r.renderRes[0], 1.0 / r.renderRes[1]);
+  v.push(r._iblSize[0], r._iblSize[1]);
+  v.push(frameCount, 0, 0, r._useIBL ? 1 : 0);
+  v.push(r._iblRotation, r._showBackground ? 1 : 0, r._maxBounces, camera.near);
+  v.push(r._iblTotalSum, r._rayEps, r._clampThreshold, 0);
+  v.push(r._sceneCounts[0], r._sceneCounts[1], r._sceneCounts[2], 0);
+  if (light) v.push(light.position[0], light.position[1], light.position[2], 1, light.emission[0], light.emission[1], light.emission[2], 1);
+  else v.push(0, 0, 0, 0, 0, 0, 0, 0);
+  const block = new Float32Array(v);
+  if (block.length !== FRAME_FLOATS) throw new Error('frame block layout');
+  return block;
+}
+
+class PathtracingRenderer {
+  constructor(parameters = {}) {
+    this.materialProfile = parameters.materialProfile || 'webgl-lean';
+    if (this.materialProfile !== 'webgl-lean' && this.materialProfile !== 'webgl-full')
+      throw new Error(`unknown material profile ${this.materialProfile}`);
+    this._h = native.create(this.materialProfile === 'webgl-full' ? 1 : 0, parameters.device || 0);
+    this._maxBounces = 8; this._useIBL = true; this._showBackground = true; this._iblRotation = 0;
+    this._backgroundColor = [0, 0, 0, 1]; this._rayEps = 0.0001; this._clampThreshold = 3.0;
+    this._exposure = 1.0; this._tonemapping = 'None'; this._enableGamma = true; this._enableFxaa = false;
+    this._pixelRatio = 1.0; this._tileRes = 4;
+    this._frameCount = 1; this._isRendering = false;
+    this._iblSize = [0, 0]; this._iblTotalSum = 0; this._sceneCounts = [0, 0, 0];
+    this._scene = null;
+    this.diagnostics = { backend: 'hip-gfx950', sampleCount: 0, mode: 'stopped', profiling: {} };
+    this.displayRes = [0, 0]; this.renderRes = [0, 0];
+    this.resize(parameters.width || 1, parameters.height || 1);
+  }
+
+  get maxBounces() { return this._maxBounces; } set maxBounces(v) { this._maxBounces = v; this.resetAccumulation(); }
+  get useIBL() { return this._useIBL; } set useIBL(v) { this._useIBL = v; this.resetAccumulation(); }
+  get showBackground() { return this._showBackground; } set showBackground(v) { this._showBackground = v; this.resetAccumulation(); }
+  get iblRotation() { return this._iblRotation / Math.PI * 180.0; } set iblRotation(v) { this._iblRotation = v / 180.0 * Math.PI; this.resetAccumulation(); }
+  get backgroundColor() { return this._backgroundColor; } set backgroundColor(v) { this._backgroundColor = v; this.resetAccumulation(); }
+  get rayEps() { return this._rayEps; } set rayEps(v) { this._rayEps = v; this.resetAccumulation(); }
+  get clampThreshold() { return this._clampThreshold; } set clampThreshold(v) { this._clampThreshold = v; this.resetAccumulation(); }
+  get exposure() { return this._exposure; } set exposure(v) { this._exposure = v; this.resetAccumulation(); }
+  get tonemapping() { return this._tonemapping; } set tonemapping(v) { this._tonemapping = v; }
+  get enableGamma() { return this._enableGamma; } set enableGamma(v) { this._enableGamma = v; this.resetAccumulation(); }
+  get enableFxaa() { return this._enableFxaa; } set enableFxaa(v) { this._enableFxaa = v; }
+  get pixelRatio() { return this._pixelRatio; } set pixelRatio(v) { this._pixelRatio = v; this.resize(this.displayRes[0], this.displayRes[1]); }
+  get tileRes() { return this._tileRes; } set tileRes(v) { this._tileRes = Math.max(2, Math.floor(v) || 2); }
+
+  // renderer.ts:914-958; the BVH is built by the engine when the scene carries no prebuilt one
+  async setScene(scene, progress) {
+    this.stopRendering();
+    if (progress) progress({ phase: 'Building BVH' });
+    let nodes = scene.bvhNodes, idx = scene.bvhTriangleIndices;
+    if (!nodes) {
+      const t0 = Date.now();
+      const r = native.buildBvh(scene.bvhPositionBuffer, scene.bvhIndexBuffer, 4);
+      nodes = r.nodeData; idx = r.triangleIndices;
+      this.diagnostics.profiling.bvhBuildMs = Date.now() - t0;
+    }
+    if (progress) progress({ phase: 'Uploading scene' });
+    const mats = new Float32Array(scene.materials.length * 68);
+    scene.materials.forEach((m, i) => mats.set(m.data, i * 68));
+    native.uploadScene(this._h, scene.vertexBuffer, scene.triangleIndexBuffer, nodes, idx, mats);
+    this._scene = scene;
+    this._sceneCounts = [scene.triangleIndexBuffer.length / 3, nodes.length / 16, idx.length];
+    this.diagnostics.bvh = { nodes: nodes.length / 16, indices: idx.length };
+    if (progress) progress({ phase: 'Ready' });
+    this.resetAccumulation();
+  }
+
+  // renderer.ts:840-865
+  setIBL(texture) {
+    const img = texture.image;
+    this._iblTotalSum = native.uploadIbl(this._h, img.data, img.width, img.height);
+    this._iblSize = [img.width, img.height];
+    this.resetAccumulation();
+  }
+
+  resize(width, height) {
+    this.displayRes = [width, height];
+    this.renderRes = [Math.ceil(width * this._pixelRatio), Math.ceil(height * this._pixelRatio)];
+    native.resize(this._h, this.renderRes[0], this.renderRes[1]);
+    this.resetAccumulation();
+  }
+
+  resetAccumulation() { this._frameCount = 1; this.diagnostics.sampleCount = 0; }
+  stopRendering() { this._isRendering = false; this.diagnostics.mode = 'stopped'; }
+  setInteractionMode(flag) { this.diagnostics.mode = flag ? 'interactive' : 'settling'; this.resetAccumulation(); }
+
+  // renderer.ts:512-565: frames 1 .. numSamples-1 are rendered (the loop stops
+  // once the frame counter reaches numSamples). Resolves when finished.
+  async render(camera, numSamples, tileFinishedCB, frameFinishedCB, renderingFinishedCB, framesPerCall) {
+    if (numSamples < 0) throw new Error('continuous rendering needs a sample limit in this engine');
+    this.resetAccumulation();
+    this._isRendering = true;
+    this.diagnostics.mode = 'accumulating';
+    native.setFrame(this._h, packFrameBlock(this, camera, 1));
+    const step = framesPerCall || Math.max(1, numSamples - 1);
+    while (this._isRendering && this._frameCount < numSamples) {
+      const n = Math.min(step, numSamples - this._frameCount);
+      await native.render(this._h, this._frameCount, n);
+      this._frameCount += n;
+      this.diagnostics.sampleCount = this._frameCount - 1;
+      if (tileFinishedCB) tileFinishedCB();
+      if (frameFinishedCB) frameFinishedCB(this._frameCount);
+    }
+    this.stopRendering();
+    if (renderingFinishedCB) renderingFinishedCB();
+  }
+
+  readAccumulation() {
+    const out = new Float32Array(this.renderRes[0] * this.renderRes[1] * 4);
+    native.readAccum(this._h, out);
+    return out;
+  }
+
+  counters() { return native.counters(this._h); }
+}
+
+module.exports = { PathtracingRenderer, packFrameBlock, native };

@@ -2,7 +2,8 @@
 //
 // All HIP code here is compiled with -ffp-contract=off and correctly rounded
 // float division and sqrt (hipcc defaults), so these helpers give the same
-// bits as the CPU oracle for +,-,*,/,sqrt. Definitions that GLSL leaves to
+// bits as the CPU oracle for +,-,*,/,sqrt. Helpers are __host__ __device__ so
+// the shading code can also be compiled for the CPU (tests/devhost). Definitions that GLSL leaves to
 // the implementation are pinned the same way as in the oracle:
 //   normalize(v) = v * (1/sqrt(dot(v,v))), dot = (x*x'+y*y')+z*z',
 //   min/max = the GLSL select forms (NaN-sensitive, unlike fminf/fmaxf),
@@ -11,7 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define DI __device__ __forceinline__
+#define DI __host__ __device__ __forceinline__
+#define DD __device__ __forceinline__
 
 struct f3 {
   float x, y, z;
@@ -81,7 +83,7 @@ DI Rng rng_seed(int seed, int px, int py) {
 
 // Wave-level (64-lane) reduction of a 64-bit counter, then one atomic.
 // Must be reached by all 64 lanes of the wave.
-DI void wave_add_u64(unsigned long long* dst, unsigned long long v) {
+DD void wave_add_u64(unsigned long long* dst, unsigned long long v) {
   unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {

@@ -95,29 +95,37 @@ DI bool tri_test(f3 org, f3 dir, f3 p0, f3 p1, f3 p2, float tfar, float& t, floa
   return true;
 }
 
+// Geometry a traversal reads, copied into registers once per ray.
+struct Geo {
+  const float4* __restrict__ nodes;
+  const float4* __restrict__ tri;
+  int32_t N, M;
+};
+DI Geo geo_of(const SceneView& S) { return Geo{S.nodes, S.tri, S.N, S.M}; }
+
 // Stack traversal in the exact visit order of SH/bvh.glsl:149-212. With
 // ANY_HIT it returns at the first accepted triangle; the occlusion boolean
 // equals the reference's closest-hit based isOccluded() because both visit
 // the same nodes with the same tfar until the first hit.
 template <bool ANY_HIT>
-DI RayHit traverse(const SceneView& S, f3 org, f3 dir, float tfar, LaneStack& st, unsigned& nv, unsigned& nt) {
+DI RayHit traverse(const Geo G, f3 org, f3 dir, float tfar, LaneStack& st, unsigned& nv, unsigned& nt) {
   f3 inv = f3{1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
   RayHit h{-1, tfar, 0.0f, 0.0f};
   int top = 1;
   st.put(0, 0);
   for (int step = 0; step < kMaxSteps && top > 0; step++) {
     int ni = st.get(--top);
-    if (ni < 0 || ni >= S.N) continue;
+    if (ni < 0 || ni >= G.N) continue;
     nv++;
-    const float4* nd = S.nodes + (size_t)ni * 4;
+    const float4* nd = G.nodes + (size_t)ni * 4;
     float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
     int cnt = (int)c.w;
     int first = (int)d.w;
     if (cnt > 0) {
-      int end = min(first + min(cnt, kMaxLeafTris), S.M);
+      int end = min(first + min(cnt, kMaxLeafTris), G.M);
       for (int i = first; i < end; i++) {
         if (i < 0) continue;
-        const float4* tp = S.tri + (size_t)i * 3;
+        const float4* tp = G.tri + (size_t)i * 3;
         float4 q0 = tp[0];
         int id = __float_as_int(q0.w);
         if (id < 0) continue;
@@ -134,7 +142,7 @@ DI RayHit traverse(const SceneView& S, f3 org, f3 dir, float tfar, LaneStack& st
       float d0 = slab(xyz(a), xyz(b), org, inv, h.t);
       float d1 = slab(xyz(c), xyz(d), org, inv, h.t);
       auto push = [&](int child) {
-        if (top < kStackSize && child >= 0 && child < S.N && child != ni) st.put(top++, child);
+        if (top < kStackSize && child >= 0 && child < G.N && child != ni) st.put(top++, child);
       };
       if (d0 < kBVH_FAR && d1 < kBVH_FAR) {
         if (d0 < d1) { push(right); push(left); }

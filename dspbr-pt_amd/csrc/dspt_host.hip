@@ -1,8 +1,17 @@
 // Host runtime and C ABI (include/dspt.h) of the gfx950 path-tracing engine.
+//
+// Resources: resident scenes and IBLs (ids), render jobs (scene + IBL +
+// frame block + resolution + frame count + accumulation). Work runs in
+// passes: a pass packs runs of consecutive frames of one or more jobs into
+// the path pool (up to max_paths paths in flight) and executes the wavefront
+// bounce loop once for all of them. The single-image API (dspt_upload_scene /
+// dspt_upload_ibl / dspt_set_frame / dspt_render) drives an implicit job on
+// scene 0 / IBL 0; the batched and served modes create jobs explicitly.
 // Kernels are in dspt_kernels.hip (same translation unit).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -25,18 +34,71 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-#define HIPCHK(expr)                                                                               \
-  do {                                                                                             \
-    hipError_t e_ = (expr);                                                                        \
+#define HIPCHK(expr)                                                                                       \
+  do {                                                                                                     \
+    hipError_t e_ = (expr);                                                                                \
     if (e_ != hipSuccess) return fail(DSPT_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
   } while (0)
 
 constexpr int kNumCounters = 32;
+constexpr int kMaxJobs = 4096;
+constexpr int kMaxTasks = 4096;
 
-template <class T>
-struct DBuf {
-  T* p = nullptr;
-  size_t n = 0;
+struct Allocs {
+  std::vector<void*> ptrs;
+  size_t bytes = 0;
+  template <class T>
+  int get(T** out, size_t n) {
+    void* p = nullptr;
+    size_t sz = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&p, sz);
+    if (e != hipSuccess)
+      return fail(DSPT_ERR_MEMORY, std::string("hipMalloc(") + std::to_string(sz) + "): " + hipGetErrorString(e));
+    ptrs.push_back(p);
+    bytes += sz;
+    *out = (T*)p;
+    return 0;
+  }
+  void release() {
+    for (void* p : ptrs) (void)hipFree(p);
+    ptrs.clear();
+    bytes = 0;
+  }
+};
+
+struct SceneRes {
+  Allocs mem;
+  float4 *nodes = nullptr, *tri = nullptr, *verts = nullptr, *mats = nullptr;
+  uint32_t* tris = nullptr;
+  int32_t N = 0, M = 0, T = 0, K = 0;
+  int64_t V = 0;
+  int users = 0;
+};
+
+struct IblRes {
+  Allocs mem;
+  float4* env = nullptr;
+  float *pdf = nullptr, *cdf = nullptr, *ypdf = nullptr, *ycdf = nullptr;
+  int w = 0, h = 0;
+  double total = 0.0;
+  int users = 0;
+};
+
+struct JobRes {
+  Allocs mem;
+  int scene = -1, ibl = -1;
+  FrameParams F{};
+  int W = 0, H = 0;
+  float4* accum = nullptr;
+  int first = 1;   // next frame to render
+  int last = 0;    // last frame to render (inclusive)
+  bool record = false;
+  float4* rec_L = nullptr;
+  uint32_t* rec_sig = nullptr;
+  int32_t* rec_prim = nullptr;
+  int rec_first = 1, rec_count = 0;
+  bool alive = false;
+  int64_t npix() const { return (int64_t)W * H; }
 };
 
 }  // namespace
@@ -45,42 +107,34 @@ struct dspt_ctx {
   int device = 0;
   int profile = 0;
   hipStream_t stream = nullptr;
-  std::vector<void*> scene_allocs, ibl_allocs, state_allocs, frame_allocs;
-  size_t bytes_scene = 0, bytes_ibl = 0, bytes_state = 0, bytes_frame = 0;
-  // scene
-  float4* nodes = nullptr;
-  float4* tri = nullptr;
-  float4* verts = nullptr;
-  uint32_t* tris = nullptr;
-  float4* mats = nullptr;
-  int32_t N = 0, M = 0, T = 0, K = 0;
-  int64_t V = 0;
-  bool have_scene = false;
-  // ibl
-  float4* env = nullptr;
-  float *pdf = nullptr, *cdf = nullptr, *ypdf = nullptr, *ycdf = nullptr;
-  int ew = 0, eh = 0;
-  bool have_ibl = false;
-  // frame
+  std::vector<SceneRes*> scenes;
+  std::vector<IblRes*> ibls;
+  std::vector<JobRes*> jobs;
+  // single-image API state (implicit job on scene 0 / IBL 0)
   FrameParams F{};
   bool have_frame = false;
   int W = 0, H = 0;
-  float4* accum = nullptr;
-  // path state
+  int legacy_job = -1;
+  bool record = false;
+  int last_frames = 0;
+  // pass machinery
+  DevJob* d_jobs = nullptr;
+  PassTask* d_tasks = nullptr;
+  PassTask* h_tasks = nullptr;  // pinned staging
+  DevJob* h_jobs = nullptr;     // pinned staging
+  bool jobs_dirty = true;
+  Allocs state_mem;
   int64_t cap = 0;
+  bool cap_record = false;
   PathState ps{};
   int32_t *qa = nullptr, *qb = nullptr, *shq = nullptr, *counts = nullptr;
   int* spill = nullptr;
   int trace_blocks = 0;
+  int trace_blocks_max = 2048;
+  int shade_blocks_max = 8192;
   unsigned long long* cnt = nullptr;
   int64_t max_paths = 8 << 20;
-  // sample recording
-  bool record = false;
-  float4* rec_L = nullptr;
-  uint32_t* rec_sig = nullptr;
-  int32_t* rec_prim = nullptr;
-  int64_t rec_n = 0;
-  int last_frames = 0;
+  int64_t budget = 0;  // admission budget for job memory (0 = unlimited)
   // kernel timing
   bool timing = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
@@ -89,34 +143,6 @@ struct dspt_ctx {
 };
 
 namespace {
-
-template <class T>
-int dalloc(dspt_ctx* c, std::vector<void*>& list, size_t& bytes, T** out, size_t n) {
-  void* p = nullptr;
-  size_t sz = std::max<size_t>(n, 1) * sizeof(T);
-  hipError_t e = hipMalloc(&p, sz);
-  if (e != hipSuccess) return fail(DSPT_ERR_MEMORY, std::string("hipMalloc(") + std::to_string(sz) + "): " + hipGetErrorString(e));
-  list.push_back(p);
-  bytes += sz;
-  *out = (T*)p;
-  return 0;
-}
-void free_list(std::vector<void*>& list, size_t& bytes) {
-  for (void* p : list) (void)hipFree(p);
-  list.clear();
-  bytes = 0;
-}
-
-SceneView view_of(dspt_ctx* c) {
-  SceneView S;
-  S.nodes = c->nodes; S.tri = c->tri; S.verts = c->verts; S.tris = c->tris; S.mats = c->mats;
-  S.N = c->N; S.M = c->M; S.T = c->T; S.K = c->K;
-  S.env = c->env; S.pdf = c->pdf; S.cdf = c->cdf; S.ypdf = c->ypdf; S.ycdf = c->ycdf;
-  S.ew = c->ew; S.eh = c->eh;
-  S.has_ibl = c->have_ibl ? 1 : 0;
-  S.full_profile = c->profile == DSPT_PROFILE_FULL;
-  return S;
-}
 
 int grid_for(int64_t n, int block) { return (int)std::max<int64_t>(1, (n + block - 1) / block); }
 
@@ -131,10 +157,10 @@ hipEvent_t take_event(dspt_ctx* c) {
   return e;
 }
 
-// Launch wrapper: with timing on, brackets the launch with events on the
-// context stream (the stream the kernel runs on).
-template <class F>
-void timed(dspt_ctx* c, int kclass, F&& launch) {
+// With timing on, brackets the launch with events on the context stream
+// (the stream the kernel runs on).
+template <class Fn>
+void timed(dspt_ctx* c, int kclass, Fn&& launch) {
   if (!c->timing) { launch(); return; }
   hipEvent_t a = take_event(c), b = take_event(c);
   (void)hipEventRecord(a, c->stream);
@@ -158,31 +184,133 @@ void drain_events(dspt_ctx* c) {
   c->ev_pending.clear();
 }
 
-// Path state for `cap` paths in flight.
-int ensure_state(dspt_ctx* c, int64_t cap) {
-  if (c->cap >= cap && ((c->ps.sig != nullptr) == c->record)) return 0;
+int64_t job_bytes(int64_t npix, bool record, int64_t frames) {
+  int64_t b = npix * 16;
+  if (record) b += frames * npix * (16 + 8);
+  return b;
+}
+
+int64_t memory_in_use(dspt_ctx* c) {
+  int64_t b = (int64_t)c->state_mem.bytes;
+  for (auto* s : c->scenes) if (s) b += (int64_t)s->mem.bytes;
+  for (auto* i : c->ibls) if (i) b += (int64_t)i->mem.bytes;
+  for (auto* j : c->jobs) if (j) b += (int64_t)j->mem.bytes;
+  return b;
+}
+
+// Path pool for `cap` paths in flight.
+int ensure_state(dspt_ctx* c, int64_t cap, bool record) {
+  if (c->cap >= cap && c->cap_record == record) return 0;
   HIPCHK(hipStreamSynchronize(c->stream));
-  free_list(c->state_allocs, c->bytes_state);
+  c->state_mem.release();
   c->cap = 0;
-  int r = 0;
   PathState& ps = c->ps;
   ps = PathState{};
-  auto& L = c->state_allocs;
-  auto& B = c->bytes_state;
-  if ((r = dalloc(c, L, B, &ps.org, cap)) || (r = dalloc(c, L, B, &ps.dir, cap)) || (r = dalloc(c, L, B, &ps.hit, cap)) ||
-      (r = dalloc(c, L, B, &ps.pw, cap)) || (r = dalloc(c, L, B, &ps.L, cap)) || (r = dalloc(c, L, B, &ps.rng, cap)) ||
-      (r = dalloc(c, L, B, &ps.sh_org, cap)) || (r = dalloc(c, L, B, &ps.sh_dir, cap)) || (r = dalloc(c, L, B, &ps.sh_c, cap)) ||
-      (r = dalloc(c, L, B, &ps.sh_pw, cap)) || (r = dalloc(c, L, B, &ps.sh_pdir, cap)) || (r = dalloc(c, L, B, &ps.sh_pc, cap)) ||
-      (r = dalloc(c, L, B, &c->qa, cap)) || (r = dalloc(c, L, B, &c->qb, cap)) || (r = dalloc(c, L, B, &c->shq, cap)))
+  Allocs& A = c->state_mem;
+  int r = 0;
+  if ((r = A.get(&ps.org, cap)) || (r = A.get(&ps.dir, cap)) || (r = A.get(&ps.hit, cap)) || (r = A.get(&ps.pw, cap)) ||
+      (r = A.get(&ps.L, cap)) || (r = A.get(&ps.rng, cap)) || (r = A.get(&ps.job, cap)) || (r = A.get(&ps.sh_org, cap)) ||
+      (r = A.get(&ps.sh_dir, cap)) || (r = A.get(&ps.sh_c, cap)) || (r = A.get(&ps.sh_pw, cap)) ||
+      (r = A.get(&ps.sh_pdir, cap)) || (r = A.get(&ps.sh_pc, cap)) || (r = A.get(&c->qa, cap)) || (r = A.get(&c->qb, cap)) ||
+      (r = A.get(&c->shq, cap)))
     return r;
-  if (c->record) {
-    if ((r = dalloc(c, L, B, &ps.sig, cap)) || (r = dalloc(c, L, B, &ps.prim, cap))) return r;
-  }
-  // traversal grid: enough resident waves to fill 256 CUs, capped by paths
-  int max_blocks = 2048;
-  c->trace_blocks = (int)std::min<int64_t>(max_blocks, grid_for(cap, kTraceBlock));
-  if ((r = dalloc(c, L, B, &c->spill, (size_t)c->trace_blocks * kTraceBlock * (kStackSize - kLdsStack)))) return r;
+  if (record && ((r = A.get(&ps.sig, cap)) || (r = A.get(&ps.prim, cap)))) return r;
+  c->trace_blocks = (int)std::min<int64_t>(c->trace_blocks_max, grid_for(cap, kTraceBlock));
+  if ((r = A.get(&c->spill, (size_t)c->trace_blocks * kTraceBlock * (kStackSize - kLdsStack)))) return r;
   c->cap = cap;
+  c->cap_record = record;
+  return 0;
+}
+
+SceneView view_of(dspt_ctx* c, const JobRes* j) {
+  SceneView S{};
+  const SceneRes* s = c->scenes[j->scene];
+  S.nodes = s->nodes; S.tri = s->tri; S.verts = s->verts; S.tris = s->tris; S.mats = s->mats;
+  S.N = s->N; S.M = s->M; S.T = s->T; S.K = s->K;
+  const IblRes* ib = (j->ibl >= 0 && j->ibl < (int)c->ibls.size()) ? c->ibls[j->ibl] : nullptr;
+  if (ib) {
+    S.env = ib->env; S.pdf = ib->pdf; S.cdf = ib->cdf; S.ypdf = ib->ypdf; S.ycdf = ib->ycdf;
+    S.ew = ib->w; S.eh = ib->h; S.has_ibl = 1;
+  }
+  S.full_profile = c->profile == DSPT_PROFILE_FULL;
+  return S;
+}
+
+int sync_job_table(dspt_ctx* c) {
+  if (!c->jobs_dirty) return 0;
+  HIPCHK(hipStreamSynchronize(c->stream));  // staging buffer may be in use by a prior copy
+  for (size_t i = 0; i < c->jobs.size(); i++) {
+    JobRes* j = c->jobs[i];
+    DevJob d{};
+    if (j && j->alive) {
+      d.S = view_of(c, j);
+      d.F = j->F;
+      d.W = j->W; d.H = j->H; d.npix = (int32_t)j->npix();
+      d.accum = j->accum;
+      d.rec_L = j->record ? j->rec_L : nullptr;
+      d.rec_sig = j->rec_sig; d.rec_prim = j->rec_prim;
+      d.rec_first = j->rec_first;
+    }
+    c->h_jobs[i] = d;
+  }
+  if (!c->jobs.empty())
+    HIPCHK(hipMemcpyAsync(c->d_jobs, c->h_jobs, c->jobs.size() * sizeof(DevJob), hipMemcpyHostToDevice, c->stream));
+  c->jobs_dirty = false;
+  return 0;
+}
+
+// Runs one pass over `tasks` (frames of jobs), then advances each job.
+int run_pass(dspt_ctx* c, const std::vector<PassTask>& tasks) {
+  if (tasks.empty()) return 0;
+  if ((int)tasks.size() > kMaxTasks) return fail(DSPT_ERR_ARG, "too many tasks in one pass");
+  int64_t P = 0;
+  int maxB = 0, max_npix = 0;
+  bool record = false;
+  for (const PassTask& t : tasks) {
+    JobRes* j = c->jobs[t.job];
+    P += (int64_t)t.nframes * t.npix;
+    maxB = std::max(maxB, (int)j->F.max_bounces);
+    max_npix = std::max(max_npix, t.npix);
+    record = record || j->record;
+  }
+  if (P >= (1LL << 31)) return fail(DSPT_ERR_ARG, "pass exceeds 2^31 paths");
+  if (int r = ensure_state(c, std::max<int64_t>(P, c->cap), record || c->cap_record)) return r;
+  if (int r = sync_job_table(c)) return r;
+  HIPCHK(hipStreamSynchronize(c->stream));  // task staging reuse
+  std::memcpy(c->h_tasks, tasks.data(), tasks.size() * sizeof(PassTask));
+  HIPCHK(hipMemcpyAsync(c->d_tasks, c->h_tasks, tasks.size() * sizeof(PassTask), hipMemcpyHostToDevice, c->stream));
+  PathState ps = c->ps;
+  if (!record) { ps.sig = nullptr; ps.prim = nullptr; }
+  const int shade_blocks = (int)std::min<int64_t>(grid_for(P, 256), c->shade_blocks_max);
+  const int trace_blocks = (int)std::min<int64_t>(c->trace_blocks, grid_for(P, kTraceBlock));
+  timed(c, DSPT_K_RAYGEN, [&] {
+    k_raygen<<<grid_for(P, 256), 256, 0, c->stream>>>(ps, c->d_jobs, c->d_tasks, (int)tasks.size(), P, c->counts);
+  });
+  int32_t *qin = nullptr, *qout = c->qa;
+  int32_t *cin = c->counts + 0, *cout = c->counts + 1, *csh = c->counts + 2;
+  for (int it = 0; it <= maxB; it++) {
+    timed(c, DSPT_K_TRACE_EXT, [&] {
+      k_trace_ext<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, ps, qin, cin, c->spill, c->cnt);
+    });
+    k_fill_i32<<<1, 64, 0, c->stream>>>(cout, 1, 0);
+    k_fill_i32<<<1, 64, 0, c->stream>>>(csh, 1, 0);
+    timed(c, DSPT_K_SHADE, [&] {
+      k_shade<<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, ps, it, qin, cin, qout, cout, c->shq, csh, c->cnt);
+    });
+    if (it < maxB)
+      timed(c, DSPT_K_TRACE_SHADOW, [&] {
+        k_trace_shadow<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, ps, c->shq, csh, c->spill, c->cnt);
+      });
+    qin = qout;
+    qout = (qout == c->qa) ? c->qb : c->qa;
+    std::swap(cin, cout);
+  }
+  HIPCHK(hipGetLastError());
+  timed(c, DSPT_K_ACCUM, [&] {
+    k_accumulate<<<dim3(grid_for(max_npix, 256), (unsigned)tasks.size()), 256, 0, c->stream>>>(ps, c->d_jobs, c->d_tasks);
+  });
+  HIPCHK(hipGetLastError());
+  for (const PassTask& t : tasks) c->jobs[t.job]->first = t.first_frame + t.nframes;
   return 0;
 }
 
@@ -198,12 +326,198 @@ int validate_nodes(const float* nodes, int64_t n) {
   return 0;
 }
 
+int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, const float* nodes, int64_t N,
+               const int32_t* bvh_idx, int64_t M, const float* mats, int32_t K, SceneRes** out) {
+  if (!vertices || !tri_idx || !nodes || !bvh_idx || !mats) return fail(DSPT_ERR_ARG, "NULL scene buffer");
+  if (V <= 0 || T <= 0 || N <= 0 || M <= 0 || K <= 0) return fail(DSPT_ERR_ARG, "empty scene");
+  if (T >= (1LL << 24) || N >= (1LL << 30) || M >= (1LL << 30))
+    return fail(DSPT_ERR_UNSUPPORTED, "scene exceeds float-encoded index range");
+  for (int64_t i = 0; i < 3 * T; i++)
+    if ((int64_t)tri_idx[i] >= V) return fail(DSPT_ERR_ARG, "triangle index " + std::to_string(i) + " out of range");
+  for (int64_t v = 0; v < V; v++) {
+    float mi = vertices[v * 20 + 3];
+    if (!(mi >= 0.0f) || mi >= (float)K) return fail(DSPT_ERR_ARG, "vertex " + std::to_string(v) + ": material index out of range");
+  }
+  for (int32_t k = 0; k < K; k++)
+    for (int i = 44; i < 60; i++)
+      if (mats[(size_t)k * 68 + i] >= 0.0f) return fail(DSPT_ERR_UNSUPPORTED, "material " + std::to_string(k) + " uses a texture");
+  if (int r = validate_nodes(nodes, N)) return r;
+  HIPCHK(hipSetDevice(c->device));
+  SceneRes* s = new SceneRes();
+  Allocs& A = s->mem;
+  int r = 0;
+  if ((r = A.get(&s->nodes, (size_t)N * 4)) || (r = A.get(&s->tri, (size_t)M * 3)) || (r = A.get(&s->verts, (size_t)V * 5)) ||
+      (r = A.get(&s->tris, (size_t)T * 3)) || (r = A.get(&s->mats, (size_t)K * 17))) {
+    A.release();
+    delete s;
+    return r;
+  }
+  int32_t* d_bvh = nullptr;
+  hipError_t e = hipMalloc(&d_bvh, (size_t)M * sizeof(int32_t));
+  if (e != hipSuccess) { A.release(); delete s; return fail(DSPT_ERR_MEMORY, "bvh index staging"); }
+  HIPCHK(hipMemcpyAsync(s->nodes, nodes, (size_t)N * 64, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(s->verts, vertices, (size_t)V * 80, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(s->tris, tri_idx, (size_t)T * 12, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(s->mats, mats, (size_t)K * 272, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_bvh, bvh_idx, (size_t)M * 4, hipMemcpyHostToDevice, c->stream));
+  k_prep_tris<<<grid_for(M, 256), 256, 0, c->stream>>>(d_bvh, s->tris, s->verts, (int32_t)M, (int32_t)T, s->tri);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipFree(d_bvh));
+  s->N = (int32_t)N; s->M = (int32_t)M; s->T = (int32_t)T; s->K = K; s->V = V;
+  *out = s;
+  return 0;
+}
+
+int make_ibl(dspt_ctx* c, const float* rgba, int32_t w, int32_t h, IblRes** out) {
+  if (!rgba || w <= 0 || h <= 0) return fail(DSPT_ERR_ARG, "invalid IBL image");
+  HIPCHK(hipSetDevice(c->device));
+  IblRes* b = new IblRes();
+  size_t n = (size_t)w * h;
+  Allocs& A = b->mem;
+  int r = 0;
+  if ((r = A.get(&b->env, n)) || (r = A.get(&b->pdf, n)) || (r = A.get(&b->cdf, n)) || (r = A.get(&b->ypdf, (size_t)h)) ||
+      (r = A.get(&b->ycdf, (size_t)h))) {
+    A.release();
+    delete b;
+    return r;
+  }
+  float* lum = nullptr;
+  float* rowsum = nullptr;
+  double* total = nullptr;
+  HIPCHK(hipMalloc(&lum, n * 4));
+  HIPCHK(hipMalloc(&rowsum, (size_t)h * 4));
+  HIPCHK(hipMalloc(&total, 8));
+  HIPCHK(hipMemcpyAsync(b->env, rgba, n * 16, hipMemcpyHostToDevice, c->stream));
+  k_ibl_luminance<<<grid_for((int64_t)n, 256), 256, 0, c->stream>>>(b->env, (int64_t)n, lum);
+  k_ibl_rows<<<grid_for(h, 64), 64, 0, c->stream>>>(lum, w, h, b->pdf, b->cdf, rowsum);
+  k_ibl_marginal<<<1, 64, 0, c->stream>>>(rowsum, h, b->ypdf, b->ycdf, total);
+  HIPCHK(hipGetLastError());
+  double tot = 0;
+  HIPCHK(hipMemcpyAsync(&tot, total, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  (void)hipFree(lum);
+  (void)hipFree(rowsum);
+  (void)hipFree(total);
+  b->w = w; b->h = h; b->total = tot;
+  *out = b;
+  return 0;
+}
+
+template <class T>
+int put_slot(std::vector<T*>& v, T* x, int want = -1) {
+  if (want >= 0) {
+    if ((int)v.size() <= want) v.resize(want + 1, nullptr);
+    v[want] = x;
+    return want;
+  }
+  for (size_t i = 0; i < v.size(); i++)
+    if (!v[i]) { v[i] = x; return (int)i; }
+  v.push_back(x);
+  return (int)v.size() - 1;
+}
+
+int check_job_frame(dspt_ctx* c, const FrameParams& F, int scene, int ibl) {
+  if (!(F.max_bounces >= 0.0f) || F.max_bounces > 1024.0f) return fail(DSPT_ERR_ARG, "max_bounces out of range");
+  if (scene < 0 || scene >= (int)c->scenes.size() || !c->scenes[scene]) return fail(DSPT_ERR_ARG, "unknown scene id");
+  bool needs_ibl = F.use_ibl > 0.0f || F.bg_from_ibl > 0.0f;
+  const IblRes* b = (ibl >= 0 && ibl < (int)c->ibls.size()) ? c->ibls[ibl] : nullptr;
+  if (ibl >= 0 && !b) return fail(DSPT_ERR_ARG, "unknown IBL id");
+  if (needs_ibl && !b) return fail(DSPT_ERR_STATE, "IBL enabled but no IBL uploaded");
+  if (b && needs_ibl && ((int)F.ibl_res[0] != b->w || (int)F.ibl_res[1] != b->h))
+    return fail(DSPT_ERR_ARG, "u_ibl_resolution does not match the IBL");
+  return 0;
+}
+
+int create_job(dspt_ctx* c, int scene, int ibl, const FrameParams& F, int W, int H, int num_frames, bool record, bool admit,
+               int want_id, int* out_id) {
+  if (W <= 0 || H <= 0 || (int64_t)W * H > (1LL << 28)) return fail(DSPT_ERR_ARG, "invalid resolution");
+  if (num_frames < 0) return fail(DSPT_ERR_ARG, "num_frames < 0");
+  if (int r = check_job_frame(c, F, scene, ibl)) return r;
+  int64_t npix = (int64_t)W * H;
+  int64_t need = job_bytes(npix, record, num_frames);
+  if (admit && c->budget > 0 && memory_in_use(c) + need > c->budget)
+    return fail(DSPT_ERR_MEMORY, "admission: job needs " + std::to_string(need) + " B over the memory budget");
+  if ((want_id < 0 ? (int)c->jobs.size() : want_id) >= kMaxJobs) return fail(DSPT_ERR_MEMORY, "too many jobs");
+  JobRes* j = new JobRes();
+  int r = 0;
+  if ((r = j->mem.get(&j->accum, (size_t)npix))) { delete j; return r; }
+  if (record && num_frames > 0) {
+    if ((r = j->mem.get(&j->rec_L, (size_t)(npix * num_frames))) || (r = j->mem.get(&j->rec_sig, (size_t)(npix * num_frames))) ||
+        (r = j->mem.get(&j->rec_prim, (size_t)(npix * num_frames)))) {
+      j->mem.release();
+      delete j;
+      return r;
+    }
+  }
+  k_fill_f4<<<grid_for(npix, 256), 256, 0, c->stream>>>(j->accum, npix, make_float4(0, 0, 0, 0));
+  j->scene = scene; j->ibl = ibl; j->F = F; j->W = W; j->H = H;
+  j->first = 1; j->last = num_frames;
+  j->record = record; j->rec_first = 1; j->rec_count = num_frames;
+  j->alive = true;
+  c->scenes[scene]->users++;
+  if (ibl >= 0) c->ibls[ibl]->users++;
+  *out_id = put_slot(c->jobs, j, want_id);
+  c->jobs_dirty = true;
+  return 0;
+}
+
+void destroy_job(dspt_ctx* c, int id) {
+  JobRes* j = c->jobs[id];
+  (void)hipStreamSynchronize(c->stream);
+  if (j->scene >= 0 && c->scenes[j->scene]) c->scenes[j->scene]->users--;
+  if (j->ibl >= 0 && j->ibl < (int)c->ibls.size() && c->ibls[j->ibl]) c->ibls[j->ibl]->users--;
+  j->mem.release();
+  delete j;
+  c->jobs[id] = nullptr;
+  c->jobs_dirty = true;
+}
+
+bool job_ok(dspt_ctx* c, int id) { return id >= 0 && id < (int)c->jobs.size() && c->jobs[id] && c->jobs[id]->alive; }
+
+// One pass over the given jobs' pending frames: each job contributes up to
+// `per_job` consecutive frames while the pool has room (FIFO over the list;
+// a first task is always taken even if it alone exceeds max_paths).
+int schedule_pass(dspt_ctx* c, const std::vector<int>& ids, int64_t max_paths, int per_job, int* frames_done) {
+  std::vector<PassTask> tasks;
+  int64_t used = 0;
+  for (int id : ids) {
+    if (!job_ok(c, id)) continue;
+    JobRes* j = c->jobs[id];
+    int remaining = j->last - j->first + 1;
+    if (remaining <= 0) continue;
+    int64_t npix = j->npix();
+    int64_t room = (max_paths - used) / npix;
+    int nf = (int)std::min<int64_t>(std::min<int64_t>(remaining, per_job > 0 ? per_job : remaining), room);
+    if (nf <= 0) {
+      if (!tasks.empty()) continue;
+      nf = 1;
+    }
+    if ((int)tasks.size() >= kMaxTasks) break;
+    PassTask t;
+    t.job = id; t.first_frame = j->first; t.nframes = nf; t.npix = (int32_t)npix; t.offset = used;
+    tasks.push_back(t);
+    used += (int64_t)nf * npix;
+  }
+  int r = run_pass(c, tasks);
+  if (r) return r;
+  int n = 0;
+  for (auto& t : tasks) n += t.nframes;
+  if (frames_done) *frames_done = n;
+  return 0;
+}
+
+int legacy_job(dspt_ctx* c, int* out) {
+  if (c->legacy_job >= 0 && job_ok(c, c->legacy_job)) { *out = c->legacy_job; return 0; }
+  return fail(DSPT_ERR_STATE, "no render target (dspt_resize)");
+}
+
 }  // namespace
 
 extern "C" {
 
 const char* dspt_last_error(void) { return g_err.c_str(); }
-const char* dspt_version(void) { return "dspt 0.1.0 gfx950"; }
+const char* dspt_version(void) { return "dspt 0.2.0 gfx950"; }
 
 int dspt_create(int profile, int device, dspt_ctx** out) {
   if (!out) return fail(DSPT_ERR_ARG, "out is NULL");
@@ -217,16 +531,24 @@ int dspt_create(int profile, int device, dspt_ctx** out) {
   c->device = device;
   c->profile = profile;
   if (const char* mp = std::getenv("DSPT_MAX_PATHS")) c->max_paths = std::max<int64_t>(65536, std::atoll(mp));
-  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) { delete c; return fail(DSPT_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+  bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
   void* p = nullptr;
-  e = hipMalloc(&p, kNumCounters * sizeof(unsigned long long));
-  if (e != hipSuccess) { (void)hipStreamDestroy(c->stream); delete c; return fail(DSPT_ERR_MEMORY, "counter allocation"); }
-  c->cnt = (unsigned long long*)p;
-  (void)hipMemset(c->cnt, 0, kNumCounters * sizeof(unsigned long long));
-  e = hipMalloc(&p, 8 * sizeof(int32_t));
-  if (e != hipSuccess) { (void)hipFree(c->cnt); (void)hipStreamDestroy(c->stream); delete c; return fail(DSPT_ERR_MEMORY, "count allocation"); }
-  c->counts = (int32_t*)p;
+  ok = ok && hipMalloc(&p, kNumCounters * sizeof(unsigned long long)) == hipSuccess;
+  if (ok) { c->cnt = (unsigned long long*)p; ok = hipMemset(c->cnt, 0, kNumCounters * sizeof(unsigned long long)) == hipSuccess; }
+  ok = ok && hipMalloc(&p, 8 * sizeof(int32_t)) == hipSuccess;
+  if (ok) c->counts = (int32_t*)p;
+  ok = ok && hipMalloc(&p, kMaxJobs * sizeof(DevJob)) == hipSuccess;
+  if (ok) c->d_jobs = (DevJob*)p;
+  ok = ok && hipMalloc(&p, kMaxTasks * sizeof(PassTask)) == hipSuccess;
+  if (ok) c->d_tasks = (PassTask*)p;
+  ok = ok && hipHostMalloc(&p, kMaxJobs * sizeof(DevJob), hipHostMallocDefault) == hipSuccess;
+  if (ok) c->h_jobs = (DevJob*)p;
+  ok = ok && hipHostMalloc(&p, kMaxTasks * sizeof(PassTask), hipHostMallocDefault) == hipSuccess;
+  if (ok) c->h_tasks = (PassTask*)p;
+  if (!ok) {
+    dspt_destroy(c);
+    return fail(DSPT_ERR_MEMORY, "context allocation failed");
+  }
   *out = c;
   return 0;
 }
@@ -234,110 +556,111 @@ int dspt_create(int profile, int device, dspt_ctx** out) {
 int dspt_destroy(dspt_ctx* c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
-  free_list(c->scene_allocs, c->bytes_scene);
-  free_list(c->ibl_allocs, c->bytes_ibl);
-  free_list(c->state_allocs, c->bytes_state);
-  free_list(c->frame_allocs, c->bytes_frame);
-  if (c->rec_L) (void)hipFree(c->rec_L);
-  if (c->rec_sig) (void)hipFree(c->rec_sig);
-  if (c->rec_prim) (void)hipFree(c->rec_prim);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   drain_events(c);
+  for (size_t i = 0; i < c->jobs.size(); i++)
+    if (c->jobs[i]) { c->jobs[i]->mem.release(); delete c->jobs[i]; }
+  for (auto* s : c->scenes) if (s) { s->mem.release(); delete s; }
+  for (auto* b : c->ibls) if (b) { b->mem.release(); delete b; }
+  c->state_mem.release();
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-  (void)hipFree(c->cnt);
-  (void)hipFree(c->counts);
-  (void)hipStreamDestroy(c->stream);
+  if (c->cnt) (void)hipFree(c->cnt);
+  if (c->counts) (void)hipFree(c->counts);
+  if (c->d_jobs) (void)hipFree(c->d_jobs);
+  if (c->d_tasks) (void)hipFree(c->d_tasks);
+  if (c->h_jobs) (void)hipHostFree(c->h_jobs);
+  if (c->h_tasks) (void)hipHostFree(c->h_tasks);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
 }
 
+// ------------------------------------------------------------------ resources
+int dspt_scene_create(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, const float* nodes,
+                      int64_t N, const int32_t* bvh_idx, int64_t M, const float* mats, int32_t K, int32_t* scene_id) {
+  if (!c || !scene_id) return fail(DSPT_ERR_ARG, "NULL argument");
+  SceneRes* s = nullptr;
+  if (int r = make_scene(c, vertices, V, tri_idx, T, nodes, N, bvh_idx, M, mats, K, &s)) return r;
+  *scene_id = put_slot(c->scenes, s);
+  return 0;
+}
+
+int dspt_scene_destroy(dspt_ctx* c, int32_t id) {
+  if (!c || id < 0 || id >= (int)c->scenes.size() || !c->scenes[id]) return fail(DSPT_ERR_ARG, "unknown scene id");
+  if (c->scenes[id]->users > 0) return fail(DSPT_ERR_STATE, "scene is used by live jobs");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->scenes[id]->mem.release();
+  delete c->scenes[id];
+  c->scenes[id] = nullptr;
+  return 0;
+}
+
+int dspt_ibl_create(dspt_ctx* c, const float* rgba, int32_t w, int32_t h, double* total_sum, int32_t* ibl_id) {
+  if (!c || !ibl_id) return fail(DSPT_ERR_ARG, "NULL argument");
+  IblRes* b = nullptr;
+  if (int r = make_ibl(c, rgba, w, h, &b)) return r;
+  if (total_sum) *total_sum = b->total;
+  *ibl_id = put_slot(c->ibls, b);
+  return 0;
+}
+
+int dspt_ibl_destroy(dspt_ctx* c, int32_t id) {
+  if (!c || id < 0 || id >= (int)c->ibls.size() || !c->ibls[id]) return fail(DSPT_ERR_ARG, "unknown IBL id");
+  if (c->ibls[id]->users > 0) return fail(DSPT_ERR_STATE, "IBL is used by live jobs");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->ibls[id]->mem.release();
+  delete c->ibls[id];
+  c->ibls[id] = nullptr;
+  return 0;
+}
+
+// Single-image API: scene 0 / IBL 0 are replaced in place; the implicit job
+// is re-pointed at them (its accumulation restarts).
 int dspt_upload_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, const float* nodes,
                       int64_t N, const int32_t* bvh_idx, int64_t M, const float* mats, int32_t K) {
   if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
-  if (!vertices || !tri_idx || !nodes || !bvh_idx || !mats) return fail(DSPT_ERR_ARG, "NULL scene buffer");
-  if (V <= 0 || T <= 0 || N <= 0 || M <= 0 || K <= 0) return fail(DSPT_ERR_ARG, "empty scene");
-  if (T >= (1LL << 24) || N >= (1LL << 30) || M >= (1LL << 30)) return fail(DSPT_ERR_UNSUPPORTED, "scene exceeds float-encoded index range");
-  for (int64_t i = 0; i < 3 * T; i++)
-    if ((int64_t)tri_idx[i] >= V) return fail(DSPT_ERR_ARG, "triangle index " + std::to_string(i) + " out of range");
-  for (int64_t v = 0; v < V; v++) {
-    float mi = vertices[v * 20 + 3];
-    if (!(mi >= 0.0f) || mi >= (float)K) return fail(DSPT_ERR_ARG, "vertex " + std::to_string(v) + ": material index out of range");
+  SceneRes* s = nullptr;
+  if (int r = make_scene(c, vertices, V, tri_idx, T, nodes, N, bvh_idx, M, mats, K, &s)) return r;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int users = 0;
+  if (!c->scenes.empty() && c->scenes[0]) {
+    users = c->scenes[0]->users;
+    c->scenes[0]->mem.release();
+    delete c->scenes[0];
   }
-  for (int32_t k = 0; k < K; k++)
-    for (int i = 44; i < 60; i++)
-      if (mats[(size_t)k * 68 + i] >= 0.0f) return fail(DSPT_ERR_UNSUPPORTED, "material " + std::to_string(k) + " uses a texture");
-  if (int r = validate_nodes(nodes, N)) return r;
-  HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  free_list(c->scene_allocs, c->bytes_scene);
-  c->have_scene = false;
-  int r = 0;
-  auto& L = c->scene_allocs;
-  auto& B = c->bytes_scene;
-  if ((r = dalloc(c, L, B, &c->nodes, (size_t)N * 4)) || (r = dalloc(c, L, B, &c->tri, (size_t)M * 3)) ||
-      (r = dalloc(c, L, B, &c->verts, (size_t)V * 5)) || (r = dalloc(c, L, B, &c->tris, (size_t)T * 3)) ||
-      (r = dalloc(c, L, B, &c->mats, (size_t)K * 17)))
-    return r;
-  int32_t* d_bvh = nullptr;
-  HIPCHK(hipMalloc(&d_bvh, (size_t)M * sizeof(int32_t)));
-  HIPCHK(hipMemcpyAsync(c->nodes, nodes, (size_t)N * 64, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->verts, vertices, (size_t)V * 80, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->tris, tri_idx, (size_t)T * 12, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->mats, mats, (size_t)K * 272, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(d_bvh, bvh_idx, (size_t)M * 4, hipMemcpyHostToDevice, c->stream));
-  k_prep_tris<<<grid_for(M, 256), 256, 0, c->stream>>>(d_bvh, c->tris, c->verts, (int32_t)M, (int32_t)T, c->tri);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipFree(d_bvh));
-  c->N = (int32_t)N; c->M = (int32_t)M; c->T = (int32_t)T; c->K = K; c->V = V;
-  c->have_scene = true;
+  s->users = users;
+  put_slot(c->scenes, s, 0);
+  c->jobs_dirty = true;
   return 0;
 }
 
 int dspt_upload_ibl(dspt_ctx* c, const float* rgba, int32_t w, int32_t h, double* total_sum) {
   if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
-  if (!rgba || w <= 0 || h <= 0) return fail(DSPT_ERR_ARG, "invalid IBL image");
-  HIPCHK(hipSetDevice(c->device));
+  IblRes* b = nullptr;
+  if (int r = make_ibl(c, rgba, w, h, &b)) return r;
   HIPCHK(hipStreamSynchronize(c->stream));
-  free_list(c->ibl_allocs, c->bytes_ibl);
-  c->have_ibl = false;
-  size_t n = (size_t)w * h;
-  int r = 0;
-  auto& L = c->ibl_allocs;
-  auto& B = c->bytes_ibl;
-  float* lum = nullptr;
-  float* rowsum = nullptr;
-  double* total = nullptr;
-  if ((r = dalloc(c, L, B, &c->env, n)) || (r = dalloc(c, L, B, &c->pdf, n)) || (r = dalloc(c, L, B, &c->cdf, n)) ||
-      (r = dalloc(c, L, B, &c->ypdf, (size_t)h)) || (r = dalloc(c, L, B, &c->ycdf, (size_t)h)))
-    return r;
-  HIPCHK(hipMalloc(&lum, n * 4));
-  HIPCHK(hipMalloc(&rowsum, (size_t)h * 4));
-  HIPCHK(hipMalloc(&total, 8));
-  HIPCHK(hipMemcpyAsync(c->env, rgba, n * 16, hipMemcpyHostToDevice, c->stream));
-  k_ibl_luminance<<<grid_for((int64_t)n, 256), 256, 0, c->stream>>>(c->env, (int64_t)n, lum);
-  k_ibl_rows<<<grid_for(h, 64), 64, 0, c->stream>>>(lum, w, h, c->pdf, c->cdf, rowsum);
-  k_ibl_marginal<<<1, 64, 0, c->stream>>>(rowsum, h, c->ypdf, c->ycdf, total);
-  HIPCHK(hipGetLastError());
-  double tot = 0;
-  HIPCHK(hipMemcpyAsync(&tot, total, 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  (void)hipFree(lum);
-  (void)hipFree(rowsum);
-  (void)hipFree(total);
-  if (total_sum) *total_sum = tot;
-  c->ew = w; c->eh = h;
-  c->have_ibl = true;
+  int users = 0;
+  if (!c->ibls.empty() && c->ibls[0]) {
+    users = c->ibls[0]->users;
+    c->ibls[0]->mem.release();
+    delete c->ibls[0];
+  }
+  b->users = users;
+  put_slot(c->ibls, b, 0);
+  if (total_sum) *total_sum = b->total;
+  c->jobs_dirty = true;
   return 0;
 }
 
 int dspt_read_ibl_tables(dspt_ctx* c, float* pdf, float* cdf, float* ypdf, float* ycdf) {
-  if (!c || !c->have_ibl) return fail(DSPT_ERR_STATE, "no IBL uploaded");
-  size_t n = (size_t)c->ew * c->eh;
-  if (pdf) HIPCHK(hipMemcpy(pdf, c->pdf, n * 4, hipMemcpyDeviceToHost));
-  if (cdf) HIPCHK(hipMemcpy(cdf, c->cdf, n * 4, hipMemcpyDeviceToHost));
-  if (ypdf) HIPCHK(hipMemcpy(ypdf, c->ypdf, (size_t)c->eh * 4, hipMemcpyDeviceToHost));
-  if (ycdf) HIPCHK(hipMemcpy(ycdf, c->ycdf, (size_t)c->eh * 4, hipMemcpyDeviceToHost));
+  if (!c || c->ibls.empty() || !c->ibls[0]) return fail(DSPT_ERR_STATE, "no IBL uploaded");
+  IblRes* b = c->ibls[0];
+  size_t n = (size_t)b->w * b->h;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (pdf) HIPCHK(hipMemcpy(pdf, b->pdf, n * 4, hipMemcpyDeviceToHost));
+  if (cdf) HIPCHK(hipMemcpy(cdf, b->cdf, n * 4, hipMemcpyDeviceToHost));
+  if (ypdf) HIPCHK(hipMemcpy(ypdf, b->ypdf, (size_t)b->h * 4, hipMemcpyDeviceToHost));
+  if (ycdf) HIPCHK(hipMemcpy(ycdf, b->ycdf, (size_t)b->h * 4, hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -345,22 +668,20 @@ int dspt_resize(dspt_ctx* c, int32_t w, int32_t h) {
   if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
   if (w <= 0 || h <= 0 || (int64_t)w * h > (1LL << 28)) return fail(DSPT_ERR_ARG, "invalid resolution");
   HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  free_list(c->frame_allocs, c->bytes_frame);
-  c->accum = nullptr;
-  int r = dalloc(c, c->frame_allocs, c->bytes_frame, &c->accum, (size_t)w * h);
-  if (r) return r;
-  c->W = w; c->H = h;
-  k_fill_f4<<<grid_for((int64_t)w * h, 256), 256, 0, c->stream>>>(c->accum, (int64_t)w * h, make_float4(0, 0, 0, 0));
-  HIPCHK(hipGetLastError());
+  c->W = w;
+  c->H = h;
+  if (c->legacy_job >= 0 && job_ok(c, c->legacy_job)) destroy_job(c, c->legacy_job);
+  c->legacy_job = -1;
   return 0;
 }
 
 int dspt_set_frame(dspt_ctx* c, const float* block) {
   if (!c || !block) return fail(DSPT_ERR_ARG, "NULL argument");
   static_assert(sizeof(FrameParams) == DSPT_FRAME_FLOATS * 4, "frame block layout");
-  std::memcpy(&c->F, block, sizeof(FrameParams));
-  if (!(c->F.max_bounces >= 0.0f) || c->F.max_bounces > 1024.0f) return fail(DSPT_ERR_ARG, "max_bounces out of range");
+  FrameParams F;
+  std::memcpy(&F, block, sizeof(FrameParams));
+  if (!(F.max_bounces >= 0.0f) || F.max_bounces > 1024.0f) return fail(DSPT_ERR_ARG, "max_bounces out of range");
+  c->F = F;
   c->have_frame = true;
   return 0;
 }
@@ -373,79 +694,38 @@ int dspt_set_record_samples(dspt_ctx* c, int enable) {
 
 int dspt_render(dspt_ctx* c, int32_t first_frame, int32_t num_frames) {
   if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
-  if (!c->have_scene) return fail(DSPT_ERR_STATE, "render before dspt_upload_scene");
+  if (c->scenes.empty() || !c->scenes[0]) return fail(DSPT_ERR_STATE, "render before dspt_upload_scene");
   if (!c->have_frame) return fail(DSPT_ERR_STATE, "render before dspt_set_frame");
-  if (!c->accum) return fail(DSPT_ERR_STATE, "render before dspt_resize");
+  if (c->W <= 0) return fail(DSPT_ERR_STATE, "render before dspt_resize");
   if (first_frame < 1 || num_frames < 0) return fail(DSPT_ERR_ARG, "frames are 1-based and num_frames >= 0");
-  bool needs_ibl = c->F.use_ibl > 0.0f || c->F.bg_from_ibl > 0.0f;
-  if (needs_ibl && !c->have_ibl) return fail(DSPT_ERR_STATE, "IBL enabled but no IBL uploaded");
-  if (needs_ibl && ((int)c->F.ibl_res[0] != c->ew || (int)c->F.ibl_res[1] != c->eh))
-    return fail(DSPT_ERR_ARG, "u_ibl_resolution does not match the uploaded IBL");
   HIPCHK(hipSetDevice(c->device));
-  const int W = c->W, H = c->H;
-  const int64_t npix = (int64_t)W * H;
+  int ibl = (!c->ibls.empty() && c->ibls[0]) ? 0 : -1;
+  if (int r = check_job_frame(c, c->F, 0, ibl)) return r;
+  // the implicit job is rebuilt when the recording request changes
+  if (c->legacy_job >= 0 && job_ok(c, c->legacy_job)) {
+    JobRes* j = c->jobs[c->legacy_job];
+    bool rec_fit = !c->record || (j->record && j->rec_count >= num_frames);
+    if (!rec_fit || j->record != c->record || j->ibl != ibl) { destroy_job(c, c->legacy_job); c->legacy_job = -1; }
+  }
+  if (c->legacy_job < 0 || !job_ok(c, c->legacy_job)) {
+    int id = -1;
+    if (int r = create_job(c, 0, ibl, c->F, c->W, c->H, num_frames, c->record, false, -1, &id)) return r;
+    c->legacy_job = id;
+  }
+  JobRes* j = c->jobs[c->legacy_job];
+  j->F = c->F;
+  j->ibl = ibl;
+  j->rec_first = first_frame;
+  c->jobs_dirty = true;
   if (first_frame == 1)
-    k_fill_f4<<<grid_for(npix, 256), 256, 0, c->stream>>>(c->accum, npix, make_float4(0, 0, 0, 0));
-  if (num_frames == 0) return 0;
-  int64_t slots = std::max<int64_t>(1, std::min<int64_t>(num_frames, c->max_paths / npix));
-  int64_t cap = slots * npix;
-  if (int r = ensure_state(c, cap)) return r;
-  if (c->record) {
-    int64_t need = (int64_t)num_frames * npix;
-    if (c->rec_n < need) {
-      if (c->rec_L) (void)hipFree(c->rec_L);
-      if (c->rec_sig) (void)hipFree(c->rec_sig);
-      if (c->rec_prim) (void)hipFree(c->rec_prim);
-      c->rec_L = nullptr; c->rec_sig = nullptr; c->rec_prim = nullptr; c->rec_n = 0;
-      HIPCHK(hipMalloc(&c->rec_L, need * sizeof(float4)));
-      HIPCHK(hipMalloc(&c->rec_sig, need * sizeof(uint32_t)));
-      HIPCHK(hipMalloc(&c->rec_prim, need * sizeof(int32_t)));
-      c->rec_n = need;
-    }
+    k_fill_f4<<<grid_for(j->npix(), 256), 256, 0, c->stream>>>(j->accum, j->npix(), make_float4(0, 0, 0, 0));
+  j->first = first_frame;
+  j->last = first_frame + num_frames - 1;
+  std::vector<int> ids{c->legacy_job};
+  while (j->first <= j->last) {
+    int done = 0;
+    if (int r = schedule_pass(c, ids, c->max_paths, 0, &done)) return r;
   }
-  SceneView S = view_of(c);
-  FrameParams F = c->F;
-  const int maxB = (int)F.max_bounces;
-  const int shade_blocks = (int)std::min<int64_t>(grid_for(cap, 256), 8192);
-  for (int64_t f0 = 0; f0 < num_frames; f0 += slots) {
-    int ns = (int)std::min<int64_t>(slots, num_frames - f0);
-    int64_t P = (int64_t)ns * npix;
-    int ff = first_frame + (int)f0;
-    F.frame_count = (float)ff;
-    timed(c, DSPT_K_RAYGEN, [&] {
-      k_raygen<<<grid_for(P, 256), 256, 0, c->stream>>>(c->ps, F, W, H, (int)npix, ff, P, c->counts, c->cnt);
-    });
-    int32_t *qin = nullptr, *qout = c->qa;
-    int32_t *cin = c->counts + 0, *cout = c->counts + 1, *csh = c->counts + 2;
-    for (int it = 0; it <= maxB; it++) {
-      timed(c, DSPT_K_TRACE_EXT, [&] {
-        k_trace_ext<<<c->trace_blocks, kTraceBlock, 0, c->stream>>>(S, c->ps, qin, cin, c->spill, c->cnt);
-      });
-      k_fill_i32<<<1, 64, 0, c->stream>>>(cout, 1, 0);
-      k_fill_i32<<<1, 64, 0, c->stream>>>(csh, 1, 0);
-      timed(c, DSPT_K_SHADE, [&] {
-        k_shade<<<shade_blocks, 256, 0, c->stream>>>(S, F, c->ps, it, (int)npix, qin, cin, qout, cout, c->shq, csh, c->cnt);
-      });
-      if (it < maxB)
-        timed(c, DSPT_K_TRACE_SHADOW, [&] {
-          k_trace_shadow<<<c->trace_blocks, kTraceBlock, 0, c->stream>>>(S, c->ps, c->shq, csh, c->spill, c->cnt);
-        });
-      // next iteration consumes qout
-      qin = qout;
-      qout = (qout == c->qa) ? c->qb : c->qa;
-      std::swap(cin, cout);
-    }
-    HIPCHK(hipGetLastError());
-    timed(c, DSPT_K_ACCUM, [&] {
-      k_accumulate<<<grid_for(npix, 256), 256, 0, c->stream>>>(c->ps.L, c->accum, (int)npix, ns, ff);
-    });
-    if (c->record) {
-      HIPCHK(hipMemcpyAsync(c->rec_L + f0 * npix, c->ps.L, P * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->rec_sig + f0 * npix, c->ps.sig, P * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->rec_prim + f0 * npix, c->ps.prim, P * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
-    }
-  }
-  HIPCHK(hipGetLastError());
   c->last_frames = num_frames;
   return 0;
 }
@@ -458,23 +738,122 @@ int dspt_synchronize(dspt_ctx* c) {
 
 int dspt_read_accum(dspt_ctx* c, float* rgba) {
   if (!c || !rgba) return fail(DSPT_ERR_ARG, "NULL argument");
-  if (!c->accum) return fail(DSPT_ERR_STATE, "no render target (dspt_resize)");
-  HIPCHK(hipMemcpyAsync(rgba, c->accum, (size_t)c->W * c->H * 16, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return 0;
+  int id;
+  if (int r = legacy_job(c, &id)) return r;
+  return dspt_job_read_accum(c, id, rgba);
 }
 
 int dspt_read_samples(dspt_ctx* c, float* rgba, uint32_t* sigs, int32_t* prim) {
   if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
-  if (!c->record || !c->rec_L) return fail(DSPT_ERR_STATE, "sample recording was not enabled for the last render");
-  size_t n = (size_t)c->last_frames * c->W * c->H;
+  int id;
+  if (int r = legacy_job(c, &id)) return r;
+  JobRes* j = c->jobs[id];
+  if (!j->record) return fail(DSPT_ERR_STATE, "sample recording was not enabled for the last render");
+  size_t n = (size_t)c->last_frames * j->npix();
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (rgba) HIPCHK(hipMemcpy(rgba, c->rec_L, n * 16, hipMemcpyDeviceToHost));
-  if (sigs) HIPCHK(hipMemcpy(sigs, c->rec_sig, n * 4, hipMemcpyDeviceToHost));
-  if (prim) HIPCHK(hipMemcpy(prim, c->rec_prim, n * 4, hipMemcpyDeviceToHost));
+  if (rgba) HIPCHK(hipMemcpy(rgba, j->rec_L, n * 16, hipMemcpyDeviceToHost));
+  if (sigs) HIPCHK(hipMemcpy(sigs, j->rec_sig, n * 4, hipMemcpyDeviceToHost));
+  if (prim) HIPCHK(hipMemcpy(prim, j->rec_prim, n * 4, hipMemcpyDeviceToHost));
   return 0;
 }
 
+// ------------------------------------------------------------------ jobs
+int dspt_job_create(dspt_ctx* c, int32_t scene_id, int32_t ibl_id, const float* block, int32_t W, int32_t H,
+                    int32_t num_frames, int32_t record_samples, int32_t* job_id) {
+  if (!c || !block || !job_id) return fail(DSPT_ERR_ARG, "NULL argument");
+  FrameParams F;
+  std::memcpy(&F, block, sizeof(FrameParams));
+  HIPCHK(hipSetDevice(c->device));
+  int id = -1;
+  if (int r = create_job(c, scene_id, ibl_id, F, W, H, num_frames, record_samples != 0, true, -1, &id)) return r;
+  *job_id = id;
+  return 0;
+}
+
+int dspt_job_destroy(dspt_ctx* c, int32_t id) {
+  if (!c || !job_ok(c, id)) return fail(DSPT_ERR_ARG, "unknown job id");
+  if (id == c->legacy_job) c->legacy_job = -1;
+  destroy_job(c, id);
+  return 0;
+}
+
+int dspt_job_status(dspt_ctx* c, int32_t id, int32_t* frames_done, int32_t* frames_total) {
+  if (!c || !job_ok(c, id)) return fail(DSPT_ERR_ARG, "unknown job id");
+  JobRes* j = c->jobs[id];
+  if (frames_done) *frames_done = j->first - 1;
+  if (frames_total) *frames_total = j->last;
+  return 0;
+}
+
+int dspt_job_read_accum(dspt_ctx* c, int32_t id, float* rgba) {
+  if (!c || !rgba || !job_ok(c, id)) return fail(DSPT_ERR_ARG, "unknown job id or NULL output");
+  JobRes* j = c->jobs[id];
+  HIPCHK(hipMemcpyAsync(rgba, j->accum, (size_t)j->npix() * 16, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int dspt_job_read_samples(dspt_ctx* c, int32_t id, float* rgba, uint32_t* sigs, int32_t* prim) {
+  if (!c || !job_ok(c, id)) return fail(DSPT_ERR_ARG, "unknown job id");
+  JobRes* j = c->jobs[id];
+  if (!j->record) return fail(DSPT_ERR_STATE, "job was created without sample recording");
+  size_t n = (size_t)j->rec_count * j->npix();
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (rgba) HIPCHK(hipMemcpy(rgba, j->rec_L, n * 16, hipMemcpyDeviceToHost));
+  if (sigs) HIPCHK(hipMemcpy(sigs, j->rec_sig, n * 4, hipMemcpyDeviceToHost));
+  if (prim) HIPCHK(hipMemcpy(prim, j->rec_prim, n * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int dspt_jobs_step(dspt_ctx* c, const int32_t* job_ids, int32_t n, int64_t max_paths, int32_t max_frames_per_job,
+                   int32_t* frames_done) {
+  if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
+  std::vector<int> ids;
+  if (job_ids) ids.assign(job_ids, job_ids + std::max(0, n));
+  else
+    for (size_t i = 0; i < c->jobs.size(); i++)
+      if (job_ok(c, (int)i) && (int)i != c->legacy_job) ids.push_back((int)i);
+  HIPCHK(hipSetDevice(c->device));
+  return schedule_pass(c, ids, max_paths > 0 ? max_paths : c->max_paths, max_frames_per_job, frames_done);
+}
+
+int dspt_jobs_run(dspt_ctx* c, const int32_t* job_ids, int32_t n, int64_t max_paths) {
+  if (!c || !job_ids) return fail(DSPT_ERR_ARG, "NULL argument");
+  for (;;) {
+    int done = 0;
+    if (int r = dspt_jobs_step(c, job_ids, n, max_paths, 0, &done)) return r;
+    if (done == 0) break;
+  }
+  return 0;
+}
+
+int dspt_set_memory_budget(dspt_ctx* c, int64_t bytes) {
+  if (!c || bytes < 0) return fail(DSPT_ERR_ARG, "invalid budget");
+  c->budget = bytes;
+  return 0;
+}
+
+int dspt_set_launch_config(dspt_ctx* c, int64_t max_paths, int32_t trace_blocks, int32_t shade_blocks) {
+  if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (max_paths > 0) c->max_paths = std::max<int64_t>(65536, max_paths);
+  if (trace_blocks > 0 && trace_blocks != c->trace_blocks_max) {
+    c->trace_blocks_max = trace_blocks;
+    c->cap = 0;  // reallocate the spill area with the pool
+  }
+  if (shade_blocks > 0) c->shade_blocks_max = shade_blocks;
+  return 0;
+}
+
+int dspt_get_launch_config(dspt_ctx* c, int64_t* max_paths, int32_t* trace_blocks, int32_t* shade_blocks) {
+  if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
+  if (max_paths) *max_paths = c->max_paths;
+  if (trace_blocks) *trace_blocks = c->trace_blocks_max;
+  if (shade_blocks) *shade_blocks = c->shade_blocks_max;
+  return 0;
+}
+
+// ------------------------------------------------------------------ counters
 int dspt_get_counters(dspt_ctx* c, int64_t* out) {
   if (!c || !out) return fail(DSPT_ERR_ARG, "NULL argument");
   unsigned long long h[kNumCounters];
@@ -483,7 +862,7 @@ int dspt_get_counters(dspt_ctx* c, int64_t* out) {
   for (int i = 0; i < DSPT_CNT_COUNT; i++) out[i] = (int64_t)h[i];
   out[DSPT_CNT_NODE_VISITS] = (int64_t)(h[4] + h[10]);
   out[DSPT_CNT_TRI_TESTS] = (int64_t)(h[5] + h[11]);
-  // every traced extension-kernel ray that is not an extension ray is primary
+  // every ray the closest-hit kernel traced that is not an extension ray is a camera ray
   out[DSPT_CNT_RAYS_PRIMARY] = (int64_t)h[9] - (int64_t)h[2];
   return 0;
 }
@@ -522,11 +901,15 @@ int dspt_get_kernel_stats(dspt_ctx* c, double* out) {
   return 0;
 }
 
+// ------------------------------------------------------------------ traversal probes
 static int trace_common(dspt_ctx* c, const float* rays, int64_t n, int any, int32_t* tri, float* t, float* u, float* v) {
   if (!c || !rays || !tri) return fail(DSPT_ERR_ARG, "NULL argument");
-  if (!c->have_scene) return fail(DSPT_ERR_STATE, "trace before dspt_upload_scene");
+  if (c->scenes.empty() || !c->scenes[0]) return fail(DSPT_ERR_STATE, "trace before dspt_upload_scene");
   if (n <= 0) return 0;
   HIPCHK(hipSetDevice(c->device));
+  JobRes tmp;
+  tmp.scene = 0;
+  SceneView S = view_of(c, &tmp);
   float4* d_rays = nullptr;
   int32_t* d_tri = nullptr;
   float *d_t = nullptr, *d_u = nullptr, *d_v = nullptr;
@@ -541,7 +924,7 @@ static int trace_common(dspt_ctx* c, const float* rays, int64_t n, int any, int3
   }
   HIPCHK(hipMalloc(&d_spill, (size_t)blocks * kTraceBlock * (kStackSize - kLdsStack) * 4));
   HIPCHK(hipMemcpyAsync(d_rays, rays, n * 32, hipMemcpyHostToDevice, c->stream));
-  k_trace_rays<<<blocks, kTraceBlock, 0, c->stream>>>(view_of(c), d_rays, n, any, d_tri, d_t, d_u, d_v, d_spill, c->cnt);
+  k_trace_rays<<<blocks, kTraceBlock, 0, c->stream>>>(S, d_rays, n, any, d_tri, d_t, d_u, d_v, d_spill, c->cnt);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(tri, d_tri, n * 4, hipMemcpyDeviceToHost, c->stream));
   if (t) {
@@ -599,7 +982,7 @@ int dspt_build_bvh(const float* positions, int64_t nv, const uint32_t* indices, 
 
 int dspt_memory_usage(dspt_ctx* c, int64_t* bytes) {
   if (!c || !bytes) return fail(DSPT_ERR_ARG, "NULL argument");
-  *bytes = (int64_t)(c->bytes_scene + c->bytes_ibl + c->bytes_state + c->bytes_frame);
+  *bytes = memory_in_use(c);
   return 0;
 }
 

@@ -1,16 +1,19 @@
 // Wavefront path-tracing kernels for gfx950 (wave64, VALU only).
 //
-// One frame batch = `nslots` consecutive frames of one image, one path per
-// pixel per frame (path id p = slot * npix + pixel). Per bounce:
-//   trace_ext   closest hit of the extension rays in the active queue
-//   shade       render state, material, RR, emission, BSDF sample, NEE sample
-//               + BSDF eval/pdf; appends extension rays and shadow rays
+// A pass renders a list of tasks (runs of consecutive frames of one job; a
+// job = one scene + IBL + camera/frame block + resolution). Every path
+// carries its job index, so paths of several scenes, resolutions and frame
+// counts share one set of launches. Per bounce:
+//   trace_ext    closest hit of the extension rays in the active queue
+//   shade        render state, material, RR, emission, BSDF sample, NEE sample
+//                + BSDF eval/pdf; appends extension rays and shadow rays
 //   trace_shadow any-hit test of the NEE shadow rays, adds the deferred
-//               NEE radiance in the reference's summation order
-// then accumulate folds the finished samples into the running mean frame by
-// frame (renderer.frag:153-156), so the result does not depend on the batch
-// size.
+//                NEE radiance in the reference's summation order
+// then accumulate folds each task's finished samples into its job's running
+// mean frame by frame (renderer.frag:153-156), so results do not depend on
+// how frames and jobs were packed into passes.
 #include "dspt_kernels.hpp"
+#include "dev_shade.hpp"
 
 // --------------------------------------------------------------------------
 // Scene preparation: per BVH index slot, the triangle's three positions in
@@ -79,15 +82,25 @@ __global__ void k_fill_f4(float4* __restrict__ p, int64_t n, float4 v) {
 }
 
 // Camera rays, renderer.frag:129-142 + rng_init (rng.glsl:23-29).
-__global__ void k_raygen(PathState ps, FrameParams F, int W, int H, int npix, int first_frame, int64_t P,
-                         int32_t* __restrict__ counts, unsigned long long* __restrict__ cnt) {
+__global__ void k_raygen(PathState ps, const DevJob* __restrict__ jobs, const PassTask* __restrict__ tasks, int ntasks,
+                         int64_t P, int32_t* __restrict__ counts) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p == 0) { counts[0] = (int32_t)P; }
   if (p >= P) return;
-  int slot = (int)(p / npix);
-  int pix = (int)(p - (int64_t)slot * npix);
+  int lo = 0, hi = ntasks - 1;  // last task with offset <= p
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].offset <= p) lo = mid; else hi = mid - 1;
+  }
+  const PassTask T = tasks[lo];
+  const DevJob* J = jobs + T.job;
+  int64_t local = p - T.offset;
+  int slot = (int)(local / T.npix);
+  int pix = (int)(local - (int64_t)slot * T.npix);
+  int W = J->W, H = J->H;
   int px = pix % W, py = pix / W;
-  int frame = first_frame + slot;
+  int frame = T.first_frame + slot;
+  const FrameParams& F = J->F;
   Rng rng = rng_seed(frame * (int)F.max_bounces, px, py);
   float r0 = rng.uniform(), r1 = rng.uniform();
   float vux = (2.0f * (float(px) + 0.5f)) / float(W) - 1.0f;
@@ -104,6 +117,7 @@ __global__ void k_raygen(PathState ps, FrameParams F, int W, int H, int npix, in
   ps.pw[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
   ps.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   ps.rng[p] = make_uint2(rng.x, rng.y);
+  ps.job[p] = T.job;
   if (ps.sig) ps.sig[p] = 2166136261u;
 }
 
@@ -121,7 +135,7 @@ DI int64_t xcd_block(int64_t b, int64_t nb) {
   return x * per + k;
 }
 
-__global__ __launch_bounds__(kTraceBlock) void k_trace_ext(SceneView S, PathState ps, const int32_t* __restrict__ queue,
+__global__ __launch_bounds__(kTraceBlock) void k_trace_ext(const DevJob* __restrict__ jobs, PathState ps, const int32_t* __restrict__ queue,
                                                            const int32_t* __restrict__ count_ptr, int* __restrict__ spill,
                                                            unsigned long long* __restrict__ cnt) {
   __shared__ int smem[kLdsStack * kTraceBlock];
@@ -136,7 +150,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_ext(SceneView S, PathStat
   for (int64_t q = lb * blockDim.x + threadIdx.x; q < count; q += nb * blockDim.x) {
     int p = queue ? queue[q] : (int)q;
     float4 o = ps.org[p], d = ps.dir[p];
-    RayHit h = traverse<false>(S, xyz(o), xyz(d), kTFAR_MAX, st, nv, nt);
+    RayHit h = traverse<false>(geo_of(jobs[ps.job[p]].S), xyz(o), xyz(d), kTFAR_MAX, st, nv, nt);
     ps.hit[p] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
     nr++;
   }
@@ -145,7 +159,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_ext(SceneView S, PathStat
   wave_add_u64(&cnt[9], nr);
 }
 
-__global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, PathState ps, const int32_t* __restrict__ queue,
+__global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(const DevJob* __restrict__ jobs, PathState ps, const int32_t* __restrict__ queue,
                                                               const int32_t* __restrict__ count_ptr, int* __restrict__ spill,
                                                               unsigned long long* __restrict__ cnt) {
   __shared__ int smem[kLdsStack * kTraceBlock];
@@ -159,19 +173,20 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(SceneView S, PathS
   int64_t lb = xcd_block(blockIdx.x, nb);
   for (int64_t q = lb * blockDim.x + threadIdx.x; q < count; q += nb * blockDim.x) {
     int p = queue[q];
+    const Geo G = geo_of(jobs[ps.job[p]].S);
     float4 so = ps.sh_org[p];
     int flags = __float_as_int(so.w);
     f3 org = xyz(so);
     f3 contrib = sp3(0.0f);
     if (flags & 1) {
       float4 sd = ps.sh_dir[p];
-      RayHit h = traverse<true>(S, org, xyz(sd), sd.w, st, nv, nt);
+      RayHit h = traverse<true>(G, org, xyz(sd), sd.w, st, nv, nt);
       nr++;
       if (h.tri < 0) contrib = xyz(ps.sh_c[p]);
     }
     if (flags & 2) {
       float4 pd = ps.sh_pdir[p];
-      RayHit h = traverse<true>(S, org, xyz(pd), pd.w, st, nv, nt);
+      RayHit h = traverse<true>(G, org, xyz(pd), pd.w, st, nv, nt);
       nr++;
       if (h.tri < 0) contrib = contrib + xyz(ps.sh_pc[p]);
     }
@@ -195,118 +210,19 @@ DI float4 finish(f3 L, float thr, bool do_clamp) {
   return make_float4(L.x, L.y, L.z, 1.0f);
 }
 
-struct Hitpoint {
-  f3 pos, ng_raw, n, t;
-  float tw;
-  f3 vc;
-  float vcw;
-  uint32_t mat;
-};
-
-// fillRenderState (renderer.frag:72-91) minus the material.
-DI Hitpoint fetch_hitpoint(const SceneView& S, f3 org, f3 dir, int tri, float t, float u, float v) {
-  Hitpoint hp;
-  hp.pos = org + dir * t;
-  uint32_t vi[3];
-  for (int k = 0; k < 3; k++) vi[k] = S.tris[(size_t)tri * 3 + k];
-  const float4* v0 = S.verts + (size_t)vi[0] * 5;
-  const float4* v1 = S.verts + (size_t)vi[1] * 5;
-  const float4* v2 = S.verts + (size_t)vi[2] * 5;
-  float4 p0 = v0[0], p1 = v1[0], p2 = v2[0];
-  hp.ng_raw = nrm3(cross3(xyz(p1) - xyz(p0), xyz(p2) - xyz(p0)));
-  float w0 = 1.0f - u - v;
-  hp.n = nrm3(w0 * xyz(v0[1]) + u * xyz(v1[1]) + v * xyz(v2[1]));
-  float4 t0 = v0[3], t1 = v1[3], t2 = v2[3];
-  float hand = (t0.w == t1.w && t0.w == t2.w) ? t0.w : 0.0f;
-  f3 tn = nrm3(w0 * xyz(t0) + u * xyz(t1) + v * xyz(t2));
-  if (len3(tn) > 0.99f && fabsf(hand) > 0.99f) {
-    hp.t = tn; hp.tw = -hand;
-  } else {
-    // get_onb(n)[0], utils.glsl:104-109
-    hp.t = nrm3(cross3(hp.n, f3{-hp.n.z, hp.n.x, -hp.n.y}));
-    hp.tw = 1.0f;
-  }
-  float4 c0 = v0[4], c1 = v1[4], c2 = v2[4];
-  hp.vc = w0 * xyz(c0) + u * xyz(c1) + v * xyz(c2);
-  hp.vcw = w0 * c0.w + u * c1.w + v * c2.w;
-  hp.mat = (uint32_t)p0.w;
-  return hp;
-}
-
-struct Shading {
-  bsdf::GltfInputs in;
-  f3 n, ng;          // flipped toward wi (fix_normals)
-  f3 tanA;           // anisotropy tangent
-  bool backside, thin;
-  float cutout;
-  f3 emissive, attColor;
-  float attDist;
-};
-
-// configure_gltf_material (material.glsl:5-110) for untextured materials.
-DI Shading configure(const SceneView& S, const Hitpoint& hp, f3 wi) {
-  const float4* m = S.mats + (size_t)hp.mat * 17;
-  float4 m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4], m5 = m[5], m6 = m[6], m7 = m[7], m8 = m[8], m9 = m[9],
-         m10 = m[10], m15 = m[15], m16 = m[16];
-  Shading sh;
-  bsdf::GltfInputs& g = sh.in;
-  g.baseColor = xyz(m0);
-  float opacity = 1.0f;
-  float l4 = sqrtf(hp.vc.x * hp.vc.x + hp.vc.y * hp.vc.y + hp.vc.z * hp.vc.z + hp.vcw * hp.vcw);
-  if (l4 > 0.0f) { g.baseColor = g.baseColor * hp.vc; opacity = opacity * hp.vcw; }
-  sh.cutout = m2.x * opacity;
-  if (m6.w > 0.0f) sh.cutout = (sh.cutout < m6.w) ? 0.0f : 1.0f;
-  if (m6.w == 1.0f) sh.cutout = 1.0f;
-  g.metallic = m0.w * 1.0f;
-  g.roughness = m1.x * 1.0f;
-  g.anisotropy = m1.y * 1.0f;
-  g.transmission = m1.w * 1.0f;
-  g.ior = m2.w;
-  g.specularColor = xyz(m3);
-  g.specular = m3.w * 1.0f;
-  g.sheenColor = xyz(m4);
-  g.sheenRoughness = m4.w * 1.0f;
-  g.clearcoat = m6.x * 1.0f;
-  g.clearcoatRoughness = m6.y * 1.0f;
-  g.diffuseTransmission = m6.z * 1.0f;
-  g.diffuseTransmissionColor = xyz(m15);
-  g.iridescence = m10.x * 1.0f;
-  g.iridescenceIor = m10.y;
-  g.iridescenceThickness = gmix(m10.z, m10.w, 1.0f);
-  g.dispersion = m16.x;
-  sh.thin = m8.w > 0.5f;
-  g.thin = sh.thin;
-  sh.emissive = xyz(m5);
-  sh.attColor = xyz(m7);
-  sh.attDist = m7.w;
-  // fix_normals (utils.glsl:88-98)
-  f3 n = hp.n, ng = hp.ng_raw;
-  sh.backside = false;
-  if (dot3(wi, ng) < 0.0f) { ng = -ng; sh.backside = true; }
-  if (dot3(ng, n) < 0.0f) n = -n;
-  sh.n = n; sh.ng = ng;
-  // anisotropy direction -> tangent rotation by atan2 + pi (material.glsl:74-83)
-  float rot = atan2f(m9.y, m9.x);
-  float angle = rot + kPI;
-  f3 t = hp.t;
-  if (angle > 0.0f) {
-    f3 b = cross3(n, hp.t) * hp.tw;
-    t = hp.t * cosf(angle) + b * sinf(angle);
-  }
-  sh.tanA = t;
-  return sh;
-}
-
-__global__ __launch_bounds__(256) void k_shade(SceneView S, FrameParams F, PathState ps, int iter, int npix,
+__global__ __launch_bounds__(256) void k_shade(const DevJob* __restrict__ jobs, PathState ps, int iter,
                                                const int32_t* __restrict__ queue_in, const int32_t* __restrict__ count_in,
                                                int32_t* __restrict__ queue_out, int32_t* __restrict__ count_out,
                                                int32_t* __restrict__ shq, int32_t* __restrict__ count_sh,
                                                unsigned long long* __restrict__ cnt) {
   const int count = *count_in;
-  const int maxB = (int)F.max_bounces;
   unsigned n_paths = 0, n_hits = 0, n_miss = 0, n_nee = 0, n_ext = 0;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < count; q += (int64_t)gridDim.x * blockDim.x) {
     int p = queue_in ? queue_in[q] : (int)q;
+    const DevJob* J = jobs + ps.job[p];
+    const SceneView& S = J->S;
+    const FrameParams& F = J->F;
+    const int maxB = (int)F.max_bounces;
     float4 hit = ps.hit[p];
     int tri = __float_as_int(hit.x);
     float4 o4 = ps.org[p], d4 = ps.dir[p];
@@ -474,21 +390,31 @@ __global__ __launch_bounds__(256) void k_shade(SceneView S, FrameParams F, PathS
   wave_add_u64(&cnt[2], n_ext);
 }
 
-// Running mean (renderer.frag:153-156), frame by frame in frame order.
-__global__ void k_accumulate(const float4* __restrict__ samples, float4* __restrict__ accum, int npix, int nslots,
-                             int first_frame) {
+// Running mean (renderer.frag:153-156), frame by frame in frame order, per
+// task (grid.y = task); also copies recorded samples for parity checks.
+__global__ void k_accumulate(PathState ps, const DevJob* __restrict__ jobs, const PassTask* __restrict__ tasks) {
+  const PassTask T = tasks[blockIdx.y];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npix) return;
-  float4 a = accum[i];
-  for (int s = 0; s < nslots; s++) {
-    float4 c = samples[(size_t)s * npix + i];
-    float N = float(first_frame + s);
+  if (i >= T.npix) return;
+  const DevJob* J = jobs + T.job;
+  float4* acc = J->accum;
+  float4 a = acc[i];
+  for (int s = 0; s < T.nframes; s++) {
+    int64_t p = T.offset + (int64_t)s * T.npix + i;
+    float4 c = ps.L[p];
+    int frame = T.first_frame + s;
+    float N = float(frame);
     a.x = (a.x * (N - 1.0f) + c.x) / N;
     a.y = (a.y * (N - 1.0f) + c.y) / N;
     a.z = (a.z * (N - 1.0f) + c.z) / N;
     a.w = (a.w * (N - 1.0f) + c.w) / N;
+    if (J->rec_L) {
+      int64_t r = (int64_t)(frame - J->rec_first) * T.npix + i;
+      J->rec_L[r] = c;
+      if (ps.sig) { J->rec_sig[r] = ps.sig[p]; J->rec_prim[r] = ps.prim[p]; }
+    }
   }
-  accum[i] = a;
+  acc[i] = a;
 }
 
 // Standalone traversal on caller rays {org.xyz, tfar, dir.xyz, pad}.
@@ -504,7 +430,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneView S, const f
   unsigned nv = 0, nt = 0;
   for (int64_t i = gtid; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     float4 a = rays[i * 2], b = rays[i * 2 + 1];
-    RayHit h = any ? traverse<true>(S, xyz(a), xyz(b), a.w, st, nv, nt) : traverse<false>(S, xyz(a), xyz(b), a.w, st, nv, nt);
+    const Geo G = geo_of(S);
+    RayHit h = any ? traverse<true>(G, xyz(a), xyz(b), a.w, st, nv, nt) : traverse<false>(G, xyz(a), xyz(b), a.w, st, nv, nt);
     tri[i] = h.tri;
     if (t) { t[i] = h.t; u[i] = h.u; v[i] = h.v; }
   }

@@ -53,6 +53,20 @@ SIGNATURES = {
     "dspt_memory_usage": (_c_int, [_vp, _i64p]),
     "dspt_set_kernel_timing": (_c_int, [_vp, _c_int]),
     "dspt_get_kernel_stats": (_c_int, [_vp, _dp]),
+    "dspt_scene_create": (_c_int, [_vp, _fp, _c_i64, _u32p, _c_i64, _fp, _c_i64, _i32p, _c_i64, _fp, _c_i32, _i32p]),
+    "dspt_scene_destroy": (_c_int, [_vp, _c_i32]),
+    "dspt_ibl_create": (_c_int, [_vp, _fp, _c_i32, _c_i32, _dp, _i32p]),
+    "dspt_ibl_destroy": (_c_int, [_vp, _c_i32]),
+    "dspt_job_create": (_c_int, [_vp, _c_i32, _c_i32, _fp, _c_i32, _c_i32, _c_i32, _c_i32, _i32p]),
+    "dspt_job_destroy": (_c_int, [_vp, _c_i32]),
+    "dspt_job_status": (_c_int, [_vp, _c_i32, _i32p, _i32p]),
+    "dspt_job_read_accum": (_c_int, [_vp, _c_i32, _fp]),
+    "dspt_job_read_samples": (_c_int, [_vp, _c_i32, _fp, _u32p, _i32p]),
+    "dspt_jobs_step": (_c_int, [_vp, _i32p, _c_i32, _c_i64, _c_i32, _i32p]),
+    "dspt_jobs_run": (_c_int, [_vp, _i32p, _c_i32, _c_i64]),
+    "dspt_set_memory_budget": (_c_int, [_vp, _c_i64]),
+    "dspt_set_launch_config": (_c_int, [_vp, _c_i64, _c_i32, _c_i32]),
+    "dspt_get_launch_config": (_c_int, [_vp, _i64p, _i32p, _i32p]),
 }
 KERNEL_CLASSES = ["raygen", "trace_ext", "shade", "trace_shadow", "accumulate"]
 
@@ -230,6 +244,91 @@ class Engine:
         out = out.reshape(8, 4)
         return {k: {"launches": int(out[i, 0]), "total_ms": float(out[i, 1]), "node_visits": int(out[i, 2]),
                     "tri_tests": int(out[i, 3])} for i, k in enumerate(KERNEL_CLASSES)}
+
+    # ---- resident scenes / IBLs and jobs (batched + served modes)
+    def scene_create(self, verts, tris, nodes, bvh_idx, mats):
+        verts, nodes, mats = f32(verts), f32(nodes), f32(mats)
+        tris = np.ascontiguousarray(tris, dtype=np.uint32)
+        bvh_idx = np.ascontiguousarray(bvh_idx, dtype=np.int32)
+        sid = ctypes.c_int32(-1)
+        check(self._L.dspt_scene_create(self.h, ptr(verts), verts.size // 20, ptr(tris, ctypes.c_uint32), tris.size // 3,
+                                        ptr(nodes), nodes.size // 16, ptr(bvh_idx, ctypes.c_int32), bvh_idx.size,
+                                        ptr(mats), mats.size // 68, ctypes.byref(sid)))
+        return sid.value
+
+    def scene_destroy(self, sid):
+        check(self._L.dspt_scene_destroy(self.h, int(sid)))
+
+    def ibl_create(self, rgba, w, h):
+        rgba = f32(rgba)
+        tot = ctypes.c_double(0)
+        iid = ctypes.c_int32(-1)
+        check(self._L.dspt_ibl_create(self.h, ptr(rgba), int(w), int(h), ctypes.byref(tot), ctypes.byref(iid)))
+        return iid.value, tot.value
+
+    def ibl_destroy(self, iid):
+        check(self._L.dspt_ibl_destroy(self.h, int(iid)))
+
+    def job_create(self, scene, ibl, block, w, h, frames, record=False):
+        block = f32(block)
+        jid = ctypes.c_int32(-1)
+        check(self._L.dspt_job_create(self.h, int(scene), int(ibl), ptr(block), int(w), int(h), int(frames),
+                                      1 if record else 0, ctypes.byref(jid)))
+        self._job_dims = getattr(self, "_job_dims", {})
+        self._job_dims[jid.value] = (int(w), int(h), int(frames))
+        return jid.value
+
+    def job_destroy(self, jid):
+        check(self._L.dspt_job_destroy(self.h, int(jid)))
+        getattr(self, "_job_dims", {}).pop(jid, None)
+
+    def job_status(self, jid):
+        d = ctypes.c_int32(0)
+        t = ctypes.c_int32(0)
+        check(self._L.dspt_job_status(self.h, int(jid), ctypes.byref(d), ctypes.byref(t)))
+        return d.value, t.value
+
+    def job_read_accum(self, jid):
+        w, h, _ = self._job_dims[jid]
+        out = np.zeros(w * h * 4, np.float32)
+        check(self._L.dspt_job_read_accum(self.h, int(jid), ptr(out)))
+        return out.reshape(h, w, 4)
+
+    def job_read_samples(self, jid):
+        w, h, nf = self._job_dims[jid]
+        n = nf * w * h
+        rgba = np.zeros(n * 4, np.float32)
+        sig = np.zeros(n, np.uint32)
+        prim = np.zeros(n, np.int32)
+        check(self._L.dspt_job_read_samples(self.h, int(jid), ptr(rgba), ptr(sig, ctypes.c_uint32), ptr(prim, ctypes.c_int32)))
+        return rgba.reshape(nf, h, w, 4), sig.reshape(nf, h, w), prim.reshape(nf, h, w)
+
+    def jobs_step(self, job_ids=None, max_paths=0, max_frames_per_job=0):
+        done = ctypes.c_int32(0)
+        if job_ids is None:
+            check(self._L.dspt_jobs_step(self.h, None, 0, int(max_paths), int(max_frames_per_job), ctypes.byref(done)))
+        else:
+            ids = np.ascontiguousarray(job_ids, np.int32)
+            check(self._L.dspt_jobs_step(self.h, ptr(ids, ctypes.c_int32), ids.size, int(max_paths),
+                                         int(max_frames_per_job), ctypes.byref(done)))
+        return done.value
+
+    def jobs_run(self, job_ids, max_paths=0):
+        ids = np.ascontiguousarray(job_ids, np.int32)
+        check(self._L.dspt_jobs_run(self.h, ptr(ids, ctypes.c_int32), ids.size, int(max_paths)))
+
+    def set_memory_budget(self, nbytes):
+        check(self._L.dspt_set_memory_budget(self.h, int(nbytes)))
+
+    def set_launch_config(self, max_paths=0, trace_blocks=0, shade_blocks=0):
+        check(self._L.dspt_set_launch_config(self.h, int(max_paths), int(trace_blocks), int(shade_blocks)))
+
+    def launch_config(self):
+        mp = ctypes.c_int64(0)
+        tb = ctypes.c_int32(0)
+        sb = ctypes.c_int32(0)
+        check(self._L.dspt_get_launch_config(self.h, ctypes.byref(mp), ctypes.byref(tb), ctypes.byref(sb)))
+        return {"max_paths": mp.value, "trace_blocks": tb.value, "shade_blocks": sb.value}
 
     def memory_usage(self):
         b = ctypes.c_int64(0)

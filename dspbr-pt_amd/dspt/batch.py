@@ -1,0 +1,190 @@
+"""Batched and served rendering on top of the engine's job API.
+
+BatchRenderer  many scenes / cameras / resolutions / frame counts rendered
+               together: every pass packs pending frames of all jobs into one
+               set of wavefront launches (`dspt_jobs_step`), with the launch
+               configuration chosen by a small autotuner. `run_plain` is the
+               one-job-at-a-time, one-frame-per-pass loop used as the
+               comparison point.
+Server         continuous batching: requests arrive over time, are admitted
+               when the device-memory budget allows (the engine refuses a job
+               with DSPT_ERR_MEMORY otherwise and the request waits), share
+               passes with the jobs already running, and report progressive
+               per-frame completion; latency = arrival -> last frame.
+"""
+import time
+
+import numpy as np
+
+from . import _lib
+from .renderer import pack_frame_block
+
+ERR_MEMORY = -4
+
+
+class JobSpec:
+    """One render request: scene key, IBL key, camera, resolution, frames."""
+
+    def __init__(self, scene, ibl, camera, width, height, frames, max_bounces=8, arrival=0.0, record=False, **frame_kw):
+        self.scene, self.ibl, self.camera = scene, ibl, camera
+        self.width, self.height, self.frames, self.max_bounces = width, height, frames, max_bounces
+        self.arrival, self.record, self.frame_kw = arrival, record, frame_kw
+
+    @property
+    def paths(self):
+        return self.width * self.height * self.frames
+
+
+class BatchRenderer:
+    def __init__(self, engine):
+        self.eng = engine
+        self.scenes = {}  # key -> (id, counts)
+        self.ibls = {}    # key -> (id, (w, h), total)
+
+    def add_scene(self, key, scene):
+        from ._lib import build_bvh
+        nodes, bidx = scene.bvhNodes, scene.bvhTriangleIndices
+        if nodes is None:
+            nodes, bidx = build_bvh(scene.bvhPositionBuffer, scene.bvhIndexBuffer)
+        sid = self.eng.scene_create(scene.vertexBuffer, scene.triangleIndexBuffer, nodes, bidx,
+                                    scene.getFlatMaterialBuffer())
+        self.scenes[key] = (sid, (scene.num_triangles, len(nodes) // 16, len(bidx)))
+        return sid
+
+    def add_ibl(self, key, ibl):
+        img = ibl["image"]
+        iid, tot = self.eng.ibl_create(img["data"], img["width"], img["height"])
+        self.ibls[key] = (iid, (img["width"], img["height"]), tot)
+        return iid
+
+    def block(self, spec):
+        sid, counts = self.scenes[spec.scene]
+        iid, isize, tot = self.ibls[spec.ibl] if spec.ibl is not None else (-1, (0, 0), 0.0)
+        kw = dict(spec.frame_kw)
+        return pack_frame_block(spec.camera, spec.width, spec.height, max_bounces=spec.max_bounces, ibl_size=isize,
+                                ibl_total_sum=tot, scene_counts=(counts[0], counts[1], counts[2], 0),
+                                use_ibl=kw.pop("use_ibl", spec.ibl is not None),
+                                show_background=kw.pop("show_background", spec.ibl is not None), **kw)
+
+    def create_job(self, spec):
+        sid = self.scenes[spec.scene][0]
+        iid = self.ibls[spec.ibl][0] if spec.ibl is not None else -1
+        return self.eng.job_create(sid, iid, self.block(spec), spec.width, spec.height, spec.frames, spec.record)
+
+    def run_batched(self, job_ids, max_paths=0, max_frames_per_job=0):
+        """Passes over all listed jobs until none has pending frames; returns passes run."""
+        passes = 0
+        while self.eng.jobs_step(job_ids, max_paths, max_frames_per_job) > 0:
+            passes += 1
+        self.eng.synchronize()
+        return passes
+
+    def run_plain(self, job_ids):
+        """One job at a time, one frame per pass (the frame loop of the reference's renderer)."""
+        passes = 0
+        for j in job_ids:
+            while self.eng.jobs_step([j], 0, 1) > 0:
+                passes += 1
+        self.eng.synchronize()
+        return passes
+
+    def autotune(self, make_jobs, candidates=None, budget_s=20.0):
+        """Times candidate launch configurations on a fresh copy of the workload
+        (make_jobs() -> job ids; they are destroyed afterwards) and keeps the
+        fastest. Returns (best, table)."""
+        if candidates is None:
+            candidates = [dict(max_paths=mp, trace_blocks=tb) for mp in (4 << 20, 8 << 20, 16 << 20)
+                          for tb in (1024, 2048, 4096)]
+        table = []
+        t_end = time.perf_counter() + budget_s
+        for cand in candidates:
+            if time.perf_counter() > t_end and table:
+                break
+            self.eng.set_launch_config(cand["max_paths"], cand["trace_blocks"], 0)
+            ids = make_jobs()
+            for _ in range(2):  # warm the pool allocation, then time one pass set
+                self.eng.jobs_step(ids, 0, 0)
+            self.eng.synchronize()
+            t0 = time.perf_counter()
+            frames = 0
+            for _ in range(3):
+                frames += self.eng.jobs_step(ids, 0, 0)
+            self.eng.synchronize()
+            dt = time.perf_counter() - t0
+            paths = 0
+            for j in ids:
+                d, _ = self.eng.job_status(j)
+                paths += 0  # rate from the frames completed in the timed passes
+            w = sum(self.eng._job_dims[j][0] * self.eng._job_dims[j][1] for j in ids) / max(1, len(ids))
+            rate = frames * w / dt if dt > 0 else 0.0
+            table.append((rate, cand))
+            for j in ids:
+                self.eng.job_destroy(j)
+        best = max(table, key=lambda x: x[0])[1]
+        self.eng.set_launch_config(best["max_paths"], best["trace_blocks"], 0)
+        return best, table
+
+
+class Server:
+    """Continuous-batching request loop over resident scenes."""
+
+    def __init__(self, batch, max_paths=0, frames_per_pass=1):
+        self.b = batch
+        self.max_paths = max_paths
+        self.frames_per_pass = frames_per_pass  # progressive: every live job advances this many frames per pass
+
+    def run(self, specs, clock=time.perf_counter, on_progress=None):
+        """specs: JobSpec list with .arrival in seconds from start. Returns per-job
+        records {arrival, admitted, first_frame, done, frames, paths}."""
+        pending = sorted(range(len(specs)), key=lambda i: specs[i].arrival)
+        live = {}  # job id -> spec index
+        rec = [dict(arrival=specs[i].arrival, admitted=None, first_frame=None, done=None, frames=specs[i].frames,
+                    paths=specs[i].paths) for i in range(len(specs))]
+        t0 = clock()
+        k = 0
+        while k < len(pending) or live:
+            now = clock() - t0
+            # admit arrived requests while memory allows
+            while k < len(pending) and specs[pending[k]].arrival <= now:
+                i = pending[k]
+                try:
+                    jid = self.b.create_job(specs[i])
+                except _lib.DsptError as e:
+                    if e.code == ERR_MEMORY and live:
+                        break  # wait for a running job to finish
+                    raise
+                live[jid] = i
+                rec[i]["admitted"] = now
+                rec[i]["job"] = jid
+                k += 1
+            if not live:
+                # idle until the next arrival
+                nxt = specs[pending[k]].arrival
+                while clock() - t0 < nxt:
+                    time.sleep(min(0.002, max(0.0, nxt - (clock() - t0))))
+                continue
+            ids = list(live)
+            self.b.eng.jobs_step(ids, self.max_paths, self.frames_per_pass)
+            self.b.eng.synchronize()
+            now = clock() - t0
+            for jid in ids:
+                i = live[jid]
+                d, tot = self.b.eng.job_status(jid)
+                if d >= 1 and rec[i]["first_frame"] is None:
+                    rec[i]["first_frame"] = now
+                if on_progress:
+                    on_progress(jid, d, tot)
+                if d >= tot:
+                    rec[i]["done"] = now
+                    if on_progress is None or not getattr(on_progress, "keep", False):
+                        self.b.eng.job_destroy(jid)
+                    del live[jid]
+        return rec
+
+
+def latency_stats(rec):
+    lat = np.array([r["done"] - r["arrival"] for r in rec if r["done"] is not None])
+    ttf = np.array([r["first_frame"] - r["arrival"] for r in rec if r["first_frame"] is not None])
+    return {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "first_frame_p50_ms": float(np.percentile(ttf, 50) * 1e3),
+            "first_frame_p99_ms": float(np.percentile(ttf, 99) * 1e3), "jobs": int(len(lat))}

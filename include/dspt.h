@@ -146,6 +146,43 @@ int dspt_build_bvh(const float* positions, int64_t num_vertices, const uint32_t*
 /* Device memory currently held by the context, bytes. */
 int dspt_memory_usage(dspt_ctx* ctx, int64_t* bytes);
 
+/* ---- Resident scenes / IBLs and render jobs (batched and served modes).
+ * No counterpart in the reference, which renders one scene at a time; these
+ * generalise setScene/setIBL/render to many scenes kept resident in HBM. */
+int dspt_scene_create(dspt_ctx* ctx, const float* vertices, int64_t num_vertices, const uint32_t* tri_indices,
+                      int64_t num_triangles, const float* bvh_nodes, int64_t num_nodes, const int32_t* bvh_indices,
+                      int64_t num_bvh_indices, const float* materials, int32_t num_materials, int32_t* scene_id);
+int dspt_scene_destroy(dspt_ctx* ctx, int32_t scene_id);
+int dspt_ibl_create(dspt_ctx* ctx, const float* rgba, int32_t width, int32_t height, double* total_sum, int32_t* ibl_id);
+int dspt_ibl_destroy(dspt_ctx* ctx, int32_t ibl_id);
+
+/* A job renders frames 1..num_frames of (scene, IBL or -1, frame block, W x H)
+ * into its own accumulation buffer. Admission: fails with DSPT_ERR_MEMORY when
+ * the job's buffers would exceed the memory budget (dspt_set_memory_budget). */
+int dspt_job_create(dspt_ctx* ctx, int32_t scene_id, int32_t ibl_id, const float block[DSPT_FRAME_FLOATS], int32_t width,
+                    int32_t height, int32_t num_frames, int32_t record_samples, int32_t* job_id);
+int dspt_job_destroy(dspt_ctx* ctx, int32_t job_id);
+int dspt_job_status(dspt_ctx* ctx, int32_t job_id, int32_t* frames_done, int32_t* frames_total);
+int dspt_job_read_accum(dspt_ctx* ctx, int32_t job_id, float* rgba);
+int dspt_job_read_samples(dspt_ctx* ctx, int32_t job_id, float* rgba, uint32_t* sigs, int32_t* prim);
+
+/* One pass: packs consecutive pending frames of the listed jobs (all live
+ * jobs when job_ids is NULL) into at most max_paths paths in flight (<= 0:
+ * the context default), up to max_frames_per_job frames each (<= 0: no cap),
+ * and renders them with one set of wavefront launches. frames_done receives
+ * the number of job-frames completed (0 when nothing was pending). Each
+ * job's result does not depend on how its frames were packed. */
+int dspt_jobs_step(dspt_ctx* ctx, const int32_t* job_ids, int32_t n, int64_t max_paths, int32_t max_frames_per_job,
+                   int32_t* frames_done);
+/* Steps until the listed jobs have no pending frames. */
+int dspt_jobs_run(dspt_ctx* ctx, const int32_t* job_ids, int32_t n, int64_t max_paths);
+int dspt_set_memory_budget(dspt_ctx* ctx, int64_t bytes);
+
+/* Launch configuration: paths in flight per pass, traversal and shading grid
+ * caps (<= 0 keeps a value). Chosen by the autotuner in the host layer. */
+int dspt_set_launch_config(dspt_ctx* ctx, int64_t max_paths, int32_t trace_blocks, int32_t shade_blocks);
+int dspt_get_launch_config(dspt_ctx* ctx, int64_t* max_paths, int32_t* trace_blocks, int32_t* shade_blocks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,85 @@
+"""The N-API addon + Node PathtracingRenderer drop-in (addon/): loads on the
+CPU and reports engine errors as JS Errors; on the GPU it renders the same
+image as the engine driven from Python (same BVH builder, same frames)."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "addon", "build", "dspt.node")
+
+
+def _node(script, *args, timeout=120):
+    return subprocess.run(["node", "-e", script, *args], capture_output=True, text=True, timeout=timeout, cwd=ROOT)
+
+
+@pytest.fixture(scope="module")
+def addon_built():
+    if not os.path.exists(ADDON):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "addon")], check=True)
+    return ADDON
+
+
+def test_addon_exports_and_error_channel(addon_built):
+    r = _node("const d=require('./addon/build/dspt.node'); console.log(JSON.stringify(Object.keys(d)));"
+              "try { d.create(9, 0) } catch (e) { console.log(e.message) }")
+    assert r.returncode == 0, r.stderr
+    keys = json.loads(r.stdout.splitlines()[0])
+    assert {"create", "uploadScene", "uploadIbl", "resize", "setFrame", "render", "readAccum", "buildBvh"} <= set(keys)
+    assert "dspt error -" in r.stdout.splitlines()[1]
+
+
+def test_addon_bvh_builder_matches_python(addon_built, tmp_path):
+    from dspt import build_bvh, scenes
+    wl = scenes.sphere_field(n_spheres=4)
+    pos = np.asarray(wl.scene.bvhPositionBuffer, np.float32)
+    ind = np.asarray(wl.scene.bvhIndexBuffer, np.uint32)
+    pos.tofile(tmp_path / "p.f32")
+    ind.tofile(tmp_path / "i.u32")
+    r = _node("const fs=require('fs'); const d=require('./addon/build/dspt.node');"
+              "const rd=(f,T)=>{const b=fs.readFileSync(f); return new T(b.buffer.slice(b.byteOffset,b.byteOffset+b.byteLength))};"
+              "const o=d.buildBvh(rd(process.argv[1],Float32Array), rd(process.argv[2],Uint32Array), 4);"
+              "fs.writeFileSync(process.argv[3], Buffer.from(o.nodeData.buffer));"
+              "fs.writeFileSync(process.argv[4], Buffer.from(o.triangleIndices.buffer));",
+              str(tmp_path / "p.f32"), str(tmp_path / "i.u32"), str(tmp_path / "n.f32"), str(tmp_path / "b.i32"))
+    assert r.returncode == 0, r.stderr
+    nodes, bidx = build_bvh(pos, ind)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "n.f32", np.float32), nodes)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "b.i32", np.int32), bidx)
+
+
+@pytest.mark.gpu
+def test_node_render_matches_python_engine(addon_built, tmp_path):
+    from dspt import Engine, build_bvh, pack_frame_block, scenes
+    wl = scenes.sphere_field(n_spheres=16)
+    s = wl.scene
+    W, H, F = 64, 48, 3
+    img = wl.ibl["image"]
+    np.asarray(s.vertexBuffer, np.float32).tofile(tmp_path / "verts.f32")
+    np.asarray(s.triangleIndexBuffer, np.uint32).tofile(tmp_path / "tris.u32")
+    np.asarray(s.bvhPositionBuffer, np.float32).tofile(tmp_path / "bvhpos.f32")
+    s.getFlatMaterialBuffer().astype(np.float32).tofile(tmp_path / "mats.f32")
+    np.asarray(img["data"], np.float32).tofile(tmp_path / "ibl.f32")
+    meta = {"profile": "webgl-lean", "width": W, "height": H, "maxBounces": 8, "iblWidth": img["width"],
+            "iblHeight": img["height"], "fov": wl.camera.fov, "near": wl.camera.near,
+            "matrixWorld": [float(x) for x in wl.camera.matrixWorld],
+            "position": dict(zip("xyz", wl.camera.position)), "numSamples": F + 1, "framesPerCall": 2}
+    (tmp_path / "meta.json").write_text(json.dumps(meta))
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "node_render.js"), str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    assert info["sampleCount"] == F  # frames 1 .. numSamples-1, as the reference's loop
+    node_acc = np.fromfile(tmp_path / "accum.f32", np.float32).reshape(H, W, 4)
+    nodes, bidx = build_bvh(s.bvhPositionBuffer, s.bvhIndexBuffer)
+    eng = Engine(0)
+    eng.upload_scene(s.vertexBuffer, s.triangleIndexBuffer, nodes, bidx, s.getFlatMaterialBuffer())
+    tot = eng.upload_ibl(img["data"], img["width"], img["height"])
+    eng.resize(W, H)
+    eng.set_frame(pack_frame_block(wl.camera, W, H, max_bounces=8, ibl_size=(img["width"], img["height"]),
+                                   ibl_total_sum=tot, scene_counts=(s.num_triangles, len(nodes) // 16, len(bidx), 0)))
+    eng.render(1, F)
+    np.testing.assert_array_equal(eng.read_accum().view(np.uint32), node_acc.view(np.uint32))
