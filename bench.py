@@ -68,21 +68,38 @@ def cpu_baseline(wl, block, budget_s=20.0):
     osc = O.OracleScene(s.vertexBuffer, s.triangleIndexBuffer, s.bvhNodes, s.bvhTriangleIndices, s.getFlatMaterialBuffer(),
                         wl.ibl, 1 if wl.profile == "webgl-full" else 0)
     W, H = wl.width, wl.height
-    rows, spent, paths = 8, 0.0, 0
-    y0 = H // 2
-    while spent < budget_s * 0.5 and rows <= H:
+    rows, spent, paths, frame, frames_full = 8, 0.0, 0, 1, 0
+    while spent < budget_s * 0.75:
+        r = min(rows, H)
+        y = max(0, H // 2 - r // 2)
         t0 = time.time()
-        osc.render(block, W, H, 1, 1, rect=(0, y0 - rows // 2, W, y0 - rows // 2 + rows), record=False, threads=threads)
+        osc.render(block, W, H, frame, 1, rect=(0, y, W, y + r), record=False, threads=threads)
         dt = time.time() - t0
         spent += dt
-        paths += W * rows
-        if dt < budget_s * 0.25:
+        paths += W * r
+        if r == H:
+            frame += 1
+            frames_full += 1
+        elif dt < budget_s / 8:
             rows *= 2
         else:
             break
     return {"value": paths / spent / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"oracle FP32 restatement, 1 spp over centred full-width row bands of the same "
-                      f"{W}x{H} frame, {paths} paths in {spent:.1f}s"}
+            "sample": f"oracle FP32 restatement over centred full-width row bands (doubling from 8 rows) then "
+                      f"{frames_full} whole frames of the same {W}x{H} image, {paths} paths in {spent:.1f}s"}
+
+
+def calibrate_copy(mib):
+    """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
+    per dispatch give the scale between the counters and real HBM bytes."""
+    import torch
+    n = mib << 20
+    a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read": n, "bytes_written": n, "dispatches": 5}), flush=True)
 
 
 def main():
@@ -94,11 +111,17 @@ def main():
     ap.add_argument("--spp", type=int, default=None, help="override frames per job")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine is measured on one GPU only; --gpus must be 1 "
                                    "(multi-GPU sharding is design-only)"}))
         sys.exit(2)
+
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     from dspt import Engine, PROFILE_FULL, PROFILE_LEAN
     wl = make_workload(args.workload)

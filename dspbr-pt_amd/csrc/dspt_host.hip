@@ -1,12 +1,21 @@
 // Host runtime and C ABI (include/dspt.h) of the gfx950 path-tracing engine.
 //
 // Resources: resident scenes and IBLs (ids), render jobs (scene + IBL +
-// frame block + resolution + frame count + accumulation). Work runs in
-// passes: a pass packs runs of consecutive frames of one or more jobs into
-// the path pool (up to max_paths paths in flight) and executes the wavefront
-// bounce loop once for all of them. The single-image API (dspt_upload_scene /
-// dspt_upload_ibl / dspt_set_frame / dspt_render) drives an implicit job on
-// scene 0 / IBL 0; the batched and served modes create jobs explicitly.
+// frame block + resolution + frame range + accumulation). All rendering
+// goes through one path pool: work items (job, frame, pixel) are issued in
+// the order of a host-maintained task list, and a slot freed by a finished
+// path is refilled on the next bounce. The host runs the pool in chunks of
+// a few bounces; after each chunk the device folds every complete frame
+// into its job's running mean and the host reads back the progress one
+// chunk late (the GPU always has the next chunk queued). Two schedules:
+//   STREAM  frames are appended round-robin over the listed jobs ahead of
+//           the pool, so the pool stays full across frames and jobs;
+//   PASS    frame-synchronous passes: up to a pool's worth of frames is
+//           issued only when the pool is idle (the plain loop).
+// A job's frame f is scheduled only when frame f - ring_frames has been
+// folded, which bounds the ring of finished-but-unfolded frames.
+// The single-image API (dspt_upload_scene / dspt_upload_ibl /
+// dspt_set_frame / dspt_render) drives an implicit job on scene 0 / IBL 0.
 // Kernels are in dspt_kernels.hip (same translation unit).
 #include <hip/hip_runtime.h>
 
@@ -16,6 +25,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -42,7 +52,8 @@ int fail(int code, const std::string& msg) {
 
 constexpr int kNumCounters = 32;
 constexpr int kMaxJobs = 4096;
-constexpr int kMaxTasks = 4096;
+constexpr int kMaxTasks = 8192;
+constexpr int64_t kStallChunks = 4096;
 
 struct Allocs {
   std::vector<void*> ptrs;
@@ -90,15 +101,27 @@ struct JobRes {
   FrameParams F{};
   int W = 0, H = 0;
   float4* accum = nullptr;
-  int first = 1;   // next frame to render
-  int last = 0;    // last frame to render (inclusive)
+  float4* ring = nullptr;
+  int32_t* done = nullptr;
+  int ring_frames = 1;
+  int frame_begin = 1, frame_end = 1;  // current frame range [begin, end)
+  int next_sched = 1;                  // next frame to schedule
+  int acc_known = 1;                   // frames below are folded (latest readback)
+  int64_t epoch = 0;                   // first chunk whose readback describes this range
   bool record = false;
   float4* rec_L = nullptr;
   uint32_t* rec_sig = nullptr;
   int32_t* rec_prim = nullptr;
   int rec_first = 1, rec_count = 0;
-  bool alive = false;
+  bool alive = false, cancel = false;
   int64_t npix() const { return (int64_t)W * H; }
+  bool complete() const { return acc_known >= frame_end; }
+  bool in_flight() const { return next_sched > acc_known; }
+};
+
+struct Readback {
+  StreamCtl ctl;
+  int32_t prog[kMaxJobs];
 };
 
 }  // namespace
@@ -117,23 +140,43 @@ struct dspt_ctx {
   int legacy_job = -1;
   bool record = false;
   int last_frames = 0;
-  // pass machinery
+  // job table
   DevJob* d_jobs = nullptr;
-  PassTask* d_tasks = nullptr;
-  PassTask* h_tasks = nullptr;  // pinned staging
-  DevJob* h_jobs = nullptr;     // pinned staging
+  DevJob* h_jobs = nullptr;  // pinned staging
   bool jobs_dirty = true;
+  // path pool
   Allocs state_mem;
   int64_t cap = 0;
-  bool cap_record = false;
   PathState ps{};
-  int32_t *qa = nullptr, *qb = nullptr, *shq = nullptr, *counts = nullptr;
+  int32_t *qa = nullptr, *qb = nullptr, *shq = nullptr, *free_list = nullptr;
   int* spill = nullptr;
   int trace_blocks = 0;
   int trace_blocks_max = 2048;
   int shade_blocks_max = 8192;
-  unsigned long long* cnt = nullptr;
   int64_t max_paths = 8 << 20;
+  unsigned long long* cnt = nullptr;
+  // scheduler
+  StreamCtl* d_ctl = nullptr;
+  StreamSched* d_sched = nullptr;
+  int32_t* d_prog = nullptr;
+  int32_t* d_joblist = nullptr;
+  PassTask* d_tasks = nullptr;
+  Readback* h_rb = nullptr;        // pinned [2]
+  StreamSched* h_sched = nullptr;  // pinned [2]
+  PassTask* h_tasks = nullptr;     // pinned [2][kMaxTasks]
+  int32_t* h_joblist = nullptr;    // pinned [2][kMaxJobs]
+  hipEvent_t ev_chunk[2] = {nullptr, nullptr};
+  int64_t limit_at[2] = {0, 0};
+  int64_t chunk_no = 0;
+  bool have_prev = false;
+  int qsel = 0;
+  int64_t sched_limit = 0, next_known = 0, limit_known = 0, alive_known = 0;
+  std::deque<PassTask> tasks;
+  int mode = DSPT_SCHED_STREAM;
+  int pass_frames = 0;
+  int chunk_iters = 2;
+  float ring_factor = 8.0f;
+  size_t rr = 0;
   int64_t budget = 0;  // admission budget for job memory (0 = unlimited)
   // kernel timing
   bool timing = false;
@@ -184,8 +227,15 @@ void drain_events(dspt_ctx* c) {
   c->ev_pending.clear();
 }
 
-int64_t job_bytes(int64_t npix, bool record, int64_t frames) {
-  int64_t b = npix * 16;
+int ring_frames_for(const dspt_ctx* c, int64_t npix, int frames) {
+  double want = std::ceil((double)c->ring_factor * (double)c->max_paths / (double)npix) + 2.0;
+  int64_t r = (int64_t)std::min<double>(want, 1 << 20);
+  r = std::min<int64_t>(r, std::max(frames, 1));
+  return (int)std::max<int64_t>(r, 1);
+}
+
+int64_t job_bytes(int64_t npix, bool record, int64_t frames, int ring) {
+  int64_t b = npix * 16 + (int64_t)ring * npix * 16 + (int64_t)ring * 4;
   if (record) b += frames * npix * (16 + 8);
   return b;
 }
@@ -196,30 +246,6 @@ int64_t memory_in_use(dspt_ctx* c) {
   for (auto* i : c->ibls) if (i) b += (int64_t)i->mem.bytes;
   for (auto* j : c->jobs) if (j) b += (int64_t)j->mem.bytes;
   return b;
-}
-
-// Path pool for `cap` paths in flight.
-int ensure_state(dspt_ctx* c, int64_t cap, bool record) {
-  if (c->cap >= cap && c->cap_record == record) return 0;
-  HIPCHK(hipStreamSynchronize(c->stream));
-  c->state_mem.release();
-  c->cap = 0;
-  PathState& ps = c->ps;
-  ps = PathState{};
-  Allocs& A = c->state_mem;
-  int r = 0;
-  if ((r = A.get(&ps.org, cap)) || (r = A.get(&ps.dir, cap)) || (r = A.get(&ps.hit, cap)) || (r = A.get(&ps.pw, cap)) ||
-      (r = A.get(&ps.L, cap)) || (r = A.get(&ps.rng, cap)) || (r = A.get(&ps.job, cap)) || (r = A.get(&ps.sh_org, cap)) ||
-      (r = A.get(&ps.sh_dir, cap)) || (r = A.get(&ps.sh_c, cap)) || (r = A.get(&ps.sh_pw, cap)) ||
-      (r = A.get(&ps.sh_pdir, cap)) || (r = A.get(&ps.sh_pc, cap)) || (r = A.get(&c->qa, cap)) || (r = A.get(&c->qb, cap)) ||
-      (r = A.get(&c->shq, cap)))
-    return r;
-  if (record && ((r = A.get(&ps.sig, cap)) || (r = A.get(&ps.prim, cap)))) return r;
-  c->trace_blocks = (int)std::min<int64_t>(c->trace_blocks_max, grid_for(cap, kTraceBlock));
-  if ((r = A.get(&c->spill, (size_t)c->trace_blocks * kTraceBlock * (kStackSize - kLdsStack)))) return r;
-  c->cap = cap;
-  c->cap_record = record;
-  return 0;
 }
 
 SceneView view_of(dspt_ctx* c, const JobRes* j) {
@@ -247,6 +273,7 @@ int sync_job_table(dspt_ctx* c) {
       d.F = j->F;
       d.W = j->W; d.H = j->H; d.npix = (int32_t)j->npix();
       d.accum = j->accum;
+      d.ring = j->ring; d.done = j->done; d.ring_frames = j->ring_frames; d.frame_end = j->frame_end;
       d.rec_L = j->record ? j->rec_L : nullptr;
       d.rec_sig = j->rec_sig; d.rec_prim = j->rec_prim;
       d.rec_first = j->rec_first;
@@ -259,58 +286,255 @@ int sync_job_table(dspt_ctx* c) {
   return 0;
 }
 
-// Runs one pass over `tasks` (frames of jobs), then advances each job.
-int run_pass(dspt_ctx* c, const std::vector<PassTask>& tasks) {
-  if (tasks.empty()) return 0;
-  if ((int)tasks.size() > kMaxTasks) return fail(DSPT_ERR_ARG, "too many tasks in one pass");
-  int64_t P = 0;
-  int maxB = 0, max_npix = 0;
-  bool record = false;
-  for (const PassTask& t : tasks) {
-    JobRes* j = c->jobs[t.job];
-    P += (int64_t)t.nframes * t.npix;
-    maxB = std::max(maxB, (int)j->F.max_bounces);
-    max_npix = std::max(max_npix, t.npix);
-    record = record || j->record;
+bool job_ok(dspt_ctx* c, int id) { return id >= 0 && id < (int)c->jobs.size() && c->jobs[id] && c->jobs[id]->alive; }
+
+// The pool is idle once the latest readback shows every scheduled item
+// issued and no path alive, and nothing was scheduled after that chunk.
+bool pool_idle(const dspt_ctx* c) {
+  return c->next_known >= c->limit_known && c->alive_known == 0 && c->limit_known == c->sched_limit;
+}
+
+// Path pool of c->max_paths slots (reallocated only while idle).
+int ensure_pool(dspt_ctx* c) {
+  if (c->cap == c->max_paths && c->cap > 0) return 0;
+  if (!pool_idle(c)) return 0;  // keep the current pool until it drains
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int64_t cap = c->max_paths;
+  c->state_mem.release();
+  c->cap = 0;
+  PathState& ps = c->ps;
+  ps = PathState{};
+  Allocs& A = c->state_mem;
+  int r = 0;
+  if ((r = A.get(&ps.org, cap)) || (r = A.get(&ps.dir, cap)) || (r = A.get(&ps.hit, cap)) || (r = A.get(&ps.pw, cap)) ||
+      (r = A.get(&ps.L, cap)) || (r = A.get(&ps.rng, cap)) || (r = A.get(&ps.job, cap)) || (r = A.get(&ps.bounce, cap)) ||
+      (r = A.get(&ps.fp, cap)) || (r = A.get(&ps.sig, cap)) || (r = A.get(&ps.prim, cap)) || (r = A.get(&ps.sh_org, cap)) ||
+      (r = A.get(&ps.sh_dir, cap)) || (r = A.get(&ps.sh_c, cap)) || (r = A.get(&ps.sh_pw, cap)) ||
+      (r = A.get(&ps.sh_pdir, cap)) || (r = A.get(&ps.sh_pc, cap)) || (r = A.get(&c->qa, cap)) || (r = A.get(&c->qb, cap)) ||
+      (r = A.get(&c->shq, cap)) || (r = A.get(&c->free_list, cap)))
+    return r;
+  c->trace_blocks = (int)std::min<int64_t>(c->trace_blocks_max, grid_for(cap, kTraceBlock));
+  if ((r = A.get(&c->spill, (size_t)c->trace_blocks * kTraceBlock * (kStackSize - kLdsStack)))) return r;
+  k_stream_init<<<grid_for(cap, 256), 256, 0, c->stream>>>(c->d_ctl, c->free_list, (int32_t)cap);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->cap = cap;
+  c->qsel = 0;
+  c->sched_limit = c->next_known = c->limit_known = c->alive_known = 0;
+  c->limit_at[0] = c->limit_at[1] = 0;
+  c->tasks.clear();
+  c->have_prev = false;
+  return 0;
+}
+
+bool eligible(const dspt_ctx* c, const JobRes* j) {
+  (void)c;
+  return j && j->alive && !j->cancel && j->next_sched < j->frame_end && j->next_sched < j->acc_known + j->ring_frames;
+}
+
+bool append_frames(dspt_ctx* c, int id, int nf) {
+  JobRes* j = c->jobs[id];
+  const int64_t npix = j->npix();
+  if (!c->tasks.empty()) {
+    PassTask& b = c->tasks.back();
+    if (b.job == id && b.first_frame + b.nframes == j->next_sched && b.offset + (int64_t)b.nframes * b.npix == c->sched_limit) {
+      b.nframes += nf;
+      c->sched_limit += (int64_t)nf * npix;
+      j->next_sched += nf;
+      return true;
+    }
   }
-  if (P >= (1LL << 31)) return fail(DSPT_ERR_ARG, "pass exceeds 2^31 paths");
-  if (int r = ensure_state(c, std::max<int64_t>(P, c->cap), record || c->cap_record)) return r;
+  if ((int)c->tasks.size() >= kMaxTasks) return false;
+  PassTask t;
+  t.job = id; t.first_frame = j->next_sched; t.nframes = nf; t.npix = (int32_t)npix; t.offset = c->sched_limit;
+  c->tasks.push_back(t);
+  c->sched_limit += (int64_t)nf * npix;
+  j->next_sched += nf;
+  return true;
+}
+
+// Appends frames of the listed jobs to the task list (see the file comment).
+void schedule(dspt_ctx* c, const std::vector<int>& ids) {
+  if (ids.empty()) return;
+  if (c->mode == DSPT_SCHED_PASS) {
+    if (!pool_idle(c)) return;
+    int64_t room = c->cap;
+    bool any = false;
+    for (int id : ids) {
+      if (!job_ok(c, id)) continue;
+      JobRes* j = c->jobs[id];
+      if (!eligible(c, j)) continue;
+      int64_t npix = j->npix();
+      int64_t nf = std::min<int64_t>(j->frame_end - j->next_sched, j->acc_known + j->ring_frames - j->next_sched);
+      if (c->pass_frames > 0) nf = std::min<int64_t>(nf, c->pass_frames);
+      nf = std::min<int64_t>(nf, room / npix);
+      if (nf <= 0) {
+        if (any) continue;
+        nf = 1;  // a frame larger than the pool streams through it
+      }
+      if (!append_frames(c, id, (int)nf)) break;
+      any = true;
+      room -= nf * npix;
+      if (room <= 0) break;
+    }
+    return;
+  }
+  const int64_t target = c->next_known + (int64_t)(c->chunk_iters + 2) * c->cap;
+  const size_t n = ids.size();
+  bool progress = true;
+  while (c->sched_limit < target && progress) {
+    progress = false;
+    for (size_t k = 0; k < n && c->sched_limit < target; k++) {
+      int id = ids[(c->rr + k) % n];
+      if (!job_ok(c, id) || !eligible(c, c->jobs[id])) continue;
+      if (!append_frames(c, id, 1)) return;
+      progress = true;
+    }
+    c->rr = (c->rr + 1) % n;
+  }
+}
+
+void consume_readback(dspt_ctx* c, int s, int64_t chunk) {
+  const Readback& rb = c->h_rb[s];
+  c->next_known = rb.ctl.next_item;
+  c->alive_known = c->cap - rb.ctl.free_count;
+  c->limit_known = c->limit_at[s];
+  const int n = std::min<int>((int)c->jobs.size(), kMaxJobs);
+  for (int i = 0; i < n; i++) {
+    JobRes* j = c->jobs[i];
+    if (!j || !j->alive || j->epoch > chunk) continue;
+    j->acc_known = std::max(j->acc_known, std::min(rb.prog[i], j->frame_end));
+  }
+}
+
+// Enqueues one chunk (schedule, chunk_iters bounces, fold), then waits for
+// the previous chunk and takes its readback.
+int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
+  if (int r = ensure_pool(c)) return r;
+  if (c->cap <= 0) return fail(DSPT_ERR_STATE, "path pool not allocated");
   if (int r = sync_job_table(c)) return r;
-  HIPCHK(hipStreamSynchronize(c->stream));  // task staging reuse
-  std::memcpy(c->h_tasks, tasks.data(), tasks.size() * sizeof(PassTask));
-  HIPCHK(hipMemcpyAsync(c->d_tasks, c->h_tasks, tasks.size() * sizeof(PassTask), hipMemcpyHostToDevice, c->stream));
-  PathState ps = c->ps;
-  if (!record) { ps.sig = nullptr; ps.prim = nullptr; }
+  if (schedule_new) schedule(c, ids);
+  while (!c->tasks.empty()) {
+    const PassTask& t = c->tasks.front();
+    if (t.offset + (int64_t)t.nframes * t.npix > c->next_known) break;
+    c->tasks.pop_front();
+  }
+  const int s = (int)(c->chunk_no & 1);
+  const int nt = (int)c->tasks.size();
+  PassTask* ht = c->h_tasks + (size_t)s * kMaxTasks;
+  for (int i = 0; i < nt; i++) ht[i] = c->tasks[i];
+  c->h_sched[s].limit = c->sched_limit;
+  c->h_sched[s].ntasks = nt;
+  c->limit_at[s] = c->sched_limit;
+  int32_t* hj = c->h_joblist + (size_t)s * kMaxJobs;
+  int nj = 0;
+  int64_t max_npix = 0;
+  for (size_t i = 0; i < c->jobs.size() && nj < kMaxJobs; i++) {
+    JobRes* j = c->jobs[i];
+    if (j && j->alive && j->in_flight()) {
+      hj[nj++] = (int32_t)i;
+      max_npix = std::max(max_npix, j->npix());
+    }
+  }
+  if (nt > 0) HIPCHK(hipMemcpyAsync(c->d_tasks, ht, (size_t)nt * sizeof(PassTask), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_sched, c->h_sched + s, sizeof(StreamSched), hipMemcpyHostToDevice, c->stream));
+  if (nj > 0) HIPCHK(hipMemcpyAsync(c->d_joblist, hj, (size_t)nj * 4, hipMemcpyHostToDevice, c->stream));
+  const int64_t P = c->cap;
   const int shade_blocks = (int)std::min<int64_t>(grid_for(P, 256), c->shade_blocks_max);
-  const int trace_blocks = (int)std::min<int64_t>(c->trace_blocks, grid_for(P, kTraceBlock));
-  timed(c, DSPT_K_RAYGEN, [&] {
-    k_raygen<<<grid_for(P, 256), 256, 0, c->stream>>>(ps, c->d_jobs, c->d_tasks, (int)tasks.size(), P, c->counts);
-  });
-  int32_t *qin = nullptr, *qout = c->qa;
-  int32_t *cin = c->counts + 0, *cout = c->counts + 1, *csh = c->counts + 2;
-  for (int it = 0; it <= maxB; it++) {
+  const int trace_blocks = c->trace_blocks;
+  PathState ps = c->ps;
+  StreamCtl* ctl = c->d_ctl;
+  for (int k = 0; k < c->chunk_iters; k++) {
+    const int a = c->qsel;
+    int32_t* qin = a == 0 ? c->qa : c->qb;
+    int32_t* qout = a == 0 ? c->qb : c->qa;
+    timed(c, DSPT_K_RAYGEN, [&] {
+      k_plan<<<1, 64, 0, c->stream>>>(ctl, c->d_sched, a);
+      k_regen<<<shade_blocks, 256, 0, c->stream>>>(ps, c->d_jobs, c->d_tasks, c->d_sched, ctl, c->free_list, qin, c->cnt);
+    });
     timed(c, DSPT_K_TRACE_EXT, [&] {
-      k_trace_ext<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, ps, qin, cin, c->spill, c->cnt);
+      k_trace_ext<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, ps, qin, &ctl->qcount[a], c->spill, c->cnt);
     });
-    k_fill_i32<<<1, 64, 0, c->stream>>>(cout, 1, 0);
-    k_fill_i32<<<1, 64, 0, c->stream>>>(csh, 1, 0);
     timed(c, DSPT_K_SHADE, [&] {
-      k_shade<<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, ps, it, qin, cin, qout, cout, c->shq, csh, c->cnt);
+      k_shade<<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, ps, qin, &ctl->qcount[a], qout, &ctl->qcount[a ^ 1], c->shq,
+                                                    &ctl->sh_count, c->free_list, &ctl->free_count, c->cnt);
     });
-    if (it < maxB)
-      timed(c, DSPT_K_TRACE_SHADOW, [&] {
-        k_trace_shadow<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, ps, c->shq, csh, c->spill, c->cnt);
-      });
-    qin = qout;
-    qout = (qout == c->qa) ? c->qb : c->qa;
-    std::swap(cin, cout);
+    timed(c, DSPT_K_TRACE_SHADOW, [&] {
+      k_trace_shadow<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, ps, c->shq, &ctl->sh_count, c->spill, c->cnt);
+    });
+    c->qsel ^= 1;
+  }
+  if (nj > 0) {
+    timed(c, DSPT_K_ACCUM, [&] {
+      k_accum_ready<<<dim3(grid_for(max_npix, 256), (unsigned)nj), 256, 0, c->stream>>>(c->d_jobs, c->d_joblist, c->d_prog);
+      k_advance<<<1, 256, 0, c->stream>>>(c->d_jobs, c->d_joblist, nj, c->d_prog);
+    });
   }
   HIPCHK(hipGetLastError());
-  timed(c, DSPT_K_ACCUM, [&] {
-    k_accumulate<<<dim3(grid_for(max_npix, 256), (unsigned)tasks.size()), 256, 0, c->stream>>>(ps, c->d_jobs, c->d_tasks);
-  });
-  HIPCHK(hipGetLastError());
-  for (const PassTask& t : tasks) c->jobs[t.job]->first = t.first_frame + t.nframes;
+  const int njobs = std::min<int>((int)c->jobs.size(), kMaxJobs);
+  HIPCHK(hipMemcpyAsync(&c->h_rb[s].ctl, ctl, sizeof(StreamCtl), hipMemcpyDeviceToHost, c->stream));
+  if (njobs > 0) HIPCHK(hipMemcpyAsync(c->h_rb[s].prog, c->d_prog, (size_t)njobs * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipEventRecord(c->ev_chunk[s], c->stream));
+  if (c->have_prev) {
+    HIPCHK(hipEventSynchronize(c->ev_chunk[s ^ 1]));
+    consume_readback(c, s ^ 1, c->chunk_no - 1);
+  }
+  c->have_prev = true;
+  c->chunk_no++;
+  return 0;
+}
+
+// Waits for the last enqueued chunk and takes its readback.
+int settle(dspt_ctx* c) {
+  if (!c->have_prev) return 0;
+  const int s = (int)((c->chunk_no - 1) & 1);
+  HIPCHK(hipEventSynchronize(c->ev_chunk[s]));
+  consume_readback(c, s, c->chunk_no - 1);
+  return 0;
+}
+
+// Progress fingerprint of the pool as of the latest readback.
+int64_t progress_mark(const dspt_ctx* c) {
+  int64_t m = c->next_known * 1000003 + c->alive_known;
+  for (const JobRes* j : c->jobs)
+    if (j && j->alive) m = m * 31 + j->acc_known;
+  return m;
+}
+
+// Runs chunks until every listed job has folded its whole frame range. A
+// pool with work always retires a path within max_bounces + 1 bounces, so
+// kStallChunks chunks without any change in the readback is an error.
+int run_until_complete(dspt_ctx* c, const std::vector<int>& ids) {
+  auto all_done = [&] {
+    for (int id : ids)
+      if (job_ok(c, id) && !c->jobs[id]->cancel && !c->jobs[id]->complete()) return false;
+    return true;
+  };
+  int64_t mark = progress_mark(c), still = 0;
+  while (!all_done()) {
+    if (int r = run_chunk(c, ids, true)) return r;
+    int64_t m = progress_mark(c);
+    still = (m == mark) ? still + 1 : 0;
+    mark = m;
+    if (still > kStallChunks) return fail(DSPT_ERR_STATE, "scheduler made no progress");
+  }
+  return 0;
+}
+
+// Runs chunks without scheduling until job `id` has nothing in flight.
+int drain_job(dspt_ctx* c, int id) {
+  JobRes* j = c->jobs[id];
+  if (j->in_flight()) {
+    if (int r = settle(c)) return r;
+  }
+  int64_t mark = progress_mark(c), still = 0;
+  while (j->in_flight()) {
+    if (int r = run_chunk(c, {}, false)) return r;
+    int64_t m = progress_mark(c);
+    still = (m == mark) ? still + 1 : 0;
+    mark = m;
+    if (still > kStallChunks) return fail(DSPT_ERR_STATE, "drain made no progress");
+  }
   return 0;
 }
 
@@ -325,7 +549,6 @@ int validate_nodes(const float* nodes, int64_t n) {
   }
   return 0;
 }
-
 int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, const float* nodes, int64_t N,
                const int32_t* bvh_idx, int64_t M, const float* mats, int32_t K, SceneRes** out) {
   if (!vertices || !tri_idx || !nodes || !bvh_idx || !mats) return fail(DSPT_ERR_ARG, "NULL scene buffer");
@@ -435,13 +658,25 @@ int create_job(dspt_ctx* c, int scene, int ibl, const FrameParams& F, int W, int
   if (num_frames < 0) return fail(DSPT_ERR_ARG, "num_frames < 0");
   if (int r = check_job_frame(c, F, scene, ibl)) return r;
   int64_t npix = (int64_t)W * H;
-  int64_t need = job_bytes(npix, record, num_frames);
+  int ring = ring_frames_for(c, npix, num_frames);
+  int64_t need = job_bytes(npix, record, num_frames, ring);
   if (admit && c->budget > 0 && memory_in_use(c) + need > c->budget)
     return fail(DSPT_ERR_MEMORY, "admission: job needs " + std::to_string(need) + " B over the memory budget");
-  if ((want_id < 0 ? (int)c->jobs.size() : want_id) >= kMaxJobs) return fail(DSPT_ERR_MEMORY, "too many jobs");
+  int slot_id = want_id;
+  if (slot_id < 0) {
+    slot_id = (int)c->jobs.size();
+    for (size_t i = 0; i < c->jobs.size(); i++)
+      if (!c->jobs[i]) { slot_id = (int)i; break; }
+  }
+  if (slot_id >= kMaxJobs) return fail(DSPT_ERR_MEMORY, "too many jobs");
   JobRes* j = new JobRes();
   int r = 0;
-  if ((r = j->mem.get(&j->accum, (size_t)npix))) { delete j; return r; }
+  if ((r = j->mem.get(&j->accum, (size_t)npix)) || (r = j->mem.get(&j->ring, (size_t)(npix * ring))) ||
+      (r = j->mem.get(&j->done, (size_t)ring))) {
+    j->mem.release();
+    delete j;
+    return r;
+  }
   if (record && num_frames > 0) {
     if ((r = j->mem.get(&j->rec_L, (size_t)(npix * num_frames))) || (r = j->mem.get(&j->rec_sig, (size_t)(npix * num_frames))) ||
         (r = j->mem.get(&j->rec_prim, (size_t)(npix * num_frames)))) {
@@ -451,59 +686,33 @@ int create_job(dspt_ctx* c, int scene, int ibl, const FrameParams& F, int W, int
     }
   }
   k_fill_f4<<<grid_for(npix, 256), 256, 0, c->stream>>>(j->accum, npix, make_float4(0, 0, 0, 0));
+  k_fill_i32<<<grid_for(ring, 256), 256, 0, c->stream>>>(j->done, ring, 0);
+  k_fill_i32<<<1, 64, 0, c->stream>>>(c->d_prog + slot_id, 1, 1);
   j->scene = scene; j->ibl = ibl; j->F = F; j->W = W; j->H = H;
-  j->first = 1; j->last = num_frames;
+  j->ring_frames = ring;
+  j->frame_begin = 1; j->frame_end = num_frames + 1;
+  j->next_sched = 1; j->acc_known = 1;
+  j->epoch = c->chunk_no;
   j->record = record; j->rec_first = 1; j->rec_count = num_frames;
   j->alive = true;
   c->scenes[scene]->users++;
   if (ibl >= 0) c->ibls[ibl]->users++;
-  *out_id = put_slot(c->jobs, j, want_id);
+  *out_id = put_slot(c->jobs, j, slot_id);
   c->jobs_dirty = true;
   return 0;
 }
 
-void destroy_job(dspt_ctx* c, int id) {
+int destroy_job(dspt_ctx* c, int id) {
   JobRes* j = c->jobs[id];
-  (void)hipStreamSynchronize(c->stream);
+  j->cancel = true;
+  if (int r = drain_job(c, id)) return r;
+  HIPCHK(hipStreamSynchronize(c->stream));
   if (j->scene >= 0 && c->scenes[j->scene]) c->scenes[j->scene]->users--;
   if (j->ibl >= 0 && j->ibl < (int)c->ibls.size() && c->ibls[j->ibl]) c->ibls[j->ibl]->users--;
   j->mem.release();
   delete j;
   c->jobs[id] = nullptr;
   c->jobs_dirty = true;
-}
-
-bool job_ok(dspt_ctx* c, int id) { return id >= 0 && id < (int)c->jobs.size() && c->jobs[id] && c->jobs[id]->alive; }
-
-// One pass over the given jobs' pending frames: each job contributes up to
-// `per_job` consecutive frames while the pool has room (FIFO over the list;
-// a first task is always taken even if it alone exceeds max_paths).
-int schedule_pass(dspt_ctx* c, const std::vector<int>& ids, int64_t max_paths, int per_job, int* frames_done) {
-  std::vector<PassTask> tasks;
-  int64_t used = 0;
-  for (int id : ids) {
-    if (!job_ok(c, id)) continue;
-    JobRes* j = c->jobs[id];
-    int remaining = j->last - j->first + 1;
-    if (remaining <= 0) continue;
-    int64_t npix = j->npix();
-    int64_t room = (max_paths - used) / npix;
-    int nf = (int)std::min<int64_t>(std::min<int64_t>(remaining, per_job > 0 ? per_job : remaining), room);
-    if (nf <= 0) {
-      if (!tasks.empty()) continue;
-      nf = 1;
-    }
-    if ((int)tasks.size() >= kMaxTasks) break;
-    PassTask t;
-    t.job = id; t.first_frame = j->first; t.nframes = nf; t.npix = (int32_t)npix; t.offset = used;
-    tasks.push_back(t);
-    used += (int64_t)nf * npix;
-  }
-  int r = run_pass(c, tasks);
-  if (r) return r;
-  int n = 0;
-  for (auto& t : tasks) n += t.nframes;
-  if (frames_done) *frames_done = n;
   return 0;
 }
 
@@ -512,12 +721,24 @@ int legacy_job(dspt_ctx* c, int* out) {
   return fail(DSPT_ERR_STATE, "no render target (dspt_resize)");
 }
 
+std::vector<int> listed_jobs(dspt_ctx* c, const int32_t* job_ids, int32_t n) {
+  std::vector<int> ids;
+  if (job_ids) {
+    for (int i = 0; i < n; i++)
+      if (job_ok(c, job_ids[i])) ids.push_back(job_ids[i]);
+  } else {
+    for (size_t i = 0; i < c->jobs.size(); i++)
+      if (job_ok(c, (int)i) && (int)i != c->legacy_job) ids.push_back((int)i);
+  }
+  return ids;
+}
+
 }  // namespace
 
 extern "C" {
 
 const char* dspt_last_error(void) { return g_err.c_str(); }
-const char* dspt_version(void) { return "dspt 0.2.0 gfx950"; }
+const char* dspt_version(void) { return "dspt 0.3.0 gfx950"; }
 
 int dspt_create(int profile, int device, dspt_ctx** out) {
   if (!out) return fail(DSPT_ERR_ARG, "out is NULL");
@@ -533,18 +754,31 @@ int dspt_create(int profile, int device, dspt_ctx** out) {
   if (const char* mp = std::getenv("DSPT_MAX_PATHS")) c->max_paths = std::max<int64_t>(65536, std::atoll(mp));
   bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
   void* p = nullptr;
-  ok = ok && hipMalloc(&p, kNumCounters * sizeof(unsigned long long)) == hipSuccess;
-  if (ok) { c->cnt = (unsigned long long*)p; ok = hipMemset(c->cnt, 0, kNumCounters * sizeof(unsigned long long)) == hipSuccess; }
-  ok = ok && hipMalloc(&p, 8 * sizeof(int32_t)) == hipSuccess;
-  if (ok) c->counts = (int32_t*)p;
-  ok = ok && hipMalloc(&p, kMaxJobs * sizeof(DevJob)) == hipSuccess;
-  if (ok) c->d_jobs = (DevJob*)p;
-  ok = ok && hipMalloc(&p, kMaxTasks * sizeof(PassTask)) == hipSuccess;
-  if (ok) c->d_tasks = (PassTask*)p;
-  ok = ok && hipHostMalloc(&p, kMaxJobs * sizeof(DevJob), hipHostMallocDefault) == hipSuccess;
-  if (ok) c->h_jobs = (DevJob*)p;
-  ok = ok && hipHostMalloc(&p, kMaxTasks * sizeof(PassTask), hipHostMallocDefault) == hipSuccess;
-  if (ok) c->h_tasks = (PassTask*)p;
+  auto dev = [&](size_t bytes) -> void* {
+    void* q = nullptr;
+    if (ok && hipMalloc(&q, bytes) != hipSuccess) { ok = false; q = nullptr; }
+    return q;
+  };
+  auto host = [&](size_t bytes) -> void* {
+    void* q = nullptr;
+    if (ok && hipHostMalloc(&q, bytes, hipHostMallocDefault) != hipSuccess) { ok = false; q = nullptr; }
+    return q;
+  };
+  c->cnt = (unsigned long long*)dev(kNumCounters * sizeof(unsigned long long));
+  if (ok) ok = hipMemset(c->cnt, 0, kNumCounters * sizeof(unsigned long long)) == hipSuccess;
+  c->d_jobs = (DevJob*)dev(kMaxJobs * sizeof(DevJob));
+  c->d_tasks = (PassTask*)dev(kMaxTasks * sizeof(PassTask));
+  c->d_ctl = (StreamCtl*)dev(sizeof(StreamCtl));
+  c->d_sched = (StreamSched*)dev(sizeof(StreamSched));
+  c->d_prog = (int32_t*)dev(kMaxJobs * sizeof(int32_t));
+  c->d_joblist = (int32_t*)dev(kMaxJobs * sizeof(int32_t));
+  c->h_jobs = (DevJob*)host(kMaxJobs * sizeof(DevJob));
+  c->h_rb = (Readback*)host(2 * sizeof(Readback));
+  c->h_sched = (StreamSched*)host(2 * sizeof(StreamSched));
+  c->h_tasks = (PassTask*)host(2 * (size_t)kMaxTasks * sizeof(PassTask));
+  c->h_joblist = (int32_t*)host(2 * (size_t)kMaxJobs * sizeof(int32_t));
+  for (int i = 0; i < 2 && ok; i++) ok = hipEventCreateWithFlags(&c->ev_chunk[i], hipEventDisableTiming) == hipSuccess;
+  (void)p;
   if (!ok) {
     dspt_destroy(c);
     return fail(DSPT_ERR_MEMORY, "context allocation failed");
@@ -564,12 +798,12 @@ int dspt_destroy(dspt_ctx* c) {
   for (auto* b : c->ibls) if (b) { b->mem.release(); delete b; }
   c->state_mem.release();
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-  if (c->cnt) (void)hipFree(c->cnt);
-  if (c->counts) (void)hipFree(c->counts);
-  if (c->d_jobs) (void)hipFree(c->d_jobs);
-  if (c->d_tasks) (void)hipFree(c->d_tasks);
-  if (c->h_jobs) (void)hipHostFree(c->h_jobs);
-  if (c->h_tasks) (void)hipHostFree(c->h_tasks);
+  for (hipEvent_t e : c->ev_chunk) if (e) (void)hipEventDestroy(e);
+  for (void* d : {(void*)c->cnt, (void*)c->d_jobs, (void*)c->d_tasks, (void*)c->d_ctl, (void*)c->d_sched, (void*)c->d_prog,
+                  (void*)c->d_joblist})
+    if (d) (void)hipFree(d);
+  for (void* h : {(void*)c->h_jobs, (void*)c->h_rb, (void*)c->h_sched, (void*)c->h_tasks, (void*)c->h_joblist})
+    if (h) (void)hipHostFree(h);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -614,13 +848,22 @@ int dspt_ibl_destroy(dspt_ctx* c, int32_t id) {
   return 0;
 }
 
-// Single-image API: scene 0 / IBL 0 are replaced in place; the implicit job
-// is re-pointed at them (its accumulation restarts).
+// Runs chunks without scheduling until nothing is in flight.
+static int drain_all(dspt_ctx* c) {
+  for (size_t i = 0; i < c->jobs.size(); i++)
+    if (job_ok(c, (int)i) && c->jobs[i]->in_flight())
+      if (int r = drain_job(c, (int)i)) return r;
+  return 0;
+}
+
+// Single-image API: scene 0 / IBL 0 are replaced in place (after the work in
+// flight has drained); the implicit job is re-pointed at them.
 int dspt_upload_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, const float* nodes,
                       int64_t N, const int32_t* bvh_idx, int64_t M, const float* mats, int32_t K) {
   if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
   SceneRes* s = nullptr;
   if (int r = make_scene(c, vertices, V, tri_idx, T, nodes, N, bvh_idx, M, mats, K, &s)) return r;
+  if (int r = drain_all(c)) return r;
   HIPCHK(hipStreamSynchronize(c->stream));
   int users = 0;
   if (!c->scenes.empty() && c->scenes[0]) {
@@ -638,6 +881,7 @@ int dspt_upload_ibl(dspt_ctx* c, const float* rgba, int32_t w, int32_t h, double
   if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
   IblRes* b = nullptr;
   if (int r = make_ibl(c, rgba, w, h, &b)) return r;
+  if (int r = drain_all(c)) return r;
   HIPCHK(hipStreamSynchronize(c->stream));
   int users = 0;
   if (!c->ibls.empty() && c->ibls[0]) {
@@ -670,7 +914,8 @@ int dspt_resize(dspt_ctx* c, int32_t w, int32_t h) {
   HIPCHK(hipSetDevice(c->device));
   c->W = w;
   c->H = h;
-  if (c->legacy_job >= 0 && job_ok(c, c->legacy_job)) destroy_job(c, c->legacy_job);
+  if (c->legacy_job >= 0 && job_ok(c, c->legacy_job))
+    if (int r = destroy_job(c, c->legacy_job)) return r;
   c->legacy_job = -1;
   return 0;
 }
@@ -698,6 +943,7 @@ int dspt_render(dspt_ctx* c, int32_t first_frame, int32_t num_frames) {
   if (!c->have_frame) return fail(DSPT_ERR_STATE, "render before dspt_set_frame");
   if (c->W <= 0) return fail(DSPT_ERR_STATE, "render before dspt_resize");
   if (first_frame < 1 || num_frames < 0) return fail(DSPT_ERR_ARG, "frames are 1-based and num_frames >= 0");
+  if ((int64_t)first_frame + num_frames >= (1LL << 30)) return fail(DSPT_ERR_ARG, "frame index out of range");
   HIPCHK(hipSetDevice(c->device));
   int ibl = (!c->ibls.empty() && c->ibls[0]) ? 0 : -1;
   if (int r = check_job_frame(c, c->F, 0, ibl)) return r;
@@ -705,27 +951,34 @@ int dspt_render(dspt_ctx* c, int32_t first_frame, int32_t num_frames) {
   if (c->legacy_job >= 0 && job_ok(c, c->legacy_job)) {
     JobRes* j = c->jobs[c->legacy_job];
     bool rec_fit = !c->record || (j->record && j->rec_count >= num_frames);
-    if (!rec_fit || j->record != c->record || j->ibl != ibl) { destroy_job(c, c->legacy_job); c->legacy_job = -1; }
+    if (!rec_fit || j->record != c->record || j->ibl != ibl) {
+      if (int r = destroy_job(c, c->legacy_job)) return r;
+      c->legacy_job = -1;
+    }
   }
   if (c->legacy_job < 0 || !job_ok(c, c->legacy_job)) {
     int id = -1;
     if (int r = create_job(c, 0, ibl, c->F, c->W, c->H, num_frames, c->record, false, -1, &id)) return r;
     c->legacy_job = id;
   }
-  JobRes* j = c->jobs[c->legacy_job];
+  if (num_frames == 0) return 0;
+  const int id = c->legacy_job;
+  JobRes* j = c->jobs[id];
   j->F = c->F;
   j->ibl = ibl;
   j->rec_first = first_frame;
+  j->frame_begin = first_frame;
+  j->frame_end = first_frame + num_frames;
+  j->next_sched = first_frame;
+  j->acc_known = first_frame;
+  j->cancel = false;
+  j->epoch = c->chunk_no;
   c->jobs_dirty = true;
+  k_fill_i32<<<1, 64, 0, c->stream>>>(c->d_prog + id, 1, first_frame);
   if (first_frame == 1)
     k_fill_f4<<<grid_for(j->npix(), 256), 256, 0, c->stream>>>(j->accum, j->npix(), make_float4(0, 0, 0, 0));
-  j->first = first_frame;
-  j->last = first_frame + num_frames - 1;
-  std::vector<int> ids{c->legacy_job};
-  while (j->first <= j->last) {
-    int done = 0;
-    if (int r = schedule_pass(c, ids, c->max_paths, 0, &done)) return r;
-  }
+  HIPCHK(hipGetLastError());
+  if (int r = run_until_complete(c, {id})) return r;
   c->last_frames = num_frames;
   return 0;
 }
@@ -733,7 +986,7 @@ int dspt_render(dspt_ctx* c, int32_t first_frame, int32_t num_frames) {
 int dspt_synchronize(dspt_ctx* c) {
   if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
   HIPCHK(hipStreamSynchronize(c->stream));
-  return 0;
+  return settle(c);
 }
 
 int dspt_read_accum(dspt_ctx* c, float* rgba) {
@@ -773,15 +1026,14 @@ int dspt_job_create(dspt_ctx* c, int32_t scene_id, int32_t ibl_id, const float* 
 int dspt_job_destroy(dspt_ctx* c, int32_t id) {
   if (!c || !job_ok(c, id)) return fail(DSPT_ERR_ARG, "unknown job id");
   if (id == c->legacy_job) c->legacy_job = -1;
-  destroy_job(c, id);
-  return 0;
+  return destroy_job(c, id);
 }
 
 int dspt_job_status(dspt_ctx* c, int32_t id, int32_t* frames_done, int32_t* frames_total) {
   if (!c || !job_ok(c, id)) return fail(DSPT_ERR_ARG, "unknown job id");
   JobRes* j = c->jobs[id];
-  if (frames_done) *frames_done = j->first - 1;
-  if (frames_total) *frames_total = j->last;
+  if (frames_done) *frames_done = j->acc_known - j->frame_begin;
+  if (frames_total) *frames_total = j->frame_end - j->frame_begin;
   return 0;
 }
 
@@ -805,26 +1057,24 @@ int dspt_job_read_samples(dspt_ctx* c, int32_t id, float* rgba, uint32_t* sigs, 
   return 0;
 }
 
-int dspt_jobs_step(dspt_ctx* c, const int32_t* job_ids, int32_t n, int64_t max_paths, int32_t max_frames_per_job,
-                   int32_t* frames_done) {
+int dspt_jobs_step(dspt_ctx* c, const int32_t* job_ids, int32_t n, int32_t* active) {
   if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
-  std::vector<int> ids;
-  if (job_ids) ids.assign(job_ids, job_ids + std::max(0, n));
-  else
-    for (size_t i = 0; i < c->jobs.size(); i++)
-      if (job_ok(c, (int)i) && (int)i != c->legacy_job) ids.push_back((int)i);
   HIPCHK(hipSetDevice(c->device));
-  return schedule_pass(c, ids, max_paths > 0 ? max_paths : c->max_paths, max_frames_per_job, frames_done);
-}
-
-int dspt_jobs_run(dspt_ctx* c, const int32_t* job_ids, int32_t n, int64_t max_paths) {
-  if (!c || !job_ids) return fail(DSPT_ERR_ARG, "NULL argument");
-  for (;;) {
-    int done = 0;
-    if (int r = dspt_jobs_step(c, job_ids, n, max_paths, 0, &done)) return r;
-    if (done == 0) break;
+  std::vector<int> ids = listed_jobs(c, job_ids, n);
+  if (int r = run_chunk(c, ids, true)) return r;
+  if (active) {
+    int k = 0;
+    for (int id : ids)
+      if (!c->jobs[id]->complete()) k++;
+    *active = k;
   }
   return 0;
+}
+
+int dspt_jobs_run(dspt_ctx* c, const int32_t* job_ids, int32_t n) {
+  if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
+  HIPCHK(hipSetDevice(c->device));
+  return run_until_complete(c, listed_jobs(c, job_ids, n));
 }
 
 int dspt_set_memory_budget(dspt_ctx* c, int64_t bytes) {
@@ -835,11 +1085,16 @@ int dspt_set_memory_budget(dspt_ctx* c, int64_t bytes) {
 
 int dspt_set_launch_config(dspt_ctx* c, int64_t max_paths, int32_t trace_blocks, int32_t shade_blocks) {
   if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
-  HIPCHK(hipStreamSynchronize(c->stream));
-  if (max_paths > 0) c->max_paths = std::max<int64_t>(65536, max_paths);
-  if (trace_blocks > 0 && trace_blocks != c->trace_blocks_max) {
-    c->trace_blocks_max = trace_blocks;
-    c->cap = 0;  // reallocate the spill area with the pool
+  if (max_paths > (1LL << 30)) return fail(DSPT_ERR_ARG, "max_paths above 2^30");
+  const bool repool = (max_paths > 0 && std::max<int64_t>(65536, max_paths) != c->max_paths) ||
+                      (trace_blocks > 0 && trace_blocks != c->trace_blocks_max);
+  if (repool) {
+    if (int r = drain_all(c)) return r;
+    if (int r = settle(c)) return r;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (max_paths > 0) c->max_paths = std::max<int64_t>(65536, max_paths);
+    if (trace_blocks > 0) c->trace_blocks_max = trace_blocks;
+    c->cap = 0;  // reallocated (with the spill area) on the next chunk
   }
   if (shade_blocks > 0) c->shade_blocks_max = shade_blocks;
   return 0;
@@ -853,6 +1108,27 @@ int dspt_get_launch_config(dspt_ctx* c, int64_t* max_paths, int32_t* trace_block
   return 0;
 }
 
+int dspt_set_schedule(dspt_ctx* c, int32_t mode, int32_t pass_frames, int32_t chunk_iters, float ring_factor) {
+  if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
+  if (mode != DSPT_SCHED_STREAM && mode != DSPT_SCHED_PASS) return fail(DSPT_ERR_ARG, "unknown schedule mode");
+  if (chunk_iters < 0 || chunk_iters > 64) return fail(DSPT_ERR_ARG, "chunk_iters must be in 1..64 (0 keeps)");
+  if (!(ring_factor >= 0.0f) || ring_factor > 1024.0f) return fail(DSPT_ERR_ARG, "ring_factor must be in [1, 1024] (0 keeps)");
+  c->mode = mode;
+  c->pass_frames = std::max(0, pass_frames);
+  if (chunk_iters > 0) c->chunk_iters = chunk_iters;
+  if (ring_factor > 0.0f) c->ring_factor = std::max(1.0f, ring_factor);
+  return 0;
+}
+
+int dspt_get_schedule(dspt_ctx* c, int32_t* mode, int32_t* pass_frames, int32_t* chunk_iters, float* ring_factor) {
+  if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
+  if (mode) *mode = c->mode;
+  if (pass_frames) *pass_frames = c->pass_frames;
+  if (chunk_iters) *chunk_iters = c->chunk_iters;
+  if (ring_factor) *ring_factor = c->ring_factor;
+  return 0;
+}
+
 // ------------------------------------------------------------------ counters
 int dspt_get_counters(dspt_ctx* c, int64_t* out) {
   if (!c || !out) return fail(DSPT_ERR_ARG, "NULL argument");
@@ -862,8 +1138,7 @@ int dspt_get_counters(dspt_ctx* c, int64_t* out) {
   for (int i = 0; i < DSPT_CNT_COUNT; i++) out[i] = (int64_t)h[i];
   out[DSPT_CNT_NODE_VISITS] = (int64_t)(h[4] + h[10]);
   out[DSPT_CNT_TRI_TESTS] = (int64_t)(h[5] + h[11]);
-  // every ray the closest-hit kernel traced that is not an extension ray is a camera ray
-  out[DSPT_CNT_RAYS_PRIMARY] = (int64_t)h[9] - (int64_t)h[2];
+  out[DSPT_CNT_RAYS_PRIMARY] = (int64_t)h[12];  // camera rays issued by the refill
   return 0;
 }
 

@@ -1,17 +1,21 @@
 // Wavefront path-tracing kernels for gfx950 (wave64, VALU only).
 //
-// A pass renders a list of tasks (runs of consecutive frames of one job; a
-// job = one scene + IBL + camera/frame block + resolution). Every path
-// carries its job index, so paths of several scenes, resolutions and frame
-// counts share one set of launches. Per bounce:
+// A fixed pool of path slots serves a stream of work items (job, frame,
+// pixel); a job = one scene + IBL + camera/frame block + resolution. Every
+// path carries its job index, so paths of several scenes, resolutions and
+// frame counts share one set of launches. Per bounce:
+//   plan + regen refill the slots freed on the previous bounce with camera
+//                rays of the next items (renderer.frag:129-147)
 //   trace_ext    closest hit of the extension rays in the active queue
 //   shade        render state, material, RR, emission, BSDF sample, NEE sample
-//                + BSDF eval/pdf; appends extension rays and shadow rays
+//                + BSDF eval/pdf; appends extension rays and shadow rays;
+//                a finished path writes its sample into its job's frame ring
+//                and frees its slot
 //   trace_shadow any-hit test of the NEE shadow rays, adds the deferred
 //                NEE radiance in the reference's summation order
-// then accumulate folds each task's finished samples into its job's running
-// mean frame by frame (renderer.frag:153-156), so results do not depend on
-// how frames and jobs were packed into passes.
+// accum_ready folds every complete frame into its job's running mean in
+// frame order (renderer.frag:153-156), so results do not depend on how
+// items of different frames and jobs were interleaved in the pool.
 #include "dspt_kernels.hpp"
 #include "dev_shade.hpp"
 
@@ -81,44 +85,87 @@ __global__ void k_fill_f4(float4* __restrict__ p, int64_t n, float4 v) {
   if (i < n) p[i] = v;
 }
 
-// Camera rays, renderer.frag:129-142 + rng_init (rng.glsl:23-29).
-__global__ void k_raygen(PathState ps, const DevJob* __restrict__ jobs, const PassTask* __restrict__ tasks, int ntasks,
-                         int64_t P, int32_t* __restrict__ counts) {
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p == 0) { counts[0] = (int32_t)P; }
-  if (p >= P) return;
-  int lo = 0, hi = ntasks - 1;  // last task with offset <= p
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (tasks[mid].offset <= p) lo = mid; else hi = mid - 1;
+__global__ void k_stream_init(StreamCtl* __restrict__ ctl, int32_t* __restrict__ free_list, int32_t P) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P) free_list[i] = P - 1 - i;
+  if (i == 0) {
+    StreamCtl z{};
+    z.free_count = P;
+    *ctl = z;
   }
-  const PassTask T = tasks[lo];
-  const DevJob* J = jobs + T.job;
-  int64_t local = p - T.offset;
-  int slot = (int)(local / T.npix);
-  int pix = (int)(local - (int64_t)slot * T.npix);
-  int W = J->W, H = J->H;
-  int px = pix % W, py = pix / W;
-  int frame = T.first_frame + slot;
-  const FrameParams& F = J->F;
-  Rng rng = rng_seed(frame * (int)F.max_bounces, px, py);
-  float r0 = rng.uniform(), r1 = rng.uniform();
-  float vux = (2.0f * (float(px) + 0.5f)) / float(W) - 1.0f;
-  float vuy = (2.0f * (float(py) + 0.5f)) / float(H) - 1.0f;
-  float offx = r0 * F.inv_res[0], offy = r1 * F.inv_res[1];
-  float aspect = F.inv_res[1] / F.inv_res[0];
-  float ux = (vux * aspect + offx) * F.cam[3];
-  float uy = (vuy * 1.0f + offy) * F.cam[3];
-  f3 d = nrm3(f3{ux, uy, -F.focal_length});
-  const float* m = F.view;
-  f3 wd = f3{m[0] * d.x + m[4] * d.y + m[8] * d.z, m[1] * d.x + m[5] * d.y + m[9] * d.z, m[2] * d.x + m[6] * d.y + m[10] * d.z};
-  ps.org[p] = make_float4(F.cam[0], F.cam[1], F.cam[2], 0.0f);
-  ps.dir[p] = make_float4(wd.x, wd.y, wd.z, 0.0f);
-  ps.pw[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-  ps.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  ps.rng[p] = make_uint2(rng.x, rng.y);
-  ps.job[p] = T.job;
-  if (ps.sig) ps.sig[p] = 2166136261u;
+}
+
+// One thread: sizes this bounce's refill (free slots vs scheduled items) and
+// resets the queues the bounce appends to.
+__global__ void k_plan(StreamCtl* __restrict__ ctl, const StreamSched* __restrict__ sch, int a) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int F = ctl->free_count, Q = ctl->qcount[a];
+  int64_t avail = sch->limit - ctl->next_item;
+  if (avail < 0) avail = 0;
+  const int n = (int)(avail < (int64_t)F ? avail : (int64_t)F);
+  ctl->regen_n = n;
+  ctl->item0 = ctl->next_item;
+  ctl->free_base = F - n;
+  ctl->q_base = Q;
+  ctl->next_item += n;
+  ctl->free_count = F - n;
+  ctl->qcount[a] = Q + n;
+  ctl->qcount[a ^ 1] = 0;
+  ctl->sh_count = 0;
+}
+
+// Camera rays for the items of this refill, renderer.frag:129-142 +
+// rng_init (rng.glsl:23-29). Item -> task by binary search over the task
+// table (offsets ascending).
+__global__ void k_regen(PathState ps, const DevJob* __restrict__ jobs, const PassTask* __restrict__ tasks,
+                        const StreamSched* __restrict__ sch, const StreamCtl* __restrict__ ctl,
+                        const int32_t* __restrict__ free_list, int32_t* __restrict__ queue, unsigned long long* __restrict__ cnt) {
+  const int n = ctl->regen_n;
+  const int64_t item0 = ctl->item0;
+  const int fb = ctl->free_base, qb = ctl->q_base;
+  const int ntasks = sch->ntasks;
+  unsigned issued = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int64_t item = item0 + i;
+    int lo = 0, hi = ntasks - 1;  // last task with offset <= item
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (tasks[mid].offset <= item) lo = mid; else hi = mid - 1;
+    }
+    const PassTask T = tasks[lo];
+    const DevJob* J = jobs + T.job;
+    int64_t local = item - T.offset;
+    int fslot = (int)(local / T.npix);
+    int pix = (int)(local - (int64_t)fslot * T.npix);
+    int W = J->W, H = J->H;
+    int px = pix % W, py = pix / W;
+    int frame = T.first_frame + fslot;
+    const FrameParams& F = J->F;
+    Rng rng = rng_seed(frame * (int)F.max_bounces, px, py);
+    float r0 = rng.uniform(), r1 = rng.uniform();
+    float vux = (2.0f * (float(px) + 0.5f)) / float(W) - 1.0f;
+    float vuy = (2.0f * (float(py) + 0.5f)) / float(H) - 1.0f;
+    float offx = r0 * F.inv_res[0], offy = r1 * F.inv_res[1];
+    float aspect = F.inv_res[1] / F.inv_res[0];
+    float ux = (vux * aspect + offx) * F.cam[3];
+    float uy = (vuy * 1.0f + offy) * F.cam[3];
+    f3 d = nrm3(f3{ux, uy, -F.focal_length});
+    const float* m = F.view;
+    f3 wd = f3{m[0] * d.x + m[4] * d.y + m[8] * d.z, m[1] * d.x + m[5] * d.y + m[9] * d.z, m[2] * d.x + m[6] * d.y + m[10] * d.z};
+    const int p = free_list[fb + i];
+    ps.org[p] = make_float4(F.cam[0], F.cam[1], F.cam[2], 0.0f);
+    ps.dir[p] = make_float4(wd.x, wd.y, wd.z, 0.0f);
+    ps.pw[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    ps.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    ps.rng[p] = make_uint2(rng.x, rng.y);
+    ps.job[p] = T.job;
+    ps.bounce[p] = 0;
+    ps.fp[p] = make_int2(frame, pix);
+    if (J->rec_L) ps.sig[p] = 2166136261u;
+    queue[qb + i] = p;
+    issued++;
+  }
+  wave_add_u64(&cnt[12], issued);
 }
 
 // XCD-aware block order: consecutive queue ranges go to the same XCD (the
@@ -148,7 +195,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_ext(const DevJob* __restr
   int64_t nb = gridDim.x;
   int64_t lb = xcd_block(blockIdx.x, nb);
   for (int64_t q = lb * blockDim.x + threadIdx.x; q < count; q += nb * blockDim.x) {
-    int p = queue ? queue[q] : (int)q;
+    int p = queue[q];
     float4 o = ps.org[p], d = ps.dir[p];
     RayHit h = traverse<false>(geo_of(jobs[ps.job[p]].S), xyz(o), xyz(d), kTFAR_MAX, st, nv, nt);
     ps.hit[p] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
@@ -210,178 +257,210 @@ DI float4 finish(f3 L, float thr, bool do_clamp) {
   return make_float4(L.x, L.y, L.z, 1.0f);
 }
 
-__global__ __launch_bounds__(256) void k_shade(const DevJob* __restrict__ jobs, PathState ps, int iter,
+// atomicAdd(addr, 1) for every active lane, one atomic per distinct address
+// in the wave (lanes of one frame share a counter).
+DD void atomic_inc_grouped(int32_t* addr) {
+  const uint64_t a = (uint64_t)(uintptr_t)addr;
+  for (;;) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    const bool mine = a == (((uint64_t)hi << 32) | lo);
+    const unsigned long long m = __ballot(mine);
+    if (mine) {
+      if ((int)__lane_id() == __ffsll((long long)m) - 1) atomicAdd(addr, (int32_t)__popcll(m));
+      break;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_shade(const DevJob* __restrict__ jobs, PathState ps,
                                                const int32_t* __restrict__ queue_in, const int32_t* __restrict__ count_in,
                                                int32_t* __restrict__ queue_out, int32_t* __restrict__ count_out,
                                                int32_t* __restrict__ shq, int32_t* __restrict__ count_sh,
+                                               int32_t* __restrict__ free_list, int32_t* __restrict__ free_count,
                                                unsigned long long* __restrict__ cnt) {
   const int count = *count_in;
   unsigned n_paths = 0, n_hits = 0, n_miss = 0, n_nee = 0, n_ext = 0;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < count; q += (int64_t)gridDim.x * blockDim.x) {
-    int p = queue_in ? queue_in[q] : (int)q;
+    const int p = queue_in[q];
     const DevJob* J = jobs + ps.job[p];
-    const SceneView& S = J->S;
-    const FrameParams& F = J->F;
-    const int maxB = (int)F.max_bounces;
-    float4 hit = ps.hit[p];
-    int tri = __float_as_int(hit.x);
-    float4 o4 = ps.org[p], d4 = ps.dir[p];
-    f3 org = xyz(o4), dir = xyz(d4);
-    float4 pw4 = ps.pw[p];
-    f3 pw = xyz(pw4);
-    float last_pdf = pw4.w;
-    float4 L4 = ps.L[p];
-    f3 L = xyz(L4);
-    uint32_t sig = ps.sig ? ps.sig[p] : 0u;
-    const bool rec = ps.sig != nullptr;
+    const bool rec = J->rec_L != nullptr;
+    uint32_t sig = rec ? ps.sig[p] : 0u;
+    float4 out;
+    bool term = true;
+    do {
+      const SceneView& S = J->S;
+      const FrameParams& F = J->F;
+      const int maxB = (int)F.max_bounces;
+      const int bounce = ps.bounce[p];
+      float4 hit = ps.hit[p];
+      int tri = __float_as_int(hit.x);
+      float4 o4 = ps.org[p], d4 = ps.dir[p];
+      f3 org = xyz(o4), dir = xyz(d4);
+      float4 pw4 = ps.pw[p];
+      f3 pw = xyz(pw4);
+      float last_pdf = pw4.w;
+      float4 L4 = ps.L[p];
+      f3 L = xyz(L4);
 
-    if (tri < 0) {
-      n_paths++;
-      if (rec) sig_mix(sig, 0xFFFFFFFFu);
-      if (iter == 0) {
-        // primary miss: background, returned unclamped (misptdl.glsl:41-46)
-        float4 out;
-        if (F.bg_from_ibl > 0.0f) { f3 e = ibl_eval(S, F, dir) * pw; out = make_float4(e.x, e.y, e.z, 1.0f); }
-        else out = make_float4(powf(F.bg[0], 2.2f), powf(F.bg[1], 2.2f), powf(F.bg[2], 2.2f), F.bg[3]);
-        ps.L[p] = out;
-        if (rec) ps.prim[p] = -1;
+      if (tri < 0) {
+        if (rec) sig_mix(sig, 0xFFFFFFFFu);
+        if (bounce == 0) {
+          // primary miss: background, returned unclamped (misptdl.glsl:41-46)
+          if (F.bg_from_ibl > 0.0f) { f3 e = ibl_eval(S, F, dir) * pw; out = make_float4(e.x, e.y, e.z, 1.0f); }
+          else out = make_float4(powf(F.bg[0], 2.2f), powf(F.bg[1], 2.2f), powf(F.bg[2], 2.2f), F.bg[3]);
+          if (rec) ps.prim[p] = -1;
+        } else {
+          n_miss++;
+          float ip = ibl_pdf(S, F, dir);
+          float mis = last_pdf / (last_pdf + ip);
+          L = L + ibl_eval(S, F, dir) * pw * mis * F.use_ibl;
+          out = finish(L, F.clamp_threshold, true);
+        }
+        break;
+      }
+      if (rec) { sig_mix(sig, (uint32_t)tri); if (bounce == 0) ps.prim[p] = tri; }
+      if (bounce >= maxB) {
+        out = finish(L, F.clamp_threshold, true);
+        break;
+      }
+      uint2 rs = ps.rng[p];
+      Rng rng{rs.x, rs.y};
+      // Russian roulette (renderer.frag:119-123)
+      pw = pw * (1.0f / (1.0f - 0.1f));
+      if (bounce > 2 && rng.uniform() <= 0.1f) {
+        if (rec) sig_mix(sig, 1u);
+        out = finish(L, F.clamp_threshold, true);
+        break;
+      }
+      n_hits++;
+      Hitpoint hp = fetch_hitpoint(S, org, dir, tri, hit.y, hit.z, hit.w);
+      f3 wi = -dir;
+      Shading sh = configure(S, hp, wi);
+      // Beer-Lambert absorption inside thick media (misptdl.glsl:60-63)
+      if (sh.backside && !sh.thin) {
+        f3 sigma = -vlog(sh.attColor) / sh.attDist;
+        pw = pw * vexp(-sigma * hit.y);
+      }
+      L = L + sh.emissive * 1.0f * pw;
+
+      // BSDF bounce (renderer.frag:94-116)
+      f3 wo, bw = sp3(1.0f);
+      float pdf = 1.0f;
+      bsdf::Closure cl;
+      bsdf::Frame3 fr;
+      bool have_closure = false;
+      if (rng.uniform() > sh.cutout) {
+        wo = -wi;
+        if (rec) sig_mix(sig, 2u);
       } else {
-        n_miss++;
-        float ip = ibl_pdf(S, F, dir);
-        float mis = last_pdf / (last_pdf + ip);
-        L = L + ibl_eval(S, F, dir) * pw * mis * F.use_ibl;
-        ps.L[p] = finish(L, F.clamp_threshold, true);
-      }
-      if (rec) ps.sig[p] = sig;
-      continue;
-    }
-    if (rec) { sig_mix(sig, (uint32_t)tri); if (iter == 0) ps.prim[p] = tri; }
-    if (iter >= maxB) {
-      n_paths++;
-      ps.L[p] = finish(L, F.clamp_threshold, true);
-      if (rec) ps.sig[p] = sig;
-      continue;
-    }
-    uint2 rs = ps.rng[p];
-    Rng rng{rs.x, rs.y};
-    // Russian roulette (renderer.frag:119-123)
-    pw = pw * (1.0f / (1.0f - 0.1f));
-    if (iter > 2 && rng.uniform() <= 0.1f) {
-      n_paths++;
-      if (rec) { sig_mix(sig, 1u); ps.sig[p] = sig; }
-      ps.L[p] = finish(L, F.clamp_threshold, true);
-      continue;
-    }
-    n_hits++;
-    Hitpoint hp = fetch_hitpoint(S, org, dir, tri, hit.y, hit.z, hit.w);
-    f3 wi = -dir;
-    Shading sh = configure(S, hp, wi);
-    // Beer-Lambert absorption inside thick media (misptdl.glsl:60-63)
-    if (sh.backside && !sh.thin) {
-      f3 sigma = -vlog(sh.attColor) / sh.attDist;
-      pw = pw * vexp(-sigma * hit.y);
-    }
-    L = L + sh.emissive * 1.0f * pw;
-
-    // BSDF bounce (renderer.frag:94-116)
-    f3 wo, bw = sp3(1.0f);
-    float pdf = 1.0f;
-    bsdf::Closure cl;
-    bsdf::Frame3 fr;
-    bool have_closure = false;
-    if (rng.uniform() > sh.cutout) {
-      wo = -wi;
-      if (rec) sig_mix(sig, 2u);
-    } else {
-      float lx = rng.uniform(), ly = rng.uniform(), bz = rng.uniform();
-      float comp = rng.uniform();
-      cl = bsdf::build_closure(sh.in, sh.tanA, S.full_profile != 0);
-      fr = bsdf::aniso_frame(sh.n, cl.tanA);
-      have_closure = true;
-      bsdf::Sample bs = bsdf::sample(cl, fr, sh.n, wi, comp, lx, ly, bz);
-      pdf = bs.pdf;
-      bw = bs.w;
-      wo = bs.dir;
-      if (rec) sig_mix(sig, 3u + (uint32_t)(int)bs.specular);
-      if (pdf < kEPS_PDF) {
-        n_paths++;
-        if (rec) { sig_mix(sig, 7u); ps.sig[p] = sig; }
-        ps.L[p] = make_float4(L.x, L.y, L.z, 1.0f);
-        continue;
-      }
-    }
-    last_pdf = pdf;
-    f3 gn = dot3(hp.ng_raw, wo) < 0.0f ? -hp.ng_raw : hp.ng_raw;
-    f3 pos = hp.pos + gn * F.ray_eps;
-
-    // next-event estimation (misptdl.glsl:2-32); radiance added after the
-    // shadow test
-    float n0 = rng.uniform(), n1 = rng.uniform();
-    int flags = 0;
-    f3 c_ibl = sp3(0.0f), c_pt = sp3(0.0f);
-    f3 ldir = sp3(0.0f);
-    float4 pdir = make_float4(0, 0, 0, 0);
-    f3 nn = sh.backside ? -sh.n : sh.n;
-    if (F.use_ibl > 0.0f) {
-      float ipdf;
-      f3 d = ibl_sample(S, F, n0, n1, ipdf);
-      float cosNL = sat(dot3(d, nn));
-      if (cosNL > kEPS_COS && ipdf > kEPS_PDF) {
-        if (!have_closure) {
-          cl = bsdf::build_closure(sh.in, sh.tanA, S.full_profile != 0);
-          fr = bsdf::aniso_frame(sh.n, cl.tanA);
-          have_closure = true;
+        float lx = rng.uniform(), ly = rng.uniform(), bz = rng.uniform();
+        float comp = rng.uniform();
+        cl = bsdf::build_closure(sh.in, sh.tanA, S.full_profile != 0);
+        fr = bsdf::aniso_frame(sh.n, cl.tanA);
+        have_closure = true;
+        bsdf::Sample bs = bsdf::sample(cl, fr, sh.n, wi, comp, lx, ly, bz);
+        pdf = bs.pdf;
+        bw = bs.w;
+        wo = bs.dir;
+        if (rec) sig_mix(sig, 3u + (uint32_t)(int)bs.specular);
+        if (pdf < kEPS_PDF) {
+          // absorbed: returned unclamped
+          if (rec) sig_mix(sig, 7u);
+          out = make_float4(L.x, L.y, L.z, 1.0f);
+          break;
         }
-        f3 e = bsdf::eval(cl, fr, sh.n, wi, d) / gmax(fabsf(dot3(sh.n, d)), kEPS);
-        float bp = bsdf::pdf(cl, fr, sh.n, wi, d);
-        f3 Li = ibl_eval(S, F, d) * e * cosNL / ipdf;
-        if (bp > kEPS_PDF) Li = Li * (ipdf / (ipdf + bp));
-        c_ibl = Li;
-        ldir = nrm3(d);
-        flags |= 1;
-        n_nee++;
       }
-    }
-    if (F.light_pos[3] > 0.5f) {
-      f3 lp = f3{F.light_pos[0], F.light_pos[1], F.light_pos[2]};
-      f3 ld = lp - pos;
-      float dist2 = dot3(ld, ld);
-      ld = nrm3(ld);
-      float cosNL = sat(dot3(ld, nn));
-      f3 dd = lp - pos;
-      pdir = make_float4(0, 0, 0, len3(dd));
-      f3 dn = nrm3(dd);
-      pdir.x = dn.x; pdir.y = dn.y; pdir.z = dn.z;
-      flags |= 2;
-      if (cosNL > kEPS_COS) {
-        if (!have_closure) {
-          cl = bsdf::build_closure(sh.in, sh.tanA, S.full_profile != 0);
-          fr = bsdf::aniso_frame(sh.n, cl.tanA);
-          have_closure = true;
+      last_pdf = pdf;
+      f3 gn = dot3(hp.ng_raw, wo) < 0.0f ? -hp.ng_raw : hp.ng_raw;
+      f3 pos = hp.pos + gn * F.ray_eps;
+
+      // next-event estimation (misptdl.glsl:2-32); radiance added after the
+      // shadow test
+      float n0 = rng.uniform(), n1 = rng.uniform();
+      int flags = 0;
+      f3 c_ibl = sp3(0.0f), c_pt = sp3(0.0f);
+      f3 ldir = sp3(0.0f);
+      float4 pdir = make_float4(0, 0, 0, 0);
+      f3 nn = sh.backside ? -sh.n : sh.n;
+      if (F.use_ibl > 0.0f) {
+        float ipdf;
+        f3 d = ibl_sample(S, F, n0, n1, ipdf);
+        float cosNL = sat(dot3(d, nn));
+        if (cosNL > kEPS_COS && ipdf > kEPS_PDF) {
+          if (!have_closure) {
+            cl = bsdf::build_closure(sh.in, sh.tanA, S.full_profile != 0);
+            fr = bsdf::aniso_frame(sh.n, cl.tanA);
+            have_closure = true;
+          }
+          f3 e = bsdf::eval(cl, fr, sh.n, wi, d) / gmax(fabsf(dot3(sh.n, d)), kEPS);
+          float bp = bsdf::pdf(cl, fr, sh.n, wi, d);
+          f3 Li = ibl_eval(S, F, d) * e * cosNL / ipdf;
+          if (bp > kEPS_PDF) Li = Li * (ipdf / (ipdf + bp));
+          c_ibl = Li;
+          ldir = nrm3(d);
+          flags |= 1;
+          n_nee++;
         }
-        f3 e = bsdf::eval(cl, fr, sh.n, wi, ld) / gmax(fabsf(dot3(sh.n, ld)), kEPS);
-        c_pt = e * (f3{F.light_emission[0], F.light_emission[1], F.light_emission[2]} / dist2) * cosNL;
       }
+      if (F.light_pos[3] > 0.5f) {
+        f3 lp = f3{F.light_pos[0], F.light_pos[1], F.light_pos[2]};
+        f3 ld = lp - pos;
+        float dist2 = dot3(ld, ld);
+        ld = nrm3(ld);
+        float cosNL = sat(dot3(ld, nn));
+        f3 dd = lp - pos;
+        pdir = make_float4(0, 0, 0, len3(dd));
+        f3 dn = nrm3(dd);
+        pdir.x = dn.x; pdir.y = dn.y; pdir.z = dn.z;
+        flags |= 2;
+        if (cosNL > kEPS_COS) {
+          if (!have_closure) {
+            cl = bsdf::build_closure(sh.in, sh.tanA, S.full_profile != 0);
+            fr = bsdf::aniso_frame(sh.n, cl.tanA);
+            have_closure = true;
+          }
+          f3 e = bsdf::eval(cl, fr, sh.n, wi, ld) / gmax(fabsf(dot3(sh.n, ld)), kEPS);
+          c_pt = e * (f3{F.light_emission[0], F.light_emission[1], F.light_emission[2]} / dist2) * cosNL;
+        }
+      }
+      if (flags) {
+        int slot = atomicAdd(count_sh, 1);
+        shq[slot] = p;
+        ps.sh_org[p] = make_float4(pos.x, pos.y, pos.z, __int_as_float(flags));
+        ps.sh_dir[p] = make_float4(ldir.x, ldir.y, ldir.z, kTFAR_MAX);
+        ps.sh_c[p] = make_float4(c_ibl.x, c_ibl.y, c_ibl.z, 0.0f);
+        ps.sh_pw[p] = make_float4(pw.x, pw.y, pw.z, 0.0f);
+        if (flags & 2) { ps.sh_pdir[p] = pdir; ps.sh_pc[p] = make_float4(c_pt.x, c_pt.y, c_pt.z, 0.0f); }
+      }
+      pw = pw * bw;
+      int slot = atomicAdd(count_out, 1);
+      queue_out[slot] = p;
+      n_ext++;
+      ps.org[p] = make_float4(pos.x, pos.y, pos.z, 0.0f);
+      ps.dir[p] = make_float4(wo.x, wo.y, wo.z, 0.0f);
+      ps.pw[p] = make_float4(pw.x, pw.y, pw.z, last_pdf);
+      ps.L[p] = make_float4(L.x, L.y, L.z, L4.w);
+      ps.rng[p] = make_uint2(rng.x, rng.y);
+      ps.bounce[p] = bounce + 1;
+      if (rec) ps.sig[p] = sig;
+      term = false;
+    } while (false);
+    if (term) {
+      // finished sample -> the job's frame ring (+ records); free the slot
+      n_paths++;
+      const int2 fp = ps.fp[p];
+      const int rs = fp.x % J->ring_frames;
+      J->ring[(int64_t)rs * J->npix + fp.y] = out;
+      if (rec) {
+        int64_t r = (int64_t)(fp.x - J->rec_first) * J->npix + fp.y;
+        J->rec_L[r] = out;
+        J->rec_sig[r] = sig;
+        J->rec_prim[r] = ps.prim[p];
+      }
+      atomic_inc_grouped(J->done + rs);
+      free_list[atomicAdd(free_count, 1)] = p;
     }
-    if (flags) {
-      int slot = atomicAdd(count_sh, 1);
-      shq[slot] = p;
-      ps.sh_org[p] = make_float4(pos.x, pos.y, pos.z, __int_as_float(flags));
-      ps.sh_dir[p] = make_float4(ldir.x, ldir.y, ldir.z, kTFAR_MAX);
-      ps.sh_c[p] = make_float4(c_ibl.x, c_ibl.y, c_ibl.z, 0.0f);
-      ps.sh_pw[p] = make_float4(pw.x, pw.y, pw.z, 0.0f);
-      if (flags & 2) { ps.sh_pdir[p] = pdir; ps.sh_pc[p] = make_float4(c_pt.x, c_pt.y, c_pt.z, 0.0f); }
-    }
-    pw = pw * bw;
-    int slot = atomicAdd(count_out, 1);
-    queue_out[slot] = p;
-    n_ext++;
-    ps.org[p] = make_float4(pos.x, pos.y, pos.z, 0.0f);
-    ps.dir[p] = make_float4(wo.x, wo.y, wo.z, 0.0f);
-    ps.pw[p] = make_float4(pw.x, pw.y, pw.z, last_pdf);
-    ps.L[p] = make_float4(L.x, L.y, L.z, L4.w);
-    ps.rng[p] = make_uint2(rng.x, rng.y);
-    if (rec) ps.sig[p] = sig;
   }
   wave_add_u64(&cnt[0], n_paths);
   wave_add_u64(&cnt[6], n_hits);
@@ -390,31 +469,46 @@ __global__ __launch_bounds__(256) void k_shade(const DevJob* __restrict__ jobs, 
   wave_add_u64(&cnt[2], n_ext);
 }
 
-// Running mean (renderer.frag:153-156), frame by frame in frame order, per
-// task (grid.y = task); also copies recorded samples for parity checks.
-__global__ void k_accumulate(PathState ps, const DevJob* __restrict__ jobs, const PassTask* __restrict__ tasks) {
-  const PassTask T = tasks[blockIdx.y];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= T.npix) return;
-  const DevJob* J = jobs + T.job;
-  float4* acc = J->accum;
-  float4 a = acc[i];
-  for (int s = 0; s < T.nframes; s++) {
-    int64_t p = T.offset + (int64_t)s * T.npix + i;
-    float4 c = ps.L[p];
-    int frame = T.first_frame + s;
-    float N = float(frame);
+// Running mean (renderer.frag:153-156) over every frame that is complete,
+// in frame order, starting at the job's next frame to fold (grid.y = job).
+__global__ void k_accum_ready(const DevJob* __restrict__ jobs, const int32_t* __restrict__ job_list,
+                              const int32_t* __restrict__ prog) {
+  const int jid = job_list[blockIdx.y];
+  const DevJob* J = jobs + jid;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int npix = J->npix;
+  if (i >= npix) return;
+  const int R = J->ring_frames, end = J->frame_end;
+  int f = prog[jid];
+  if (f >= end || J->done[f % R] != npix) return;
+  float4 a = J->accum[i];
+  // at most R frames: the ring slot after them still holds the first one
+  for (int k = 0; k < R && f < end && J->done[f % R] == npix; k++) {
+    float4 c = J->ring[(int64_t)(f % R) * npix + i];
+    float N = float(f);
     a.x = (a.x * (N - 1.0f) + c.x) / N;
     a.y = (a.y * (N - 1.0f) + c.y) / N;
     a.z = (a.z * (N - 1.0f) + c.z) / N;
     a.w = (a.w * (N - 1.0f) + c.w) / N;
-    if (J->rec_L) {
-      int64_t r = (int64_t)(frame - J->rec_first) * T.npix + i;
-      J->rec_L[r] = c;
-      if (ps.sig) { J->rec_sig[r] = ps.sig[p]; J->rec_prim[r] = ps.prim[p]; }
-    }
+    f++;
   }
-  acc[i] = a;
+  J->accum[i] = a;
+}
+
+// After k_accum_ready: advances each job's next frame to fold and releases
+// the ring slots of the folded frames.
+__global__ void k_advance(const DevJob* __restrict__ jobs, const int32_t* __restrict__ job_list, int n, int32_t* __restrict__ prog) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int jid = job_list[k];
+    const DevJob* J = jobs + jid;
+    const int R = J->ring_frames, end = J->frame_end, npix = J->npix;
+    int f = prog[jid];
+    for (int k = 0; k < R && f < end && J->done[f % R] == npix; k++) {
+      J->done[f % R] = 0;
+      f++;
+    }
+    prog[jid] = f;
+  }
 }
 
 // Standalone traversal on caller rays {org.xyz, tfar, dir.xyz, pad}.

@@ -1,39 +1,68 @@
-// Kernel declarations and the device-side job / pass / path-state layouts
+// Kernel declarations and the device-side job / task / path-state layouts
 // shared by the wavefront kernels and the host runtime.
 #pragma once
 #include "dev_scene.hpp"
 
 // One render job as the kernels see it: its scene + IBL, its 52-float frame
-// block, resolution, accumulation target and optional per-sample recording.
+// block, resolution, accumulation target, the ring of finished frames that
+// are not yet folded into the running mean, and optional per-sample records.
 struct DevJob {
   SceneView S;
   FrameParams F;
   int32_t W, H, npix, rec_first;  // rec_first: first frame held by the record buffers
   float4* accum;
+  float4* ring;                   // [frame % ring_frames][pixel] finished samples
+  int32_t* done;                  // [frame % ring_frames] finished paths of that frame
+  int32_t ring_frames;
+  int32_t frame_end;              // one past the last frame of the current range
   float4* rec_L;                  // [frame - rec_first][pixel] finished samples (or null)
   uint32_t* rec_sig;
   int32_t* rec_prim;
 };
 
-// A contiguous run of paths in one pass: frames first..first+nframes-1 of
-// one job, path ids [offset, offset + nframes*npix).
+// A contiguous run of work items: frames first..first+nframes-1 of one job,
+// item ids [offset, offset + nframes*npix), frame-major then pixel.
 struct PassTask {
   int32_t job, first_frame, nframes, npix;
   int64_t offset;
 };
 
-// Structure-of-arrays path state, capacity = paths in flight. Arrays are
-// indexed by path id.
+// Device-resident control block of the path pool. Items (job, frame, pixel)
+// are issued in id order; a slot freed by a finished path is refilled with
+// the next item on the following bounce.
+struct StreamCtl {
+  int64_t next_item;   // first item not yet issued
+  int64_t item0;       // first item of the current refill
+  int32_t free_count;  // entries on the free-slot stack
+  int32_t regen_n;     // items issued by the current refill
+  int32_t free_base;   // free-stack index of the first slot the refill takes
+  int32_t q_base;      // queue index where the refill appends
+  int32_t qcount[2];   // extension-ray queues (ping-pong)
+  int32_t sh_count;    // shadow-ray queue
+  int32_t pad;
+};
+
+// Written by the host before each chunk of bounces.
+struct StreamSched {
+  int64_t limit;   // items [next_item, limit) may be issued
+  int32_t ntasks;  // entries in the task table
+  int32_t pad;
+};
+
+// Structure-of-arrays path state, capacity = pool size. Arrays are indexed
+// by slot.
 struct PathState {
-  float4* org;   // extension ray origin
-  float4* dir;   // extension ray direction
-  float4* hit;   // (tri id bits, t, u, v) of the last extension ray
-  float4* pw;    // path weight xyz, last BSDF pdf in w
-  float4* L;     // radiance xyz (+alpha); the finished sample at exit
-  uint2* rng;    // generator state
-  int32_t* job;  // job index of the path
-  uint32_t* sig; // path signature (only when some job records samples)
-  int32_t* prim; // primary hit triangle (only when recording)
+  float4* org;    // extension ray origin
+  float4* dir;    // extension ray direction
+  float4* hit;    // (tri id bits, t, u, v) of the last extension ray
+  float4* pw;     // path weight xyz, last BSDF pdf in w
+  float4* L;      // radiance xyz (+alpha)
+  uint2* rng;     // generator state
+  int32_t* job;   // job index of the path
+  int32_t* bounce;  // bounce index of the current extension ray (0 = camera ray)
+  int2* fp;       // (frame, pixel) of the path
+  uint32_t* sig;  // path signature (recording jobs only)
+  int32_t* prim;  // primary hit triangle (recording jobs only)
   float4* sh_org;   // shadow origin xyz, flags bits in w (1 = IBL ray, 2 = point light ray)
   float4* sh_dir;   // IBL shadow direction, tmax
   float4* sh_c;     // IBL NEE radiance before path weight
@@ -49,13 +78,18 @@ __global__ void k_ibl_rows(const float* f, int w, int h, float* pdf, float* cdf,
 __global__ void k_ibl_marginal(const float* rowsum, int h, float* ypdf, float* ycdf, double* total);
 __global__ void k_fill_i32(int32_t* p, int n, int32_t v);
 __global__ void k_fill_f4(float4* p, int64_t n, float4 v);
-__global__ void k_raygen(PathState ps, const DevJob* jobs, const PassTask* tasks, int ntasks, int64_t P, int32_t* counts);
+__global__ void k_stream_init(StreamCtl* ctl, int32_t* free_list, int32_t P);
+__global__ void k_plan(StreamCtl* ctl, const StreamSched* sch, int a);
+__global__ void k_regen(PathState ps, const DevJob* jobs, const PassTask* tasks, const StreamSched* sch, const StreamCtl* ctl,
+                        const int32_t* free_list, int32_t* queue, unsigned long long* cnt);
 __global__ void k_trace_ext(const DevJob* jobs, PathState ps, const int32_t* queue, const int32_t* count_ptr, int* spill,
                             unsigned long long* cnt);
 __global__ void k_trace_shadow(const DevJob* jobs, PathState ps, const int32_t* queue, const int32_t* count_ptr, int* spill,
                                unsigned long long* cnt);
-__global__ void k_shade(const DevJob* jobs, PathState ps, int iter, const int32_t* queue_in, const int32_t* count_in,
-                        int32_t* queue_out, int32_t* count_out, int32_t* shq, int32_t* count_sh, unsigned long long* cnt);
-__global__ void k_accumulate(PathState ps, const DevJob* jobs, const PassTask* tasks);
+__global__ void k_shade(const DevJob* jobs, PathState ps, const int32_t* queue_in, const int32_t* count_in, int32_t* queue_out,
+                        int32_t* count_out, int32_t* shq, int32_t* count_sh, int32_t* free_list, int32_t* free_count,
+                        unsigned long long* cnt);
+__global__ void k_accum_ready(const DevJob* jobs, const int32_t* job_list, const int32_t* prog);
+__global__ void k_advance(const DevJob* jobs, const int32_t* job_list, int n, int32_t* prog);
 __global__ void k_trace_rays(SceneView S, const float4* rays, int64_t n, int any, int32_t* tri, float* t, float* u, float* v,
                              int* spill, unsigned long long* cnt);

@@ -62,12 +62,16 @@ SIGNATURES = {
     "dspt_job_status": (_c_int, [_vp, _c_i32, _i32p, _i32p]),
     "dspt_job_read_accum": (_c_int, [_vp, _c_i32, _fp]),
     "dspt_job_read_samples": (_c_int, [_vp, _c_i32, _fp, _u32p, _i32p]),
-    "dspt_jobs_step": (_c_int, [_vp, _i32p, _c_i32, _c_i64, _c_i32, _i32p]),
-    "dspt_jobs_run": (_c_int, [_vp, _i32p, _c_i32, _c_i64]),
+    "dspt_jobs_step": (_c_int, [_vp, _i32p, _c_i32, _i32p]),
+    "dspt_jobs_run": (_c_int, [_vp, _i32p, _c_i32]),
     "dspt_set_memory_budget": (_c_int, [_vp, _c_i64]),
     "dspt_set_launch_config": (_c_int, [_vp, _c_i64, _c_i32, _c_i32]),
     "dspt_get_launch_config": (_c_int, [_vp, _i64p, _i32p, _i32p]),
+    "dspt_set_schedule": (_c_int, [_vp, _c_i32, _c_i32, _c_i32, ctypes.c_float]),
+    "dspt_get_schedule": (_c_int, [_vp, _i32p, _i32p, _i32p, _fp]),
 }
+SCHED_STREAM = 0
+SCHED_PASS = 1
 KERNEL_CLASSES = ["raygen", "trace_ext", "shade", "trace_shadow", "accumulate"]
 
 
@@ -303,19 +307,32 @@ class Engine:
         check(self._L.dspt_job_read_samples(self.h, int(jid), ptr(rgba), ptr(sig, ctypes.c_uint32), ptr(prim, ctypes.c_int32)))
         return rgba.reshape(nf, h, w, 4), sig.reshape(nf, h, w), prim.reshape(nf, h, w)
 
-    def jobs_step(self, job_ids=None, max_paths=0, max_frames_per_job=0):
-        done = ctypes.c_int32(0)
+    def jobs_step(self, job_ids=None):
+        """One chunk of bounces over the listed jobs (all when None); returns
+        how many of them are not finished (as of the previous chunk)."""
+        active = ctypes.c_int32(0)
         if job_ids is None:
-            check(self._L.dspt_jobs_step(self.h, None, 0, int(max_paths), int(max_frames_per_job), ctypes.byref(done)))
+            check(self._L.dspt_jobs_step(self.h, None, 0, ctypes.byref(active)))
         else:
             ids = np.ascontiguousarray(job_ids, np.int32)
-            check(self._L.dspt_jobs_step(self.h, ptr(ids, ctypes.c_int32), ids.size, int(max_paths),
-                                         int(max_frames_per_job), ctypes.byref(done)))
-        return done.value
+            check(self._L.dspt_jobs_step(self.h, ptr(ids, ctypes.c_int32), ids.size, ctypes.byref(active)))
+        return active.value
 
-    def jobs_run(self, job_ids, max_paths=0):
-        ids = np.ascontiguousarray(job_ids, np.int32)
-        check(self._L.dspt_jobs_run(self.h, ptr(ids, ctypes.c_int32), ids.size, int(max_paths)))
+    def jobs_run(self, job_ids=None):
+        if job_ids is None:
+            check(self._L.dspt_jobs_run(self.h, None, 0))
+        else:
+            ids = np.ascontiguousarray(job_ids, np.int32)
+            check(self._L.dspt_jobs_run(self.h, ptr(ids, ctypes.c_int32), ids.size))
+
+    def set_schedule(self, mode=SCHED_STREAM, pass_frames=0, chunk_iters=0, ring_factor=0.0):
+        check(self._L.dspt_set_schedule(self.h, int(mode), int(pass_frames), int(chunk_iters), float(ring_factor)))
+
+    def schedule(self):
+        m, pf, ci = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
+        rf = ctypes.c_float(0)
+        check(self._L.dspt_get_schedule(self.h, ctypes.byref(m), ctypes.byref(pf), ctypes.byref(ci), ctypes.byref(rf)))
+        return {"mode": m.value, "pass_frames": pf.value, "chunk_iters": ci.value, "ring_factor": rf.value}
 
     def set_memory_budget(self, nbytes):
         check(self._L.dspt_set_memory_budget(self.h, int(nbytes)))

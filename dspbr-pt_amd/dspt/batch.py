@@ -1,16 +1,18 @@
 """Batched and served rendering on top of the engine's job API.
 
 BatchRenderer  many scenes / cameras / resolutions / frame counts rendered
-               together: every pass packs pending frames of all jobs into one
-               set of wavefront launches (`dspt_jobs_step`), with the launch
-               configuration chosen by a small autotuner. `run_plain` is the
-               one-job-at-a-time, one-frame-per-pass loop used as the
-               comparison point.
+               together through one path pool (`dspt_jobs_run` with the
+               STREAM schedule: finished paths are refilled with samples of
+               any job on the next bounce), with the pool size, traversal
+               grid and chunk length chosen by a small autotuner.
+               `run_plain` is the one-job-at-a-time loop with
+               frame-synchronous passes (PASS schedule), optionally one frame
+               per pass as in the reference's render loop.
 Server         continuous batching: requests arrive over time, are admitted
                when the device-memory budget allows (the engine refuses a job
                with DSPT_ERR_MEMORY otherwise and the request waits), share
-               passes with the jobs already running, and report progressive
-               per-frame completion; latency = arrival -> last frame.
+               the pool with the jobs already running, and report progressive
+               per-frame completion; latency = arrival -> last frame folded.
 """
 import time
 
@@ -71,71 +73,66 @@ class BatchRenderer:
         iid = self.ibls[spec.ibl][0] if spec.ibl is not None else -1
         return self.eng.job_create(sid, iid, self.block(spec), spec.width, spec.height, spec.frames, spec.record)
 
-    def run_batched(self, job_ids, max_paths=0, max_frames_per_job=0):
-        """Passes over all listed jobs until none has pending frames; returns passes run."""
-        passes = 0
-        while self.eng.jobs_step(job_ids, max_paths, max_frames_per_job) > 0:
-            passes += 1
+    def run_batched(self, job_ids):
+        """All listed jobs through the pool together (STREAM schedule)."""
+        self.eng.set_schedule(_lib.SCHED_STREAM)
+        self.eng.jobs_run(job_ids)
         self.eng.synchronize()
-        return passes
 
-    def run_plain(self, job_ids):
-        """One job at a time, one frame per pass (the frame loop of the reference's renderer)."""
-        passes = 0
+    def run_plain(self, job_ids, pass_frames=0):
+        """One job at a time in frame-synchronous passes (PASS schedule);
+        pass_frames=1 renders one frame per pass like the reference's loop."""
+        self.eng.set_schedule(_lib.SCHED_PASS, pass_frames)
         for j in job_ids:
-            while self.eng.jobs_step([j], 0, 1) > 0:
-                passes += 1
+            self.eng.jobs_run([j])
         self.eng.synchronize()
-        return passes
+        self.eng.set_schedule(_lib.SCHED_STREAM)
 
-    def autotune(self, make_jobs, candidates=None, budget_s=20.0):
-        """Times candidate launch configurations on a fresh copy of the workload
-        (make_jobs() -> job ids; they are destroyed afterwards) and keeps the
-        fastest. Returns (best, table)."""
+    def autotune(self, make_jobs, candidates=None, chunks=24, warm_chunks=6):
+        """Times candidate launch configurations on a fresh copy of the
+        workload (make_jobs() -> job ids; destroyed afterwards): Mpaths/s over
+        `chunks` chunks after `warm_chunks`. Keeps the fastest; returns
+        (best, table)."""
         if candidates is None:
-            candidates = [dict(max_paths=mp, trace_blocks=tb) for mp in (4 << 20, 8 << 20, 16 << 20)
-                          for tb in (1024, 2048, 4096)]
+            candidates = [dict(max_paths=mp, trace_blocks=tb, chunk_iters=ci)
+                          for mp in (4 << 20, 8 << 20, 16 << 20) for tb in (1024, 2048) for ci in (2, 4)]
         table = []
-        t_end = time.perf_counter() + budget_s
+        self.eng.set_schedule(_lib.SCHED_STREAM)
         for cand in candidates:
-            if time.perf_counter() > t_end and table:
-                break
             self.eng.set_launch_config(cand["max_paths"], cand["trace_blocks"], 0)
+            self.eng.set_schedule(_lib.SCHED_STREAM, 0, cand["chunk_iters"])
             ids = make_jobs()
-            for _ in range(2):  # warm the pool allocation, then time one pass set
-                self.eng.jobs_step(ids, 0, 0)
+            for _ in range(warm_chunks):
+                self.eng.jobs_step(ids)
             self.eng.synchronize()
+            p0 = self.eng.counters()["paths"]
             t0 = time.perf_counter()
-            frames = 0
-            for _ in range(3):
-                frames += self.eng.jobs_step(ids, 0, 0)
+            for _ in range(chunks):
+                if self.eng.jobs_step(ids) == 0:
+                    break
             self.eng.synchronize()
             dt = time.perf_counter() - t0
-            paths = 0
-            for j in ids:
-                d, _ = self.eng.job_status(j)
-                paths += 0  # rate from the frames completed in the timed passes
-            w = sum(self.eng._job_dims[j][0] * self.eng._job_dims[j][1] for j in ids) / max(1, len(ids))
-            rate = frames * w / dt if dt > 0 else 0.0
+            rate = (self.eng.counters()["paths"] - p0) / dt / 1e6 if dt > 0 else 0.0
             table.append((rate, cand))
             for j in ids:
                 self.eng.job_destroy(j)
         best = max(table, key=lambda x: x[0])[1]
         self.eng.set_launch_config(best["max_paths"], best["trace_blocks"], 0)
+        self.eng.set_schedule(_lib.SCHED_STREAM, 0, best["chunk_iters"])
         return best, table
 
 
 class Server:
     """Continuous-batching request loop over resident scenes."""
 
-    def __init__(self, batch, max_paths=0, frames_per_pass=1):
+    def __init__(self, batch):
         self.b = batch
-        self.max_paths = max_paths
-        self.frames_per_pass = frames_per_pass  # progressive: every live job advances this many frames per pass
 
-    def run(self, specs, clock=time.perf_counter, on_progress=None):
+    def run(self, specs, clock=time.perf_counter, on_progress=None, keep=False):
         """specs: JobSpec list with .arrival in seconds from start. Returns per-job
         records {arrival, admitted, first_frame, done, frames, paths}."""
+        eng = self.b.eng
+        eng.set_schedule(_lib.SCHED_STREAM)
         pending = sorted(range(len(specs)), key=lambda i: specs[i].arrival)
         live = {}  # job id -> spec index
         rec = [dict(arrival=specs[i].arrival, admitted=None, first_frame=None, done=None, frames=specs[i].frames,
@@ -164,21 +161,21 @@ class Server:
                     time.sleep(min(0.002, max(0.0, nxt - (clock() - t0))))
                 continue
             ids = list(live)
-            self.b.eng.jobs_step(ids, self.max_paths, self.frames_per_pass)
-            self.b.eng.synchronize()
+            eng.jobs_step(ids)
             now = clock() - t0
             for jid in ids:
                 i = live[jid]
-                d, tot = self.b.eng.job_status(jid)
+                d, tot = eng.job_status(jid)
                 if d >= 1 and rec[i]["first_frame"] is None:
                     rec[i]["first_frame"] = now
                 if on_progress:
                     on_progress(jid, d, tot)
                 if d >= tot:
                     rec[i]["done"] = now
-                    if on_progress is None or not getattr(on_progress, "keep", False):
-                        self.b.eng.job_destroy(jid)
+                    if not keep:
+                        eng.job_destroy(jid)
                     del live[jid]
+        eng.synchronize()
         return rec
 
 

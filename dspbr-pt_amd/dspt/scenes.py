@@ -363,3 +363,35 @@ def batch_scene(seed, n_tris, kind):
     else:
         raise ValueError(kind)
     return mb.finish(scene)
+
+
+BATCH_CAMERAS = {
+    "spheres": ((0.0, 3.0, 12.0), (0.0, 1.0, 0.0)),
+    "terrain": ((0.0, 6.0, 14.0), (0.0, 0.0, 0.0)),
+    "soup": ((0.0, 5.0, 18.0), (0.0, 5.0, 0.0)),
+}
+
+
+def batch_mix(seed=3, n_jobs=32, tri_range=(10_000, 2_000_000), side_range=(256, 1024), spp_range=(16, 256),
+              ibl_size=(1024, 512), max_bounces=8, profile="webgl-full"):
+    """The uneven batched mix: n_jobs scenes with log-uniform triangle counts
+    in tri_range, a kind drawn from {spheres, terrain, soup}, a square
+    resolution with log-uniform pixel count (side in side_range, a multiple
+    of 16), log-uniform spp in spp_range and a seeded log-normal IBL each.
+    Draw order per job: n_tris, kind, side, spp, scene seed."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kinds = ["spheres", "terrain", "soup"]
+    out = []
+    for k in range(n_jobs):
+        n_tris = int(round(math.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1])))))
+        kind = kinds[int(rng.integers(0, 3))]
+        side = math.exp(rng.uniform(math.log(side_range[0]), math.log(side_range[1])))
+        side = int(min(side_range[1], max(side_range[0], 16 * round(side / 16))))
+        spp = int(round(math.exp(rng.uniform(math.log(spp_range[0]), math.log(spp_range[1])))))
+        sseed = int(rng.integers(0, 2**31 - 1))
+        scene = batch_scene(sseed, n_tris, kind)
+        pos, tgt = BATCH_CAMERAS[kind]
+        cam = ViewportCamera(position=pos, target=tgt, fov=45, near=0.01)
+        ibl = lognormal_ibl(*ibl_size, seed=sseed + 1)
+        out.append(Workload(f"batch{k}_{kind}", scene, ibl, cam, side, side, spp, max_bounces, profile))
+    return out

@@ -113,13 +113,13 @@ int dspt_get_counters(dspt_ctx* ctx, int64_t* out);
 int dspt_reset_counters(dspt_ctx* ctx);
 
 /* Per-kernel timing with HIP events recorded on the engine stream around
- * every launch of dspt_render(). Kernel classes: */
+ * every launch of the bounce loop. Kernel classes: */
 enum {
-  DSPT_K_RAYGEN = 0,
+  DSPT_K_RAYGEN = 0,       /* refill of freed slots with camera rays (plan + regen) */
   DSPT_K_TRACE_EXT = 1,    /* closest-hit traversal of camera + extension rays */
   DSPT_K_SHADE = 2,
   DSPT_K_TRACE_SHADOW = 3, /* any-hit traversal of NEE shadow rays */
-  DSPT_K_ACCUM = 4,
+  DSPT_K_ACCUM = 4,        /* running mean of complete frames (accum_ready + advance) */
   DSPT_K_COUNT = 8
 };
 /* Stats per class, double[DSPT_K_COUNT * 4]: {launches, total_ms,
@@ -166,22 +166,38 @@ int dspt_job_status(dspt_ctx* ctx, int32_t job_id, int32_t* frames_done, int32_t
 int dspt_job_read_accum(dspt_ctx* ctx, int32_t job_id, float* rgba);
 int dspt_job_read_samples(dspt_ctx* ctx, int32_t job_id, float* rgba, uint32_t* sigs, int32_t* prim);
 
-/* One pass: packs consecutive pending frames of the listed jobs (all live
- * jobs when job_ids is NULL) into at most max_paths paths in flight (<= 0:
- * the context default), up to max_frames_per_job frames each (<= 0: no cap),
- * and renders them with one set of wavefront launches. frames_done receives
- * the number of job-frames completed (0 when nothing was pending). Each
- * job's result does not depend on how its frames were packed. */
-int dspt_jobs_step(dspt_ctx* ctx, const int32_t* job_ids, int32_t n, int64_t max_paths, int32_t max_frames_per_job,
-                   int32_t* frames_done);
-/* Steps until the listed jobs have no pending frames. */
-int dspt_jobs_run(dspt_ctx* ctx, const int32_t* job_ids, int32_t n, int64_t max_paths);
+/* One chunk of the path pool (a few bounces; see dspt_set_schedule) over the
+ * listed jobs (all live jobs when job_ids is NULL): schedules their pending
+ * frames, runs the bounces and folds every complete frame into its job's
+ * running mean. Progress (dspt_job_status) is that of the previous chunk;
+ * *active receives how many listed jobs are not finished yet. A job's
+ * result does not depend on which other jobs shared the pool. */
+int dspt_jobs_step(dspt_ctx* ctx, const int32_t* job_ids, int32_t n, int32_t* active);
+/* Runs chunks until the listed jobs have folded all their frames. */
+int dspt_jobs_run(dspt_ctx* ctx, const int32_t* job_ids, int32_t n);
 int dspt_set_memory_budget(dspt_ctx* ctx, int64_t bytes);
 
-/* Launch configuration: paths in flight per pass, traversal and shading grid
- * caps (<= 0 keeps a value). Chosen by the autotuner in the host layer. */
+/* Launch configuration: path-pool size (paths in flight), traversal and
+ * shading grid caps (<= 0 keeps a value). Chosen by the autotuner in the
+ * host layer. A new pool size takes effect once the work in flight drains. */
 int dspt_set_launch_config(dspt_ctx* ctx, int64_t max_paths, int32_t trace_blocks, int32_t shade_blocks);
 int dspt_get_launch_config(dspt_ctx* ctx, int64_t* max_paths, int32_t* trace_blocks, int32_t* shade_blocks);
+
+/* Scheduling of the path pool.
+ *   DSPT_SCHED_STREAM  frames of the listed jobs are queued round-robin ahead
+ *                      of the pool; a slot freed by a finished path takes the
+ *                      next sample on the next bounce (default).
+ *   DSPT_SCHED_PASS    frame-synchronous passes: a new pass (up to a pool of
+ *                      frames, FIFO over the listed jobs, at most pass_frames
+ *                      frames of a job when > 0) starts only when the pool is
+ *                      idle; pass_frames = 1 is the reference's one frame per
+ *                      draw loop (RT:595-714).
+ * chunk_iters: bounces per chunk (0 keeps); ring_factor: per-job ring of
+ * unfolded frames, in pools (ring_factor * max_paths / pixels + 2 frames;
+ * applies to jobs created afterwards; 0 keeps). */
+enum { DSPT_SCHED_STREAM = 0, DSPT_SCHED_PASS = 1 };
+int dspt_set_schedule(dspt_ctx* ctx, int32_t mode, int32_t pass_frames, int32_t chunk_iters, float ring_factor);
+int dspt_get_schedule(dspt_ctx* ctx, int32_t* mode, int32_t* pass_frames, int32_t* chunk_iters, float* ring_factor);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,161 @@
+"""The path pool's scheduling never changes a job's result.
+
+Each work item (job, frame, pixel) is computed from its own seed, and frames
+are folded into the running mean in frame order, so a job's samples and
+accumulation must be bit-identical whether it runs alone, shares the pool
+with other jobs, runs in frame-synchronous passes (one frame per pass, the
+reference's loop), goes through the single-image API, or wraps a small frame
+ring in a small pool. The single-image path is itself checked against the
+oracle in test_gpu_parity.py. Also: memory admission, destroying a job with
+work in flight, and the continuous-batching server loop.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix():
+    from dspt import scenes
+    return scenes.batch_mix(seed=11, n_jobs=3, tri_range=(2_000, 20_000), side_range=(32, 64), spp_range=(3, 6),
+                            ibl_size=(128, 64), max_bounces=6)
+
+
+@pytest.fixture(scope="module")
+def mix():
+    from dspt import build_bvh
+    wls = _mix()
+    for w in wls:
+        w.scene.bvhNodes, w.scene.bvhTriangleIndices = build_bvh(w.scene.bvhPositionBuffer, w.scene.bvhIndexBuffer)
+    return wls
+
+
+def _batch(eng, wls):
+    from dspt.batch import BatchRenderer, JobSpec
+    b = BatchRenderer(eng)
+    specs = []
+    for k, w in enumerate(wls):
+        b.add_scene(k, w.scene)
+        b.add_ibl(k, w.ibl)
+        specs.append(JobSpec(k, k, w.camera, w.width, w.height, w.spp, max_bounces=w.max_bounces, record=True))
+    return b, specs
+
+
+def _result(eng, jid):
+    acc = eng.job_read_accum(jid)
+    s, sig, prim = eng.job_read_samples(jid)
+    return acc, s, sig, prim
+
+
+def _same(a, b):
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_job_result_independent_of_schedule(mix):
+    from dspt import Engine, PROFILE_FULL, SCHED_PASS, SCHED_STREAM
+    eng = Engine(PROFILE_FULL)
+    b, specs = _batch(eng, mix)
+    alone = b.create_job(specs[0])
+    eng.jobs_run([alone])
+    ref = _result(eng, alone)
+    assert eng.job_status(alone) == (specs[0].frames, specs[0].frames)
+    assert np.isfinite(ref[0]).all() and ref[0][..., 3].min() > 0
+
+    ids = [b.create_job(s) for s in specs]  # job 0 again, together with the others
+    eng.jobs_run(ids)
+    _same(_result(eng, ids[0]), ref)
+
+    eng.set_schedule(SCHED_PASS, 1)  # one frame per pass
+    j = b.create_job(specs[0])
+    eng.jobs_run([j])
+    _same(_result(eng, j), ref)
+    eng.set_schedule(SCHED_STREAM)
+
+    # single-image API on the same scene: frames 1..n of the implicit job
+    w = mix[0]
+    eng2 = Engine(PROFILE_FULL)
+    eng2.upload_scene(w.scene.vertexBuffer, w.scene.triangleIndexBuffer, w.scene.bvhNodes, w.scene.bvhTriangleIndices,
+                      w.scene.getFlatMaterialBuffer())
+    img = w.ibl["image"]
+    eng2.upload_ibl(img["data"], img["width"], img["height"])
+    eng2.resize(w.width, w.height)
+    eng2.set_frame(b.block(specs[0]))
+    eng2.set_record_samples(True)
+    eng2.render(1, specs[0].frames)
+    eng2.synchronize()
+    s2, sig2, prim2 = eng2.read_samples(specs[0].frames)
+    np.testing.assert_array_equal(eng2.read_accum(), ref[0])
+    np.testing.assert_array_equal(s2, ref[1])
+    np.testing.assert_array_equal(sig2, ref[2])
+
+
+def test_small_pool_and_ring_wrap(mix):
+    """A 65536-path pool and a 1-pool ring force slot reuse within frames and
+    ring-slot reuse across frames."""
+    from dspt import Engine, PROFILE_FULL, SCHED_STREAM
+    from dspt.batch import JobSpec
+    w = mix[1]
+    eng = Engine(PROFILE_FULL)
+    b, _ = _batch(eng, [w])
+    spec = JobSpec(0, 0, w.camera, 192, 128, 9, max_bounces=w.max_bounces, record=True)
+    ref_job = b.create_job(spec)
+    eng.jobs_run([ref_job])
+    ref = _result(eng, ref_job)
+
+    eng.set_launch_config(65536, 256, 256)
+    eng.set_schedule(SCHED_STREAM, 0, 3, 1.0)
+    j = b.create_job(spec)  # ring of ceil(65536 / 24576) + 2 = 5 frames < 9
+    eng.jobs_run([j])
+    _same(_result(eng, j), ref)
+
+
+def test_destroy_with_work_in_flight(mix):
+    from dspt import Engine, PROFILE_FULL
+    eng = Engine(PROFILE_FULL)
+    b, specs = _batch(eng, mix)
+    ref_job = b.create_job(specs[2])
+    eng.jobs_run([ref_job])
+    ref = _result(eng, ref_job)
+    a = b.create_job(specs[1])
+    c = b.create_job(specs[2])
+    for _ in range(3):
+        eng.jobs_step([a, c])
+    eng.job_destroy(a)
+    eng.jobs_run([c])
+    _same(_result(eng, c), ref)
+
+
+def test_admission_by_memory_budget(mix):
+    from dspt import DsptError, Engine, PROFILE_FULL
+    from dspt.batch import JobSpec
+    eng = Engine(PROFILE_FULL)
+    b, specs = _batch(eng, mix[:1])
+    small = b.create_job(specs[0])
+    eng.set_memory_budget(eng.memory_usage() + (1 << 20))
+    big = JobSpec(0, 0, mix[0].camera, 1024, 1024, 64, max_bounces=4)
+    with pytest.raises(DsptError) as e:
+        b.create_job(big)
+    assert e.value.code == -4
+    eng.job_destroy(small)
+    eng.set_memory_budget(0)
+    j = b.create_job(big)  # unlimited again
+    eng.job_destroy(j)
+
+
+def test_server_loop_completes_every_request(mix):
+    from dspt import Engine, PROFILE_FULL
+    from dspt.batch import JobSpec, Server, latency_stats
+    eng = Engine(PROFILE_FULL)
+    b, specs = _batch(eng, mix)
+    reqs = []
+    rng = np.random.default_rng(5)
+    t = 0.0
+    for k in range(12):
+        t += rng.exponential(0.004)
+        s = specs[k % len(specs)]
+        reqs.append(JobSpec(s.scene, s.ibl, s.camera, s.width, s.height, s.frames, max_bounces=s.max_bounces, arrival=t))
+    rec = Server(b).run(reqs)
+    assert all(r["done"] is not None and r["done"] >= r["arrival"] for r in rec)
+    st = latency_stats(rec)
+    assert st["jobs"] == len(reqs) and st["p99_ms"] >= st["p50_ms"] > 0
