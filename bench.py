@@ -11,7 +11,15 @@ profile. One step = one complete 64-spp render of that frame (a job).
 Prints one JSON line: Mpaths/s (value), Mrays/s, p50/p99 job latency, the
 roofline of the dominant kernel (closest-hit traversal, HBM-bound, from HIP
 events on the engine stream) and the CPU baseline (the oracle on the host
-cores over a bounded sample of the same workload).
+cores over a bounded sample of the same workload). Two more objects:
+  batched  the uneven 32-scene mix (10k-2M triangles, 256^2-1024^2,
+           16-256 spp, 1024x512 IBL each) rendered through the shared path
+           pool vs the plain one-job-at-a-time loop (frame-synchronous
+           passes), Mpaths/s and p50/p99 time-to-N-spp of each, plus the
+           reference's one-frame-per-pass loop on a bounded subset;
+  serving  Poisson arrival streams over the full-material scene with
+           resolution/spp mixes, admission by a device-memory budget,
+           p50/p99 time-to-N-spp and time-to-first-frame against Mpaths/s.
 """
 import argparse
 import json
@@ -89,6 +97,182 @@ def cpu_baseline(wl, block, budget_s=20.0):
                       f"{frames_full} whole frames of the same {W}x{H} image, {paths} paths in {spent:.1f}s"}
 
 
+def batched_vs_plain(args):
+    """Config-3 style comparison; every job is rendered by both schedules and
+    one job's accumulation is compared bit for bit."""
+    from dspt import Engine, PROFILE_FULL, SCHED_PASS, SCHED_STREAM, POLICY_FIFO, POLICY_RR, build_bvh, scenes
+    from dspt.batch import BatchRenderer, JobSpec
+    t0 = time.time()
+    mix = scenes.batch_mix(seed=3, n_jobs=args.batch_jobs)
+    for w in mix:
+        w.scene.bvhNodes, w.scene.bvhTriangleIndices = build_bvh(w.scene.bvhPositionBuffer, w.scene.bvhIndexBuffer)
+    log(f"[bench] batched mix: {len(mix)} scenes, {sum(w.scene.num_triangles for w in mix)} tris, "
+        f"{sum(w.width * w.height * w.spp for w in mix) / 1e6:.1f} M paths, built in {time.time() - t0:.1f}s")
+    eng = Engine(PROFILE_FULL)
+    eng.set_schedule(SCHED_STREAM, 0, 0, 2.0)  # rings of 2 pools: the pool is shared by many jobs
+    b = BatchRenderer(eng)
+    specs = []
+    for k, w in enumerate(mix):
+        b.add_scene(k, w.scene)
+        b.add_ibl(k, w.ibl)
+        specs.append(JobSpec(k, k, w.camera, w.width, w.height, w.spp, max_bounces=w.max_bounces))
+    total = sum(s.paths for s in specs)
+
+    # autotune the pool on the first jobs' early chunks
+    sub = specs[: min(8, len(specs))]
+    best, table = b.autotune(lambda: [b.create_job(s) for s in sub],
+                             candidates=[dict(max_paths=mp, trace_blocks=tb, chunk_iters=2)
+                                         for mp in (4 << 20, 8 << 20, 16 << 20) for tb in (1024, 2048)],
+                             chunks=16, warm_chunks=4)
+    log(f"[bench] autotune: best {best}; " + ", ".join(f"{r:.0f}" for r, _ in table))
+
+    def run(mode_fn, **kw):
+        done_at = {}
+        eng.synchronize()
+        eng.reset_counters()
+        eng.set_kernel_timing(True)
+        t_start = time.perf_counter()
+        mode_fn(done_at, t_start, **kw)
+        eng.synchronize()
+        el = time.perf_counter() - t_start
+        run.kms = {k: round(v["total_ms"], 1) for k, v in eng.kernel_stats().items()}
+        eng.set_kernel_timing(False)
+        lat = np.array(sorted(done_at.values()))
+        return el, lat
+
+    def batched(done_at, t_start, policy=POLICY_FIFO):
+        ids = [b.create_job(s) for s in specs]
+        eng.set_schedule(SCHED_STREAM, 0, best["chunk_iters"], 2.0)
+        eng.set_policy(policy)
+        live = set(ids)
+        while live:
+            eng.jobs_step(sorted(live))
+            now = time.perf_counter() - t_start
+            for j in list(live):
+                d, tot = eng.job_status(j)
+                if d >= tot:
+                    done_at[j] = now
+                    live.discard(j)
+        run.ids = ids
+
+    def plain(done_at, t_start):
+        ids = []
+        eng.set_schedule(SCHED_PASS, 0)
+        for s in specs:
+            j = b.create_job(s)
+            eng.jobs_run([j])
+            eng.synchronize()
+            done_at[j] = time.perf_counter() - t_start
+            ids.append(j)
+        eng.set_schedule(SCHED_STREAM)
+        run.ids = ids
+
+    el_b, lat_b = run(batched)
+    ids_b, kms_b = run.ids, run.kms
+    acc_b = eng.job_read_accum(ids_b[0])
+    for j in ids_b:
+        eng.job_destroy(j)
+    el_r, lat_r = run(batched, policy=POLICY_RR)
+    kms_r = run.kms
+    for j in run.ids:
+        eng.job_destroy(j)
+    eng.set_policy(POLICY_RR)
+    el_p, lat_p = run(plain)
+    ids_p, kms_p = run.ids, run.kms
+    acc_p = eng.job_read_accum(ids_p[0])
+    for j in ids_p:
+        eng.job_destroy(j)
+    # the reference's loop (one frame per pass) on a bounded subset
+    fsub = [s for s in specs if s.paths <= 8e6][:4] or specs[:1]
+    eng.set_schedule(SCHED_PASS, 1)
+    t1 = time.perf_counter()
+    fids = []
+    for s in fsub:
+        j = b.create_job(s)
+        eng.jobs_run([j])
+        fids.append(j)
+    eng.synchronize()
+    el_f = time.perf_counter() - t1
+    eng.set_schedule(SCHED_STREAM)
+    t2 = time.perf_counter()
+    sids = [b.create_job(s) for s in fsub]
+    eng.jobs_run(sids)
+    eng.synchronize()
+    el_fs = time.perf_counter() - t2
+    for j in fids + sids:
+        eng.job_destroy(j)
+    fpaths = sum(s.paths for s in fsub)
+    q = lambda a, p: round(float(np.percentile(a, p)) * 1e3, 1)
+    res = {
+        "workload": f"{len(specs)} scenes {sum(w.scene.num_triangles for w in mix)} tris, "
+                    f"{min(s.width for s in specs)}^2-{max(s.width for s in specs)}^2, "
+                    f"{min(s.frames for s in specs)}-{max(s.frames for s in specs)} spp, depth 8, webgl-full",
+        "paths": total,
+        "batched_mpaths_per_s": round(total / el_b / 1e6, 2),
+        "plain_mpaths_per_s": round(total / el_p / 1e6, 2),
+        "speedup": round(el_p / el_b, 3),
+        "batched_latency_ms": {"p50": q(lat_b, 50), "p99": q(lat_b, 99)},
+        "plain_latency_ms": {"p50": q(lat_p, 50), "p99": q(lat_p, 99)},
+        "batched_policy": "fifo",
+        "round_robin": {"mpaths_per_s": round(total / el_r / 1e6, 2),
+                        "latency_ms": {"p50": q(lat_r, 50), "p99": q(lat_r, 99)}},
+        "kernel_ms": {"batched": kms_b, "round_robin": kms_r, "plain": kms_p},
+        "identical_accumulation": bool(np.array_equal(acc_b, acc_p)),
+        "autotuned": best,
+        "frame_loop_subset": {"jobs": len(fsub), "paths": fpaths,
+                              "one_frame_per_pass_mpaths_per_s": round(fpaths / el_f / 1e6, 2),
+                              "batched_mpaths_per_s": round(fpaths / el_fs / 1e6, 2)},
+    }
+    log(f"[bench] batched {res['batched_mpaths_per_s']} vs plain {res['plain_mpaths_per_s']} Mpaths/s")
+    return res
+
+
+def serving(wl, args):
+    """Poisson arrivals over the resident full-material scene; each request
+    draws a resolution and spp from the mixes below."""
+    from dspt import Engine, PROFILE_FULL
+    from dspt.batch import BatchRenderer, JobSpec, Server, latency_stats
+    eng = Engine(PROFILE_FULL)
+    b = BatchRenderer(eng)
+    b.add_scene(0, wl.scene)
+    b.add_ibl(0, wl.ibl)
+    resolutions = [(256, 256), (512, 512), (1024, 576), (1920, 1080)]
+    spps = [4, 8, 16, 32]
+    # warm the pool and the kernels
+    w0 = b.create_job(JobSpec(0, 0, wl.camera, 256, 256, 4, max_bounces=wl.max_bounces))
+    eng.jobs_run([w0])
+    eng.job_destroy(w0)
+    eng.set_memory_budget(eng.memory_usage() + (6 << 30))
+    out = []
+    for rate in args.serve_rates:
+        rng = np.random.default_rng(int(rate))
+        t, reqs = 0.0, []
+        while True:
+            t += rng.exponential(1.0 / rate)
+            if t > args.serve_seconds:
+                break
+            w, h = resolutions[int(rng.integers(0, len(resolutions)))]
+            reqs.append(JobSpec(0, 0, wl.camera, w, h, spps[int(rng.integers(0, len(spps)))], max_bounces=wl.max_bounces,
+                                arrival=t))
+        if not reqs:
+            continue
+        c0 = eng.counters()["paths"]
+        t0 = time.perf_counter()
+        rec = Server(b).run(reqs)
+        el = time.perf_counter() - t0
+        st = latency_stats(rec)
+        paths = sum(r.paths for r in reqs)
+        st.update({"rate_jobs_per_s": rate, "mpaths_per_s": round(paths / el / 1e6, 2), "wall_s": round(el, 2),
+                   "paths": paths})
+        assert eng.counters()["paths"] - c0 >= paths
+        st = {k: (round(v, 1) if isinstance(v, float) else v) for k, v in st.items()}
+        log(f"[bench] serving {rate}/s: {st}")
+        out.append(st)
+    eng.set_memory_budget(0)
+    return {"scene": f"{wl.name} {wl.scene.num_triangles} tris, depth {wl.max_bounces}",
+            "resolutions": resolutions, "spp": spps, "budget_bytes_for_jobs": 6 << 30, "sweep": out}
+
+
 def calibrate_copy(mib):
     """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
     per dispatch give the scale between the counters and real HBM bytes."""
@@ -111,6 +295,11 @@ def main():
     ap.add_argument("--spp", type=int, default=None, help="override frames per job")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain comparison")
+    ap.add_argument("--batch-jobs", type=int, default=32)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
+    ap.add_argument("--serve-rates", type=float, nargs="*", default=[5.0, 25.0, 100.0, 200.0])
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -185,7 +374,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{wl.name} {wl.width}x{wl.height} {spp}spp depth{wl.max_bounces} "
-                               f"{s.num_triangles} tris {wl.profile}", "mode": "plain (one job at a time)"},
+                               f"{s.num_triangles} tris {wl.profile}", "mode": "single job, streaming path pool"},
         "mrays_per_s": round(mrays, 2),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 1),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 1)},
@@ -196,6 +385,8 @@ def main():
                      "kernel": "k_trace_ext", "avg_launch_ms": round(avg_ms, 4),
                      "alg_bytes_per_launch": round(bytes_per_launch)},
     }
+    out["batched"] = None if args.no_batched else batched_vs_plain(args)
+    out["serving"] = None if args.no_serving else serving(wl, args)
     if not args.no_cpu_baseline:
         t0 = time.time()
         out["cpu_baseline"] = cpu_baseline(wl, block, args.cpu_budget)

@@ -205,7 +205,15 @@ DI f3 film_sensitivity(float opd, f3 shift) {
             -0.96926599740982056f * q.x + 1.87601077556610107f * q.y + 0.04155600070953369f * q.z,
             0.05564339831471443f * q.x - 0.20402589440345764f * q.y + 1.05722522735595703f * q.z};
 }
-DI f3 film_reflectance(float outside, float filmIor, f3 baseF0, float thick, float cos1) {
+#ifndef DSPT_FILM_INLINE
+#define DSPT_FILM_INLINE 1
+#endif
+#if DSPT_FILM_INLINE
+#define FILM_FN DI
+#else
+#define FILM_FN DN
+#endif
+FILM_FN f3 film_reflectance(float outside, float filmIor, f3 baseF0, float thick, float cos1) {
   float fi = gclamp(filmIor, 1.0f, 2.5f);
   float c1 = gclamp(cos1, 0.0f, 1.0f);
   float c2sq = 1.0f - outside / fi * (outside / fi) * (1.0f - c1 * c1);
@@ -654,11 +662,18 @@ DI Sample sample_dielectric(const Closure& c, const Frame3& F, f3 n, f3 v, float
 
 // sampleMaterialBsdfMixture (KF:3200-3567). u = {component, lobe.x, lobe.y, boundary}.
 // `lobe_id` reports the branch taken (diagnostics / path signatures).
+// sampleMaterialBsdfMixture (KF:3200-3567). The lobe choice, the
+// dielectric sample and the mixture pdf each have one call site (the
+// branches of the reference share them), which keeps the shading kernel's
+// code small.
 DI Sample sample(const Closure& c, const Frame3& F, f3 n, f3 v, float cu, float u0, float u1, float eu) {
   Sample s;
-  f3 dir;
-  float spec;
-  if (thin_transmissive(c)) {
+  f3 dir = sp3(0.0f);
+  float spec = 0.0f;
+  const bool thin = thin_transmissive(c);
+  bool dielectric = false;  // the transmission lobe: sample_dielectric
+  float deu = eu, transP = 0.0f;
+  if (thin) {
     float tb = thin_base_prob(c, n, v);
     float ts = 1.0f - tb;
     float blw = base_layer_weight(c);
@@ -672,75 +687,74 @@ DI Sample sample(const Closure& c, const Frame3& F, f3 n, f3 v, float cu, float 
     if (cu < pd) { dir = cosine_dir(n, u0, u1); spec = 0.0f; }
     else if (cu < c1) { dir = cosine_dir(n, u0, u1); spec = 0.0f; }
     else if (cu < c2) { dir = cosine_dir(-n, u0, u1); spec = 0.0f; }
-    else if (cu < c3) {
-      s = sample_dielectric(c, F, n, v, u0, u1, 1.0f);
-      float mp = pdf(c, F, n, v, s.dir);
-      s.pdf = mp;
-      s.w = dielectric_transmission(c, F, n, v, s.dir) / gmax(mp, E5);
-      return s;
-    } else if (cu < c3 + psp) { dir = reflect3(-v, ggx_half(c, F, u0, u1)); spec = 1.0f; }
+    else if (cu < c3) { dielectric = true; deu = 1.0f; }
+    else if (cu < c3 + psp) { dir = reflect3(-v, ggx_half(c, F, u0, u1)); spec = 1.0f; }
     else { dir = reflect3(-v, ggx_half_iso(n, c.ccRough, u0, u1)); spec = 1.0f; }
-    float nd = dot3(n, dir);
-    float nl = gmax(nd, 0.0f);
-    if (nl <= 0.0f) {
-      if (nd < 0.0f) {
-        float p = gmax(pdf(c, F, n, v, dir), E5);
-        s.dir = dir; s.pdf = p; s.w = diffuse_transmission(c, n, dir) / gmax(p, E5); s.specular = 0.0f;
-        return s;
-      }
+  } else {
+    float dvf = max3c(schlick(c.dielF0, sp3(c.dielF90), gmax(dot3(n, v), 0.0f)));
+    float omf = 1.0f - dvf;
+    float dpw = c.diffuseWeight * omf, spw = c.sheenWeight * omf;
+    float specw = c.specularWeight + c.dielectricWeight * (1.0f - c.transparency) * dvf;
+    float bpw = dpw + spw + c.diffTransWeight;
+    float tot = gmax(bpw + specw + c.transmissionWeight + c.clearcoatWeight, E5);
+    float baseP = bpw / tot, specP = specw / tot;
+    transP = c.transmissionWeight / tot;
+    if (cu < baseP) {
+      float bu = cu / gmax(baseP, E5);
+      float dp = bpw > 0.0f ? dpw / bpw : 0.0f;
+      float sp = bpw > 0.0f ? spw / bpw : 0.0f;
+      if (bu < dp) dir = cosine_dir(n, u0, u1);
+      else if (bu < dp + sp) dir = cosine_dir(n, u0, u1);
+      else dir = cosine_dir(-n, u0, u1);
+      spec = 0.0f;
+    } else if (cu < baseP + specP) {
+      dir = reflect3(-v, ggx_half(c, F, u0, u1)); spec = 1.0f;
+    } else if (cu < baseP + specP + transP) {
+      dielectric = true;
+    } else {
+      dir = reflect3(-v, ggx_half_iso(n, c.ccRough, u0, u1)); spec = 1.0f;
+    }
+  }
+  float nd = 0.0f, nl = 0.0f;
+  bool need_pdf;
+  f3 pdir;
+  if (dielectric) {
+    s = sample_dielectric(c, F, n, v, u0, u1, deu);
+    need_pdf = thin;
+    pdir = s.dir;
+  } else {
+    nd = dot3(n, dir);
+    nl = gmax(nd, 0.0f);
+    if (nl <= 0.0f && !(nd < 0.0f)) {
       f3 fb = cosine_dir(n, u0, u1);
       s.dir = fb; s.pdf = gmax(gmax(dot3(n, fb), 0.0f) * INV_PI, E5); s.w = sp3(0.0f); s.specular = 0.0f;
       return s;
     }
-    float mp = pdf(c, F, n, v, dir);
-    f3 rb = reflected_base(c, F, n, v, dir) / gmax(nl, E5);
-    f3 cce = nl > 0.0f ? clearcoat(c, n, v, dir) / gmax(nl, E5) : sp3(0.0f);
-    float bw = clearcoat_base_weight(c, n, v, dir);
-    s.dir = dir; s.pdf = mp; s.w = (rb * bw + cce) * nl / gmax(mp, E5); s.specular = spec;
-    return s;
+    need_pdf = true;
+    pdir = dir;
   }
-  float dvf = max3c(schlick(c.dielF0, sp3(c.dielF90), gmax(dot3(n, v), 0.0f)));
-  float omf = 1.0f - dvf;
-  float dpw = c.diffuseWeight * omf, spw = c.sheenWeight * omf;
-  float specw = c.specularWeight + c.dielectricWeight * (1.0f - c.transparency) * dvf;
-  float bpw = dpw + spw + c.diffTransWeight;
-  float tot = gmax(bpw + specw + c.transmissionWeight + c.clearcoatWeight, E5);
-  float baseP = bpw / tot, specP = specw / tot, transP = c.transmissionWeight / tot;
-  if (cu < baseP) {
-    float bu = cu / gmax(baseP, E5);
-    float dp = bpw > 0.0f ? dpw / bpw : 0.0f;
-    float sp = bpw > 0.0f ? spw / bpw : 0.0f;
-    if (bu < dp) dir = cosine_dir(n, u0, u1);
-    else if (bu < dp + sp) dir = cosine_dir(n, u0, u1);
-    else dir = cosine_dir(-n, u0, u1);
-    spec = 0.0f;
-  } else if (cu < baseP + specP) {
-    dir = reflect3(-v, ggx_half(c, F, u0, u1)); spec = 1.0f;
-  } else if (cu < baseP + specP + transP) {
-    s = sample_dielectric(c, F, n, v, u0, u1, eu);
-    s.w = s.w / gmax(transP, E5);
-    if (s.pdf < 1.0f) s.pdf = s.pdf * transP;
-    return s;
-  } else {
-    dir = reflect3(-v, ggx_half_iso(n, c.ccRough, u0, u1)); spec = 1.0f;
-  }
-  float nd = dot3(n, dir);
-  float nl = gmax(nd, 0.0f);
-  if (nl <= 0.0f) {
-    if (nd < 0.0f) {
-      float p = gmax(pdf(c, F, n, v, dir), E5);
-      s.dir = dir; s.pdf = p; s.w = diffuse_transmission(c, n, dir) / p; s.specular = 0.0f;
-      return s;
+  const float mp = need_pdf ? pdf(c, F, n, v, pdir) : 0.0f;
+  if (dielectric) {
+    if (thin) {
+      s.pdf = mp;
+      s.w = dielectric_transmission(c, F, n, v, s.dir) / gmax(mp, E5);
+    } else {
+      s.w = s.w / gmax(transP, E5);
+      if (s.pdf < 1.0f) s.pdf = s.pdf * transP;
     }
-    f3 fb = cosine_dir(n, u0, u1);
-    s.dir = fb; s.pdf = gmax(gmax(dot3(n, fb), 0.0f) * INV_PI, E5); s.w = sp3(0.0f); s.specular = 0.0f;
     return s;
   }
-  float mp = pdf(c, F, n, v, dir);
+  if (nl <= 0.0f) {  // nd < 0: diffuse transmission
+    float p = gmax(mp, E5);
+    s.dir = dir; s.pdf = p; s.specular = 0.0f;
+    s.w = thin ? diffuse_transmission(c, n, dir) / gmax(p, E5) : diffuse_transmission(c, n, dir) / p;
+    return s;
+  }
   f3 rb = reflected_base(c, F, n, v, dir) / gmax(nl, E5);
   f3 cce = nl > 0.0f ? clearcoat(c, n, v, dir) / gmax(nl, E5) : sp3(0.0f);
   float bw = clearcoat_base_weight(c, n, v, dir);
-  s.dir = dir; s.pdf = mp; s.w = (rb * bw + cce) * fabsf(nd) / mp; s.specular = spec;
+  s.dir = dir; s.pdf = mp; s.specular = spec;
+  s.w = thin ? (rb * bw + cce) * nl / gmax(mp, E5) : (rb * bw + cce) * fabsf(nd) / mp;
   return s;
 }
 

@@ -14,6 +14,25 @@
 
 #define DI __host__ __device__ __forceinline__
 #define DD __device__ __forceinline__
+// Out-of-line device functions: large leaf computations with register-only
+// arguments that several BSDF lobes call (one copy in the kernel's code).
+#define DN __host__ __device__ __attribute__((noinline))
+
+// Loads through pointers that the kernels read from memory (scene tables)
+// are generic (FLAT) unless the compiler is told they address global
+// memory; ldg() marks them as global loads. Host builds load directly.
+#ifndef DSPT_LDG
+#define DSPT_LDG 1
+#endif
+DI float4 ldg(const float4* p) {
+#if defined(__HIP_DEVICE_COMPILE__) && DSPT_LDG
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const v4f x = *(const __attribute__((address_space(1))) v4f*)p;
+  return make_float4(x.x, x.y, x.z, x.w);
+#else
+  return *p;
+#endif
+}
 
 struct f3 {
   float x, y, z;

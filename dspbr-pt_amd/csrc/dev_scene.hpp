@@ -62,7 +62,16 @@ struct LaneStack {
     if (i < kLdsStack) lds[i * kTraceBlock] = v;
     else spill[i - kLdsStack] = v;
   }
-  DI int get(int i) const { return i < kLdsStack ? lds[i * kTraceBlock] : spill[i - kLdsStack]; }
+  // The spill read is a non-temporal global load so the compiler cannot
+  // merge the two reads into one generic (FLAT) load of a selected pointer.
+  DI int get(int i) const {
+    if (i < kLdsStack) return lds[i * kTraceBlock];
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_nontemporal_load((const __attribute__((address_space(1))) int*)(spill + (i - kLdsStack)));
+#else
+    return spill[i - kLdsStack];
+#endif
+  }
 };
 
 // Slab test, SH/bvh.glsl:31-41 (GLSL min/max select forms, NaN guarded).
@@ -103,58 +112,87 @@ struct Geo {
 };
 DI Geo geo_of(const SceneView& S) { return Geo{S.nodes, S.tri, S.N, S.M}; }
 
-// Stack traversal in the exact visit order of SH/bvh.glsl:149-212. With
-// ANY_HIT it returns at the first accepted triangle; the occlusion boolean
-// equals the reference's closest-hit based isOccluded() because both visit
-// the same nodes with the same tfar until the first hit.
-template <bool ANY_HIT>
-DI RayHit traverse(const Geo G, f3 org, f3 dir, float tfar, LaneStack& st, unsigned& nv, unsigned& nt) {
-  f3 inv = f3{1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
-  RayHit h{-1, tfar, 0.0f, 0.0f};
-  int top = 1;
+// Resumable traversal state of one ray (the loop variables of
+// SH/bvh.glsl:149-212).
+struct RayState {
+  f3 org, dir, inv;
+  RayHit h;
+  int top, step;
+};
+
+DI void ray_init(RayState& r, LaneStack& st, f3 org, f3 dir, float tfar) {
+  r.org = org;
+  r.dir = dir;
+  r.inv = f3{1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
+  r.h = RayHit{-1, tfar, 0.0f, 0.0f};
+  r.top = 1;
+  r.step = 0;
   st.put(0, 0);
-  for (int step = 0; step < kMaxSteps && top > 0; step++) {
-    int ni = st.get(--top);
-    if (ni < 0 || ni >= G.N) continue;
-    nv++;
-    const float4* nd = G.nodes + (size_t)ni * 4;
-    float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
-    int cnt = (int)c.w;
-    int first = (int)d.w;
-    if (cnt > 0) {
-      int end = min(first + min(cnt, kMaxLeafTris), G.M);
-      for (int i = first; i < end; i++) {
-        if (i < 0) continue;
-        const float4* tp = G.tri + (size_t)i * 3;
-        float4 q0 = tp[0];
-        int id = __float_as_int(q0.w);
-        if (id < 0) continue;
-        float4 q1 = tp[1], q2 = tp[2];
-        nt++;
-        float t, u, v;
-        if (tri_test(org, dir, xyz(q0), xyz(q1), xyz(q2), h.t, t, u, v)) {
-          h.t = t; h.tri = id; h.u = u; h.v = v;
-          if (ANY_HIT) return h;
-        }
-      }
-    } else {
-      int left = (int)a.w, right = (int)b.w;
-      float d0 = slab(xyz(a), xyz(b), org, inv, h.t);
-      float d1 = slab(xyz(c), xyz(d), org, inv, h.t);
-      auto push = [&](int child) {
-        if (top < kStackSize && child >= 0 && child < G.N && child != ni) st.put(top++, child);
-      };
-      if (d0 < kBVH_FAR && d1 < kBVH_FAR) {
-        if (d0 < d1) { push(right); push(left); }
-        else { push(left); push(right); }
-      } else if (d0 < kBVH_FAR) {
-        push(left);
-      } else if (d1 < kBVH_FAR) {
-        push(right);
+}
+
+DI bool ray_done(const RayState& r) { return !(r.step < kMaxSteps && r.top > 0); }
+
+// One iteration of the stack traversal, in the exact visit order of
+// SH/bvh.glsl:149-212: pop a node; a leaf tests its triangles, an inner
+// node pushes the children it overlaps, far child first (ties: right child
+// popped first). Returns true when the ray is finished. With ANY_HIT it
+// finishes at the first accepted triangle; the occlusion boolean equals the
+// reference's closest-hit based isOccluded() because both visit the same
+// nodes with the same tfar until the first hit.
+template <bool ANY_HIT>
+DI bool trav_step(const Geo& G, RayState& r, LaneStack& st, unsigned& nv, unsigned& nt) {
+  if (ray_done(r)) return true;
+  r.step++;
+  const int ni = st.get(--r.top);
+  if (ni < 0 || ni >= G.N) return ray_done(r);
+  nv++;
+  const float4* nd = G.nodes + (size_t)ni * 4;
+  const float4 a = ldg(nd), b = ldg(nd + 1), c = ldg(nd + 2), d = ldg(nd + 3);
+  const int cnt = (int)c.w;
+  const int first = (int)d.w;
+  if (cnt > 0) {
+    const int end = min(first + min(cnt, kMaxLeafTris), G.M);
+    for (int i = first; i < end; i++) {
+      if (i < 0) continue;
+      const float4* tp = G.tri + (size_t)i * 3;
+      const float4 q0 = ldg(tp);
+      const int id = __float_as_int(q0.w);
+      if (id < 0) continue;
+      const float4 q1 = ldg(tp + 1), q2 = ldg(tp + 2);
+      nt++;
+      float t, u, v;
+      if (tri_test(r.org, r.dir, xyz(q0), xyz(q1), xyz(q2), r.h.t, t, u, v)) {
+        r.h.t = t; r.h.tri = id; r.h.u = u; r.h.v = v;
+        if (ANY_HIT) return true;
       }
     }
+  } else {
+    const int left = (int)a.w, right = (int)b.w;
+    const float d0 = slab(xyz(a), xyz(b), r.org, r.inv, r.h.t);
+    const float d1 = slab(xyz(c), xyz(d), r.org, r.inv, r.h.t);
+    auto push = [&](int child) {
+      if (r.top < kStackSize && child >= 0 && child < G.N && child != ni) st.put(r.top++, child);
+    };
+    if (d0 < kBVH_FAR && d1 < kBVH_FAR) {
+      if (d0 < d1) { push(right); push(left); }
+      else { push(left); push(right); }
+    } else if (d0 < kBVH_FAR) {
+      push(left);
+    } else if (d1 < kBVH_FAR) {
+      push(right);
+    }
   }
-  return h;
+  return ray_done(r);
+}
+
+// Whole traversal of one ray.
+template <bool ANY_HIT>
+DI RayHit traverse(const Geo G, f3 org, f3 dir, float tfar, LaneStack& st, unsigned& nv, unsigned& nt) {
+  RayState r;
+  ray_init(r, st, org, dir, tfar);
+  while (!trav_step<ANY_HIT>(G, r, st, nv, nt)) {
+  }
+  return r.h;
 }
 
 // ---------------------------------------------------------------- IBL

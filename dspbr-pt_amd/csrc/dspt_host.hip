@@ -173,6 +173,7 @@ struct dspt_ctx {
   int64_t sched_limit = 0, next_known = 0, limit_known = 0, alive_known = 0;
   std::deque<PassTask> tasks;
   int mode = DSPT_SCHED_STREAM;
+  int policy = DSPT_POLICY_RR;
   int pass_frames = 0;
   int chunk_iters = 2;
   float ring_factor = 8.0f;
@@ -296,8 +297,7 @@ bool pool_idle(const dspt_ctx* c) {
 
 // Path pool of c->max_paths slots (reallocated only while idle).
 int ensure_pool(dspt_ctx* c) {
-  if (c->cap == c->max_paths && c->cap > 0) return 0;
-  if (!pool_idle(c)) return 0;  // keep the current pool until it drains
+  if (c->cap > 0 && (c->cap == c->max_paths || !pool_idle(c))) return 0;  // resize once idle
   HIPCHK(hipStreamSynchronize(c->stream));
   const int64_t cap = c->max_paths;
   c->state_mem.release();
@@ -380,21 +380,46 @@ void schedule(dspt_ctx* c, const std::vector<int>& ids) {
     return;
   }
   const int64_t target = c->next_known + (int64_t)(c->chunk_iters + 2) * c->cap;
-  const size_t n = ids.size();
-  bool progress = true;
-  while (c->sched_limit < target && progress) {
-    progress = false;
-    for (size_t k = 0; k < n && c->sched_limit < target; k++) {
-      int id = ids[(c->rr + k) % n];
-      if (!job_ok(c, id) || !eligible(c, c->jobs[id])) continue;
-      if (!append_frames(c, id, 1)) return;
-      progress = true;
+  if (c->policy == DSPT_POLICY_RR) {
+    const size_t n = ids.size();
+    bool progress = true;
+    while (c->sched_limit < target && progress) {
+      progress = false;
+      for (size_t k = 0; k < n && c->sched_limit < target; k++) {
+        int id = ids[(c->rr + k) % n];
+        if (!job_ok(c, id) || !eligible(c, c->jobs[id])) continue;
+        if (!append_frames(c, id, 1)) return;
+        progress = true;
+      }
+      c->rr = (c->rr + 1) % n;
     }
-    c->rr = (c->rr + 1) % n;
+    return;
+  }
+  // FIFO (listed order) or least remaining work first: whole runs of frames
+  // of one job before the next, so few scenes share the caches at a time.
+  std::vector<int> order;
+  for (int id : ids)
+    if (job_ok(c, id)) order.push_back(id);
+  if (c->policy == DSPT_POLICY_SRPT)
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      const JobRes* x = c->jobs[a];
+      const JobRes* y = c->jobs[b];
+      return (int64_t)(x->frame_end - x->next_sched) * x->npix() < (int64_t)(y->frame_end - y->next_sched) * y->npix();
+    });
+  for (int id : order) {
+    JobRes* j = c->jobs[id];
+    while (c->sched_limit < target && eligible(c, j)) {
+      int64_t want = (target - c->sched_limit + j->npix() - 1) / j->npix();
+      int64_t nf = std::min<int64_t>({want, (int64_t)(j->frame_end - j->next_sched),
+                                      (int64_t)(j->acc_known + j->ring_frames - j->next_sched)});
+      if (nf <= 0 || !append_frames(c, id, (int)nf)) return;
+    }
+    if (c->sched_limit >= target) return;
   }
 }
 
 void consume_readback(dspt_ctx* c, int s, int64_t chunk) {
+  if (c->cap <= 0) return;  // the pool was released after this chunk
   const Readback& rb = c->h_rb[s];
   c->next_known = rb.ctl.next_item;
   c->alive_known = c->cap - rb.ctl.free_count;
@@ -453,14 +478,16 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
       k_regen<<<shade_blocks, 256, 0, c->stream>>>(ps, c->d_jobs, c->d_tasks, c->d_sched, ctl, c->free_list, qin, c->cnt);
     });
     timed(c, DSPT_K_TRACE_EXT, [&] {
-      k_trace_ext<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, ps, qin, &ctl->qcount[a], c->spill, c->cnt);
+      k_trace_ext<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, ps, qin, &ctl->qcount[a], &ctl->ext_next, c->spill,
+                                                                c->cnt);
     });
     timed(c, DSPT_K_SHADE, [&] {
       k_shade<<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, ps, qin, &ctl->qcount[a], qout, &ctl->qcount[a ^ 1], c->shq,
                                                     &ctl->sh_count, c->free_list, &ctl->free_count, c->cnt);
     });
     timed(c, DSPT_K_TRACE_SHADOW, [&] {
-      k_trace_shadow<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, ps, c->shq, &ctl->sh_count, c->spill, c->cnt);
+      k_trace_shadow<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, ps, c->shq, &ctl->sh_count, &ctl->sh_next,
+                                                                   c->spill, c->cnt);
     });
     c->qsel ^= 1;
   }
@@ -1095,6 +1122,7 @@ int dspt_set_launch_config(dspt_ctx* c, int64_t max_paths, int32_t trace_blocks,
     if (max_paths > 0) c->max_paths = std::max<int64_t>(65536, max_paths);
     if (trace_blocks > 0) c->trace_blocks_max = trace_blocks;
     c->cap = 0;  // reallocated (with the spill area) on the next chunk
+    c->have_prev = false;
   }
   if (shade_blocks > 0) c->shade_blocks_max = shade_blocks;
   return 0;
@@ -1117,6 +1145,14 @@ int dspt_set_schedule(dspt_ctx* c, int32_t mode, int32_t pass_frames, int32_t ch
   c->pass_frames = std::max(0, pass_frames);
   if (chunk_iters > 0) c->chunk_iters = chunk_iters;
   if (ring_factor > 0.0f) c->ring_factor = std::max(1.0f, ring_factor);
+  return 0;
+}
+
+int dspt_set_policy(dspt_ctx* c, int32_t policy) {
+  if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
+  if (policy != DSPT_POLICY_RR && policy != DSPT_POLICY_FIFO && policy != DSPT_POLICY_SRPT)
+    return fail(DSPT_ERR_ARG, "unknown scheduling policy");
+  c->policy = policy;
   return 0;
 }
 

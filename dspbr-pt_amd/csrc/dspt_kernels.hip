@@ -19,6 +19,17 @@
 #include "dspt_kernels.hpp"
 #include "dev_shade.hpp"
 
+// Occupancy target of the shading kernel (waves per SIMD; 0 = compiler's
+// choice). A tuning knob for variant builds.
+#ifndef DSPT_SHADE_WAVES
+#define DSPT_SHADE_WAVES 0
+#endif
+#if DSPT_SHADE_WAVES > 0
+#define SHADE_BOUNDS __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DSPT_SHADE_WAVES, DSPT_SHADE_WAVES)))
+#else
+#define SHADE_BOUNDS __launch_bounds__(256)
+#endif
+
 // --------------------------------------------------------------------------
 // Scene preparation: per BVH index slot, the triangle's three positions in
 // leaf order (bvh.glsl:72-99 resolved once at upload).
@@ -112,6 +123,8 @@ __global__ void k_plan(StreamCtl* __restrict__ ctl, const StreamSched* __restric
   ctl->qcount[a] = Q + n;
   ctl->qcount[a ^ 1] = 0;
   ctl->sh_count = 0;
+  ctl->ext_next = 0;
+  ctl->sh_next = 0;
 }
 
 // Camera rays for the items of this refill, renderer.frag:129-142 +
@@ -182,66 +195,181 @@ DI int64_t xcd_block(int64_t b, int64_t nb) {
   return x * per + k;
 }
 
-__global__ __launch_bounds__(kTraceBlock) void k_trace_ext(const DevJob* __restrict__ jobs, PathState ps, const int32_t* __restrict__ queue,
-                                                           const int32_t* __restrict__ count_ptr, int* __restrict__ spill,
+// Lanes of a traversal wave take new rays from the queue whenever at least
+// kRefill of them are idle (one atomic per wave), so a wave is not held by
+// its slowest ray; every ray still runs the same step sequence.
+#ifndef DSPT_REFILL
+#define DSPT_REFILL 48
+#endif
+#ifndef DSPT_TRACE_DYNAMIC
+#define DSPT_TRACE_DYNAMIC 1
+#endif
+constexpr int kRefill = DSPT_REFILL;
+
+// Wave-uniform refill: idle lanes get consecutive queue entries. Returns the
+// queue index for this lane (or -1) and sets `exhausted` once the queue is
+// used up.
+DD int refill(bool active, int count, int32_t* next_ptr, bool& exhausted) {
+  const unsigned long long idle = __ballot(!active);
+  const int n_idle = __popcll(idle);
+  if (exhausted || n_idle < kRefill) return -1;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)idle) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(next_ptr, n_idle);
+  base = __shfl(base, leader);
+  if (base + n_idle >= count) exhausted = true;
+  if (active) return -1;
+  const int q = base + __popcll(idle & ((1ull << lane) - 1ull));
+  return q < count ? q : -1;
+}
+
+__global__ __launch_bounds__(kTraceBlock) void k_trace_ext(const DevJob* __restrict__ jobs, PathState ps,
+                                                           const int32_t* __restrict__ queue, const int32_t* __restrict__ count_ptr,
+                                                           int32_t* __restrict__ next_ptr, int* __restrict__ spill,
                                                            unsigned long long* __restrict__ cnt) {
   __shared__ int smem[kLdsStack * kTraceBlock];
   const int count = *count_ptr;
   LaneStack st;
   st.lds = smem + threadIdx.x;
-  int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   st.spill = spill + gtid * (kStackSize - kLdsStack);
   unsigned nv = 0, nt = 0, nr = 0;
-  int64_t nb = gridDim.x;
-  int64_t lb = xcd_block(blockIdx.x, nb);
-  for (int64_t q = lb * blockDim.x + threadIdx.x; q < count; q += nb * blockDim.x) {
-    int p = queue[q];
-    float4 o = ps.org[p], d = ps.dir[p];
-    RayHit h = traverse<false>(geo_of(jobs[ps.job[p]].S), xyz(o), xyz(d), kTFAR_MAX, st, nv, nt);
-    ps.hit[p] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
-    nr++;
+#if !DSPT_TRACE_DYNAMIC
+  {
+    const int64_t nb = gridDim.x, lb = xcd_block(blockIdx.x, nb);
+    for (int64_t q = lb * blockDim.x + threadIdx.x; q < count; q += nb * blockDim.x) {
+      const int p = queue[q];
+      const float4 o = ps.org[p], d = ps.dir[p];
+      const RayHit h = traverse<false>(geo_of(jobs[ps.job[p]].S), xyz(o), xyz(d), kTFAR_MAX, st, nv, nt);
+      ps.hit[p] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
+      nr++;
+    }
+    wave_add_u64(&cnt[4], nv);
+    wave_add_u64(&cnt[5], nt);
+    wave_add_u64(&cnt[9], nr);
+    (void)next_ptr;
+    return;
+  }
+#endif
+  bool active = false, exhausted = false;
+  int p = 0;
+  Geo G{};
+  RayState r;
+  for (;;) {
+    const int q = refill(active, count, next_ptr, exhausted);
+    if (q >= 0) {
+      p = queue[q];
+      G = geo_of(jobs[ps.job[p]].S);
+      const float4 o = ps.org[p], d = ps.dir[p];
+      ray_init(r, st, xyz(o), xyz(d), kTFAR_MAX);
+      active = true;
+    }
+    if (__ballot(active) == 0) {
+      if (exhausted) break;
+      continue;
+    }
+    if (active && trav_step<false>(G, r, st, nv, nt)) {
+      ps.hit[p] = make_float4(__int_as_float(r.h.tri), r.h.t, r.h.u, r.h.v);
+      nr++;
+      active = false;
+    }
   }
   wave_add_u64(&cnt[4], nv);
   wave_add_u64(&cnt[5], nt);
   wave_add_u64(&cnt[9], nr);
 }
 
-__global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(const DevJob* __restrict__ jobs, PathState ps, const int32_t* __restrict__ queue,
-                                                              const int32_t* __restrict__ count_ptr, int* __restrict__ spill,
-                                                              unsigned long long* __restrict__ cnt) {
+// Shadow rays of a path: the IBL ray (flag 1), then the point-light ray
+// (flag 2); the unoccluded contributions are summed in that order and added
+// as L += (L_ibl + L_point) * path_weight (misptdl.glsl:72-74).
+__global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(const DevJob* __restrict__ jobs, PathState ps,
+                                                              const int32_t* __restrict__ queue,
+                                                              const int32_t* __restrict__ count_ptr, int32_t* __restrict__ next_ptr,
+                                                              int* __restrict__ spill, unsigned long long* __restrict__ cnt) {
   __shared__ int smem[kLdsStack * kTraceBlock];
   const int count = *count_ptr;
   LaneStack st;
   st.lds = smem + threadIdx.x;
-  int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   st.spill = spill + gtid * (kStackSize - kLdsStack);
   unsigned nv = 0, nt = 0, nr = 0;
-  int64_t nb = gridDim.x;
-  int64_t lb = xcd_block(blockIdx.x, nb);
-  for (int64_t q = lb * blockDim.x + threadIdx.x; q < count; q += nb * blockDim.x) {
-    int p = queue[q];
-    const Geo G = geo_of(jobs[ps.job[p]].S);
-    float4 so = ps.sh_org[p];
-    int flags = __float_as_int(so.w);
-    f3 org = xyz(so);
-    f3 contrib = sp3(0.0f);
-    if (flags & 1) {
-      float4 sd = ps.sh_dir[p];
-      RayHit h = traverse<true>(G, org, xyz(sd), sd.w, st, nv, nt);
-      nr++;
-      if (h.tri < 0) contrib = xyz(ps.sh_c[p]);
+#if !DSPT_TRACE_DYNAMIC
+  {
+    const int64_t nb = gridDim.x, lb = xcd_block(blockIdx.x, nb);
+    for (int64_t q = lb * blockDim.x + threadIdx.x; q < count; q += nb * blockDim.x) {
+      const int p = queue[q];
+      const Geo G = geo_of(jobs[ps.job[p]].S);
+      const float4 so = ps.sh_org[p];
+      const int flags = __float_as_int(so.w);
+      const f3 org = xyz(so);
+      f3 contrib = sp3(0.0f);
+      if (flags & 1) {
+        const float4 sd = ps.sh_dir[p];
+        const RayHit h = traverse<true>(G, org, xyz(sd), sd.w, st, nv, nt);
+        nr++;
+        if (h.tri < 0) contrib = xyz(ps.sh_c[p]);
+      }
+      if (flags & 2) {
+        const float4 pd = ps.sh_pdir[p];
+        const RayHit h = traverse<true>(G, org, xyz(pd), pd.w, st, nv, nt);
+        nr++;
+        if (h.tri < 0) contrib = contrib + xyz(ps.sh_pc[p]);
+      }
+      const float4 L = ps.L[p];
+      const f3 w = xyz(ps.sh_pw[p]);
+      const f3 nl = xyz(L) + contrib * w;
+      ps.L[p] = make_float4(nl.x, nl.y, nl.z, L.w);
     }
-    if (flags & 2) {
-      float4 pd = ps.sh_pdir[p];
-      RayHit h = traverse<true>(G, org, xyz(pd), pd.w, st, nv, nt);
-      nr++;
-      if (h.tri < 0) contrib = contrib + xyz(ps.sh_pc[p]);
+    wave_add_u64(&cnt[10], nv);
+    wave_add_u64(&cnt[11], nt);
+    wave_add_u64(&cnt[3], nr);
+    (void)next_ptr;
+    return;
+  }
+#endif
+  bool active = false, exhausted = false;
+  int p = 0, flags = 0, phase = 0;
+  Geo G{};
+  RayState r;
+  f3 org = sp3(0.0f), contrib = sp3(0.0f);
+  for (;;) {
+    const int q = refill(active, count, next_ptr, exhausted);
+    if (q >= 0) {
+      p = queue[q];
+      G = geo_of(jobs[ps.job[p]].S);
+      const float4 so = ps.sh_org[p];
+      flags = __float_as_int(so.w);
+      org = xyz(so);
+      contrib = sp3(0.0f);
+      phase = (flags & 1) ? 0 : 1;
+      const float4 sd = phase == 0 ? ps.sh_dir[p] : ps.sh_pdir[p];
+      ray_init(r, st, org, xyz(sd), sd.w);
+      active = true;
     }
-    // L += (L_ibl + L_point) * path_weight   (misptdl.glsl:72-74)
-    float4 L = ps.L[p];
-    f3 w = xyz(ps.sh_pw[p]);
-    f3 nl = xyz(L) + contrib * w;
-    ps.L[p] = make_float4(nl.x, nl.y, nl.z, L.w);
+    if (__ballot(active) == 0) {
+      if (exhausted) break;
+      continue;
+    }
+    if (active && trav_step<true>(G, r, st, nv, nt)) {
+      nr++;
+      if (phase == 0) {
+        if (r.h.tri < 0) contrib = xyz(ps.sh_c[p]);
+      } else {
+        if (r.h.tri < 0) contrib = contrib + xyz(ps.sh_pc[p]);
+      }
+      if (phase == 0 && (flags & 2)) {
+        phase = 1;
+        const float4 pd = ps.sh_pdir[p];
+        ray_init(r, st, org, xyz(pd), pd.w);
+      } else {
+        const float4 L = ps.L[p];
+        const f3 w = xyz(ps.sh_pw[p]);
+        const f3 nl = xyz(L) + contrib * w;
+        ps.L[p] = make_float4(nl.x, nl.y, nl.z, L.w);
+        active = false;
+      }
+    }
   }
   wave_add_u64(&cnt[10], nv);
   wave_add_u64(&cnt[11], nt);
@@ -273,7 +401,7 @@ DD void atomic_inc_grouped(int32_t* addr) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_shade(const DevJob* __restrict__ jobs, PathState ps,
+__global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState ps,
                                                const int32_t* __restrict__ queue_in, const int32_t* __restrict__ count_in,
                                                int32_t* __restrict__ queue_out, int32_t* __restrict__ count_out,
                                                int32_t* __restrict__ shq, int32_t* __restrict__ count_sh,

@@ -39,7 +39,9 @@ struct StreamCtl {
   int32_t q_base;      // queue index where the refill appends
   int32_t qcount[2];   // extension-ray queues (ping-pong)
   int32_t sh_count;    // shadow-ray queue
-  int32_t pad;
+  int32_t ext_next;    // next extension-queue entry a traversal wave takes
+  int32_t sh_next;     // next shadow-queue entry a traversal wave takes
+  int32_t pad[3];
 };
 
 // Written by the host before each chunk of bounces.
@@ -82,10 +84,10 @@ __global__ void k_stream_init(StreamCtl* ctl, int32_t* free_list, int32_t P);
 __global__ void k_plan(StreamCtl* ctl, const StreamSched* sch, int a);
 __global__ void k_regen(PathState ps, const DevJob* jobs, const PassTask* tasks, const StreamSched* sch, const StreamCtl* ctl,
                         const int32_t* free_list, int32_t* queue, unsigned long long* cnt);
-__global__ void k_trace_ext(const DevJob* jobs, PathState ps, const int32_t* queue, const int32_t* count_ptr, int* spill,
-                            unsigned long long* cnt);
-__global__ void k_trace_shadow(const DevJob* jobs, PathState ps, const int32_t* queue, const int32_t* count_ptr, int* spill,
-                               unsigned long long* cnt);
+__global__ void k_trace_ext(const DevJob* jobs, PathState ps, const int32_t* queue, const int32_t* count_ptr,
+                            int32_t* next_ptr, int* spill, unsigned long long* cnt);
+__global__ void k_trace_shadow(const DevJob* jobs, PathState ps, const int32_t* queue, const int32_t* count_ptr,
+                               int32_t* next_ptr, int* spill, unsigned long long* cnt);
 __global__ void k_shade(const DevJob* jobs, PathState ps, const int32_t* queue_in, const int32_t* count_in, int32_t* queue_out,
                         int32_t* count_out, int32_t* shq, int32_t* count_sh, int32_t* free_list, int32_t* free_count,
                         unsigned long long* cnt);

@@ -69,9 +69,13 @@ SIGNATURES = {
     "dspt_get_launch_config": (_c_int, [_vp, _i64p, _i32p, _i32p]),
     "dspt_set_schedule": (_c_int, [_vp, _c_i32, _c_i32, _c_i32, ctypes.c_float]),
     "dspt_get_schedule": (_c_int, [_vp, _i32p, _i32p, _i32p, _fp]),
+    "dspt_set_policy": (_c_int, [_vp, _c_i32]),
 }
 SCHED_STREAM = 0
 SCHED_PASS = 1
+POLICY_RR = 0
+POLICY_FIFO = 1
+POLICY_SRPT = 2
 KERNEL_CLASSES = ["raygen", "trace_ext", "shade", "trace_shadow", "accumulate"]
 
 
@@ -87,12 +91,14 @@ _lib = None
 
 
 def lib():
-    """Loads libdspt.so (once). Raises if the library has not been built."""
+    """Loads libdspt.so (once; DSPT_LIBRARY may name another build of it, e.g.
+    a tuning variant). Raises if the library has not been built."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"engine library not built: {LIB_PATH} (run __graft_entry__.build())")
-        L = ctypes.CDLL(LIB_PATH)
+        path = os.environ.get("DSPT_LIBRARY") or LIB_PATH
+        if not os.path.exists(path):
+            raise RuntimeError(f"engine library not built: {path} (run __graft_entry__.build())")
+        L = ctypes.CDLL(path)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
             fn.restype = res
@@ -327,6 +333,9 @@ class Engine:
 
     def set_schedule(self, mode=SCHED_STREAM, pass_frames=0, chunk_iters=0, ring_factor=0.0):
         check(self._L.dspt_set_schedule(self.h, int(mode), int(pass_frames), int(chunk_iters), float(ring_factor)))
+
+    def set_policy(self, policy):
+        check(self._L.dspt_set_policy(self.h, int(policy)))
 
     def schedule(self):
         m, pf, ci = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)

@@ -35,8 +35,12 @@ def hipcc_path():
     raise RuntimeError("hipcc not found (ROCm is required to build the engine)")
 
 
-def build(verbose=False):
-    cmd = [hipcc_path()] + HIP_FLAGS + ["-o", LIB_PATH + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES]
+def build(verbose=False, out=None, defines=()):
+    """Compiles the library to `out` (default: the in-tree libdspt.so);
+    `defines` are extra -D tuning macros (for variant builds)."""
+    out = os.path.abspath(out or LIB_PATH)
+    cmd = ([hipcc_path()] + HIP_FLAGS + [f"-D{d}" for d in defines] + ["-o", out + ".tmp"] +
+           [os.path.join(CSRC, s) for s in SOURCES])
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, cwd=CSRC, capture_output=True, text=True)
@@ -46,9 +50,14 @@ def build(verbose=False):
         raise RuntimeError("hipcc failed:\n" + err)
     if err and verbose:
         print(err, file=sys.stderr)
-    os.replace(LIB_PATH + ".tmp", LIB_PATH)
-    return LIB_PATH
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=None)
+    ap.add_argument("-D", dest="defines", action="append", default=[])
+    a = ap.parse_args()
+    print(build(verbose=True, out=a.out, defines=a.defines))

@@ -199,6 +199,15 @@ enum { DSPT_SCHED_STREAM = 0, DSPT_SCHED_PASS = 1 };
 int dspt_set_schedule(dspt_ctx* ctx, int32_t mode, int32_t pass_frames, int32_t chunk_iters, float ring_factor);
 int dspt_get_schedule(dspt_ctx* ctx, int32_t* mode, int32_t* pass_frames, int32_t* chunk_iters, float* ring_factor);
 
+/* Which listed job's frames the STREAM schedule queues next:
+ *   DSPT_POLICY_RR    one frame per job in turn (fair progress; default);
+ *   DSPT_POLICY_FIFO  the listed order, whole runs of frames per job, the
+ *                     next job starting as soon as the pool has room (few
+ *                     scenes share the caches at a time);
+ *   DSPT_POLICY_SRPT  like FIFO, least remaining work (pixels x frames) first. */
+enum { DSPT_POLICY_RR = 0, DSPT_POLICY_FIFO = 1, DSPT_POLICY_SRPT = 2 };
+int dspt_set_policy(dspt_ctx* ctx, int32_t policy);
+
 #ifdef __cplusplus
 }
 #endif
