@@ -147,8 +147,8 @@ struct dspt_ctx {
   // path pool
   Allocs state_mem;
   int64_t cap = 0;
-  PathState ps{};
-  int32_t *qa = nullptr, *qb = nullptr, *shq = nullptr, *free_list = nullptr;
+  PathState ps[2] = {};  // alternate per bounce; hit and shadow fields shared
+  int32_t* shq = nullptr;
   int* spill = nullptr;
   int trace_blocks = 0;
   int trace_blocks_max = 2048;
@@ -167,6 +167,7 @@ struct dspt_ctx {
   int32_t* h_joblist = nullptr;    // pinned [2][kMaxJobs]
   hipEvent_t ev_chunk[2] = {nullptr, nullptr};
   int64_t limit_at[2] = {0, 0};
+  int qsel_at[2] = {0, 0};  // state buffer holding the live paths after the chunk
   int64_t chunk_no = 0;
   bool have_prev = false;
   int qsel = 0;
@@ -302,20 +303,25 @@ int ensure_pool(dspt_ctx* c) {
   const int64_t cap = c->max_paths;
   c->state_mem.release();
   c->cap = 0;
-  PathState& ps = c->ps;
-  ps = PathState{};
+  PathState& a = c->ps[0];
+  PathState& b = c->ps[1];
+  a = PathState{};
+  b = PathState{};
   Allocs& A = c->state_mem;
   int r = 0;
-  if ((r = A.get(&ps.org, cap)) || (r = A.get(&ps.dir, cap)) || (r = A.get(&ps.hit, cap)) || (r = A.get(&ps.pw, cap)) ||
-      (r = A.get(&ps.L, cap)) || (r = A.get(&ps.rng, cap)) || (r = A.get(&ps.job, cap)) || (r = A.get(&ps.bounce, cap)) ||
-      (r = A.get(&ps.fp, cap)) || (r = A.get(&ps.sig, cap)) || (r = A.get(&ps.prim, cap)) || (r = A.get(&ps.sh_org, cap)) ||
-      (r = A.get(&ps.sh_dir, cap)) || (r = A.get(&ps.sh_c, cap)) || (r = A.get(&ps.sh_pw, cap)) ||
-      (r = A.get(&ps.sh_pdir, cap)) || (r = A.get(&ps.sh_pc, cap)) || (r = A.get(&c->qa, cap)) || (r = A.get(&c->qb, cap)) ||
-      (r = A.get(&c->shq, cap)) || (r = A.get(&c->free_list, cap)))
+  for (PathState* ps : {&a, &b})
+    if ((r = A.get(&ps->org, cap)) || (r = A.get(&ps->dir, cap)) || (r = A.get(&ps->pw, cap)) || (r = A.get(&ps->L, cap)) ||
+        (r = A.get(&ps->rng, cap)) || (r = A.get(&ps->job, cap)) || (r = A.get(&ps->bounce, cap)) || (r = A.get(&ps->fp, cap)) ||
+        (r = A.get(&ps->sig, cap)) || (r = A.get(&ps->prim, cap)))
+      return r;
+  if ((r = A.get(&a.hit, cap)) || (r = A.get(&a.sh_org, cap)) || (r = A.get(&a.sh_dir, cap)) || (r = A.get(&a.sh_c, cap)) ||
+      (r = A.get(&a.sh_pw, cap)) || (r = A.get(&a.sh_pdir, cap)) || (r = A.get(&a.sh_pc, cap)) || (r = A.get(&c->shq, cap)))
     return r;
+  b.hit = a.hit; b.sh_org = a.sh_org; b.sh_dir = a.sh_dir; b.sh_c = a.sh_c; b.sh_pw = a.sh_pw; b.sh_pdir = a.sh_pdir;
+  b.sh_pc = a.sh_pc;
   c->trace_blocks = (int)std::min<int64_t>(c->trace_blocks_max, grid_for(cap, kTraceBlock));
   if ((r = A.get(&c->spill, (size_t)c->trace_blocks * kTraceBlock * (kStackSize - kLdsStack)))) return r;
-  k_stream_init<<<grid_for(cap, 256), 256, 0, c->stream>>>(c->d_ctl, c->free_list, (int32_t)cap);
+  k_stream_init<<<1, 64, 0, c->stream>>>(c->d_ctl);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   c->cap = cap;
@@ -422,7 +428,7 @@ void consume_readback(dspt_ctx* c, int s, int64_t chunk) {
   if (c->cap <= 0) return;  // the pool was released after this chunk
   const Readback& rb = c->h_rb[s];
   c->next_known = rb.ctl.next_item;
-  c->alive_known = c->cap - rb.ctl.free_count;
+  c->alive_known = rb.ctl.qcount[c->qsel_at[s]];
   c->limit_known = c->limit_at[s];
   const int n = std::min<int>((int)c->jobs.size(), kMaxJobs);
   for (int i = 0; i < n; i++) {
@@ -449,6 +455,7 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
   PassTask* ht = c->h_tasks + (size_t)s * kMaxTasks;
   for (int i = 0; i < nt; i++) ht[i] = c->tasks[i];
   c->h_sched[s].limit = c->sched_limit;
+  c->qsel_at[s] = (c->qsel + c->chunk_iters) & 1;
   c->h_sched[s].ntasks = nt;
   c->limit_at[s] = c->sched_limit;
   int32_t* hj = c->h_joblist + (size_t)s * kMaxJobs;
@@ -467,26 +474,24 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
   const int64_t P = c->cap;
   const int shade_blocks = (int)std::min<int64_t>(grid_for(P, 256), c->shade_blocks_max);
   const int trace_blocks = c->trace_blocks;
-  PathState ps = c->ps;
   StreamCtl* ctl = c->d_ctl;
+  const int32_t P32 = (int32_t)P;
   for (int k = 0; k < c->chunk_iters; k++) {
     const int a = c->qsel;
-    int32_t* qin = a == 0 ? c->qa : c->qb;
-    int32_t* qout = a == 0 ? c->qb : c->qa;
+    const PathState cur = c->ps[a], nxt = c->ps[a ^ 1];
     timed(c, DSPT_K_RAYGEN, [&] {
-      k_plan<<<1, 64, 0, c->stream>>>(ctl, c->d_sched, a);
-      k_regen<<<shade_blocks, 256, 0, c->stream>>>(ps, c->d_jobs, c->d_tasks, c->d_sched, ctl, c->free_list, qin, c->cnt);
+      k_plan<<<1, 64, 0, c->stream>>>(ctl, c->d_sched, a, P32);
+      k_regen<<<shade_blocks, 256, 0, c->stream>>>(cur, c->d_jobs, c->d_tasks, c->d_sched, ctl, c->cnt);
     });
     timed(c, DSPT_K_TRACE_EXT, [&] {
-      k_trace_ext<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, ps, qin, &ctl->qcount[a], &ctl->ext_next, c->spill,
-                                                                c->cnt);
+      k_trace_ext<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], &ctl->ext_next, c->spill, c->cnt);
     });
     timed(c, DSPT_K_SHADE, [&] {
-      k_shade<<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, ps, qin, &ctl->qcount[a], qout, &ctl->qcount[a ^ 1], c->shq,
-                                                    &ctl->sh_count, c->free_list, &ctl->free_count, c->cnt);
+      k_shade<<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1], c->shq,
+                                                    &ctl->sh_count, c->cnt);
     });
     timed(c, DSPT_K_TRACE_SHADOW, [&] {
-      k_trace_shadow<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, ps, c->shq, &ctl->sh_count, &ctl->sh_next,
+      k_trace_shadow<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, nxt, c->shq, &ctl->sh_count, &ctl->sh_next,
                                                                    c->spill, c->cnt);
     });
     c->qsel ^= 1;

@@ -96,30 +96,26 @@ __global__ void k_fill_f4(float4* __restrict__ p, int64_t n, float4 v) {
   if (i < n) p[i] = v;
 }
 
-__global__ void k_stream_init(StreamCtl* __restrict__ ctl, int32_t* __restrict__ free_list, int32_t P) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < P) free_list[i] = P - 1 - i;
-  if (i == 0) {
+__global__ void k_stream_init(StreamCtl* __restrict__ ctl) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     StreamCtl z{};
-    z.free_count = P;
     *ctl = z;
   }
 }
 
-// One thread: sizes this bounce's refill (free slots vs scheduled items) and
-// resets the queues the bounce appends to.
-__global__ void k_plan(StreamCtl* __restrict__ ctl, const StreamSched* __restrict__ sch, int a) {
+// One thread: sizes this bounce's refill (free positions vs scheduled
+// items) and resets the queues the bounce appends to.
+__global__ void k_plan(StreamCtl* __restrict__ ctl, const StreamSched* __restrict__ sch, int a, int32_t P) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const int F = ctl->free_count, Q = ctl->qcount[a];
+  const int Q = ctl->qcount[a];
+  const int F = P - Q;
   int64_t avail = sch->limit - ctl->next_item;
   if (avail < 0) avail = 0;
   const int n = (int)(avail < (int64_t)F ? avail : (int64_t)F);
   ctl->regen_n = n;
   ctl->item0 = ctl->next_item;
-  ctl->free_base = F - n;
   ctl->q_base = Q;
   ctl->next_item += n;
-  ctl->free_count = F - n;
   ctl->qcount[a] = Q + n;
   ctl->qcount[a ^ 1] = 0;
   ctl->sh_count = 0;
@@ -131,11 +127,10 @@ __global__ void k_plan(StreamCtl* __restrict__ ctl, const StreamSched* __restric
 // rng_init (rng.glsl:23-29). Item -> task by binary search over the task
 // table (offsets ascending).
 __global__ void k_regen(PathState ps, const DevJob* __restrict__ jobs, const PassTask* __restrict__ tasks,
-                        const StreamSched* __restrict__ sch, const StreamCtl* __restrict__ ctl,
-                        const int32_t* __restrict__ free_list, int32_t* __restrict__ queue, unsigned long long* __restrict__ cnt) {
+                        const StreamSched* __restrict__ sch, const StreamCtl* __restrict__ ctl, unsigned long long* __restrict__ cnt) {
   const int n = ctl->regen_n;
   const int64_t item0 = ctl->item0;
-  const int fb = ctl->free_base, qb = ctl->q_base;
+  const int qb = ctl->q_base;
   const int ntasks = sch->ntasks;
   unsigned issued = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -165,7 +160,7 @@ __global__ void k_regen(PathState ps, const DevJob* __restrict__ jobs, const Pas
     f3 d = nrm3(f3{ux, uy, -F.focal_length});
     const float* m = F.view;
     f3 wd = f3{m[0] * d.x + m[4] * d.y + m[8] * d.z, m[1] * d.x + m[5] * d.y + m[9] * d.z, m[2] * d.x + m[6] * d.y + m[10] * d.z};
-    const int p = free_list[fb + i];
+    const int p = qb + i;
     ps.org[p] = make_float4(F.cam[0], F.cam[1], F.cam[2], 0.0f);
     ps.dir[p] = make_float4(wd.x, wd.y, wd.z, 0.0f);
     ps.pw[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -175,7 +170,6 @@ __global__ void k_regen(PathState ps, const DevJob* __restrict__ jobs, const Pas
     ps.bounce[p] = 0;
     ps.fp[p] = make_int2(frame, pix);
     if (J->rec_L) ps.sig[p] = 2166136261u;
-    queue[qb + i] = p;
     issued++;
   }
   wave_add_u64(&cnt[12], issued);
@@ -225,9 +219,8 @@ DD int refill(bool active, int count, int32_t* next_ptr, bool& exhausted) {
 }
 
 __global__ __launch_bounds__(kTraceBlock) void k_trace_ext(const DevJob* __restrict__ jobs, PathState ps,
-                                                           const int32_t* __restrict__ queue, const int32_t* __restrict__ count_ptr,
-                                                           int32_t* __restrict__ next_ptr, int* __restrict__ spill,
-                                                           unsigned long long* __restrict__ cnt) {
+                                                           const int32_t* __restrict__ count_ptr, int32_t* __restrict__ next_ptr,
+                                                           int* __restrict__ spill, unsigned long long* __restrict__ cnt) {
   __shared__ int smem[kLdsStack * kTraceBlock];
   const int count = *count_ptr;
   LaneStack st;
@@ -239,7 +232,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_ext(const DevJob* __restr
   {
     const int64_t nb = gridDim.x, lb = xcd_block(blockIdx.x, nb);
     for (int64_t q = lb * blockDim.x + threadIdx.x; q < count; q += nb * blockDim.x) {
-      const int p = queue[q];
+      const int p = (int)q;
       const float4 o = ps.org[p], d = ps.dir[p];
       const RayHit h = traverse<false>(geo_of(jobs[ps.job[p]].S), xyz(o), xyz(d), kTFAR_MAX, st, nv, nt);
       ps.hit[p] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
@@ -259,7 +252,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_ext(const DevJob* __restr
   for (;;) {
     const int q = refill(active, count, next_ptr, exhausted);
     if (q >= 0) {
-      p = queue[q];
+      p = q;
       G = geo_of(jobs[ps.job[p]].S);
       const float4 o = ps.org[p], d = ps.dir[p];
       ray_init(r, st, xyz(o), xyz(d), kTFAR_MAX);
@@ -401,17 +394,16 @@ DD void atomic_inc_grouped(int32_t* addr) {
   }
 }
 
-__global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState ps,
-                                               const int32_t* __restrict__ queue_in, const int32_t* __restrict__ count_in,
-                                               int32_t* __restrict__ queue_out, int32_t* __restrict__ count_out,
-                                               int32_t* __restrict__ shq, int32_t* __restrict__ count_sh,
-                                               int32_t* __restrict__ free_list, int32_t* __restrict__ free_count,
-                                               unsigned long long* __restrict__ cnt) {
+__global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState ps, PathState nx,
+                                   const int32_t* __restrict__ count_in, int32_t* __restrict__ count_out,
+                                   int32_t* __restrict__ shq, int32_t* __restrict__ count_sh,
+                                   unsigned long long* __restrict__ cnt) {
   const int count = *count_in;
   unsigned n_paths = 0, n_hits = 0, n_miss = 0, n_nee = 0, n_ext = 0;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < count; q += (int64_t)gridDim.x * blockDim.x) {
-    const int p = queue_in[q];
-    const DevJob* J = jobs + ps.job[p];
+    const int p = (int)q;
+    const int jid = ps.job[p];
+    const DevJob* J = jobs + jid;
     const bool rec = J->rec_L != nullptr;
     uint32_t sig = rec ? ps.sig[p] : 0u;
     float4 out;
@@ -552,30 +544,33 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
           c_pt = e * (f3{F.light_emission[0], F.light_emission[1], F.light_emission[2]} / dist2) * cosNL;
         }
       }
-      if (flags) {
-        int slot = atomicAdd(count_sh, 1);
-        shq[slot] = p;
-        ps.sh_org[p] = make_float4(pos.x, pos.y, pos.z, __int_as_float(flags));
-        ps.sh_dir[p] = make_float4(ldir.x, ldir.y, ldir.z, kTFAR_MAX);
-        ps.sh_c[p] = make_float4(c_ibl.x, c_ibl.y, c_ibl.z, 0.0f);
-        ps.sh_pw[p] = make_float4(pw.x, pw.y, pw.z, 0.0f);
-        if (flags & 2) { ps.sh_pdir[p] = pdir; ps.sh_pc[p] = make_float4(c_pt.x, c_pt.y, c_pt.z, 0.0f); }
-      }
+      const f3 pw_nee = pw;  // path weight at the time of NEE
       pw = pw * bw;
-      int slot = atomicAdd(count_out, 1);
-      queue_out[slot] = p;
+      // the path continues at position o of the next state buffer
+      const int o = atomicAdd(count_out, 1);
       n_ext++;
-      ps.org[p] = make_float4(pos.x, pos.y, pos.z, 0.0f);
-      ps.dir[p] = make_float4(wo.x, wo.y, wo.z, 0.0f);
-      ps.pw[p] = make_float4(pw.x, pw.y, pw.z, last_pdf);
-      ps.L[p] = make_float4(L.x, L.y, L.z, L4.w);
-      ps.rng[p] = make_uint2(rng.x, rng.y);
-      ps.bounce[p] = bounce + 1;
-      if (rec) ps.sig[p] = sig;
+      if (flags) {
+        const int k = atomicAdd(count_sh, 1);
+        shq[k] = o;
+        ps.sh_org[o] = make_float4(pos.x, pos.y, pos.z, __int_as_float(flags));
+        ps.sh_dir[o] = make_float4(ldir.x, ldir.y, ldir.z, kTFAR_MAX);
+        ps.sh_c[o] = make_float4(c_ibl.x, c_ibl.y, c_ibl.z, 0.0f);
+        ps.sh_pw[o] = make_float4(pw_nee.x, pw_nee.y, pw_nee.z, 0.0f);
+        if (flags & 2) { ps.sh_pdir[o] = pdir; ps.sh_pc[o] = make_float4(c_pt.x, c_pt.y, c_pt.z, 0.0f); }
+      }
+      nx.org[o] = make_float4(pos.x, pos.y, pos.z, 0.0f);
+      nx.dir[o] = make_float4(wo.x, wo.y, wo.z, 0.0f);
+      nx.pw[o] = make_float4(pw.x, pw.y, pw.z, last_pdf);
+      nx.L[o] = make_float4(L.x, L.y, L.z, L4.w);
+      nx.rng[o] = make_uint2(rng.x, rng.y);
+      nx.job[o] = jid;
+      nx.bounce[o] = bounce + 1;
+      nx.fp[o] = ps.fp[p];
+      if (rec) { nx.sig[o] = sig; nx.prim[o] = ps.prim[p]; }
       term = false;
     } while (false);
     if (term) {
-      // finished sample -> the job's frame ring (+ records); free the slot
+      // finished sample -> the job's frame ring (+ records)
       n_paths++;
       const int2 fp = ps.fp[p];
       const int rs = fp.x % J->ring_frames;
@@ -587,7 +582,6 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
         J->rec_prim[r] = ps.prim[p];
       }
       atomic_inc_grouped(J->done + rs);
-      free_list[atomicAdd(free_count, 1)] = p;
     }
   }
   wave_add_u64(&cnt[0], n_paths);

@@ -28,20 +28,20 @@ struct PassTask {
 };
 
 // Device-resident control block of the path pool. Items (job, frame, pixel)
-// are issued in id order; a slot freed by a finished path is refilled with
-// the next item on the following bounce.
+// are issued in id order. Path state is dense: the live paths of a bounce
+// occupy positions [0, qcount) of the current state buffer; the paths that
+// continue are compacted into the other buffer, and the next bounce's new
+// paths are appended behind them.
 struct StreamCtl {
   int64_t next_item;   // first item not yet issued
   int64_t item0;       // first item of the current refill
-  int32_t free_count;  // entries on the free-slot stack
   int32_t regen_n;     // items issued by the current refill
-  int32_t free_base;   // free-stack index of the first slot the refill takes
-  int32_t q_base;      // queue index where the refill appends
-  int32_t qcount[2];   // extension-ray queues (ping-pong)
+  int32_t q_base;      // position where the refill appends
+  int32_t qcount[2];   // live paths in state buffer 0 / 1
   int32_t sh_count;    // shadow-ray queue
-  int32_t ext_next;    // next extension-queue entry a traversal wave takes
+  int32_t ext_next;    // next position a traversal wave takes
   int32_t sh_next;     // next shadow-queue entry a traversal wave takes
-  int32_t pad[3];
+  int32_t pad;
 };
 
 // Written by the host before each chunk of bounces.
@@ -51,8 +51,11 @@ struct StreamSched {
   int32_t pad;
 };
 
-// Structure-of-arrays path state, capacity = pool size. Arrays are indexed
-// by slot.
+// Structure-of-arrays path state, capacity = pool size, indexed by
+// position. Two instances alternate per bounce (the per-path fields are
+// double-buffered); hit and the shadow-ray fields are shared: hit is written
+// and read at the current position, the shadow fields at the position the
+// path continues at.
 struct PathState {
   float4* org;    // extension ray origin
   float4* dir;    // extension ray direction
@@ -80,17 +83,16 @@ __global__ void k_ibl_rows(const float* f, int w, int h, float* pdf, float* cdf,
 __global__ void k_ibl_marginal(const float* rowsum, int h, float* ypdf, float* ycdf, double* total);
 __global__ void k_fill_i32(int32_t* p, int n, int32_t v);
 __global__ void k_fill_f4(float4* p, int64_t n, float4 v);
-__global__ void k_stream_init(StreamCtl* ctl, int32_t* free_list, int32_t P);
-__global__ void k_plan(StreamCtl* ctl, const StreamSched* sch, int a);
+__global__ void k_stream_init(StreamCtl* ctl);
+__global__ void k_plan(StreamCtl* ctl, const StreamSched* sch, int a, int32_t P);
 __global__ void k_regen(PathState ps, const DevJob* jobs, const PassTask* tasks, const StreamSched* sch, const StreamCtl* ctl,
-                        const int32_t* free_list, int32_t* queue, unsigned long long* cnt);
-__global__ void k_trace_ext(const DevJob* jobs, PathState ps, const int32_t* queue, const int32_t* count_ptr,
-                            int32_t* next_ptr, int* spill, unsigned long long* cnt);
+                        unsigned long long* cnt);
+__global__ void k_trace_ext(const DevJob* jobs, PathState ps, const int32_t* count_ptr, int32_t* next_ptr, int* spill,
+                            unsigned long long* cnt);
 __global__ void k_trace_shadow(const DevJob* jobs, PathState ps, const int32_t* queue, const int32_t* count_ptr,
                                int32_t* next_ptr, int* spill, unsigned long long* cnt);
-__global__ void k_shade(const DevJob* jobs, PathState ps, const int32_t* queue_in, const int32_t* count_in, int32_t* queue_out,
-                        int32_t* count_out, int32_t* shq, int32_t* count_sh, int32_t* free_list, int32_t* free_count,
-                        unsigned long long* cnt);
+__global__ void k_shade(const DevJob* jobs, PathState cur, PathState nxt, const int32_t* count_in, int32_t* count_out,
+                        int32_t* shq, int32_t* count_sh, unsigned long long* cnt);
 __global__ void k_accum_ready(const DevJob* jobs, const int32_t* job_list, const int32_t* prog);
 __global__ void k_advance(const DevJob* jobs, const int32_t* job_list, int n, int32_t* prog);
 __global__ void k_trace_rays(SceneView S, const float4* rays, int64_t n, int any, int32_t* tri, float* t, float* u, float* v,
