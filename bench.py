@@ -14,9 +14,9 @@ events on the engine stream) and the CPU baseline (the oracle on the host
 cores over a bounded sample of the same workload). Two more objects:
   batched  the uneven 32-scene mix (10k-2M triangles, 256^2-1024^2,
            16-256 spp, 1024x512 IBL each) rendered through the shared path
-           pool vs the plain one-job-at-a-time loop (frame-synchronous
-           passes), Mpaths/s and p50/p99 time-to-N-spp of each, plus the
-           reference's one-frame-per-pass loop on a bounded subset;
+           pool vs one job at a time in frame-synchronous passes of a full
+           pool and vs one job at a time with one frame per pass (the
+           reference's render loop): Mpaths/s and p50/p99 time-to-N-spp;
   serving  Poisson arrival streams over the full-material scene with
            resolution/spp mixes, admission by a device-memory budget,
            p50/p99 time-to-N-spp and time-to-first-frame against Mpaths/s.
@@ -155,9 +155,9 @@ def batched_vs_plain(args):
                     live.discard(j)
         run.ids = ids
 
-    def plain(done_at, t_start):
+    def plain(done_at, t_start, pass_frames=0):
         ids = []
-        eng.set_schedule(SCHED_PASS, 0)
+        eng.set_schedule(SCHED_PASS, pass_frames)
         for s in specs:
             j = b.create_job(s)
             eng.jobs_run([j])
@@ -182,26 +182,12 @@ def batched_vs_plain(args):
     acc_p = eng.job_read_accum(ids_p[0])
     for j in ids_p:
         eng.job_destroy(j)
-    # the reference's loop (one frame per pass) on a bounded subset
-    fsub = [s for s in specs if s.paths <= 8e6][:4] or specs[:1]
-    eng.set_schedule(SCHED_PASS, 1)
-    t1 = time.perf_counter()
-    fids = []
-    for s in fsub:
-        j = b.create_job(s)
-        eng.jobs_run([j])
-        fids.append(j)
-    eng.synchronize()
-    el_f = time.perf_counter() - t1
-    eng.set_schedule(SCHED_STREAM)
-    t2 = time.perf_counter()
-    sids = [b.create_job(s) for s in fsub]
-    eng.jobs_run(sids)
-    eng.synchronize()
-    el_fs = time.perf_counter() - t2
-    for j in fids + sids:
+    # the reference's loop structure: one job at a time, one frame per pass
+    el_f, lat_f = run(plain, pass_frames=1)
+    kms_f = run.kms
+    acc_f = eng.job_read_accum(run.ids[0])
+    for j in run.ids:
         eng.job_destroy(j)
-    fpaths = sum(s.paths for s in fsub)
     q = lambda a, p: round(float(np.percentile(a, p)) * 1e3, 1)
     res = {
         "workload": f"{len(specs)} scenes {sum(w.scene.num_triangles for w in mix)} tris, "
@@ -217,11 +203,13 @@ def batched_vs_plain(args):
         "round_robin": {"mpaths_per_s": round(total / el_r / 1e6, 2),
                         "latency_ms": {"p50": q(lat_r, 50), "p99": q(lat_r, 99)}},
         "kernel_ms": {"batched": kms_b, "round_robin": kms_r, "plain": kms_p},
-        "identical_accumulation": bool(np.array_equal(acc_b, acc_p)),
+        "plain_mode": "one job at a time, frame-synchronous passes of up to a full pool",
+        "frame_loop_mode": "one job at a time, one frame per pass (the reference's render loop)",
+        "identical_accumulation": bool(np.array_equal(acc_b, acc_p) and np.array_equal(acc_b, acc_f)),
         "autotuned": best,
-        "frame_loop_subset": {"jobs": len(fsub), "paths": fpaths,
-                              "one_frame_per_pass_mpaths_per_s": round(fpaths / el_f / 1e6, 2),
-                              "batched_mpaths_per_s": round(fpaths / el_fs / 1e6, 2)},
+        "frame_loop": {"mpaths_per_s": round(total / el_f / 1e6, 2),
+                       "latency_ms": {"p50": q(lat_f, 50), "p99": q(lat_f, 99)},
+                       "speedup_batched": round(el_f / el_b, 3), "kernel_ms": kms_f},
     }
     log(f"[bench] batched {res['batched_mpaths_per_s']} vs plain {res['plain_mpaths_per_s']} Mpaths/s")
     return res

@@ -113,24 +113,31 @@ struct Geo {
 DI Geo geo_of(const SceneView& S) { return Geo{S.nodes, S.tri, S.N, S.M}; }
 
 // Resumable traversal state of one ray (the loop variables of
-// SH/bvh.glsl:149-212).
+// SH/bvh.glsl:149-212). The entry on top of the reference's stack is held in
+// `next` when it was pushed by the previous step (has_next); the stack then
+// logically holds top + has_next entries, so capacity, visit order and step
+// count are exactly those of the reference loop.
 struct RayState {
   f3 org, dir, inv;
   RayHit h;
-  int top, step;
+  int top, step, next;
+  int leaf_first, leaf_end;  // leaf reached by trav_inner, tested by trav_leaf
+  bool has_next;
 };
 
 DI void ray_init(RayState& r, LaneStack& st, f3 org, f3 dir, float tfar) {
+  (void)st;
   r.org = org;
   r.dir = dir;
   r.inv = f3{1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
   r.h = RayHit{-1, tfar, 0.0f, 0.0f};
-  r.top = 1;
+  r.top = 0;
+  r.next = 0;  // the root
+  r.has_next = true;
   r.step = 0;
-  st.put(0, 0);
 }
 
-DI bool ray_done(const RayState& r) { return !(r.step < kMaxSteps && r.top > 0); }
+DI bool ray_done(const RayState& r) { return !(r.step < kMaxSteps && (r.top > 0 || r.has_next)); }
 
 // One iteration of the stack traversal, in the exact visit order of
 // SH/bvh.glsl:149-212: pop a node; a leaf tests its triangles, an inner
@@ -143,7 +150,13 @@ template <bool ANY_HIT>
 DI bool trav_step(const Geo& G, RayState& r, LaneStack& st, unsigned& nv, unsigned& nt) {
   if (ray_done(r)) return true;
   r.step++;
-  const int ni = st.get(--r.top);
+  int ni;
+  if (r.has_next) {
+    ni = r.next;
+    r.has_next = false;
+  } else {
+    ni = st.get(--r.top);
+  }
   if (ni < 0 || ni >= G.N) return ray_done(r);
   nv++;
   const float4* nd = G.nodes + (size_t)ni * 4;
@@ -155,10 +168,9 @@ DI bool trav_step(const Geo& G, RayState& r, LaneStack& st, unsigned& nv, unsign
     for (int i = first; i < end; i++) {
       if (i < 0) continue;
       const float4* tp = G.tri + (size_t)i * 3;
-      const float4 q0 = ldg(tp);
+      const float4 q0 = ldg(tp), q1 = ldg(tp + 1), q2 = ldg(tp + 2);
       const int id = __float_as_int(q0.w);
       if (id < 0) continue;
-      const float4 q1 = ldg(tp + 1), q2 = ldg(tp + 2);
       nt++;
       float t, u, v;
       if (tri_test(r.org, r.dir, xyz(q0), xyz(q1), xyz(q2), r.h.t, t, u, v)) {
@@ -170,19 +182,91 @@ DI bool trav_step(const Geo& G, RayState& r, LaneStack& st, unsigned& nv, unsign
     const int left = (int)a.w, right = (int)b.w;
     const float d0 = slab(xyz(a), xyz(b), r.org, r.inv, r.h.t);
     const float d1 = slab(xyz(c), xyz(d), r.org, r.inv, r.h.t);
-    auto push = [&](int child) {
-      if (r.top < kStackSize && child >= 0 && child < G.N && child != ni) st.put(r.top++, child);
-    };
-    if (d0 < kBVH_FAR && d1 < kBVH_FAR) {
-      if (d0 < d1) { push(right); push(left); }
-      else { push(left); push(right); }
-    } else if (d0 < kBVH_FAR) {
-      push(left);
-    } else if (d1 < kBVH_FAR) {
-      push(right);
+    const bool h0 = d0 < kBVH_FAR, h1 = d1 < kBVH_FAR;
+    const bool both = h0 && h1;
+    const bool near_left = d0 < d1;
+    // pushes in reference order: c1 first (the far child when both overlap,
+    // else the only one), then c2 (the near child); the last push is the
+    // next node visited, so it stays in a register
+    const int c1 = both ? (near_left ? right : left) : (h0 ? left : right);
+    const int c2 = near_left ? left : right;
+    auto ok = [&](int child) { return child >= 0 && child < G.N && child != ni; };
+    const bool p1 = (h0 || h1) && r.top < kStackSize && ok(c1);
+    if (both) {
+      if (p1) st.put(r.top++, c1);
+      if (r.top < kStackSize && ok(c2)) { r.next = c2; r.has_next = true; }
+    } else if (p1) {
+      r.next = c1;
+      r.has_next = true;
     }
   }
   return ray_done(r);
+}
+
+// The same step split in two for the wavefront loop: trav_inner pops one
+// node; an inner node pushes its children (as trav_step), a leaf is handed
+// to trav_leaf, which tests its triangles before the ray pops again, so the
+// per-ray sequence of node visits, triangle tests and steps is exactly that
+// of trav_step. Returns 0 = continue, 1 = leaf pending, 2 = ray finished.
+DI int trav_inner(const Geo& G, RayState& r, LaneStack& st, unsigned& nv) {
+  if (ray_done(r)) return 2;
+  r.step++;
+  int ni;
+  if (r.has_next) {
+    ni = r.next;
+    r.has_next = false;
+  } else {
+    ni = st.get(--r.top);
+  }
+  if (ni < 0 || ni >= G.N) return ray_done(r) ? 2 : 0;
+  nv++;
+  const float4* nd = G.nodes + (size_t)ni * 4;
+  const float4 a = ldg(nd), b = ldg(nd + 1), c = ldg(nd + 2), d = ldg(nd + 3);
+  const int cnt = (int)c.w;
+  if (cnt > 0) {
+    const int first = (int)d.w;
+    r.leaf_first = first;
+    r.leaf_end = min(first + min(cnt, kMaxLeafTris), G.M);
+    return 1;
+  }
+  const int left = (int)a.w, right = (int)b.w;
+  const float d0 = slab(xyz(a), xyz(b), r.org, r.inv, r.h.t);
+  const float d1 = slab(xyz(c), xyz(d), r.org, r.inv, r.h.t);
+  const bool h0 = d0 < kBVH_FAR, h1 = d1 < kBVH_FAR;
+  const bool both = h0 && h1;
+  const bool near_left = d0 < d1;
+  const int c1 = both ? (near_left ? right : left) : (h0 ? left : right);
+  const int c2 = near_left ? left : right;
+  auto ok = [&](int child) { return child >= 0 && child < G.N && child != ni; };
+  const bool p1 = (h0 || h1) && r.top < kStackSize && ok(c1);
+  if (both) {
+    if (p1) st.put(r.top++, c1);
+    if (r.top < kStackSize && ok(c2)) { r.next = c2; r.has_next = true; }
+  } else if (p1) {
+    r.next = c1;
+    r.has_next = true;
+  }
+  return ray_done(r) ? 2 : 0;
+}
+
+// Triangle tests of the pending leaf; with ANY_HIT returns true at the first
+// accepted triangle.
+template <bool ANY_HIT>
+DI bool trav_leaf(const Geo& G, RayState& r, unsigned& nt) {
+  for (int i = r.leaf_first; i < r.leaf_end; i++) {
+    if (i < 0) continue;
+    const float4* tp = G.tri + (size_t)i * 3;
+    const float4 q0 = ldg(tp), q1 = ldg(tp + 1), q2 = ldg(tp + 2);
+    const int id = __float_as_int(q0.w);
+    if (id < 0) continue;
+    nt++;
+    float t, u, v;
+    if (tri_test(r.org, r.dir, xyz(q0), xyz(q1), xyz(q2), r.h.t, t, u, v)) {
+      r.h.t = t; r.h.tri = id; r.h.u = u; r.h.v = v;
+      if (ANY_HIT) return true;
+    }
+  }
+  return false;
 }
 
 // Whole traversal of one ray.

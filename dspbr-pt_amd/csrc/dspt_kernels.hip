@@ -29,6 +29,12 @@
 #else
 #define SHADE_BOUNDS __launch_bounds__(256)
 #endif
+// Traversal kernels: full occupancy (8 waves per SIMD, <= 64 VGPRs) keeps
+// enough node fetches in flight.
+#ifndef DSPT_TRACE_WAVES
+#define DSPT_TRACE_WAVES 8
+#endif
+#define TRACE_BOUNDS __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(DSPT_TRACE_WAVES)))
 
 // --------------------------------------------------------------------------
 // Scene preparation: per BVH index slot, the triangle's three positions in
@@ -199,6 +205,11 @@ DI int64_t xcd_block(int64_t b, int64_t nb) {
 #define DSPT_TRACE_DYNAMIC 1
 #endif
 constexpr int kRefill = DSPT_REFILL;
+// Inner-node steps a wave takes before it tests the leaves its lanes reached.
+#ifndef DSPT_INNER_RUN
+#define DSPT_INNER_RUN 4
+#endif
+constexpr int kInnerRun = DSPT_INNER_RUN;
 
 // Wave-uniform refill: idle lanes get consecutive queue entries. Returns the
 // queue index for this lane (or -1) and sets `exhausted` once the queue is
@@ -218,7 +229,7 @@ DD int refill(bool active, int count, int32_t* next_ptr, bool& exhausted) {
   return q < count ? q : -1;
 }
 
-__global__ __launch_bounds__(kTraceBlock) void k_trace_ext(const DevJob* __restrict__ jobs, PathState ps,
+__global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathState ps,
                                                            const int32_t* __restrict__ count_ptr, int32_t* __restrict__ next_ptr,
                                                            int* __restrict__ spill, unsigned long long* __restrict__ cnt) {
   __shared__ int smem[kLdsStack * kTraceBlock];
@@ -262,10 +273,30 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_ext(const DevJob* __restr
       if (exhausted) break;
       continue;
     }
-    if (active && trav_step<false>(G, r, st, nv, nt)) {
-      ps.hit[p] = make_float4(__int_as_float(r.h.tri), r.h.t, r.h.u, r.h.v);
-      nr++;
-      active = false;
+    // inner nodes until every busy lane has reached its next leaf (or at
+    // most kInnerRun steps), then the leaves together
+    bool leaf = false;
+    for (int k = 0; k < kInnerRun; k++) {
+      const bool inner = active && !leaf;
+      if (__ballot(inner) == 0) break;
+      if (inner) {
+        const int res = trav_inner(G, r, st, nv);
+        if (res == 1) {
+          leaf = true;
+        } else if (res == 2) {
+          ps.hit[p] = make_float4(__int_as_float(r.h.tri), r.h.t, r.h.u, r.h.v);
+          nr++;
+          active = false;
+        }
+      }
+    }
+    if (leaf) {
+      trav_leaf<false>(G, r, nt);
+      if (ray_done(r)) {
+        ps.hit[p] = make_float4(__int_as_float(r.h.tri), r.h.t, r.h.u, r.h.v);
+        nr++;
+        active = false;
+      }
     }
   }
   wave_add_u64(&cnt[4], nv);
@@ -276,7 +307,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_ext(const DevJob* __restr
 // Shadow rays of a path: the IBL ray (flag 1), then the point-light ray
 // (flag 2); the unoccluded contributions are summed in that order and added
 // as L += (L_ibl + L_point) * path_weight (misptdl.glsl:72-74).
-__global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(const DevJob* __restrict__ jobs, PathState ps,
+__global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, PathState ps,
                                                               const int32_t* __restrict__ queue,
                                                               const int32_t* __restrict__ count_ptr, int32_t* __restrict__ next_ptr,
                                                               int* __restrict__ spill, unsigned long long* __restrict__ cnt) {
@@ -344,7 +375,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_shadow(const DevJob* __re
       if (exhausted) break;
       continue;
     }
-    if (active && trav_step<true>(G, r, st, nv, nt)) {
+    bool leaf = false, fin = false;
+    for (int k = 0; k < kInnerRun; k++) {
+      const bool inner = active && !leaf && !fin;
+      if (__ballot(inner) == 0) break;
+      if (inner) {
+        const int res = trav_inner(G, r, st, nv);
+        if (res == 1) leaf = true;
+        else if (res == 2) fin = true;
+      }
+    }
+    if (leaf) fin = trav_leaf<true>(G, r, nt) || ray_done(r);
+    if (active && fin) {
       nr++;
       if (phase == 0) {
         if (r.h.tri < 0) contrib = xyz(ps.sh_c[p]);
