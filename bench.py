@@ -34,6 +34,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dspbr-pt_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# Memory traffic of one k_trace_ext launch on the default workload (1080p,
+# 64 spp, full-material scene), from rocprofv3 FETCH_SIZE + WRITE_SIZE passes
+# scaled by a plain-copy calibration (profiles/k_trace_ext_counters_*.md).
+TRACE_EXT_TRAFFIC_BYTES = 11.04e9
 METRIC = "Mpaths/s + Mrays/s at 1080p; p50/p99 job latency, batched vs plain, 1 GPU"
 
 
@@ -336,6 +340,7 @@ def main():
     ks = eng.kernel_stats()
     eng.set_kernel_timing(False)
 
+    default_workload = args.workload == "full_materials" and spp == 64
     npaths = wl.width * wl.height * spp * args.steps
     rays = c["rays_primary"] + c["rays_extension"] + c["rays_shadow"]
     mpaths = npaths / elapsed / 1e6
@@ -369,7 +374,8 @@ def main():
         "counters": c,
         "kernel_ms": kernel_ms,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": TRACE_EXT_TRAFFIC_BYTES if default_workload else None,
                      "kernel": "k_trace_ext", "avg_launch_ms": round(avg_ms, 4),
                      "alg_bytes_per_launch": round(bytes_per_launch)},
     }
