@@ -33,13 +33,32 @@ static napi_value throw_dspt(napi_env env, int code) {
   return NULL;
 }
 
-static dspt_ctx* get_ctx(napi_env env, napi_value v) {
+/* An engine handle: the context plus a flag set while an async call (render,
+ * jobsStep, jobsRun) owns it; the context is not re-entrant, so other calls
+ * on the handle throw until that call settles. Both sides of the flag run on
+ * the JS thread. */
+typedef struct {
+  dspt_ctx* c;
+  int busy;
+} Handle;
+
+static Handle* get_handle(napi_env env, napi_value v) {
   void* p = NULL;
   if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
     napi_throw_type_error(env, NULL, "expected an engine handle");
     return NULL;
   }
-  return (dspt_ctx*)p;
+  return (Handle*)p;
+}
+
+static dspt_ctx* get_ctx(napi_env env, napi_value v) {
+  Handle* h = get_handle(env, v);
+  if (!h) return NULL;
+  if (h->busy) {
+    napi_throw_error(env, NULL, "engine busy: an asynchronous call on this handle has not settled");
+    return NULL;
+  }
+  return h->c;
 }
 
 static int get_typed(napi_env env, napi_value v, napi_typedarray_type want, void** data, size_t* len) {
@@ -61,7 +80,11 @@ static int64_t get_i64(napi_env env, napi_value v) {
 
 static void finalize_ctx(napi_env env, void* data, void* hint) {
   (void)env; (void)hint;
-  if (data) dspt_destroy((dspt_ctx*)data);
+  Handle* h = (Handle*)data;
+  if (h) {
+    dspt_destroy(h->c);
+    free(h);
+  }
 }
 
 /* create(profile, device) -> handle */
@@ -74,8 +97,10 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   dspt_ctx* c = NULL;
   int rc = dspt_create(profile, device, &c);
   if (rc) return throw_dspt(env, rc);
+  Handle* hd = (Handle*)calloc(1, sizeof(Handle));
+  hd->c = c;
   napi_value h;
-  CALL(env, napi_create_external(env, c, finalize_ctx, NULL, &h));
+  CALL(env, napi_create_external(env, hd, finalize_ctx, NULL, &h));
   return h;
 }
 
@@ -146,38 +171,68 @@ static napi_value js_set_frame(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+enum { OP_RENDER = 0, OP_STEP = 1, OP_RUN = 2 };
+
 typedef struct {
   napi_async_work work;
   napi_deferred deferred;
   napi_ref handle_ref;
-  dspt_ctx* c;
-  int first, count, rc;
+  Handle* h;
+  int op, first, count, rc, active;
+  int32_t* ids;
+  int nids;
   char err[512];
-} RenderJob;
+} AsyncCall;
 
-static void render_execute(napi_env env, void* data) {
+static void async_execute(napi_env env, void* data) {
   (void)env;
-  RenderJob* j = (RenderJob*)data;
-  j->rc = dspt_render(j->c, j->first, j->count);
-  if (!j->rc) j->rc = dspt_synchronize(j->c);
+  AsyncCall* j = (AsyncCall*)data;
+  dspt_ctx* c = j->h->c;
+  if (j->op == OP_RENDER) {
+    j->rc = dspt_render(c, j->first, j->count);
+    if (!j->rc) j->rc = dspt_synchronize(c);
+  } else if (j->op == OP_STEP) {
+    j->rc = dspt_jobs_step(c, j->ids, j->nids, &j->active);
+  } else {
+    j->rc = dspt_jobs_run(c, j->ids, j->nids);
+    if (!j->rc) j->rc = dspt_synchronize(c);
+  }
   if (j->rc) snprintf(j->err, sizeof j->err, "dspt error %d: %s", j->rc, dspt_last_error());
 }
 
-static void render_complete(napi_env env, napi_status status, void* data) {
-  RenderJob* j = (RenderJob*)data;
+static void async_complete(napi_env env, napi_status status, void* data) {
+  AsyncCall* j = (AsyncCall*)data;
+  j->h->busy = 0;
   napi_value v;
   if (status == napi_ok && j->rc == 0) {
-    napi_get_undefined(env, &v);
+    if (j->op == OP_STEP) napi_create_int32(env, j->active, &v);
+    else napi_get_undefined(env, &v);
     napi_resolve_deferred(env, j->deferred, v);
   } else {
     napi_value msg;
-    napi_create_string_utf8(env, j->rc ? j->err : "render cancelled", NAPI_AUTO_LENGTH, &msg);
+    napi_create_string_utf8(env, j->rc ? j->err : "call cancelled", NAPI_AUTO_LENGTH, &msg);
     napi_create_error(env, NULL, msg, &v);
     napi_reject_deferred(env, j->deferred, v);
   }
   napi_delete_reference(env, j->handle_ref);
   napi_delete_async_work(env, j->work);
+  free(j->ids);
   free(j);
+}
+
+static napi_value queue_call(napi_env env, napi_value handle, AsyncCall* j, const char* what) {
+  napi_value promise, name;
+  j->h->busy = 1;
+  if (napi_create_promise(env, &j->deferred, &promise) != napi_ok ||
+      napi_create_reference(env, handle, 1, &j->handle_ref) != napi_ok ||
+      napi_create_string_utf8(env, what, NAPI_AUTO_LENGTH, &name) != napi_ok ||
+      napi_create_async_work(env, NULL, name, async_execute, async_complete, j, &j->work) != napi_ok ||
+      napi_queue_async_work(env, j->work) != napi_ok) {
+    j->h->busy = 0;
+    napi_throw_error(env, NULL, "could not queue the asynchronous call");
+    return NULL;
+  }
+  return promise;
 }
 
 /* render(h, firstFrame, numFrames) -> Promise<void> */
@@ -185,19 +240,192 @@ static napi_value js_render(napi_env env, napi_callback_info info) {
   size_t argc = 3;
   napi_value argv[3];
   CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
-  dspt_ctx* c = get_ctx(env, argv[0]);
-  if (!c) return NULL;
-  RenderJob* j = (RenderJob*)calloc(1, sizeof(RenderJob));
-  j->c = c;
+  if (!get_ctx(env, argv[0])) return NULL;
+  AsyncCall* j = (AsyncCall*)calloc(1, sizeof(AsyncCall));
+  j->h = get_handle(env, argv[0]);
+  j->op = OP_RENDER;
   j->first = (int)get_i64(env, argv[1]);
   j->count = (int)get_i64(env, argv[2]);
-  napi_value promise, name;
-  CALL(env, napi_create_promise(env, &j->deferred, &promise));
-  CALL(env, napi_create_reference(env, argv[0], 1, &j->handle_ref));
-  CALL(env, napi_create_string_utf8(env, "dspt_render", NAPI_AUTO_LENGTH, &name));
-  CALL(env, napi_create_async_work(env, NULL, name, render_execute, render_complete, j, &j->work));
-  CALL(env, napi_queue_async_work(env, j->work));
-  return promise;
+  return queue_call(env, argv[0], j, "dspt_render");
+}
+
+/* jobsStep(h, Int32Array ids | null) -> Promise<number of unfinished jobs>;
+ * jobsRun(h, Int32Array ids | null) -> Promise<void> */
+static napi_value jobs_call(napi_env env, napi_callback_info info, int op) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (!get_ctx(env, argv[0])) return NULL;
+  AsyncCall* j = (AsyncCall*)calloc(1, sizeof(AsyncCall));
+  j->h = get_handle(env, argv[0]);
+  j->op = op;
+  napi_valuetype t = napi_null;
+  if (argc > 1) napi_typeof(env, argv[1], &t);
+  if (argc > 1 && t != napi_null && t != napi_undefined) {
+    void* d;
+    size_t l;
+    if (get_typed(env, argv[1], napi_int32_array, &d, &l)) { free(j); return NULL; }
+    j->ids = (int32_t*)malloc(l * sizeof(int32_t) + 1);
+    memcpy(j->ids, d, l * sizeof(int32_t));
+    j->nids = (int)l;
+  }
+  return queue_call(env, argv[0], j, op == OP_STEP ? "dspt_jobs_step" : "dspt_jobs_run");
+}
+static napi_value js_jobs_step(napi_env env, napi_callback_info info) { return jobs_call(env, info, OP_STEP); }
+static napi_value js_jobs_run(napi_env env, napi_callback_info info) { return jobs_call(env, info, OP_RUN); }
+
+/* sceneCreate(h, verts f32, tris u32, nodes f32, bvhIdx i32, mats f32) -> scene id */
+static napi_value js_scene_create(napi_env env, napi_callback_info info) {
+  size_t argc = 6;
+  napi_value argv[6];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 6) { napi_throw_error(env, NULL, "sceneCreate expects 6 arguments"); return NULL; }
+  dspt_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return NULL;
+  void *v, *t, *n, *b, *m;
+  size_t lv, lt, ln, lb, lm;
+  if (get_typed(env, argv[1], napi_float32_array, &v, &lv) || get_typed(env, argv[2], napi_uint32_array, &t, &lt) ||
+      get_typed(env, argv[3], napi_float32_array, &n, &ln) || get_typed(env, argv[4], napi_int32_array, &b, &lb) ||
+      get_typed(env, argv[5], napi_float32_array, &m, &lm))
+    return NULL;
+  int32_t id = -1;
+  int rc = dspt_scene_create(c, (const float*)v, (int64_t)(lv / 20), (const uint32_t*)t, (int64_t)(lt / 3), (const float*)n,
+                             (int64_t)(ln / 16), (const int32_t*)b, (int64_t)lb, (const float*)m, (int32_t)(lm / 68), &id);
+  if (rc) return throw_dspt(env, rc);
+  napi_value r;
+  CALL(env, napi_create_int32(env, id, &r));
+  return r;
+}
+
+/* iblCreate(h, rgba f32, w, h) -> {id, totalSum} */
+static napi_value js_ibl_create(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  dspt_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return NULL;
+  void* d;
+  size_t l;
+  if (get_typed(env, argv[1], napi_float32_array, &d, &l)) return NULL;
+  int w = (int)get_i64(env, argv[2]), hh = (int)get_i64(env, argv[3]);
+  if ((size_t)w * hh * 4 != l) { napi_throw_range_error(env, NULL, "IBL data length != 4*w*h"); return NULL; }
+  double tot = 0;
+  int32_t id = -1;
+  int rc = dspt_ibl_create(c, (const float*)d, w, hh, &tot, &id);
+  if (rc) return throw_dspt(env, rc);
+  napi_value obj, vid, vt;
+  CALL(env, napi_create_object(env, &obj));
+  CALL(env, napi_create_int32(env, id, &vid));
+  CALL(env, napi_create_double(env, tot, &vt));
+  CALL(env, napi_set_named_property(env, obj, "id", vid));
+  CALL(env, napi_set_named_property(env, obj, "totalSum", vt));
+  return obj;
+}
+
+/* jobCreate(h, scene, ibl, Float32Array(52) block, w, h, frames) -> job id */
+static napi_value js_job_create(napi_env env, napi_callback_info info) {
+  size_t argc = 7;
+  napi_value argv[7];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 7) { napi_throw_error(env, NULL, "jobCreate expects 7 arguments"); return NULL; }
+  dspt_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return NULL;
+  void* d;
+  size_t l;
+  if (get_typed(env, argv[3], napi_float32_array, &d, &l)) return NULL;
+  if (l != DSPT_FRAME_FLOATS) { napi_throw_range_error(env, NULL, "frame block must hold 52 floats"); return NULL; }
+  int32_t id = -1;
+  int rc = dspt_job_create(c, (int32_t)get_i64(env, argv[1]), (int32_t)get_i64(env, argv[2]), (const float*)d,
+                           (int32_t)get_i64(env, argv[4]), (int32_t)get_i64(env, argv[5]), (int32_t)get_i64(env, argv[6]), 0, &id);
+  if (rc) return throw_dspt(env, rc);
+  napi_value r;
+  CALL(env, napi_create_int32(env, id, &r));
+  return r;
+}
+
+/* jobStatus(h, id) -> [framesDone, framesTotal] */
+static napi_value js_job_status(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  dspt_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return NULL;
+  int32_t d = 0, t = 0;
+  int rc = dspt_job_status(c, (int32_t)get_i64(env, argv[1]), &d, &t);
+  if (rc) return throw_dspt(env, rc);
+  napi_value arr, vd, vt;
+  CALL(env, napi_create_array_with_length(env, 2, &arr));
+  CALL(env, napi_create_int32(env, d, &vd));
+  CALL(env, napi_create_int32(env, t, &vt));
+  CALL(env, napi_set_element(env, arr, 0, vd));
+  CALL(env, napi_set_element(env, arr, 1, vt));
+  return arr;
+}
+
+/* jobReadAccum(h, id, Float32Array out) */
+static napi_value js_job_read_accum(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  dspt_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return NULL;
+  void* d;
+  size_t l;
+  if (get_typed(env, argv[2], napi_float32_array, &d, &l)) return NULL;
+  int rc = dspt_job_read_accum(c, (int32_t)get_i64(env, argv[1]), (float*)d);
+  if (rc) return throw_dspt(env, rc);
+  return NULL;
+}
+
+/* small setters: jobDestroy(h, id), sceneDestroy(h, id), iblDestroy(h, id),
+ * setMemoryBudget(h, bytes), setLaunchConfig(h, maxPaths, traceBlocks,
+ * shadeBlocks), setSchedule(h, mode, passFrames, chunkIters, ringFactor),
+ * setPolicy(h, policy) */
+enum { S_JOB_DESTROY, S_SCENE_DESTROY, S_IBL_DESTROY, S_BUDGET, S_LAUNCH, S_SCHEDULE, S_POLICY };
+static napi_value setter(napi_env env, napi_callback_info info, int what) {
+  size_t argc = 5;
+  napi_value argv[5];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  dspt_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return NULL;
+  int64_t a[4] = {0, 0, 0, 0};
+  double rf = 0.0;
+  for (size_t i = 1; i < argc && i < 5; i++) a[i - 1] = get_i64(env, argv[i]);
+  if (what == S_SCHEDULE && argc > 4) napi_get_value_double(env, argv[4], &rf);
+  int rc = 0;
+  switch (what) {
+    case S_JOB_DESTROY: rc = dspt_job_destroy(c, (int32_t)a[0]); break;
+    case S_SCENE_DESTROY: rc = dspt_scene_destroy(c, (int32_t)a[0]); break;
+    case S_IBL_DESTROY: rc = dspt_ibl_destroy(c, (int32_t)a[0]); break;
+    case S_BUDGET: rc = dspt_set_memory_budget(c, a[0]); break;
+    case S_LAUNCH: rc = dspt_set_launch_config(c, a[0], (int32_t)a[1], (int32_t)a[2]); break;
+    case S_SCHEDULE: rc = dspt_set_schedule(c, (int32_t)a[0], (int32_t)a[1], (int32_t)a[2], (float)rf); break;
+    case S_POLICY: rc = dspt_set_policy(c, (int32_t)a[0]); break;
+  }
+  if (rc) return throw_dspt(env, rc);
+  return NULL;
+}
+static napi_value js_job_destroy(napi_env e, napi_callback_info i) { return setter(e, i, S_JOB_DESTROY); }
+static napi_value js_scene_destroy(napi_env e, napi_callback_info i) { return setter(e, i, S_SCENE_DESTROY); }
+static napi_value js_ibl_destroy(napi_env e, napi_callback_info i) { return setter(e, i, S_IBL_DESTROY); }
+static napi_value js_set_budget(napi_env e, napi_callback_info i) { return setter(e, i, S_BUDGET); }
+static napi_value js_set_launch(napi_env e, napi_callback_info i) { return setter(e, i, S_LAUNCH); }
+static napi_value js_set_schedule(napi_env e, napi_callback_info i) { return setter(e, i, S_SCHEDULE); }
+static napi_value js_set_policy(napi_env e, napi_callback_info i) { return setter(e, i, S_POLICY); }
+
+/* memoryUsage(h) -> bytes */
+static napi_value js_memory_usage(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  dspt_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return NULL;
+  int64_t b = 0;
+  int rc = dspt_memory_usage(c, &b);
+  if (rc) return throw_dspt(env, rc);
+  napi_value r;
+  CALL(env, napi_create_double(env, (double)b, &r));
+  return r;
 }
 
 /* readAccum(h, Float32Array out) */
@@ -280,6 +508,21 @@ static napi_value init(napi_env env, napi_value exports) {
       {"counters", NULL, js_counters, NULL, NULL, NULL, napi_enumerable, NULL},
       {"buildBvh", NULL, js_build_bvh, NULL, NULL, NULL, napi_enumerable, NULL},
       {"version", NULL, js_version, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"sceneCreate", NULL, js_scene_create, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"sceneDestroy", NULL, js_scene_destroy, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"iblCreate", NULL, js_ibl_create, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"iblDestroy", NULL, js_ibl_destroy, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"jobCreate", NULL, js_job_create, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"jobDestroy", NULL, js_job_destroy, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"jobStatus", NULL, js_job_status, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"jobReadAccum", NULL, js_job_read_accum, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"jobsStep", NULL, js_jobs_step, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"jobsRun", NULL, js_jobs_run, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setMemoryBudget", NULL, js_set_budget, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setLaunchConfig", NULL, js_set_launch, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setSchedule", NULL, js_set_schedule, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"setPolicy", NULL, js_set_policy, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"memoryUsage", NULL, js_memory_usage, NULL, NULL, NULL, napi_enumerable, NULL},
   };
   napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
   return exports;

@@ -34,3 +34,19 @@ export declare class PathtracingRenderer {
   counters(): Float64Array;
 }
 export declare function packFrameBlock(r: PathtracingRenderer, camera: PathtracingCamera, frameCount: number): Float32Array;
+
+export interface RenderRequest {
+  scene: number; ibl: number; camera: PathtracingCamera; width: number; height: number; spp: number;
+  maxBounces?: number; useIBL?: boolean; showBackground?: boolean; backgroundColor?: number[];
+  iblRotation?: number; rayEps?: number; clampThreshold?: number;
+  onProgress?: (framesDone: number, framesTotal: number) => void;
+}
+export interface RenderResult { accumulation: Float32Array; width: number; height: number; frames: number; latencyMs: number; }
+export declare class RenderServer {
+  constructor(options?: { materialProfile?: 'webgl-lean' | 'webgl-full'; device?: number; memoryBudget?: number;
+                          policy?: 'rr' | 'fifo' | 'srpt' });
+  addScene(scene: SceneBuffers): number;
+  addIbl(texture: { image: { data: Float32Array; width: number; height: number } }): number;
+  submit(request: RenderRequest): Promise<RenderResult>;
+  memoryUsage(): number;
+}

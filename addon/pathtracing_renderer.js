@@ -135,4 +135,116 @@ class PathtracingRenderer {
   counters() { return native.counters(this._h); }
 }
 
-module.exports = { PathtracingRenderer, packFrameBlock, native };
+const POLICIES = { rr: 0, fifo: 1, srpt: 2 };
+const ERR_MEMORY = -4;
+
+function errorCode(e) {
+  const m = /dspt error (-?\d+)/.exec(e && e.message ? e.message : '');
+  return m ? parseInt(m[1], 10) : 0;
+}
+
+// Continuous-batching render server over scenes kept resident in HBM (no
+// counterpart in the reference, which renders one scene at a time). Requests
+// are admitted while the engine's memory budget allows (a refused request
+// waits for a running one to finish), share the engine's path pool with the
+// requests already running, report per-frame progress and resolve with the
+// linear RGBA32F accumulation of frames 1..spp. One loop drives the engine;
+// the JS thread stays free while the GPU works (jobsStep is asynchronous).
+class RenderServer {
+  constructor({ materialProfile = 'webgl-full', device = 0, memoryBudget = 0, policy = 'rr' } = {}) {
+    this._h = native.create(materialProfile === 'webgl-full' ? 1 : 0, device);
+    if (memoryBudget) native.setMemoryBudget(this._h, memoryBudget);
+    if (!(policy in POLICIES)) throw new Error(`unknown policy ${policy}`);
+    native.setPolicy(this._h, POLICIES[policy]);
+    this._scenes = new Map();
+    this._ibls = new Map();
+    this._pending = [];
+    this._live = new Map();
+    this._loop = null;
+  }
+
+  addScene(scene) {
+    let nodes = scene.bvhNodes, idx = scene.bvhTriangleIndices;
+    if (!nodes) {
+      const r = native.buildBvh(scene.bvhPositionBuffer, scene.bvhIndexBuffer, 4);
+      nodes = r.nodeData; idx = r.triangleIndices;
+    }
+    const mats = new Float32Array(scene.materials.length * 68);
+    scene.materials.forEach((m, i) => mats.set(m.data, i * 68));
+    const id = native.sceneCreate(this._h, scene.vertexBuffer, scene.triangleIndexBuffer, nodes, idx, mats);
+    this._scenes.set(id, { counts: [scene.triangleIndexBuffer.length / 3, nodes.length / 16, idx.length],
+                           lights: scene.lights || [] });
+    return id;
+  }
+
+  addIbl(texture) {
+    const img = texture.image;
+    const r = native.iblCreate(this._h, img.data, img.width, img.height);
+    this._ibls.set(r.id, { size: [img.width, img.height], totalSum: r.totalSum });
+    return r.id;
+  }
+
+  // req: { scene, ibl (id or -1), camera, width, height, spp, maxBounces, useIBL, showBackground,
+  //        backgroundColor, iblRotation (degrees), rayEps, clampThreshold, onProgress(done, total) }
+  submit(req) {
+    return new Promise((resolve, reject) => {
+      this._pending.push({ req, resolve, reject, arrival: Date.now() });
+      if (!this._loop) this._loop = this._run().finally(() => { this._loop = null; });
+    });
+  }
+
+  _block(req) {
+    const sc = this._scenes.get(req.scene);
+    const ib = req.ibl >= 0 ? this._ibls.get(req.ibl) : { size: [0, 0], totalSum: 0 };
+    const useIBL = req.useIBL !== undefined ? req.useIBL : req.ibl >= 0;
+    const settings = {
+      _backgroundColor: req.backgroundColor || [0, 0, 0, 1], renderRes: [req.width, req.height],
+      _iblSize: ib.size, _useIBL: useIBL,
+      _iblRotation: (req.iblRotation || 0) / 180.0 * Math.PI,
+      _showBackground: req.showBackground !== undefined ? req.showBackground : useIBL,
+      _maxBounces: req.maxBounces || 8, _iblTotalSum: ib.totalSum,
+      _rayEps: req.rayEps !== undefined ? req.rayEps : 0.0001,
+      _clampThreshold: req.clampThreshold !== undefined ? req.clampThreshold : 3.0,
+      _sceneCounts: sc.counts, _scene: { lights: sc.lights },
+    };
+    return packFrameBlock(settings, req.camera, 1);
+  }
+
+  async _run() {
+    while (this._pending.length || this._live.size) {
+      // admission in arrival order while memory allows
+      while (this._pending.length) {
+        const p = this._pending[0];
+        let id;
+        try {
+          id = native.jobCreate(this._h, p.req.scene, p.req.ibl, this._block(p.req), p.req.width, p.req.height, p.req.spp);
+        } catch (e) {
+          if (errorCode(e) === ERR_MEMORY && this._live.size) break;
+          this._pending.shift();
+          p.reject(e);
+          continue;
+        }
+        this._pending.shift();
+        this._live.set(id, p);
+      }
+      if (!this._live.size) continue;
+      await native.jobsStep(this._h, Int32Array.from(this._live.keys()));
+      for (const [id, p] of Array.from(this._live)) {
+        const [done, total] = native.jobStatus(this._h, id);
+        if (p.req.onProgress) p.req.onProgress(done, total);
+        if (done >= total) {
+          const out = new Float32Array(p.req.width * p.req.height * 4);
+          native.jobReadAccum(this._h, id, out);
+          native.jobDestroy(this._h, id);
+          this._live.delete(id);
+          p.resolve({ accumulation: out, width: p.req.width, height: p.req.height, frames: total,
+                      latencyMs: Date.now() - p.arrival });
+        }
+      }
+    }
+  }
+
+  memoryUsage() { return native.memoryUsage(this._h); }
+}
+
+module.exports = { PathtracingRenderer, RenderServer, packFrameBlock, native };

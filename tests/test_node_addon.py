@@ -83,3 +83,41 @@ def test_node_render_matches_python_engine(addon_built, tmp_path):
                                    ibl_total_sum=tot, scene_counts=(s.num_triangles, len(nodes) // 16, len(bidx), 0)))
     eng.render(1, F)
     np.testing.assert_array_equal(eng.read_accum().view(np.uint32), node_acc.view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_node_render_server_matches_python_jobs(addon_built, tmp_path):
+    """Requests served by the Node RenderServer (continuous batching, staggered
+    arrivals) produce the same images as the same jobs run from Python."""
+    from dspt import Engine, PROFILE_LEAN, build_bvh, pack_frame_block, scenes
+    wl = scenes.sphere_field(n_spheres=16)
+    s = wl.scene
+    img = wl.ibl["image"]
+    np.asarray(s.vertexBuffer, np.float32).tofile(tmp_path / "verts.f32")
+    np.asarray(s.triangleIndexBuffer, np.uint32).tofile(tmp_path / "tris.u32")
+    np.asarray(s.bvhPositionBuffer, np.float32).tofile(tmp_path / "bvhpos.f32")
+    s.getFlatMaterialBuffer().astype(np.float32).tofile(tmp_path / "mats.f32")
+    np.asarray(img["data"], np.float32).tofile(tmp_path / "ibl.f32")
+    reqs = [{"width": 64, "height": 48, "spp": 5, "delayMs": 0}, {"width": 40, "height": 40, "spp": 3, "delayMs": 5},
+            {"width": 96, "height": 32, "spp": 4, "delayMs": 10}]
+    meta = {"profile": "webgl-lean", "policy": "rr", "maxBounces": 6, "iblWidth": img["width"], "iblHeight": img["height"],
+            "fov": wl.camera.fov, "near": wl.camera.near, "matrixWorld": [float(x) for x in wl.camera.matrixWorld],
+            "position": dict(zip("xyz", wl.camera.position)), "requests": reqs}
+    (tmp_path / "meta.json").write_text(json.dumps(meta))
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "node_serve.js"), str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    assert info["frames"] == [q["spp"] for q in reqs] and info["progress"] == info["frames"]
+    nodes, bidx = build_bvh(s.bvhPositionBuffer, s.bvhIndexBuffer)
+    eng = Engine(PROFILE_LEAN)
+    sid = eng.scene_create(s.vertexBuffer, s.triangleIndexBuffer, nodes, bidx, s.getFlatMaterialBuffer())
+    iid, tot = eng.ibl_create(img["data"], img["width"], img["height"])
+    for i, q in enumerate(reqs):
+        blk = pack_frame_block(wl.camera, q["width"], q["height"], max_bounces=6, ibl_size=(img["width"], img["height"]),
+                               ibl_total_sum=tot, scene_counts=(s.num_triangles, len(nodes) // 16, len(bidx), 0))
+        j = eng.job_create(sid, iid, blk, q["width"], q["height"], q["spp"])
+        eng.jobs_run([j])
+        ref = eng.job_read_accum(j)
+        got = np.fromfile(tmp_path / f"accum{i}.f32", np.float32).reshape(q["height"], q["width"], 4)
+        np.testing.assert_array_equal(got, ref)
