@@ -21,6 +21,8 @@ struct SceneView {
   const float* __restrict__ cdf;
   const float* __restrict__ ypdf;
   const float* __restrict__ ycdf;
+  const int32_t* __restrict__ yguide;  // guide tables of ycdf (h+1) and of each cdf row (w+1 per row)
+  const int32_t* __restrict__ guide;
   int32_t ew, eh;
   int32_t has_ibl;  // 0: no IBL uploaded; radiance and pdf lookups return 0
   int32_t full_profile;
@@ -291,6 +293,30 @@ DI int lower_bound_f(const float* __restrict__ row, int size, float value) {
   }
   return first;
 }
+// lower_bound over [lo, hi) only: equals lower_bound_f over the whole
+// non-decreasing row whenever the answer lies in [lo, hi].
+DI int lower_bound_range(const float* __restrict__ row, int lo, int hi, float value) {
+  int count = hi - lo, first = lo;
+  while (count > 0) {
+    int step = count / 2;
+    int idx = first + step;
+    if (row[idx] < value) { first = idx + 1; count -= step + 1; }
+    else count = step;
+  }
+  return first;
+}
+// Guide table of a non-decreasing CDF row of n entries: guide[g] =
+// lower_bound(row, g / n) for g = 0..n. For a value in bucket g the answer
+// lies between guide[g - 2] and guide[g + 3] (two buckets of margin cover
+// any rounding of value * n), so the search runs over a few entries and
+// returns exactly lower_bound_f(row, n, value).
+DI int lower_bound_guided(const float* __restrict__ row, const int32_t* __restrict__ guide, int n, float value) {
+  int g = (int)(value * (float)n);
+  g = g < 0 ? 0 : (g >= n ? n - 1 : g);
+  const int lo = guide[g >= 2 ? g - 2 : 0];
+  const int hi = guide[g + 3 <= n ? g + 3 : n];
+  return lower_bound_range(row, lo, hi, value);
+}
 DI f3 rotate_phi(f3 d, float rot, bool inverse) {
   float a = inverse ? -rot : rot;
   float c = cosf(a), s = sinf(a);
@@ -343,11 +369,11 @@ DI float ibl_pdf(const SceneView& S, const FrameParams& F, f3 d) {
 DI f3 ibl_sample(const SceneView& S, const FrameParams& F, float r0, float r1, float& pdf) {
   int w = (int)F.ibl_res[0], h = (int)F.ibl_res[1];
   if (!S.has_ibl || w <= 0 || h <= 0) { pdf = 0.0f; return f3{0.0f, 1.0f, 0.0f}; }
-  int y = lower_bound_f(S.ycdf, h, r1);
+  int y = lower_bound_guided(S.ycdf, S.yguide, h, r1);
   float py = y < h ? S.ypdf[y] : 0.0f;
   y = y >= h ? h - 1 : y;
   const float* crow = S.cdf + (size_t)y * w;
-  int x = lower_bound_f(crow, w, r0);
+  int x = lower_bound_guided(crow, S.guide + (size_t)y * (w + 1), w, r0);
   float px = x < w ? S.pdf[(size_t)y * w + x] : 0.0f;
   x = x >= w ? w - 1 : x;
   float spdf = F.ibl_res[0] * F.ibl_res[1] * py * px;

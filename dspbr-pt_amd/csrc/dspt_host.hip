@@ -90,6 +90,7 @@ struct IblRes {
   Allocs mem;
   float4* env = nullptr;
   float *pdf = nullptr, *cdf = nullptr, *ypdf = nullptr, *ycdf = nullptr;
+  int32_t *guide = nullptr, *yguide = nullptr;
   int w = 0, h = 0;
   double total = 0.0;
   int users = 0;
@@ -258,6 +259,7 @@ SceneView view_of(dspt_ctx* c, const JobRes* j) {
   const IblRes* ib = (j->ibl >= 0 && j->ibl < (int)c->ibls.size()) ? c->ibls[j->ibl] : nullptr;
   if (ib) {
     S.env = ib->env; S.pdf = ib->pdf; S.cdf = ib->cdf; S.ypdf = ib->ypdf; S.ycdf = ib->ycdf;
+    S.guide = ib->guide; S.yguide = ib->yguide;
     S.ew = ib->w; S.eh = ib->h; S.has_ibl = 1;
   }
   S.full_profile = c->profile == DSPT_PROFILE_FULL;
@@ -632,7 +634,7 @@ int make_ibl(dspt_ctx* c, const float* rgba, int32_t w, int32_t h, IblRes** out)
   Allocs& A = b->mem;
   int r = 0;
   if ((r = A.get(&b->env, n)) || (r = A.get(&b->pdf, n)) || (r = A.get(&b->cdf, n)) || (r = A.get(&b->ypdf, (size_t)h)) ||
-      (r = A.get(&b->ycdf, (size_t)h))) {
+      (r = A.get(&b->ycdf, (size_t)h)) || (r = A.get(&b->guide, (size_t)h * (w + 1))) || (r = A.get(&b->yguide, (size_t)h + 1))) {
     A.release();
     delete b;
     return r;
@@ -647,6 +649,7 @@ int make_ibl(dspt_ctx* c, const float* rgba, int32_t w, int32_t h, IblRes** out)
   k_ibl_luminance<<<grid_for((int64_t)n, 256), 256, 0, c->stream>>>(b->env, (int64_t)n, lum);
   k_ibl_rows<<<grid_for(h, 64), 64, 0, c->stream>>>(lum, w, h, b->pdf, b->cdf, rowsum);
   k_ibl_marginal<<<1, 64, 0, c->stream>>>(rowsum, h, b->ypdf, b->ycdf, total);
+  k_ibl_guides<<<grid_for((int64_t)h * (w + 1) + h + 1, 256), 256, 0, c->stream>>>(b->cdf, b->ycdf, w, h, b->guide, b->yguide);
   HIPCHK(hipGetLastError());
   double tot = 0;
   HIPCHK(hipMemcpyAsync(&tot, total, 8, hipMemcpyDeviceToHost, c->stream));

@@ -93,6 +93,21 @@ __global__ void k_ibl_marginal(const float* __restrict__ rowsum, int h, float* _
 }
 
 // --------------------------------------------------------------------------
+// Guide tables for the IBL sampling searches: row r < rows is the cdf row r
+// (n = w entries), row `rows` is the marginal (n = h entries).
+__global__ void k_ibl_guides(const float* __restrict__ cdf, const float* __restrict__ ycdf, int w, int h,
+                             int32_t* __restrict__ guide, int32_t* __restrict__ yguide) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nrow = (int64_t)h * (w + 1);
+  if (i < nrow) {
+    const int r = (int)(i / (w + 1)), g = (int)(i - (int64_t)r * (w + 1));
+    guide[i] = lower_bound_f(cdf + (size_t)r * w, w, (float)g / (float)w);
+  } else if (i < nrow + h + 1) {
+    const int g = (int)(i - nrow);
+    yguide[g] = lower_bound_f(ycdf, h, (float)g / (float)h);
+  }
+}
+
 __global__ void k_fill_i32(int32_t* __restrict__ p, int n, int32_t v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;

@@ -81,6 +81,7 @@ __global__ void k_prep_tris(const int32_t* bvh_idx, const uint32_t* tris, const 
 __global__ void k_ibl_luminance(const float4* env, int64_t n, float* f);
 __global__ void k_ibl_rows(const float* f, int w, int h, float* pdf, float* cdf, float* rowsum);
 __global__ void k_ibl_marginal(const float* rowsum, int h, float* ypdf, float* ycdf, double* total);
+__global__ void k_ibl_guides(const float* cdf, const float* ycdf, int w, int h, int32_t* guide, int32_t* yguide);
 __global__ void k_fill_i32(int32_t* p, int n, int32_t v);
 __global__ void k_fill_f4(float4* p, int64_t n, float4 v);
 __global__ void k_stream_init(StreamCtl* ctl);
