@@ -9,6 +9,14 @@
 //   verts  float4[5*V]  20-float vertex stream, unchanged
 //   tris   u32[3*T], mats float4[17*K]
 //   env    float4[w*h], pdf/cdf f32[w*h], ypdf/ycdf f32[h]
+// One RGBA8 texture array: layers of w x h texels, row 0 first, packed
+// R | G << 8 | B << 16 | A << 24 (scene_data_webgl2.ts:233-300 uploads the
+// same bytes with texImage3D).
+struct TexArrayDesc {
+  int64_t offset;  // first texel in SceneView::texels
+  int32_t w, h, layers, pad;
+};
+
 struct SceneView {
   const float4* __restrict__ nodes;
   const float4* __restrict__ tri;
@@ -26,6 +34,10 @@ struct SceneView {
   int32_t ew, eh;
   int32_t has_ibl;  // 0: no IBL uploaded; radiance and pdf lookups return 0
   int32_t full_profile;
+  const float4* __restrict__ texinfo;       // 2 per texture (TexInfo, material.ts:483-523)
+  const TexArrayDesc* __restrict__ texarr;
+  const uint32_t* __restrict__ texels;
+  int32_t num_tex, num_texarr;
 };
 
 // The 52-float uniform block (renderer.frag:10-31).
@@ -348,6 +360,44 @@ DI f3 env_lookup(const SceneView& S, float2 uv) {
   f3 top = t00 * (1.0f - a) + t10 * a;
   f3 bot = t01 * (1.0f - a) + t11 * a;
   return top * (1.0f - b) + bot * b;
+}
+// evaluateMaterialTextureValue (scene_data_webgl2.ts:325-340):
+// texture(array, vec3(uv * scale + offset, layer)) with GL LINEAR + REPEAT on
+// RGBA8 UNORM (texel / 255, bilinear in f32, as the environment lookup). The
+// layer is round-to-nearest and clamped; an array index out of range gives
+// vec4(1) like the generated if-chain. uv set 1 is never written by the
+// reference's render state (renderer.frag:72-91), taken as (0, 0).
+DI float unorm8(uint32_t v, int k) { return float((v >> (8 * k)) & 0xFFu) / 255.0f; }
+DI float4 unorm8x4(uint32_t v) { return make_float4(unorm8(v, 0), unorm8(v, 1), unorm8(v, 2), unorm8(v, 3)); }
+DI float4 tex_value(const SceneView& S, float id, float2 uv0) {
+  const int idx = (int)id;
+  const float4 t0 = ldg(S.texinfo + 2 * (size_t)idx), t1 = ldg(S.texinfo + 2 * (size_t)idx + 1);
+  const int ai = (int)t0.z;
+  if (ai < 0 || ai >= S.num_texarr) return make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+  const float2 uv = t1.z > 0.5f ? make_float2(0.0f, 0.0f) : uv0;
+  const float tu = uv.x * t1.x + t0.x, tv = uv.y * t1.y + t0.y;
+  const TexArrayDesc A = S.texarr[ai];
+  const int w = A.w, h = A.h;
+  float lf = floorf(t0.w + 0.5f);
+  lf = gmin(gmax(lf, 0.0f), float(A.layers - 1));
+  const int layer = (int)lf;
+  const float x = tu * float(w) - 0.5f, y = tv * float(h) - 0.5f;
+  float fx = floorf(x), fy = floorf(y);
+  const float a = x - fx, b = y - fy;
+  fx = gmin(gmax(fx, -1.0e9f), 1.0e9f);  // keeps the integer texel index in range
+  fy = gmin(gmax(fy, -1.0e9f), 1.0e9f);
+  const int ix = (int)fx, iy = (int)fy;
+  int x0 = ix % w; if (x0 < 0) x0 += w;
+  int x1 = (ix + 1) % w; if (x1 < 0) x1 += w;
+  int y0 = iy % h; if (y0 < 0) y0 += h;
+  int y1 = (iy + 1) % h; if (y1 < 0) y1 += h;
+  const uint32_t* L = S.texels + A.offset + (int64_t)layer * w * h;
+  const float4 t00 = unorm8x4(L[(int64_t)y0 * w + x0]), t10 = unorm8x4(L[(int64_t)y0 * w + x1]);
+  const float4 t01 = unorm8x4(L[(int64_t)y1 * w + x0]), t11 = unorm8x4(L[(int64_t)y1 * w + x1]);
+  const float ia = 1.0f - a, ib = 1.0f - b;
+  const float4 top = make_float4(t00.x * ia + t10.x * a, t00.y * ia + t10.y * a, t00.z * ia + t10.z * a, t00.w * ia + t10.w * a);
+  const float4 bot = make_float4(t01.x * ia + t11.x * a, t01.y * ia + t11.y * a, t01.z * ia + t11.z * a, t01.w * ia + t11.w * a);
+  return make_float4(top.x * ib + bot.x * b, top.y * ib + bot.y * b, top.z * ib + bot.z * b, top.w * ib + bot.w * b);
 }
 DI f3 ibl_eval(const SceneView& S, const FrameParams& F, f3 d) {
   if (!S.has_ibl) return sp3(0.0f);

@@ -10,6 +10,7 @@ struct Hitpoint {
   float tw;
   f3 vc;
   float vcw;
+  float2 uv0;  // compute_interpolated_uv(set 0), bvh.glsl:108-115
   uint32_t mat;
 };
 
@@ -36,6 +37,8 @@ DI Hitpoint fetch_hitpoint(const SceneView& S, f3 org, f3 dir, int tri, float t,
     hp.t = nrm3(cross3(hp.n, f3{-hp.n.z, hp.n.x, -hp.n.y}));
     hp.tw = 1.0f;
   }
+  float4 a0 = v0[2], a1 = v1[2], a2 = v2[2];
+  hp.uv0 = make_float2(w0 * a0.x + u * a1.x + v * a2.x, w0 * a0.y + u * a1.y + v * a2.y);
   float4 c0 = v0[4], c1 = v1[4], c2 = v2[4];
   hp.vc = w0 * xyz(c0) + u * xyz(c1) + v * xyz(c2);
   hp.vcw = w0 * c0.w + u * c1.w + v * c2.w;
@@ -53,7 +56,40 @@ struct Shading {
   float attDist;
 };
 
-// configure_gltf_material (material.glsl:5-110) for untextured materials.
+DI f3 to_linear(float4 t) { return f3{powf(t.x, 2.2f), powf(t.y, 2.2f), powf(t.z, 2.2f)}; }
+
+// Material texture slots in the order configure_gltf_material reads them.
+enum TexSlot {
+  TS_BASE = 0, TS_TRANSMISSION, TS_DIFFTRANS, TS_DIFFTRANS_COLOR, TS_METAL_ROUGH, TS_ANISO, TS_SPEC_COLOR, TS_SPEC,
+  TS_SHEEN_COLOR, TS_SHEEN_ROUGH, TS_NORMAL, TS_ANISO_DIR, TS_EMISSIVE, TS_CLEARCOAT, TS_CLEARCOAT_ROUGH, TS_IRID,
+  TS_IRID_THICK, TS_COUNT
+};
+DI float tex_id(float4 m9, float4 m11, float4 m12, float4 m13, float4 m14, float4 m15, int k) {
+  switch (k) {
+    case TS_BASE: return m11.x;
+    case TS_TRANSMISSION: return m12.z;
+    case TS_DIFFTRANS: return m9.w;
+    case TS_DIFFTRANS_COLOR: return m15.w;
+    case TS_METAL_ROUGH: return m11.y;
+    case TS_ANISO: return m14.x;
+    case TS_SPEC_COLOR: return m12.y;
+    case TS_SPEC: return m12.x;
+    case TS_SHEEN_COLOR: return m13.z;
+    case TS_SHEEN_ROUGH: return m13.w;
+    case TS_NORMAL: return m11.z;
+    case TS_ANISO_DIR: return m14.y;
+    case TS_EMISSIVE: return m11.w;
+    case TS_CLEARCOAT: return m12.w;
+    case TS_CLEARCOAT_ROUGH: return m13.x;
+    case TS_IRID: return m14.z;
+    default: return m14.w;  // TS_IRID_THICK
+  }
+}
+
+// configure_gltf_material (material.glsl:5-110). Every factor starts as its
+// untextured value (texture value vec4(1): to_linear_rgb(1) = 1, x * 1 = x,
+// mix(a, b, 1) = b); a material with textures then replaces the factors its
+// textures modulate, through one sampling site for all slots.
 DI Shading configure(const SceneView& S, const Hitpoint& hp, f3 wi) {
   const float4* m = S.mats + (size_t)hp.mat * 17;
   float4 m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4], m5 = m[5], m6 = m[6], m7 = m[7], m8 = m[8], m9 = m[9],
@@ -62,11 +98,6 @@ DI Shading configure(const SceneView& S, const Hitpoint& hp, f3 wi) {
   bsdf::GltfInputs& g = sh.in;
   g.baseColor = xyz(m0);
   float opacity = 1.0f;
-  float l4 = sqrtf(hp.vc.x * hp.vc.x + hp.vc.y * hp.vc.y + hp.vc.z * hp.vc.z + hp.vcw * hp.vcw);
-  if (l4 > 0.0f) { g.baseColor = g.baseColor * hp.vc; opacity = opacity * hp.vcw; }
-  sh.cutout = m2.x * opacity;
-  if (m6.w > 0.0f) sh.cutout = (sh.cutout < m6.w) ? 0.0f : 1.0f;
-  if (m6.w == 1.0f) sh.cutout = 1.0f;
   g.metallic = m0.w * 1.0f;
   g.roughness = m1.x * 1.0f;
   g.anisotropy = m1.y * 1.0f;
@@ -84,26 +115,74 @@ DI Shading configure(const SceneView& S, const Hitpoint& hp, f3 wi) {
   g.iridescenceIor = m10.y;
   g.iridescenceThickness = gmix(m10.z, m10.w, 1.0f);
   g.dispersion = m16.x;
+  sh.emissive = xyz(m5);
+  f3 n = hp.n, t = hp.t;
+  bool aniso_tex = false, aniso_dir_tex = false;
+  float2 aniso_rg = make_float2(0.0f, 0.0f);
+  f3 aniso_dir = sp3(0.0f);
+  if (S.num_tex > 0) {
+    const float4 m11 = m[11], m12 = m[12], m13 = m[13], m14 = m[14];
+    for (int k = 0; k < TS_COUNT; k++) {
+      const float id = tex_id(m9, m11, m12, m13, m14, m15, k);
+      if (!(id >= 0.0f)) continue;
+      const float4 tv = tex_value(S, id, hp.uv0);
+      switch (k) {
+        case TS_BASE: g.baseColor = xyz(m0) * to_linear(tv); opacity = tv.w; break;
+        case TS_TRANSMISSION: g.transmission = m1.w * tv.x; break;
+        case TS_DIFFTRANS: g.diffuseTransmission = m6.z * tv.x; break;
+        case TS_DIFFTRANS_COLOR: g.diffuseTransmissionColor = xyz(m15) * to_linear(tv); break;
+        case TS_METAL_ROUGH: g.metallic = m0.w * tv.z; g.roughness = m1.x * tv.y; break;
+        case TS_ANISO: g.anisotropy = m1.y * tv.z; aniso_tex = true; aniso_rg = make_float2(tv.x, tv.y); break;
+        case TS_SPEC_COLOR: g.specularColor = xyz(m3) * to_linear(tv); break;
+        case TS_SPEC: g.specular = m3.w * tv.w; break;
+        case TS_SHEEN_COLOR: g.sheenColor = xyz(m4) * to_linear(tv); break;
+        case TS_SHEEN_ROUGH: g.sheenRoughness = m4.w * tv.x; break;
+        case TS_NORMAL: {
+          // to_world = get_onb(n, t) (utils.glsl:111-115); n' = normalize(tex * 2 - 1),
+          // scaled in xy; then the tangent is re-orthonormalised
+          const f3 bb = nrm3(cross3(n, t));
+          const f3 tt = cross3(bb, n);
+          f3 nt = nrm3(f3{tv.x * 2.0f - 1.0f, tv.y * 2.0f - 1.0f, tv.z * 2.0f - 1.0f});
+          nt = nrm3(nt * f3{m2.z, m2.z, 1.0f});
+          n = tt * nt.x + bb * nt.y + n * nt.z;
+          const f3 b2 = nrm3(cross3(n, t)) * hp.tw;
+          t = cross3(b2, n);
+          break;
+        }
+        case TS_ANISO_DIR: aniso_dir_tex = true; aniso_dir = f3{tv.x * 2.0f - 1.0f, tv.y * 2.0f - 1.0f, tv.z * 2.0f - 1.0f}; break;
+        case TS_EMISSIVE: sh.emissive = xyz(m5) * to_linear(tv); break;
+        case TS_CLEARCOAT: g.clearcoat = m6.x * tv.x; break;
+        case TS_CLEARCOAT_ROUGH: g.clearcoatRoughness = m6.y * tv.x; break;
+        case TS_IRID: g.iridescence = m10.x * tv.x; break;
+        default: g.iridescenceThickness = gmix(m10.z, m10.w, tv.y); break;  // TS_IRID_THICK
+      }
+    }
+  }
+  float l4 = sqrtf(hp.vc.x * hp.vc.x + hp.vc.y * hp.vc.y + hp.vc.z * hp.vc.z + hp.vcw * hp.vcw);
+  if (l4 > 0.0f) { g.baseColor = g.baseColor * hp.vc; opacity = opacity * hp.vcw; }
+  sh.cutout = m2.x * opacity;
+  if (m6.w > 0.0f) sh.cutout = (sh.cutout < m6.w) ? 0.0f : 1.0f;
+  if (m6.w == 1.0f) sh.cutout = 1.0f;
   sh.thin = m8.w > 0.5f;
   g.thin = sh.thin;
-  sh.emissive = xyz(m5);
   sh.attColor = xyz(m7);
   sh.attDist = m7.w;
   // fix_normals (utils.glsl:88-98)
-  f3 n = hp.n, ng = hp.ng_raw;
+  f3 ng = hp.ng_raw;
   sh.backside = false;
   if (dot3(wi, ng) < 0.0f) { ng = -ng; sh.backside = true; }
   if (dot3(ng, n) < 0.0f) n = -n;
   sh.n = n; sh.ng = ng;
   // anisotropy direction -> tangent rotation by atan2 + pi (material.glsl:74-83)
-  float rot = atan2f(m9.y, m9.x);
+  float ax = m9.x, ay = m9.y;
+  if (aniso_dir_tex) { ax = aniso_dir.x; ay = aniso_dir.y; }
+  else if (aniso_tex) { ax = aniso_rg.x * 2.0f - 1.0f; ay = aniso_rg.y * 2.0f - 1.0f; }
+  float rot = atan2f(ay, ax);
   float angle = rot + kPI;
-  f3 t = hp.t;
   if (angle > 0.0f) {
-    f3 b = cross3(n, hp.t) * hp.tw;
-    t = hp.t * cosf(angle) + b * sinf(angle);
+    f3 b = cross3(n, t) * hp.tw;
+    t = t * cosf(angle) + b * sinf(angle);
   }
   sh.tanA = t;
   return sh;
 }
-

@@ -84,6 +84,12 @@ struct SceneRes {
   int32_t N = 0, M = 0, T = 0, K = 0;
   int64_t V = 0;
   int users = 0;
+  int32_t max_tex_id = -1;  // largest texture id a material references
+  Allocs tex_mem;           // material textures (replaced as a whole)
+  float4* texinfo = nullptr;
+  TexArrayDesc* texarr = nullptr;
+  uint32_t* texels = nullptr;
+  int32_t num_tex = 0, num_texarr = 0;
 };
 
 struct IblRes {
@@ -245,7 +251,7 @@ int64_t job_bytes(int64_t npix, bool record, int64_t frames, int ring) {
 
 int64_t memory_in_use(dspt_ctx* c) {
   int64_t b = (int64_t)c->state_mem.bytes;
-  for (auto* s : c->scenes) if (s) b += (int64_t)s->mem.bytes;
+  for (auto* s : c->scenes) if (s) b += (int64_t)s->mem.bytes + (int64_t)s->tex_mem.bytes;
   for (auto* i : c->ibls) if (i) b += (int64_t)i->mem.bytes;
   for (auto* j : c->jobs) if (j) b += (int64_t)j->mem.bytes;
   return b;
@@ -256,6 +262,8 @@ SceneView view_of(dspt_ctx* c, const JobRes* j) {
   const SceneRes* s = c->scenes[j->scene];
   S.nodes = s->nodes; S.tri = s->tri; S.verts = s->verts; S.tris = s->tris; S.mats = s->mats;
   S.N = s->N; S.M = s->M; S.T = s->T; S.K = s->K;
+  S.texinfo = s->texinfo; S.texarr = s->texarr; S.texels = s->texels;
+  S.num_tex = s->num_tex; S.num_texarr = s->num_texarr;
   const IblRes* ib = (j->ibl >= 0 && j->ibl < (int)c->ibls.size()) ? c->ibls[j->ibl] : nullptr;
   if (ib) {
     S.env = ib->env; S.pdf = ib->pdf; S.cdf = ib->cdf; S.ypdf = ib->ypdf; S.ycdf = ib->ycdf;
@@ -595,9 +603,14 @@ int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tr
     float mi = vertices[v * 20 + 3];
     if (!(mi >= 0.0f) || mi >= (float)K) return fail(DSPT_ERR_ARG, "vertex " + std::to_string(v) + ": material index out of range");
   }
+  int32_t max_tex = -1;
   for (int32_t k = 0; k < K; k++)
-    for (int i = 44; i < 60; i++)
-      if (mats[(size_t)k * 68 + i] >= 0.0f) return fail(DSPT_ERR_UNSUPPORTED, "material " + std::to_string(k) + " uses a texture");
+    for (int i : {39, 44, 45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 55, 56, 57, 58, 59, 63}) {
+      const float id = mats[(size_t)k * 68 + i];
+      if (!std::isfinite(id) || id >= 16777216.0f)
+        return fail(DSPT_ERR_ARG, "material " + std::to_string(k) + ": texture id is not a finite value below 2^24");
+      if (id >= 0.0f) max_tex = std::max(max_tex, (int32_t)id);
+    }
   if (int r = validate_nodes(nodes, N)) return r;
   HIPCHK(hipSetDevice(c->device));
   SceneRes* s = new SceneRes();
@@ -622,6 +635,7 @@ int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tr
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipFree(d_bvh));
   s->N = (int32_t)N; s->M = (int32_t)M; s->T = (int32_t)T; s->K = K; s->V = V;
+  s->max_tex_id = max_tex;
   *out = s;
   return 0;
 }
@@ -678,6 +692,10 @@ int put_slot(std::vector<T*>& v, T* x, int want = -1) {
 int check_job_frame(dspt_ctx* c, const FrameParams& F, int scene, int ibl) {
   if (!(F.max_bounces >= 0.0f) || F.max_bounces > 1024.0f) return fail(DSPT_ERR_ARG, "max_bounces out of range");
   if (scene < 0 || scene >= (int)c->scenes.size() || !c->scenes[scene]) return fail(DSPT_ERR_ARG, "unknown scene id");
+  if (c->scenes[scene]->max_tex_id >= c->scenes[scene]->num_tex)
+    return fail(DSPT_ERR_STATE, "a material references texture " + std::to_string(c->scenes[scene]->max_tex_id) +
+                                    " but the scene has " + std::to_string(c->scenes[scene]->num_tex) +
+                                    " textures (dspt_scene_set_textures)");
   bool needs_ibl = F.use_ibl > 0.0f || F.bg_from_ibl > 0.0f;
   const IblRes* b = (ibl >= 0 && ibl < (int)c->ibls.size()) ? c->ibls[ibl] : nullptr;
   if (ibl >= 0 && !b) return fail(DSPT_ERR_ARG, "unknown IBL id");
@@ -829,7 +847,7 @@ int dspt_destroy(dspt_ctx* c) {
   drain_events(c);
   for (size_t i = 0; i < c->jobs.size(); i++)
     if (c->jobs[i]) { c->jobs[i]->mem.release(); delete c->jobs[i]; }
-  for (auto* s : c->scenes) if (s) { s->mem.release(); delete s; }
+  for (auto* s : c->scenes) if (s) { s->mem.release(); s->tex_mem.release(); delete s; }
   for (auto* b : c->ibls) if (b) { b->mem.release(); delete b; }
   c->state_mem.release();
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -859,6 +877,7 @@ int dspt_scene_destroy(dspt_ctx* c, int32_t id) {
   if (c->scenes[id]->users > 0) return fail(DSPT_ERR_STATE, "scene is used by live jobs");
   HIPCHK(hipStreamSynchronize(c->stream));
   c->scenes[id]->mem.release();
+  c->scenes[id]->tex_mem.release();
   delete c->scenes[id];
   c->scenes[id] = nullptr;
   return 0;
@@ -884,6 +903,59 @@ int dspt_ibl_destroy(dspt_ctx* c, int32_t id) {
 }
 
 // Runs chunks without scheduling until nothing is in flight.
+static int drain_all(dspt_ctx* c);
+
+int dspt_scene_set_textures(dspt_ctx* c, int32_t scene_id, const float* texinfos, int32_t num_texinfos,
+                            const int32_t* array_dims, int32_t num_arrays, const uint8_t* texels, int64_t num_texel_bytes) {
+  if (!c || scene_id < 0 || scene_id >= (int)c->scenes.size() || !c->scenes[scene_id])
+    return fail(DSPT_ERR_ARG, "unknown scene id");
+  if (num_texinfos < 0 || num_arrays < 0 || num_texel_bytes < 0) return fail(DSPT_ERR_ARG, "negative texture count");
+  if ((num_texinfos > 0 && !texinfos) || (num_arrays > 0 && (!array_dims || !texels)))
+    return fail(DSPT_ERR_ARG, "NULL texture buffer");
+  int64_t want = 0;
+  std::vector<TexArrayDesc> desc((size_t)std::max(num_arrays, 1));
+  for (int32_t a = 0; a < num_arrays; a++) {
+    const int32_t w = array_dims[3 * a], h = array_dims[3 * a + 1], l = array_dims[3 * a + 2];
+    if (w <= 0 || h <= 0 || l <= 0 || (int64_t)w * h * l > (1LL << 31))
+      return fail(DSPT_ERR_ARG, "texture array " + std::to_string(a) + ": invalid dimensions");
+    desc[a].offset = want / 4;
+    desc[a].w = w; desc[a].h = h; desc[a].layers = l; desc[a].pad = 0;
+    want += (int64_t)w * h * l * 4;
+  }
+  if (want != num_texel_bytes) return fail(DSPT_ERR_ARG, "texel byte count does not match the array dimensions");
+  for (int32_t i = 0; i < num_texinfos; i++)
+    for (int k = 0; k < 8; k++)
+      if (!std::isfinite(texinfos[(size_t)i * 8 + k]))
+        return fail(DSPT_ERR_ARG, "texture info " + std::to_string(i) + " holds a non-finite value");
+  HIPCHK(hipSetDevice(c->device));
+  if (int r = drain_all(c)) return r;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  SceneRes* s = c->scenes[scene_id];
+  s->tex_mem.release();
+  s->texinfo = nullptr; s->texarr = nullptr; s->texels = nullptr;
+  s->num_tex = 0; s->num_texarr = 0;
+  if (num_texinfos > 0 || num_arrays > 0) {
+    Allocs& A = s->tex_mem;
+    int r = 0;
+    if ((r = A.get(&s->texinfo, (size_t)num_texinfos * 2)) || (r = A.get(&s->texarr, (size_t)std::max(num_arrays, 1))) ||
+        (r = A.get(&s->texels, (size_t)std::max<int64_t>(want / 4, 1)))) {
+      A.release();
+      return r;
+    }
+    if (num_texinfos > 0)
+      HIPCHK(hipMemcpyAsync(s->texinfo, texinfos, (size_t)num_texinfos * 32, hipMemcpyHostToDevice, c->stream));
+    if (num_arrays > 0) {
+      HIPCHK(hipMemcpyAsync(s->texarr, desc.data(), (size_t)num_arrays * sizeof(TexArrayDesc), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(s->texels, texels, (size_t)want, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    s->num_tex = num_texinfos;
+    s->num_texarr = num_arrays;
+  }
+  c->jobs_dirty = true;
+  return 0;
+}
+
 static int drain_all(dspt_ctx* c) {
   for (size_t i = 0; i < c->jobs.size(); i++)
     if (job_ok(c, (int)i) && c->jobs[i]->in_flight())
@@ -904,6 +976,7 @@ int dspt_upload_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint3
   if (!c->scenes.empty() && c->scenes[0]) {
     users = c->scenes[0]->users;
     c->scenes[0]->mem.release();
+    c->scenes[0]->tex_mem.release();
     delete c->scenes[0];
   }
   s->users = users;

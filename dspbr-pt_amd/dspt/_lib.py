@@ -70,6 +70,7 @@ SIGNATURES = {
     "dspt_set_schedule": (_c_int, [_vp, _c_i32, _c_i32, _c_i32, ctypes.c_float]),
     "dspt_get_schedule": (_c_int, [_vp, _i32p, _i32p, _i32p, _fp]),
     "dspt_set_policy": (_c_int, [_vp, _c_i32]),
+    "dspt_scene_set_textures": (_c_int, [_vp, _c_i32, _fp, _c_i32, _i32p, _c_i32, ctypes.POINTER(ctypes.c_uint8), _c_i64]),
 }
 SCHED_STREAM = 0
 SCHED_PASS = 1
@@ -268,6 +269,14 @@ class Engine:
 
     def scene_destroy(self, sid):
         check(self._L.dspt_scene_destroy(self.h, int(sid)))
+
+    def scene_set_textures(self, scene_id, texinfos, array_dims, texels):
+        """texinfos f32[8*X]; array_dims i32[A, 3] (w, h, layers); texels u8 (all arrays, RGBA8, row 0 first)."""
+        ti = f32(texinfos)
+        dims = np.ascontiguousarray(array_dims, np.int32).reshape(-1)
+        tx = np.ascontiguousarray(texels, np.uint8).reshape(-1)
+        check(self._L.dspt_scene_set_textures(self.h, int(scene_id), ptr(ti), ti.size // 8, ptr(dims, ctypes.c_int32),
+                                              dims.size // 3, ptr(tx, ctypes.c_uint8), tx.size))
 
     def ibl_create(self, rgba, w, h):
         rgba = f32(rgba)

@@ -50,6 +50,9 @@ class BatchRenderer:
             nodes, bidx = build_bvh(scene.bvhPositionBuffer, scene.bvhIndexBuffer)
         sid = self.eng.scene_create(scene.vertexBuffer, scene.triangleIndexBuffer, nodes, bidx,
                                     scene.getFlatMaterialBuffer())
+        if getattr(scene, "num_textures", 0):
+            dims, texels = scene.getTextureArrays()
+            self.eng.scene_set_textures(sid, scene.getFlatTextureInfoBuffer(), dims, texels)
         self.scenes[key] = (sid, (scene.num_triangles, len(nodes) // 16, len(bidx)))
         return sid
 

@@ -155,6 +155,10 @@ class PathtracingRenderer:
             self.diagnostics["profiling"]["bvhBuildMs"] = (time.perf_counter() - t1) * 1e3
         self._engine.upload_scene(scene.vertexBuffer, scene.triangleIndexBuffer, nodes, bidx,
                                   scene.getFlatMaterialBuffer())
+        if getattr(scene, "num_textures", 0):
+            # scene_data_webgl2.ts:233-363: texture arrays + TexInfo records
+            dims, texels = scene.getTextureArrays()
+            self._engine.scene_set_textures(0, scene.getFlatTextureInfoBuffer(), dims, texels)
         self._scene = scene
         self._bvh_counts = (len(nodes) // 16, len(bidx))
         self.diagnostics["scene"] = {"triangles": scene.num_triangles, "materials": scene.num_materials,

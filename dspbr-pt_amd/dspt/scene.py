@@ -82,6 +82,27 @@ class MaterialData:
         raise AttributeError(name)
 
 
+class TexInfo:
+    """8-float texture record of material.ts:483-523: offset.xy, texture-array
+    index, layer, scale.xy (the texture's repeat), uv set, pad."""
+
+    def __init__(self):
+        self.data = np.zeros(8, np.float32)
+        self.texOffset = (0, 0)
+        self.texArrayIdx = 255
+        self.texIdx = 255
+        self.texScale = (1, 1)
+        self.uvSet = 255
+
+    def __setattr__(self, name, value):
+        slots = {"texOffset": (0, 2), "texArrayIdx": (2, 1), "texIdx": (3, 1), "texScale": (4, 2), "uvSet": (6, 1)}
+        if name in slots:
+            a, n = slots[name]
+            self.data[a:a + n] = np.asarray(value, np.float64).reshape(-1)[:n] if n > 1 else np.float32(value)
+            return
+        object.__setattr__(self, name, value)
+
+
 class Light:
     """Point light record of scene_data.ts:3-8."""
 
@@ -101,6 +122,8 @@ class PathtracingSceneData:
         self.bvhIndexBuffer = None       # u32[3*T]
         self.materials = []
         self.lights = []
+        self.texInfos = []
+        self.texArrays = {}  # "w,h" -> list of (key, RGBA8 image h x w x 4), insertion ordered
         # engine extension: a prebuilt bvh_gpu node array + index array skips the build
         self.bvhNodes = None
         self.bvhTriangleIndices = None
@@ -118,6 +141,52 @@ class PathtracingSceneData:
     def addMaterial(self, mat):
         self.materials.append(mat)
         return len(self.materials) - 1
+
+    def addTexture(self, image, offset=(0, 0), repeat=(1, 1), uvSet=0, key=None):
+        """scene_data.ts:71-101: images of one resolution share a texture array;
+        returns the TexInfo id a material slot refers to. image: u8[h, w, 4]
+        (row 0 first); key identifies an image already in its array."""
+        img = np.ascontiguousarray(image, np.uint8)
+        h, w = img.shape[0], img.shape[1]
+        res = f"{w},{h}"
+        info = TexInfo()
+        if res in self.texArrays:
+            arr = self.texArrays[res]
+            idx = next((i for i, (k, _) in enumerate(arr) if key is not None and k == key), -1)
+            if idx < 0:
+                arr.append((key, img))
+                idx = len(arr) - 1
+            info.texArrayIdx = list(self.texArrays).index(res)
+            info.texIdx = idx
+        else:
+            self.texArrays[res] = [(key, img)]
+            info.texArrayIdx = len(self.texArrays) - 1
+            info.texIdx = 0
+        info.texOffset = offset
+        info.texScale = repeat
+        info.uvSet = uvSet
+        self.texInfos.append(info)
+        return len(self.texInfos) - 1
+
+    @property
+    def num_textures(self):
+        return len(self.texInfos)
+
+    def getFlatTextureInfoBuffer(self):
+        if not self.texInfos:
+            return np.zeros(0, np.float32)
+        return np.concatenate([t.data for t in self.texInfos]).astype(np.float32)
+
+    def getTextureArrays(self):
+        """(dims i32[A, 3] = width, height, layers; texels u8, arrays concatenated)."""
+        dims, parts = [], []
+        for res, arr in self.texArrays.items():
+            w, h = (int(x) for x in res.split(","))
+            dims.append((w, h, len(arr)))
+            parts += [img.reshape(-1) for _, img in arr]
+        if not dims:
+            return np.zeros((0, 3), np.int32), np.zeros(0, np.uint8)
+        return np.asarray(dims, np.int32), np.concatenate(parts)
 
     def addLight(self, light):
         self.lights.append(light)

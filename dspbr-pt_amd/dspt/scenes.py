@@ -395,3 +395,83 @@ def batch_mix(seed=3, n_jobs=32, tri_range=(10_000, 2_000_000), side_range=(256,
         ibl = lognormal_ibl(*ibl_size, seed=sseed + 1)
         out.append(Workload(f"batch{k}_{kind}", scene, ibl, cam, side, side, spp, max_bounces, profile))
     return out
+
+
+def _checker(w, h, n, c0, c1):
+    y, x = np.mgrid[0:h, 0:w]
+    m = ((x * n // w + y * n // h) % 2).astype(bool)
+    img = np.where(m[..., None], np.asarray(c1, np.uint8), np.asarray(c0, np.uint8))
+    return np.ascontiguousarray(img.astype(np.uint8))
+
+
+def _noise_rgba(rng, w, h):
+    return rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+
+
+def _bump_normal_map(w, h, freq=4.0, amp=0.6):
+    y, x = np.mgrid[0:h, 0:w] / np.array([h, w])[:, None, None]
+    dx = amp * np.cos(2 * math.pi * freq * x) * np.sin(2 * math.pi * freq * y)
+    dy = amp * np.sin(2 * math.pi * freq * x) * np.cos(2 * math.pi * freq * y)
+    n = np.stack([-dx, -dy, np.ones_like(dx)], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    rgb = np.clip(np.round((n * 0.5 + 0.5) * 255), 0, 255).astype(np.uint8)
+    return np.concatenate([rgb, np.full((h, w, 1), 255, np.uint8)], -1)
+
+
+def textured_spheres(seed=6, n_spheres=16, width=256, height=256, spp=16, max_bounces=8, ibl_size=(512, 256),
+                     profile="webgl-full"):
+    """Sphere field whose materials use every texture slot of MaterialData
+    (material.glsl:5-110): checker / noise / normal-map RGBA8 textures in
+    three resolutions (so several texture arrays with several layers), with
+    offsets and repeats; the ground repeats its textures across the quad."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    scene = PathtracingSceneData()
+    mb = MeshBuilder()
+    checker = _checker(64, 64, 8, (230, 230, 230, 255), (40, 60, 160, 255))
+    checker2 = _checker(64, 64, 4, (200, 40, 30, 255), (250, 220, 120, 200))
+    normal = _bump_normal_map(64, 64)
+    noise = _noise_rgba(rng, 32, 32)
+    noise2 = _noise_rgba(rng, 32, 32)
+    stripes = np.zeros((96, 48, 4), np.uint8)
+    stripes[..., :] = 255
+    stripes[::8, :, :3] = 30
+    t_checker = scene.addTexture(checker, key="checker")
+    t_checker_rep = scene.addTexture(checker, offset=(0.25, 0.1), repeat=(2, 3), key="checker")
+    t_checker2 = scene.addTexture(checker2, key="checker2")
+    t_normal = scene.addTexture(normal, key="normal")
+    t_noise = scene.addTexture(noise, key="noise")
+    t_noise2 = scene.addTexture(noise2, repeat=(3, 3), key="noise2")
+    t_stripes = scene.addTexture(stripes, key="stripes")
+    g = basic_material(rng)
+    g.baseColorTextureId = t_checker_rep
+    g.metallicRoughnessTextureId = t_noise
+    g.normalTextureId = t_normal
+    gm = scene.addMaterial(g)
+    gp = np.array([[-10, 0, -10], [10, 0, -10], [10, 0, 10], [-10, 0, 10]], np.float64)
+    mb.add(gp, np.tile([0, 1, 0], (4, 1)), [[0, 0], [4, 0], [4, 4], [0, 4]], [[0, 2, 1], [0, 3, 2]], gm)
+    slots = [
+        dict(baseColorTextureId=t_checker2, emissiveTextureId=t_stripes, emissiveFactor=[0.6, 0.5, 0.3]),
+        dict(metallicRoughnessTextureId=t_noise2, normalTextureId=t_normal, baseColorTextureId=t_checker),
+        dict(clearcoatFactor=0.8, clearcoatTextureId=t_noise, clearcoatRoughnessTextureId=t_noise2),
+        dict(sheenColorFactor=[0.8, 0.7, 0.9], sheenRoughnessFactor=0.6, sheenColorTextureId=t_checker2,
+             sheenRoughnessTextureId=t_noise),
+        dict(specularFactor=0.8, specularTextureId=t_noise2, specularColorTextureId=t_checker),
+        dict(transmissionFactor=0.9, transmissionTextureId=t_stripes, thinWalled=0, attenuationDistance=1.0),
+        dict(diffuseTransmissionFactor=0.7, diffuseTransmissionTextureId=t_noise,
+             diffuseTransmissionColorTextureId=t_checker2),
+        dict(anisotropy=0.7, anisotropyTextureId=t_noise, roughnessFactor=0.4),
+        dict(anisotropy=0.6, anisotropyTextureId=t_noise2, anisotropyDirectionTextureId=t_normal, roughnessFactor=0.3),
+        dict(iridescenceFactor=0.8, iridescenceTextureId=t_noise, iridescenceThicknessTextureId=t_stripes),
+        dict(normalTextureId=t_normal, normalScale=0.5, metallicFactor=1.0, roughnessFactor=0.2),
+    ]
+    sp, sn, suv, sf = uv_sphere(1.0)
+    for k in range(n_spheres):
+        c = np.array([rng.uniform(-5, 5), rng.uniform(0, 3), rng.uniform(-5, 5)])
+        r = rng.uniform(0.3, 0.8)
+        m = basic_material(rng)
+        for name, v in slots[k % len(slots)].items():
+            setattr(m, name, v)
+        mb.add(sp * r + c, sn, suv, sf, scene.addMaterial(m))
+    mb.finish(scene)
+    cam = ViewportCamera(position=(0, 3, 12), target=(0, 1, 0), fov=45, near=0.01)
+    return Workload("textured_spheres", scene, sky_ibl(*ibl_size), cam, width, height, spp, max_bounces, profile)

@@ -66,8 +66,9 @@ int dspt_destroy(dspt_ctx* ctx);
  * (RT:872-912): vertex stream f32[20*V] (packages/lib/scene_data.ts:21),
  * triangle indices u32[3*T], BVH nodes f32[16*N] and BVH indices i32[M]
  * (packages/lib/bvh/tinybvh_builder.ts:1-5), materials f32[68*K]
- * (packages/lib/material.ts:2-481). Material textures are not supported:
- * every texture id must be < 0. */
+ * (packages/lib/material.ts:2-481). Materials that reference textures
+ * need the scene's textures (dspt_scene_set_textures) before rendering;
+ * replacing the scene drops them. */
 int dspt_upload_scene(dspt_ctx* ctx, const float* vertices, int64_t num_vertices, const uint32_t* tri_indices,
                       int64_t num_triangles, const float* bvh_nodes, int64_t num_nodes, const int32_t* bvh_indices,
                       int64_t num_bvh_indices, const float* materials, int32_t num_materials);
@@ -155,6 +156,20 @@ int dspt_scene_create(dspt_ctx* ctx, const float* vertices, int64_t num_vertices
 int dspt_scene_destroy(dspt_ctx* ctx, int32_t scene_id);
 int dspt_ibl_create(dspt_ctx* ctx, const float* rgba, int32_t width, int32_t height, double* total_sum, int32_t* ibl_id);
 int dspt_ibl_destroy(dspt_ctx* ctx, int32_t ibl_id);
+
+/* Material textures of a scene (scene 0 is the one dspt_upload_scene
+ * replaces), replacing earlier ones. Replaces the reference's texture-array
+ * and TexInfo upload (packages/lib/scene_data_webgl2.ts:233-363):
+ *   texinfos    f32[8 * num_texinfos], TexInfo of packages/lib/material.ts:483-523
+ *               (offset.xy, texture array index, layer, scale.xy, uv set, pad);
+ *   array_dims  i32[3 * num_arrays]: width, height, layers of each RGBA8 array;
+ *   texels      the arrays' RGBA8 texels concatenated in array order, each
+ *               array layer by layer, row 0 first (num_texel_bytes in total).
+ * Sampling is GL LINEAR + REPEAT without mipmaps, as the reference's arrays.
+ * Rendering a scene whose materials reference a texture id >= num_texinfos
+ * fails with DSPT_ERR_STATE. */
+int dspt_scene_set_textures(dspt_ctx* ctx, int32_t scene_id, const float* texinfos, int32_t num_texinfos,
+                            const int32_t* array_dims, int32_t num_arrays, const uint8_t* texels, int64_t num_texel_bytes);
 
 /* A job renders frames 1..num_frames of (scene, IBL or -1, frame block, W x H)
  * into its own accumulation buffer. Admission: fails with DSPT_ERR_MEMORY when

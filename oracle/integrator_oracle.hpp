@@ -61,20 +61,32 @@ template <class R> struct MClosure {  // MaterialClosure, SH/structs.glsl:80-91
 template <class R> struct RState {  // RenderState, SH/structs.glsl:93-103
   V3<R> hitPos, n, ng;
   V3<R> tangent; R tangent_w;
+  R uv0x = R(0), uv0y = R(0);  // compute_interpolated_uv(set 0)
   V3<R> wo, wi;
   MClosure<R> closure;
 };
 
-// SH/material.glsl:5-110 (with get_texture_value() == vec4(1) for untextured slots)
+// get_texture_value(id, uv0, uv1), scene_data_webgl2.ts:343-357
+template <class R> inline V4<R> gtv(const Scene& s, float id, const RState<R>& rs) {
+  if (id < 0.0f) return V4<R>{R(1), R(1), R(1), R(1)};
+  return tex_value<R>(s, id, rs.uv0x, rs.uv0y);
+}
+template <class R> inline V3<R> xyz4(const V4<R>& t) { return V3<R>(t.x, t.y, t.z); }
+
+// SH/material.glsl:5-110. TexInfo ids: MatData.texIds[0..15] = slots 44..59
+// (baseColor, metallicRoughness, normal, emissive, specular, specularColor,
+// transmission, clearcoat, clearcoatRoughness, clearcoatNormal, sheenColor,
+// sheenRoughness, anisotropy, anisotropyDirection, iridescence,
+// iridescenceThickness), plus diffuseTransmission (slot 39) and
+// diffuseTransmissionColor (slot 63).
 template <class R> MClosure<R> configure_material(const Scene& s, uint32_t matIdx, const RState<R>& rs, V3<R> vcol, R vcol_w) {
   MatData md = decode_material(s.mats + (size_t)matIdx * 68);
   MClosure<R> c;
   PbrMat<R>& m = c.material;
-  const V3<R> one(R(1));
-  // to_linear_rgb(vec3(1)) = pow(1, 2.2) = 1
-  V3<R> lin1 = vpow(one, R(2.2f));
-  m.baseColor = V3<R>(R(md.baseColor[0]), R(md.baseColor[1]), R(md.baseColor[2])) * lin1;
-  R opacity = R(1);
+  const R E22 = R(2.2f);
+  const V4<R> bct = gtv(s, md.texIds[0], rs);
+  m.baseColor = V3<R>(R(md.baseColor[0]), R(md.baseColor[1]), R(md.baseColor[2])) * vpow(xyz4(bct), E22);
+  R opacity = bct.w;
   // length(vertexColor) > 0.0 on the vec4
   R l4 = std::sqrt(vcol.x * vcol.x + vcol.y * vcol.y + vcol.z * vcol.z + vcol_w * vcol_w);
   if (l4 > R(0)) { m.baseColor = m.baseColor * vcol; opacity = opacity * vcol_w; }
@@ -82,25 +94,46 @@ template <class R> MClosure<R> configure_material(const Scene& s, uint32_t matId
   c.cutout_opacity = R(md.cutoutOpacity) * opacity;
   if (md.alphaCutoff > 0.0f) c.cutout_opacity = (c.cutout_opacity < R(md.alphaCutoff)) ? R(0) : R(1);
   if (md.alphaCutoff == 1.0f) c.cutout_opacity = R(1);
-  m.transmission = R(md.transmission) * R(1);
-  m.diffuseTransmission = R(md.diffuseTransmission) * R(1);
-  m.diffuseTransmissionColor = V3<R>(R(md.diffuseTransmissionColor[0]), R(md.diffuseTransmissionColor[1]), R(md.diffuseTransmissionColor[2])) * lin1;
+  m.transmission = R(md.transmission) * gtv(s, md.texIds[6], rs).x;
+  m.diffuseTransmission = R(md.diffuseTransmission) * gtv(s, md.diffuseTransmissionTextureId, rs).x;
+  m.diffuseTransmissionColor = V3<R>(R(md.diffuseTransmissionColor[0]), R(md.diffuseTransmissionColor[1]), R(md.diffuseTransmissionColor[2])) *
+                               vpow(xyz4(gtv(s, md.diffuseTransmissionColorTextureId, rs)), E22);
   c.thin_walled = md.thinWalled;
   m.ior = R(md.ior);
   c.double_sided = md.doubleSided;
-  m.metallic = R(md.metallic) * R(1);
-  m.roughness = R(md.roughness) * R(1);
-  m.anisotropyStrength = R(md.anisotropy) * R(1);
-  m.specularColor = V3<R>(R(md.specularColor[0]), R(md.specularColor[1]), R(md.specularColor[2])) * vpow(one, R(2.2f));
-  m.specular = R(md.specular) * R(1);
-  m.sheenRoughness = R(md.sheenRoughness) * R(1);
-  m.sheenColor = V3<R>(R(md.sheenColor[0]), R(md.sheenColor[1]), R(md.sheenColor[2])) * lin1;
+  const V4<R> orm = gtv(s, md.texIds[1], rs);
+  m.metallic = R(md.metallic) * orm.z;
+  m.roughness = R(md.roughness) * orm.y;
+  const V4<R> anis = gtv(s, md.texIds[12], rs);
+  m.anisotropyStrength = R(md.anisotropy) * anis.z;
+  m.specularColor = V3<R>(R(md.specularColor[0]), R(md.specularColor[1]), R(md.specularColor[2])) * vpow(xyz4(gtv(s, md.texIds[5], rs)), E22);
+  m.specular = R(md.specular) * gtv(s, md.texIds[4], rs).w;
+  const V4<R> shc = gtv(s, md.texIds[10], rs);
+  const V4<R> shr = gtv(s, md.texIds[11], rs);
+  m.sheenRoughness = R(md.sheenRoughness) * shr.x;
+  m.sheenColor = V3<R>(R(md.sheenColor[0]), R(md.sheenColor[1]), R(md.sheenColor[2])) * vpow(xyz4(shc), E22);
   c.n = rs.n;
   c.ng = rs.ng;
   c.t = rs.tangent; c.tw = rs.tangent_w;
-  // normal map: not supported (textures), see header
+  if (md.texIds[2] >= 0.0f) {
+    // to_world = get_onb(n, t), SH/utils.glsl:111-115
+    const V3<R> b = normalize(cross(c.n, c.t));
+    const V3<R> tt = cross(b, c.n);
+    const V4<R> nm = gtv(s, md.texIds[2], rs);
+    V3<R> nt = normalize(V3<R>(nm.x * R(2) - R(1), nm.y * R(2) - R(1), nm.z * R(2) - R(1)));
+    nt = normalize(nt * V3<R>(R(md.normalScale), R(md.normalScale), R(1)));
+    c.n = tt * nt.x + b * nt.y + c.n * nt.z;
+    const V3<R> b2 = normalize(cross(c.n, c.t)) * c.tw;
+    c.t = cross(b2, c.n);
+  }
   c.backside = fix_normals(c.n, c.ng, rs.wi);
   V3<R> adir(R(md.anisotropyDirection[0]), R(md.anisotropyDirection[1]), R(0));
+  if (md.texIds[13] >= 0.0f) {
+    const V4<R> ad = gtv(s, md.texIds[13], rs);
+    adir = V3<R>(ad.x * R(2) - R(1), ad.y * R(2) - R(1), R(0));
+  } else if (md.texIds[12] >= 0.0f) {
+    adir = V3<R>(anis.x * R(2) - R(1), anis.y * R(2) - R(1), R(0));
+  }
   m.anisotropyRotation = std::atan2(adir.y, adir.x);
   // rotation_to_tangent(angle, n, t), SH/utils.glsl:183-190
   R angle = m.anisotropyRotation + K<R>::PI;
@@ -110,19 +143,19 @@ template <class R> MClosure<R> configure_material(const Scene& s, uint32_t matId
     c.t = gt * std::cos(angle) + gb * std::sin(angle);
   }
   c.anisotropyTangent = c.t;
-  m.emissive = V3<R>(R(md.emissive[0]), R(md.emissive[1]), R(md.emissive[2])) * lin1;
+  m.emissive = V3<R>(R(md.emissive[0]), R(md.emissive[1]), R(md.emissive[2])) * vpow(xyz4(gtv(s, md.texIds[3], rs)), E22);
   m.emissiveStrength = R(1);
-  m.clearcoat = R(md.clearcoat) * R(1);
-  m.clearcoatRoughness = R(md.clearcoatRoughness) * R(1);
+  m.clearcoat = R(md.clearcoat) * gtv(s, md.texIds[7], rs).x;
+  m.clearcoatRoughness = R(md.clearcoatRoughness) * gtv(s, md.texIds[8], rs).x;
   m.clearcoatNormalScale = R(md.clearcoatNormalScale);
   m.attenuationColor = V3<R>(R(md.attenuationColor[0]), R(md.attenuationColor[1]), R(md.attenuationColor[2]));
   m.attenuationDistance = R(md.attenuationDistance);
   m.thickness = c.thin_walled ? R(0) : R(1);
   m.multiscatterColor = V3<R>(R(0));
   m.scatterAnisotropy = R(0);
-  m.iridescence = R(md.iridescence) * R(1);
+  m.iridescence = R(md.iridescence) * gtv(s, md.texIds[14], rs).x;
   m.iridescenceIor = R(md.iridescenceIor);
-  m.iridescenceThickness = gmix(R(md.iridescenceThicknessMin), R(md.iridescenceThicknessMax), R(1));
+  m.iridescenceThickness = gmix(R(md.iridescenceThicknessMin), R(md.iridescenceThicknessMax), gtv(s, md.texIds[15], rs).y);
   m.dispersion = R(md.dispersion);
   m.normalScale = R(md.normalScale);
   return c;
@@ -153,6 +186,12 @@ template <class R> void fill_render_state(const Scene& s, const Ray<R>& r, const
   V3<R> tan = normalize(w0 * t0 + u * t1 + v * t2);
   if (length(tan) > R(0.99f) && std::fabs(hand) > R(0.99f)) { rs.tangent = tan; rs.tangent_w = -hand; }
   else { rs.tangent = onb_tangent(rs.n); rs.tangent_w = R(1); }
+  // compute_interpolated_uv(tri, uv, 0), bvh.glsl:108-115 (attribute 2, .xy)
+  {
+    V3<R> a0 = vtx_attr<R>(s, tri, 0, 2), a1 = vtx_attr<R>(s, tri, 1, 2), a2 = vtx_attr<R>(s, tri, 2, 2);
+    rs.uv0x = w0 * a0.x + u * a1.x + v * a2.x;
+    rs.uv0y = w0 * a0.y + u * a1.y + v * a2.y;
+  }
   R c0w, c1w, c2w;
   V3<R> c0 = vtx_attr<R>(s, tri, 0, 4, &c0w), c1 = vtx_attr<R>(s, tri, 1, 4, &c1w), c2 = vtx_attr<R>(s, tri, 2, 4, &c2w);
   V3<R> vc = w0 * c0 + u * c1 + v * c2;

@@ -216,6 +216,16 @@ int oracle_ibl_lookup(const Scene* s, const Frame* f, const float* dirs, int n, 
 }
 
 int oracle_scene_struct_size() { return (int)sizeof(Scene); }
+
+// get_texture_value(id, uv) probes (f32): out f32[4n]
+int oracle_tex_value(const Scene* s, float id, const float* uv, int n, float* out) {
+  for (int i = 0; i < n; i++) {
+    V4<float> t{1.0f, 1.0f, 1.0f, 1.0f};
+    if (id >= 0.0f) t = tex_value<float>(*s, id, uv[2 * i], uv[2 * i + 1]);
+    out[4 * i] = t.x; out[4 * i + 1] = t.y; out[4 * i + 2] = t.z; out[4 * i + 3] = t.w;
+  }
+  return 0;
+}
 int oracle_frame_struct_size() { return (int)sizeof(Frame); }
 
 }  // extern "C"

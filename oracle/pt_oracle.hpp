@@ -142,6 +142,14 @@ struct Scene {
   const float* ypdf = nullptr;
   const float* ycdf = nullptr;
   int profile = 0;  // 0 = webgl-lean, 1 = webgl-full
+  // material textures, scene_data_webgl2.ts:233-363: TexInfo f32[8*num_tex]
+  // (material.ts:483-523), RGBA8 arrays of (w, h, layers) in texdims,
+  // texels of all arrays concatenated, row 0 first
+  const float* texinfo = nullptr;
+  int num_tex = 0;
+  const int32_t* texdims = nullptr;
+  int num_texarr = 0;
+  const uint8_t* texels = nullptr;
 };
 
 // 52-float uniform block, SH/renderer.frag:10-31 <-> renderer.ts:376-438
@@ -210,6 +218,47 @@ template <class R> inline V3<R> vtx_attr(const Scene& s, uint32_t tri, int corne
   const float* p = s.verts + (size_t)vid * 20 + attr * 4;
   if (w) *w = R(p[3]);
   return V3<R>(R(p[0]), R(p[1]), R(p[2]));
+}
+
+// get_texture_value (scene_data_webgl2.ts:325-357): texture(array,
+// vec3(uv * scale + offset, layer)) on RGBA8 UNORM with GL LINEAR + REPEAT:
+// texel / 255, bilinear weights as in the environment lookup. The layer is
+// round-to-nearest, clamped; an array index out of range yields vec4(1) (the
+// generated if-chain falls through). uv set 1 is never written by the render
+// state (renderer.frag:72-91): taken as (0, 0).
+template <class R> inline V4<R> tex_value(const Scene& s, float id, R u0, R v0) {
+  const int idx = (int)id;
+  const float* ti = s.texinfo + (size_t)idx * 8;
+  const int ai = (int)ti[2];
+  if (ai < 0 || ai >= s.num_texarr) return V4<R>{R(1), R(1), R(1), R(1)};
+  int64_t off = 0;
+  for (int k = 0; k < ai; k++) off += (int64_t)s.texdims[3 * k] * s.texdims[3 * k + 1] * s.texdims[3 * k + 2];
+  const int w = s.texdims[3 * ai], h = s.texdims[3 * ai + 1], layers = s.texdims[3 * ai + 2];
+  const R u = ti[6] > 0.5f ? R(0) : u0, v = ti[6] > 0.5f ? R(0) : v0;
+  const R tu = u * R(ti[4]) + R(ti[0]), tv = v * R(ti[5]) + R(ti[1]);
+  R lf = std::floor(R(ti[3]) + R(0.5f));
+  lf = gmin(gmax(lf, R(0)), R(layers - 1));
+  const int layer = (int)lf;
+  const R x = tu * R(w) - R(0.5f), y = tv * R(h) - R(0.5f);
+  R fx = std::floor(x), fy = std::floor(y);
+  const R a = x - fx, b = y - fy;
+  fx = gmin(gmax(fx, R(-1.0e9f)), R(1.0e9f));
+  fy = gmin(gmax(fy, R(-1.0e9f)), R(1.0e9f));
+  const int ix = (int)fx, iy = (int)fy;
+  int x0 = ix % w; if (x0 < 0) x0 += w;
+  int x1 = (ix + 1) % w; if (x1 < 0) x1 += w;
+  int y0 = iy % h; if (y0 < 0) y0 += h;
+  int y1 = (iy + 1) % h; if (y1 < 0) y1 += h;
+  const uint8_t* L = s.texels + (off + (int64_t)layer * w * h) * 4;
+  auto texel = [&](int xx, int yy, int c) { return R(L[((int64_t)yy * w + xx) * 4 + c]) / R(255); };
+  R o[4];
+  for (int c = 0; c < 4; c++) {
+    const R ia = R(1) - a, ib = R(1) - b;
+    const R top = texel(x0, y0, c) * ia + texel(x1, y0, c) * a;
+    const R bot = texel(x0, y1, c) * ia + texel(x1, y1, c) * a;
+    o[c] = top * ib + bot * b;
+  }
+  return V4<R>{o[0], o[1], o[2], o[3]};
 }
 
 // SH/bvh.glsl:137-212. Occlusion uses the same closest-hit search

@@ -24,6 +24,8 @@ class CScene(ctypes.Structure):
         ("verts", _f), ("V", ctypes.c_int), ("tris", _u32), ("T", ctypes.c_int), ("nodes", _f), ("N", ctypes.c_int),
         ("bvh_idx", _i32), ("M", ctypes.c_int), ("mats", _f), ("K", ctypes.c_int), ("env", _f), ("ew", ctypes.c_int),
         ("eh", ctypes.c_int), ("pdf", _f), ("cdf", _f), ("ypdf", _f), ("ycdf", _f), ("profile", ctypes.c_int),
+        ("texinfo", _f), ("num_tex", ctypes.c_int), ("texdims", _i32), ("num_texarr", ctypes.c_int),
+        ("texels", ctypes.POINTER(ctypes.c_uint8)),
     ]
 
 
@@ -53,6 +55,7 @@ def lib():
         L.oracle_bsdf_sample.argtypes = [_f, ctypes.c_int, _f, _f, ctypes.c_int, ctypes.c_int, _d]
         L.oracle_ibl_sample.argtypes = [ctypes.POINTER(CScene), _f, _f, ctypes.c_int, _f]
         L.oracle_ibl_lookup.argtypes = [ctypes.POINTER(CScene), _f, _f, ctypes.c_int, _f]
+        L.oracle_tex_value.argtypes = [ctypes.POINTER(CScene), ctypes.c_float, _f, ctypes.c_int, _f]
         assert L.oracle_scene_struct_size() == ctypes.sizeof(CScene)
         assert L.oracle_frame_struct_size() == 52 * 4
         _lib = L
@@ -82,6 +85,22 @@ class OracleScene:
         self.total = None
         if ibl is not None:
             self.set_ibl(ibl)
+
+    def set_textures(self, texinfos, array_dims, texels):
+        """Material textures (same arguments as the engine's scene_set_textures)."""
+        self.texinfo = np.ascontiguousarray(texinfos, np.float32).reshape(-1)
+        self.texdims = np.ascontiguousarray(array_dims, np.int32).reshape(-1)
+        self.texels = np.ascontiguousarray(texels, np.uint8).reshape(-1)
+        self.c.texinfo, self.c.num_tex = _p(self.texinfo), self.texinfo.size // 8
+        self.c.texdims, self.c.num_texarr = _p(self.texdims, ctypes.c_int32), self.texdims.size // 3
+        self.c.texels = _p(self.texels, ctypes.c_uint8)
+
+    def tex_value(self, tex_id, uv):
+        """get_texture_value for probes: f32[n, 4] at uv f32[n, 2]."""
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        out = np.zeros((uv.shape[0], 4), np.float32)
+        lib().oracle_tex_value(ctypes.byref(self.c), float(tex_id), _p(uv), uv.shape[0], _p(out))
+        return out
 
     def set_ibl(self, ibl):
         img = ibl["image"]
