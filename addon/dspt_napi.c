@@ -297,6 +297,25 @@ static napi_value js_scene_create(napi_env env, napi_callback_info info) {
   return r;
 }
 
+/* sceneSetTextures(h, sceneId, texinfos f32[8X], dims i32[3A], texels u8) */
+static napi_value js_scene_set_textures(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value argv[5];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 5) { napi_throw_error(env, NULL, "sceneSetTextures expects 5 arguments"); return NULL; }
+  dspt_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return NULL;
+  void *ti, *dm, *tx;
+  size_t lti, ldm, ltx;
+  if (get_typed(env, argv[2], napi_float32_array, &ti, &lti) || get_typed(env, argv[3], napi_int32_array, &dm, &ldm) ||
+      get_typed(env, argv[4], napi_uint8_array, &tx, &ltx))
+    return NULL;
+  int rc = dspt_scene_set_textures(c, (int32_t)get_i64(env, argv[1]), (const float*)ti, (int32_t)(lti / 8),
+                                   (const int32_t*)dm, (int32_t)(ldm / 3), (const uint8_t*)tx, (int64_t)ltx);
+  if (rc) return throw_dspt(env, rc);
+  return NULL;
+}
+
 /* iblCreate(h, rgba f32, w, h) -> {id, totalSum} */
 static napi_value js_ibl_create(napi_env env, napi_callback_info info) {
   size_t argc = 4;
@@ -510,6 +529,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"version", NULL, js_version, NULL, NULL, NULL, napi_enumerable, NULL},
       {"sceneCreate", NULL, js_scene_create, NULL, NULL, NULL, napi_enumerable, NULL},
       {"sceneDestroy", NULL, js_scene_destroy, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"sceneSetTextures", NULL, js_scene_set_textures, NULL, NULL, NULL, napi_enumerable, NULL},
       {"iblCreate", NULL, js_ibl_create, NULL, NULL, NULL, napi_enumerable, NULL},
       {"iblDestroy", NULL, js_ibl_destroy, NULL, NULL, NULL, napi_enumerable, NULL},
       {"jobCreate", NULL, js_job_create, NULL, NULL, NULL, napi_enumerable, NULL},

@@ -14,6 +14,9 @@ export interface SceneBuffers {
   lights?: { position: number[]; emission: number[] }[];
   bvhNodes?: Float32Array;             // optional prebuilt bvh_gpu nodes
   bvhTriangleIndices?: Int32Array;
+  // material textures: TexInfo records (8 f32 each, material.ts:483-523), RGBA8 texture
+  // arrays (width, height, layers per array) and their texels, row 0 first
+  textures?: { texInfos: Float32Array; arrayDims: Int32Array; texels: Uint8Array };
 }
 export declare class PathtracingRenderer {
   constructor(parameters?: { materialProfile?: 'webgl-lean' | 'webgl-full'; device?: number; width?: number; height?: number });

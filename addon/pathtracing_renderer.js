@@ -79,6 +79,8 @@ class PathtracingRenderer {
     const mats = new Float32Array(scene.materials.length * 68);
     scene.materials.forEach((m, i) => mats.set(m.data, i * 68));
     native.uploadScene(this._h, scene.vertexBuffer, scene.triangleIndexBuffer, nodes, idx, mats);
+    // scene_data_webgl2.ts:233-363: texture arrays (RGBA8, row 0 first) + TexInfo records
+    if (scene.textures) native.sceneSetTextures(this._h, 0, scene.textures.texInfos, scene.textures.arrayDims, scene.textures.texels);
     this._scene = scene;
     this._sceneCounts = [scene.triangleIndexBuffer.length / 3, nodes.length / 16, idx.length];
     this.diagnostics.bvh = { nodes: nodes.length / 16, indices: idx.length };
@@ -172,6 +174,7 @@ class RenderServer {
     const mats = new Float32Array(scene.materials.length * 68);
     scene.materials.forEach((m, i) => mats.set(m.data, i * 68));
     const id = native.sceneCreate(this._h, scene.vertexBuffer, scene.triangleIndexBuffer, nodes, idx, mats);
+    if (scene.textures) native.sceneSetTextures(this._h, id, scene.textures.texInfos, scene.textures.arrayDims, scene.textures.texels);
     this._scenes.set(id, { counts: [scene.triangleIndexBuffer.length / 3, nodes.length / 16, idx.length],
                            lights: scene.lights || [] });
     return id;
