@@ -132,22 +132,12 @@ struct Builder {
 }  // namespace
 
 // Returns 0 on success. out_nodes: 16 floats per node (DFS order, root 0).
-int dspt_bvh_build_impl(const float* pos, int64_t nv, const uint32_t* ind, int64_t nt, int max_leaf,
-                        std::vector<float>& out_nodes, std::vector<int32_t>& out_idx, std::string& err) {
-  if (nt <= 0) { err = "bvh build: no triangles"; return -1; }
-  if (nt > (1 << 24)) { err = "bvh build: more than 2^24 triangles (float-encoded indices)"; return -1; }
-  if (max_leaf < 1) max_leaf = 1;
-  if (max_leaf > 64) max_leaf = 64;
-  std::vector<Box> tb(nt);
-  std::vector<float> cent(3 * (size_t)nt);
-  for (int64_t t = 0; t < nt; t++) {
-    for (int k = 0; k < 3; k++) {
-      uint32_t v = ind[3 * t + k];
-      if ((int64_t)v >= nv) { err = "bvh build: vertex index out of range"; return -1; }
-      tb[t].grow(pos + 3 * (size_t)v);
-    }
-    for (int k = 0; k < 3; k++) cent[3 * t + k] = 0.5f * (tb[t].lo[k] + tb[t].hi[k]);
-  }
+namespace {
+
+// Shared back end: binned SAH over per-item boxes and centroids, then the
+// bvh_gpu layout in DFS order (root 0). out_idx lists item ids per leaf.
+int build_items(std::vector<Box>& tb, std::vector<float>& cent, int64_t nt, int max_leaf, std::vector<float>& out_nodes,
+                std::vector<int32_t>& out_idx) {
   std::vector<int32_t> idx(nt);
   for (int64_t t = 0; t < nt; t++) idx[t] = (int32_t)t;
   std::vector<BNode> nodes(2 * (size_t)nt);
@@ -192,4 +182,46 @@ int dspt_bvh_build_impl(const float* pos, int64_t nv, const uint32_t* ind, int64
   }
   out_idx = std::move(idx);
   return 0;
+}
+
+}  // namespace
+
+int dspt_bvh_build_impl(const float* pos, int64_t nv, const uint32_t* ind, int64_t nt, int max_leaf,
+                        std::vector<float>& out_nodes, std::vector<int32_t>& out_idx, std::string& err) {
+  if (nt <= 0) { err = "bvh build: no triangles"; return -1; }
+  if (nt > (1 << 24)) { err = "bvh build: more than 2^24 triangles (float-encoded indices)"; return -1; }
+  if (max_leaf < 1) max_leaf = 1;
+  if (max_leaf > 64) max_leaf = 64;
+  std::vector<Box> tb(nt);
+  std::vector<float> cent(3 * (size_t)nt);
+  for (int64_t t = 0; t < nt; t++) {
+    for (int k = 0; k < 3; k++) {
+      uint32_t v = ind[3 * t + k];
+      if ((int64_t)v >= nv) { err = "bvh build: vertex index out of range"; return -1; }
+      tb[t].grow(pos + 3 * (size_t)v);
+    }
+    for (int k = 0; k < 3; k++) cent[3 * t + k] = 0.5f * (tb[t].lo[k] + tb[t].hi[k]);
+  }
+  return build_items(tb, cent, nt, max_leaf, out_nodes, out_idx);
+}
+
+// Same layout over axis-aligned boxes (f32[6n]: lo.xyz, hi.xyz), e.g. the
+// world bounds of instances for a top-level BVH.
+int dspt_bvh_build_boxes_impl(const float* boxes, int64_t n, int max_leaf, std::vector<float>& out_nodes,
+                              std::vector<int32_t>& out_idx, std::string& err) {
+  if (n <= 0) { err = "bvh build: no boxes"; return -1; }
+  if (n > (1 << 24)) { err = "bvh build: more than 2^24 boxes"; return -1; }
+  if (max_leaf < 1) max_leaf = 1;
+  if (max_leaf > 64) max_leaf = 64;
+  std::vector<Box> tb(n);
+  std::vector<float> cent(3 * (size_t)n);
+  for (int64_t i = 0; i < n; i++) {
+    tb[i].grow(boxes + 6 * i);
+    tb[i].grow(boxes + 6 * i + 3);
+    for (int k = 0; k < 3; k++) {
+      if (!std::isfinite(boxes[6 * i + k]) || !std::isfinite(boxes[6 * i + 3 + k])) { err = "bvh build: non-finite box"; return -1; }
+      cent[3 * i + k] = 0.5f * (tb[i].lo[k] + tb[i].hi[k]);
+    }
+  }
+  return build_items(tb, cent, n, max_leaf, out_nodes, out_idx);
 }

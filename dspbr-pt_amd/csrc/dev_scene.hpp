@@ -17,6 +17,14 @@ struct TexArrayDesc {
   int32_t w, h, layers, pad;
 };
 
+// One instance of a two-level scene: affine transforms (rows of 3x4) and the
+// root of its mesh's bottom-level BVH in SceneView::nodes.
+struct InstanceDesc {
+  float4 w2o[3];  // world -> object
+  float4 o2w[3];  // object -> world
+  int32_t root, mesh, pad0, pad1;
+};
+
 struct SceneView {
   const float4* __restrict__ nodes;
   const float4* __restrict__ tri;
@@ -38,6 +46,12 @@ struct SceneView {
   const TexArrayDesc* __restrict__ texarr;
   const uint32_t* __restrict__ texels;
   int32_t num_tex, num_texarr;
+  // two-level scenes (engine extension: the reference flattens instances):
+  // top-level bvh_gpu nodes over instance bounds, one instance per leaf
+  const float4* __restrict__ tlas;
+  const int32_t* __restrict__ tlas_idx;
+  const InstanceDesc* __restrict__ inst;
+  int32_t TN, TM, I, pad;
 };
 
 // The 52-float uniform block (renderer.frag:10-31).
@@ -65,6 +79,20 @@ struct RayHit {
   int tri;
   float t, u, v;
 };
+
+DI f3 xf_point(const float4* m, f3 p) {
+  return f3{m[0].x * p.x + m[0].y * p.y + m[0].z * p.z + m[0].w, m[1].x * p.x + m[1].y * p.y + m[1].z * p.z + m[1].w,
+            m[2].x * p.x + m[2].y * p.y + m[2].z * p.z + m[2].w};
+}
+DI f3 xf_vec(const float4* m, f3 v) {
+  return f3{m[0].x * v.x + m[0].y * v.y + m[0].z * v.z, m[1].x * v.x + m[1].y * v.y + m[1].z * v.z,
+            m[2].x * v.x + m[2].y * v.y + m[2].z * v.z};
+}
+// transpose(m) * v: normals with the inverse-transpose of the other direction
+DI f3 xf_tvec(const float4* m, f3 v) {
+  return f3{m[0].x * v.x + m[1].x * v.y + m[2].x * v.z, m[0].y * v.x + m[1].y * v.y + m[2].y * v.z,
+            m[0].z * v.x + m[1].z * v.y + m[2].z * v.z};
+}
 
 // Per-lane traversal stack: the first kLdsStack entries live in LDS
 // (lane-interleaved, conflict-free), the rest in a global spill area that
@@ -125,6 +153,16 @@ struct Geo {
   int32_t N, M;
 };
 DI Geo geo_of(const SceneView& S) { return Geo{S.nodes, S.tri, S.N, S.M}; }
+
+// Geometry of a two-level traversal.
+struct Geo2 {
+  Geo g;
+  const float4* __restrict__ tlas;
+  const int32_t* __restrict__ tlas_idx;
+  const InstanceDesc* __restrict__ inst;
+  int32_t TN, TM, I;
+};
+DI Geo2 geo2_of(const SceneView& S) { return Geo2{geo_of(S), S.tlas, S.tlas_idx, S.inst, S.TN, S.TM, S.I}; }
 
 // Resumable traversal state of one ray (the loop variables of
 // SH/bvh.glsl:149-212). The entry on top of the reference's stack is held in
@@ -281,6 +319,115 @@ DI bool trav_leaf(const Geo& G, RayState& r, unsigned& nt) {
     }
   }
   return false;
+}
+
+// Two-level traversal (scenes with instances). The top-level BVH is walked
+// like a mesh BVH; at a top-level leaf the ray is moved into the instance's
+// object space (the direction is not renormalised, so t keeps its meaning)
+// and the instance's mesh BVH is walked on the same stack above `base`;
+// when the stack is back at `base` the world ray is restored. Leaf
+// triangles are tested by trav_leaf as for one level; `inst` records the
+// instance of the closest hit.
+struct InstState {
+  f3 worg, wdir, winv;
+  int level, base, cur, hit;  // level 1 = inside an instance; cur = its id; hit = instance of the closest hit
+};
+
+DI void inst_init(InstState& q) {
+  q.level = 0; q.base = 0; q.cur = -1; q.hit = -1;
+  q.worg = q.wdir = q.winv = f3{0.0f, 0.0f, 0.0f};
+}
+
+DI int trav_inner2(const Geo2& G, RayState& r, InstState& q, LaneStack& st, unsigned& nv) {
+  if (q.level == 1 && !r.has_next && r.top == q.base) {
+    q.level = 0;
+    r.org = q.worg; r.dir = q.wdir; r.inv = q.winv;
+  }
+  if (ray_done(r)) return 2;
+  r.step++;
+  int ni;
+  if (r.has_next) {
+    ni = r.next;
+    r.has_next = false;
+  } else {
+    ni = st.get(--r.top);
+  }
+  const bool blas = q.level == 1;
+  const float4* nodes = blas ? G.g.nodes : G.tlas;
+  const int N = blas ? G.g.N : G.TN;
+  if (ni < 0 || ni >= N) return ray_done(r) ? 2 : 0;
+  nv++;
+  const float4* nd = nodes + (size_t)ni * 4;
+  const float4 a = ldg(nd), b = ldg(nd + 1), c = ldg(nd + 2), d = ldg(nd + 3);
+  const int cnt = (int)c.w;
+  if (cnt > 0) {
+    const int first = (int)d.w;
+    if (blas) {
+      r.leaf_first = first;
+      r.leaf_end = min(first + min(cnt, kMaxLeafTris), G.g.M);
+      return 1;
+    }
+    // top-level leaf: enter its instance
+    const int k = (first >= 0 && first < G.TM) ? G.tlas_idx[first] : -1;
+    if (k < 0 || k >= G.I || r.top >= kStackSize) return ray_done(r) ? 2 : 0;
+    const InstanceDesc* D = G.inst + k;
+    const float4 w2o[3] = {ldg(&D->w2o[0]), ldg(&D->w2o[1]), ldg(&D->w2o[2])};
+    q.worg = r.org; q.wdir = r.dir; q.winv = r.inv;
+    q.level = 1;
+    q.base = r.top;
+    q.cur = k;
+    r.org = xf_point(w2o, q.worg);
+    r.dir = xf_vec(w2o, q.wdir);
+    r.inv = f3{1.0f / r.dir.x, 1.0f / r.dir.y, 1.0f / r.dir.z};
+    r.next = D->root;
+    r.has_next = true;
+    return 0;
+  }
+  const int left = (int)a.w, right = (int)b.w;
+  const float d0 = slab(xyz(a), xyz(b), r.org, r.inv, r.h.t);
+  const float d1 = slab(xyz(c), xyz(d), r.org, r.inv, r.h.t);
+  const bool h0 = d0 < kBVH_FAR, h1 = d1 < kBVH_FAR;
+  const bool both = h0 && h1;
+  const bool near_left = d0 < d1;
+  const int c1 = both ? (near_left ? right : left) : (h0 ? left : right);
+  const int c2 = near_left ? left : right;
+  auto ok = [&](int child) { return child >= 0 && child < N && child != ni; };
+  const bool p1 = (h0 || h1) && r.top < kStackSize && ok(c1);
+  if (both) {
+    if (p1) st.put(r.top++, c1);
+    if (r.top < kStackSize && ok(c2)) { r.next = c2; r.has_next = true; }
+  } else if (p1) {
+    r.next = c1;
+    r.has_next = true;
+  }
+  return ray_done(r) ? 2 : 0;
+}
+
+// trav_leaf for the two-level walk: a closer hit also records the instance.
+template <bool ANY_HIT>
+DI bool trav_leaf2(const Geo2& G, RayState& r, InstState& q, unsigned& nt) {
+  const float t0 = r.h.t;
+  const bool hit = trav_leaf<ANY_HIT>(G.g, r, nt);
+  if (r.h.t != t0 || hit) q.hit = q.cur;
+  return hit;
+}
+
+// Whole two-level traversal of one ray (standalone probes).
+template <bool ANY_HIT>
+DI RayHit traverse2(const Geo2& G, f3 org, f3 dir, float tfar, LaneStack& st, unsigned& nv, unsigned& nt, int& inst) {
+  RayState r;
+  InstState q;
+  ray_init(r, st, org, dir, tfar);
+  inst_init(q);
+  for (;;) {
+    const int res = trav_inner2(G, r, q, st, nv);
+    if (res == 2) break;
+    if (res == 1) {
+      if (trav_leaf2<ANY_HIT>(G, r, q, nt)) break;
+    }
+  }
+  inst = q.hit;
+  return r.h;
 }
 
 // Whole traversal of one ray.

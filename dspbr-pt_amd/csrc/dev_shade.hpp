@@ -14,8 +14,12 @@ struct Hitpoint {
   uint32_t mat;
 };
 
-// fillRenderState (renderer.frag:72-91) minus the material.
-DI Hitpoint fetch_hitpoint(const SceneView& S, f3 org, f3 dir, int tri, float t, float u, float v) {
+// fillRenderState (renderer.frag:72-91) minus the material. For a hit on an
+// instance of a two-level scene (inst >= 0) the mesh's object-space
+// attributes are taken to world space: positions with object->world,
+// normals with the transpose of world->object, tangents with object->world;
+// the hit position is on the world ray as in the one-level case.
+DI Hitpoint fetch_hitpoint(const SceneView& S, f3 org, f3 dir, int tri, float t, float u, float v, int inst = -1) {
   Hitpoint hp;
   hp.pos = org + dir * t;
   uint32_t vi[3];
@@ -24,12 +28,23 @@ DI Hitpoint fetch_hitpoint(const SceneView& S, f3 org, f3 dir, int tri, float t,
   const float4* v1 = S.verts + (size_t)vi[1] * 5;
   const float4* v2 = S.verts + (size_t)vi[2] * 5;
   float4 p0 = v0[0], p1 = v1[0], p2 = v2[0];
-  hp.ng_raw = nrm3(cross3(xyz(p1) - xyz(p0), xyz(p2) - xyz(p0)));
+  const InstanceDesc* D = (inst >= 0 && S.inst) ? S.inst + inst : nullptr;
+  float4 o2w[3], w2o[3];
+  if (D) {
+    for (int k = 0; k < 3; k++) { o2w[k] = D->o2w[k]; w2o[k] = D->w2o[k]; }
+  }
+  f3 q0 = xyz(p0), q1 = xyz(p1), q2 = xyz(p2);
+  if (D) { q0 = xf_point(o2w, q0); q1 = xf_point(o2w, q1); q2 = xf_point(o2w, q2); }
+  hp.ng_raw = nrm3(cross3(q1 - q0, q2 - q0));
   float w0 = 1.0f - u - v;
-  hp.n = nrm3(w0 * xyz(v0[1]) + u * xyz(v1[1]) + v * xyz(v2[1]));
+  f3 ni = w0 * xyz(v0[1]) + u * xyz(v1[1]) + v * xyz(v2[1]);
+  if (D) ni = xf_tvec(w2o, ni);
+  hp.n = nrm3(ni);
   float4 t0 = v0[3], t1 = v1[3], t2 = v2[3];
   float hand = (t0.w == t1.w && t0.w == t2.w) ? t0.w : 0.0f;
-  f3 tn = nrm3(w0 * xyz(t0) + u * xyz(t1) + v * xyz(t2));
+  f3 ti = w0 * xyz(t0) + u * xyz(t1) + v * xyz(t2);
+  if (D) ti = xf_tvec(w2o, ti);  // tangents by the normal matrix too, as scene_loader.ts:327-336 bakes them
+  f3 tn = nrm3(ti);
   if (len3(tn) > 0.99f && fabsf(hand) > 0.99f) {
     hp.t = tn; hp.tw = -hand;
   } else {

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -34,6 +35,8 @@
 
 int dspt_bvh_build_impl(const float* pos, int64_t nv, const uint32_t* ind, int64_t nt, int max_leaf,
                         std::vector<float>& out_nodes, std::vector<int32_t>& out_idx, std::string& err);
+int dspt_bvh_build_boxes_impl(const float* boxes, int64_t n, int max_leaf, std::vector<float>& out_nodes,
+                              std::vector<int32_t>& out_idx, std::string& err);
 
 namespace {
 
@@ -90,6 +93,11 @@ struct SceneRes {
   TexArrayDesc* texarr = nullptr;
   uint32_t* texels = nullptr;
   int32_t num_tex = 0, num_texarr = 0;
+  // two-level scenes
+  float4* tlas = nullptr;
+  int32_t* tlas_idx = nullptr;
+  InstanceDesc* inst = nullptr;
+  int32_t TN = 0, TM = 0, I = 0;
 };
 
 struct IblRes {
@@ -264,6 +272,8 @@ SceneView view_of(dspt_ctx* c, const JobRes* j) {
   S.N = s->N; S.M = s->M; S.T = s->T; S.K = s->K;
   S.texinfo = s->texinfo; S.texarr = s->texarr; S.texels = s->texels;
   S.num_tex = s->num_tex; S.num_texarr = s->num_texarr;
+  S.tlas = s->tlas; S.tlas_idx = s->tlas_idx; S.inst = s->inst;
+  S.TN = s->TN; S.TM = s->TM; S.I = s->I;
   const IblRes* ib = (j->ibl >= 0 && j->ibl < (int)c->ibls.size()) ? c->ibls[j->ibl] : nullptr;
   if (ib) {
     S.env = ib->env; S.pdf = ib->pdf; S.cdf = ib->cdf; S.ypdf = ib->ypdf; S.ycdf = ib->ycdf;
@@ -324,10 +334,10 @@ int ensure_pool(dspt_ctx* c) {
         (r = A.get(&ps->rng, cap)) || (r = A.get(&ps->job, cap)) || (r = A.get(&ps->bounce, cap)) || (r = A.get(&ps->fp, cap)) ||
         (r = A.get(&ps->sig, cap)) || (r = A.get(&ps->prim, cap)))
       return r;
-  if ((r = A.get(&a.hit, cap)) || (r = A.get(&a.sh_org, cap)) || (r = A.get(&a.sh_dir, cap)) || (r = A.get(&a.sh_c, cap)) ||
+  if ((r = A.get(&a.hit, cap)) || (r = A.get(&a.hinst, cap)) || (r = A.get(&a.sh_org, cap)) || (r = A.get(&a.sh_dir, cap)) || (r = A.get(&a.sh_c, cap)) ||
       (r = A.get(&a.sh_pw, cap)) || (r = A.get(&a.sh_pdir, cap)) || (r = A.get(&a.sh_pc, cap)) || (r = A.get(&c->shq, cap)))
     return r;
-  b.hit = a.hit; b.sh_org = a.sh_org; b.sh_dir = a.sh_dir; b.sh_c = a.sh_c; b.sh_pw = a.sh_pw; b.sh_pdir = a.sh_pdir;
+  b.hit = a.hit; b.hinst = a.hinst; b.sh_org = a.sh_org; b.sh_dir = a.sh_dir; b.sh_c = a.sh_c; b.sh_pw = a.sh_pw; b.sh_pdir = a.sh_pdir;
   b.sh_pc = a.sh_pc;
   c->trace_blocks = (int)std::min<int64_t>(c->trace_blocks_max, grid_for(cap, kTraceBlock));
   if ((r = A.get(&c->spill, (size_t)c->trace_blocks * kTraceBlock * (kStackSize - kLdsStack)))) return r;
@@ -486,6 +496,10 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
   const int trace_blocks = c->trace_blocks;
   StreamCtl* ctl = c->d_ctl;
   const int32_t P32 = (int32_t)P;
+  // two-level traversal only while a job on a two-level scene is live
+  bool inst = false;
+  for (const JobRes* j : c->jobs)
+    if (j && j->alive && c->scenes[j->scene] && c->scenes[j->scene]->tlas) inst = true;
   for (int k = 0; k < c->chunk_iters; k++) {
     const int a = c->qsel;
     const PathState cur = c->ps[a], nxt = c->ps[a ^ 1];
@@ -494,15 +508,24 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
       k_regen<<<shade_blocks, 256, 0, c->stream>>>(cur, c->d_jobs, c->d_tasks, c->d_sched, ctl, c->cnt);
     });
     timed(c, DSPT_K_TRACE_EXT, [&] {
-      k_trace_ext<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], &ctl->ext_next, c->spill, c->cnt);
+      if (inst)
+        k_trace_ext<true><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], &ctl->ext_next, c->spill,
+                                                                      c->cnt);
+      else
+        k_trace_ext<false><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], &ctl->ext_next, c->spill,
+                                                                       c->cnt);
     });
     timed(c, DSPT_K_SHADE, [&] {
       k_shade<<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1], c->shq,
                                                     &ctl->sh_count, c->cnt);
     });
     timed(c, DSPT_K_TRACE_SHADOW, [&] {
-      k_trace_shadow<<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, nxt, c->shq, &ctl->sh_count, &ctl->sh_next,
-                                                                   c->spill, c->cnt);
+      if (inst)
+        k_trace_shadow<true><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, nxt, c->shq, &ctl->sh_count, &ctl->sh_next,
+                                                                         c->spill, c->cnt);
+      else
+        k_trace_shadow<false><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, nxt, c->shq, &ctl->sh_count,
+                                                                          &ctl->sh_next, c->spill, c->cnt);
     });
     c->qsel ^= 1;
   }
@@ -636,6 +659,131 @@ int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tr
   HIPCHK(hipFree(d_bvh));
   s->N = (int32_t)N; s->M = (int32_t)M; s->T = (int32_t)T; s->K = K; s->V = V;
   s->max_tex_id = max_tex;
+  *out = s;
+  return 0;
+}
+
+// Two-level scene: the engine builds one bottom-level BVH per mesh (the
+// mesh's triangles, object space), concatenates them (child indices and leaf
+// ranges offset), and a top-level BVH over the instances' world bounds with
+// one instance per leaf.
+int make_scene_instanced(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T,
+                         const int64_t* mesh_first, const int64_t* mesh_count, int32_t num_meshes, const float* inst_o2w,
+                         const int32_t* inst_mesh, int32_t num_inst, const float* mats, int32_t K, SceneRes** out) {
+  if (!vertices || !tri_idx || !mesh_first || !mesh_count || !inst_o2w || !inst_mesh || !mats)
+    return fail(DSPT_ERR_ARG, "NULL scene buffer");
+  if (V <= 0 || T <= 0 || num_meshes <= 0 || num_inst <= 0 || K <= 0) return fail(DSPT_ERR_ARG, "empty scene");
+  if (T >= (1LL << 24)) return fail(DSPT_ERR_UNSUPPORTED, "meshes exceed 2^24 triangles");
+  for (int32_t m = 0; m < num_meshes; m++)
+    if (mesh_first[m] < 0 || mesh_count[m] <= 0 || mesh_first[m] + mesh_count[m] > T)
+      return fail(DSPT_ERR_ARG, "mesh " + std::to_string(m) + ": triangle range out of bounds");
+  // bottom-level BVHs, concatenated
+  std::vector<float> nodes;
+  std::vector<int32_t> bidx;
+  std::vector<int32_t> root(num_meshes);
+  std::vector<std::array<float, 6>> mbox(num_meshes);
+  std::vector<float> pos(3 * (size_t)V);
+  for (int64_t v = 0; v < V; v++)
+    for (int k = 0; k < 3; k++) pos[3 * v + k] = vertices[v * 20 + k];
+  for (int32_t m = 0; m < num_meshes; m++) {
+    std::vector<float> mn;
+    std::vector<int32_t> mi;
+    std::string err;
+    if (dspt_bvh_build_impl(pos.data(), V, tri_idx + 3 * mesh_first[m], mesh_count[m], 4, mn, mi, err))
+      return fail(DSPT_ERR_ARG, "mesh " + std::to_string(m) + ": " + err);
+    const int64_t noff = (int64_t)nodes.size() / 16, ioff = (int64_t)bidx.size();
+    if (noff + (int64_t)mn.size() / 16 >= (1LL << 24) || ioff + (int64_t)mi.size() >= (1LL << 24))
+      return fail(DSPT_ERR_UNSUPPORTED, "bottom-level BVHs exceed float-encoded index range");
+    root[m] = (int32_t)noff;
+    for (size_t i = 0; i < mn.size() / 16; i++) {
+      float* o = &mn[i * 16];
+      if (o[11] > 0.0f) o[15] += (float)ioff;       // leaf: first
+      else { o[3] += (float)noff; o[7] += (float)noff; }  // inner: children
+    }
+    for (int32_t& t : mi) t += (int32_t)mesh_first[m];
+    nodes.insert(nodes.end(), mn.begin(), mn.end());
+    bidx.insert(bidx.end(), mi.begin(), mi.end());
+    std::array<float, 6> b = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int64_t t = mesh_first[m]; t < mesh_first[m] + mesh_count[m]; t++)
+      for (int k = 0; k < 3; k++) {
+        const float* p = &pos[3 * (size_t)tri_idx[3 * t + k]];
+        for (int d = 0; d < 3; d++) { b[d] = std::min(b[d], p[d]); b[3 + d] = std::max(b[3 + d], p[d]); }
+      }
+    mbox[m] = b;
+  }
+  // instances: world -> object inverse, world bounds (slightly padded so the
+  // top level never culls what the object-space test would hit)
+  std::vector<InstanceDesc> inst(num_inst);
+  std::vector<float> boxes(6 * (size_t)num_inst);
+  for (int32_t i = 0; i < num_inst; i++) {
+    const int32_t m = inst_mesh[i];
+    if (m < 0 || m >= num_meshes) return fail(DSPT_ERR_ARG, "instance " + std::to_string(i) + ": mesh out of range");
+    const float* a = inst_o2w + 12 * (size_t)i;
+    double M[3][4];
+    for (int r = 0; r < 3; r++)
+      for (int k = 0; k < 4; k++) {
+        if (!std::isfinite(a[4 * r + k])) return fail(DSPT_ERR_ARG, "instance " + std::to_string(i) + ": non-finite transform");
+        M[r][k] = a[4 * r + k];
+      }
+    const double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                       M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+    if (!(std::fabs(det) > 1e-12)) return fail(DSPT_ERR_ARG, "instance " + std::to_string(i) + ": singular transform");
+    double Inv[3][3];
+    Inv[0][0] = (M[1][1] * M[2][2] - M[1][2] * M[2][1]) / det;
+    Inv[0][1] = (M[0][2] * M[2][1] - M[0][1] * M[2][2]) / det;
+    Inv[0][2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) / det;
+    Inv[1][0] = (M[1][2] * M[2][0] - M[1][0] * M[2][2]) / det;
+    Inv[1][1] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) / det;
+    Inv[1][2] = (M[0][2] * M[1][0] - M[0][0] * M[1][2]) / det;
+    Inv[2][0] = (M[1][0] * M[2][1] - M[1][1] * M[2][0]) / det;
+    Inv[2][1] = (M[0][1] * M[2][0] - M[0][0] * M[2][1]) / det;
+    Inv[2][2] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) / det;
+    InstanceDesc& D = inst[i];
+    for (int r = 0; r < 3; r++) {
+      const double tr = -(Inv[r][0] * M[0][3] + Inv[r][1] * M[1][3] + Inv[r][2] * M[2][3]);
+      D.w2o[r] = make_float4((float)Inv[r][0], (float)Inv[r][1], (float)Inv[r][2], (float)tr);
+      D.o2w[r] = make_float4(a[4 * r], a[4 * r + 1], a[4 * r + 2], a[4 * r + 3]);
+    }
+    D.root = root[m];
+    D.mesh = m;
+    D.pad0 = D.pad1 = 0;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int corner = 0; corner < 8; corner++) {
+      const double p[3] = {mbox[m][(corner & 1) ? 3 : 0], mbox[m][(corner & 2) ? 4 : 1], mbox[m][(corner & 4) ? 5 : 2]};
+      for (int r = 0; r < 3; r++) {
+        const double w = M[r][0] * p[0] + M[r][1] * p[1] + M[r][2] * p[2] + M[r][3];
+        lo[r] = std::min(lo[r], w);
+        hi[r] = std::max(hi[r], w);
+      }
+    }
+    for (int r = 0; r < 3; r++) {
+      const double pad = 1e-5 * (hi[r] - lo[r]) + 1e-5 * std::max(std::fabs(lo[r]), std::fabs(hi[r])) + 1e-6;
+      boxes[6 * (size_t)i + r] = (float)(lo[r] - pad);
+      boxes[6 * (size_t)i + 3 + r] = (float)(hi[r] + pad);
+    }
+  }
+  std::vector<float> tn;
+  std::vector<int32_t> ti;
+  std::string err;
+  if (dspt_bvh_build_boxes_impl(boxes.data(), num_inst, 1, tn, ti, err)) return fail(DSPT_ERR_ARG, "top level: " + err);
+  SceneRes* s = nullptr;
+  if (int r = make_scene(c, vertices, V, tri_idx, T, nodes.data(), (int64_t)nodes.size() / 16, bidx.data(), (int64_t)bidx.size(),
+                         mats, K, &s))
+    return r;
+  Allocs& A = s->mem;
+  int r = 0;
+  if ((r = A.get(&s->tlas, tn.size() / 4)) || (r = A.get(&s->tlas_idx, ti.size())) || (r = A.get(&s->inst, (size_t)num_inst))) {
+    A.release();
+    s->tex_mem.release();
+    delete s;
+    return r;
+  }
+  HIPCHK(hipMemcpy(s->tlas, tn.data(), tn.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s->tlas_idx, ti.data(), ti.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s->inst, inst.data(), inst.size() * sizeof(InstanceDesc), hipMemcpyHostToDevice));
+  s->TN = (int32_t)(tn.size() / 16);
+  s->TM = (int32_t)ti.size();
+  s->I = num_inst;
   *out = s;
   return 0;
 }
@@ -872,6 +1020,19 @@ int dspt_scene_create(dspt_ctx* c, const float* vertices, int64_t V, const uint3
   return 0;
 }
 
+int dspt_scene_create_instanced(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T,
+                                const int64_t* mesh_first_tri, const int64_t* mesh_num_tris, int32_t num_meshes,
+                                const float* instance_o2w, const int32_t* instance_mesh, int32_t num_instances,
+                                const float* mats, int32_t K, int32_t* scene_id) {
+  if (!c || !scene_id) return fail(DSPT_ERR_ARG, "NULL argument");
+  SceneRes* s = nullptr;
+  if (int r = make_scene_instanced(c, vertices, V, tri_idx, T, mesh_first_tri, mesh_num_tris, num_meshes, instance_o2w,
+                                   instance_mesh, num_instances, mats, K, &s))
+    return r;
+  *scene_id = put_slot(c->scenes, s);
+  return 0;
+}
+
 int dspt_scene_destroy(dspt_ctx* c, int32_t id) {
   if (!c || id < 0 || id >= (int)c->scenes.size() || !c->scenes[id]) return fail(DSPT_ERR_ARG, "unknown scene id");
   if (c->scenes[id]->users > 0) return fail(DSPT_ERR_STATE, "scene is used by live jobs");
@@ -970,6 +1131,30 @@ int dspt_upload_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint3
   if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
   SceneRes* s = nullptr;
   if (int r = make_scene(c, vertices, V, tri_idx, T, nodes, N, bvh_idx, M, mats, K, &s)) return r;
+  if (int r = drain_all(c)) return r;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int users = 0;
+  if (!c->scenes.empty() && c->scenes[0]) {
+    users = c->scenes[0]->users;
+    c->scenes[0]->mem.release();
+    c->scenes[0]->tex_mem.release();
+    delete c->scenes[0];
+  }
+  s->users = users;
+  put_slot(c->scenes, s, 0);
+  c->jobs_dirty = true;
+  return 0;
+}
+
+int dspt_upload_scene_instanced(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T,
+                                const int64_t* mesh_first_tri, const int64_t* mesh_num_tris, int32_t num_meshes,
+                                const float* instance_o2w, const int32_t* instance_mesh, int32_t num_instances,
+                                const float* mats, int32_t K) {
+  if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
+  SceneRes* s = nullptr;
+  if (int r = make_scene_instanced(c, vertices, V, tri_idx, T, mesh_first_tri, mesh_num_tris, num_meshes, instance_o2w,
+                                   instance_mesh, num_instances, mats, K, &s))
+    return r;
   if (int r = drain_all(c)) return r;
   HIPCHK(hipStreamSynchronize(c->stream));
   int users = 0;
@@ -1294,7 +1479,8 @@ int dspt_get_kernel_stats(dspt_ctx* c, double* out) {
 }
 
 // ------------------------------------------------------------------ traversal probes
-static int trace_common(dspt_ctx* c, const float* rays, int64_t n, int any, int32_t* tri, float* t, float* u, float* v) {
+static int trace_common(dspt_ctx* c, const float* rays, int64_t n, int any, int32_t* tri, float* t, float* u, float* v,
+                        int32_t* inst) {
   if (!c || !rays || !tri) return fail(DSPT_ERR_ARG, "NULL argument");
   if (c->scenes.empty() || !c->scenes[0]) return fail(DSPT_ERR_STATE, "trace before dspt_upload_scene");
   if (n <= 0) return 0;
@@ -1303,12 +1489,13 @@ static int trace_common(dspt_ctx* c, const float* rays, int64_t n, int any, int3
   tmp.scene = 0;
   SceneView S = view_of(c, &tmp);
   float4* d_rays = nullptr;
-  int32_t* d_tri = nullptr;
+  int32_t *d_tri = nullptr, *d_inst = nullptr;
   float *d_t = nullptr, *d_u = nullptr, *d_v = nullptr;
   int* d_spill = nullptr;
   int blocks = (int)std::min<int64_t>(2048, grid_for(n, kTraceBlock));
   HIPCHK(hipMalloc(&d_rays, n * 32));
   HIPCHK(hipMalloc(&d_tri, n * 4));
+  if (inst) HIPCHK(hipMalloc(&d_inst, n * 4));
   if (t) {
     HIPCHK(hipMalloc(&d_t, n * 4));
     HIPCHK(hipMalloc(&d_u, n * 4));
@@ -1316,9 +1503,10 @@ static int trace_common(dspt_ctx* c, const float* rays, int64_t n, int any, int3
   }
   HIPCHK(hipMalloc(&d_spill, (size_t)blocks * kTraceBlock * (kStackSize - kLdsStack) * 4));
   HIPCHK(hipMemcpyAsync(d_rays, rays, n * 32, hipMemcpyHostToDevice, c->stream));
-  k_trace_rays<<<blocks, kTraceBlock, 0, c->stream>>>(S, d_rays, n, any, d_tri, d_t, d_u, d_v, d_spill, c->cnt);
+  k_trace_rays<<<blocks, kTraceBlock, 0, c->stream>>>(S, d_rays, n, any, d_tri, d_t, d_u, d_v, d_inst, d_spill, c->cnt);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(tri, d_tri, n * 4, hipMemcpyDeviceToHost, c->stream));
+  if (inst) HIPCHK(hipMemcpyAsync(inst, d_inst, n * 4, hipMemcpyDeviceToHost, c->stream));
   if (t) {
     HIPCHK(hipMemcpyAsync(t, d_t, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(u, d_u, n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1326,19 +1514,26 @@ static int trace_common(dspt_ctx* c, const float* rays, int64_t n, int any, int3
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   (void)hipFree(d_rays); (void)hipFree(d_tri); (void)hipFree(d_spill);
+  if (d_inst) (void)hipFree(d_inst);
   if (d_t) { (void)hipFree(d_t); (void)hipFree(d_u); (void)hipFree(d_v); }
   return 0;
 }
 
 int dspt_trace_closest(dspt_ctx* c, const float* rays, int64_t n, int32_t* tri, float* t, float* u, float* v) {
   if (!t || !u || !v) return fail(DSPT_ERR_ARG, "NULL output");
-  return trace_common(c, rays, n, 0, tri, t, u, v);
+  return trace_common(c, rays, n, 0, tri, t, u, v, nullptr);
+}
+
+int dspt_trace_closest_instanced(dspt_ctx* c, const float* rays, int64_t n, int32_t* inst, int32_t* tri, float* t, float* u,
+                                 float* v) {
+  if (!inst || !t || !u || !v) return fail(DSPT_ERR_ARG, "NULL output");
+  return trace_common(c, rays, n, 0, tri, t, u, v, inst);
 }
 
 int dspt_trace_occluded(dspt_ctx* c, const float* rays, int64_t n, int32_t* occluded) {
   if (!occluded) return fail(DSPT_ERR_ARG, "NULL output");
   std::vector<int32_t> tri(std::max<int64_t>(n, 1));
-  int r = trace_common(c, rays, n, 1, tri.data(), nullptr, nullptr, nullptr);
+  int r = trace_common(c, rays, n, 1, tri.data(), nullptr, nullptr, nullptr, nullptr);
   if (r) return r;
   for (int64_t i = 0; i < n; i++) occluded[i] = tri[i] >= 0 ? 1 : 0;
   return 0;

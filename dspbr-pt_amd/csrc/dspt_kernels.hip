@@ -218,9 +218,6 @@ DI int64_t xcd_block(int64_t b, int64_t nb) {
 #ifndef DSPT_REFILL
 #define DSPT_REFILL 48
 #endif
-#ifndef DSPT_TRACE_DYNAMIC
-#define DSPT_TRACE_DYNAMIC 1
-#endif
 constexpr int kRefill = DSPT_REFILL;
 // Inner-node steps a wave takes before it tests the leaves its lanes reached.
 #ifndef DSPT_INNER_RUN
@@ -246,9 +243,43 @@ DD int refill(bool active, int count, int32_t* next_ptr, bool& exhausted) {
   return q < count ? q : -1;
 }
 
-__global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathState ps,
-                                                           const int32_t* __restrict__ count_ptr, int32_t* __restrict__ next_ptr,
-                                                           int* __restrict__ spill, unsigned long long* __restrict__ cnt) {
+// Per-lane ray walker: one-level scenes, or (INST) scenes whose jobs may be
+// two-level (a per-lane choice by the scene's top-level BVH).
+template <bool INST> struct Walker;
+template <> struct Walker<false> {
+  Geo G;
+  RayState r;
+  DD void start(const SceneView& S, LaneStack& st, f3 o, f3 d, float tfar) {
+    G = geo_of(S);
+    ray_init(r, st, o, d, tfar);
+  }
+  DD int inner(LaneStack& st, unsigned& nv) { return trav_inner(G, r, st, nv); }
+  template <bool ANY> DD bool leaf(unsigned& nt) { return trav_leaf<ANY>(G, r, nt); }
+  DD void restart(LaneStack& st, f3 o, f3 d, float tfar) { ray_init(r, st, o, d, tfar); }
+  DD int inst() const { return -1; }
+};
+template <> struct Walker<true> {
+  Geo2 G;
+  RayState r;
+  InstState q;
+  DD void start(const SceneView& S, LaneStack& st, f3 o, f3 d, float tfar) {
+    G = geo2_of(S);
+    ray_init(r, st, o, d, tfar);
+    inst_init(q);
+  }
+  DD int inner(LaneStack& st, unsigned& nv) { return G.tlas ? trav_inner2(G, r, q, st, nv) : trav_inner(G.g, r, st, nv); }
+  template <bool ANY> DD bool leaf(unsigned& nt) { return G.tlas ? trav_leaf2<ANY>(G, r, q, nt) : trav_leaf<ANY>(G.g, r, nt); }
+  DD void restart(LaneStack& st, f3 o, f3 d, float tfar) {
+    ray_init(r, st, o, d, tfar);
+    inst_init(q);
+  }
+  DD int inst() const { return q.hit; }
+};
+
+template <bool INST>
+__global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathState ps, const int32_t* __restrict__ count_ptr,
+                                         int32_t* __restrict__ next_ptr, int* __restrict__ spill,
+                                         unsigned long long* __restrict__ cnt) {
   __shared__ int smem[kLdsStack * kTraceBlock];
   const int count = *count_ptr;
   LaneStack st;
@@ -256,34 +287,21 @@ __global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathSt
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   st.spill = spill + gtid * (kStackSize - kLdsStack);
   unsigned nv = 0, nt = 0, nr = 0;
-#if !DSPT_TRACE_DYNAMIC
-  {
-    const int64_t nb = gridDim.x, lb = xcd_block(blockIdx.x, nb);
-    for (int64_t q = lb * blockDim.x + threadIdx.x; q < count; q += nb * blockDim.x) {
-      const int p = (int)q;
-      const float4 o = ps.org[p], d = ps.dir[p];
-      const RayHit h = traverse<false>(geo_of(jobs[ps.job[p]].S), xyz(o), xyz(d), kTFAR_MAX, st, nv, nt);
-      ps.hit[p] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
-      nr++;
-    }
-    wave_add_u64(&cnt[4], nv);
-    wave_add_u64(&cnt[5], nt);
-    wave_add_u64(&cnt[9], nr);
-    (void)next_ptr;
-    return;
-  }
-#endif
   bool active = false, exhausted = false;
   int p = 0;
-  Geo G{};
-  RayState r;
+  Walker<INST> w;
+  auto finish_ray = [&]() {
+    ps.hit[p] = make_float4(__int_as_float(w.r.h.tri), w.r.h.t, w.r.h.u, w.r.h.v);
+    if (INST) ps.hinst[p] = w.inst();
+    nr++;
+    active = false;
+  };
   for (;;) {
     const int q = refill(active, count, next_ptr, exhausted);
     if (q >= 0) {
       p = q;
-      G = geo_of(jobs[ps.job[p]].S);
       const float4 o = ps.org[p], d = ps.dir[p];
-      ray_init(r, st, xyz(o), xyz(d), kTFAR_MAX);
+      w.start(jobs[ps.job[p]].S, st, xyz(o), xyz(d), kTFAR_MAX);
       active = true;
     }
     if (__ballot(active) == 0) {
@@ -297,23 +315,14 @@ __global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathSt
       const bool inner = active && !leaf;
       if (__ballot(inner) == 0) break;
       if (inner) {
-        const int res = trav_inner(G, r, st, nv);
-        if (res == 1) {
-          leaf = true;
-        } else if (res == 2) {
-          ps.hit[p] = make_float4(__int_as_float(r.h.tri), r.h.t, r.h.u, r.h.v);
-          nr++;
-          active = false;
-        }
+        const int res = w.inner(st, nv);
+        if (res == 1) leaf = true;
+        else if (res == 2) finish_ray();
       }
     }
     if (leaf) {
-      trav_leaf<false>(G, r, nt);
-      if (ray_done(r)) {
-        ps.hit[p] = make_float4(__int_as_float(r.h.tri), r.h.t, r.h.u, r.h.v);
-        nr++;
-        active = false;
-      }
+      w.template leaf<false>(nt);
+      if (ray_done(w.r)) finish_ray();
     }
   }
   wave_add_u64(&cnt[4], nv);
@@ -324,10 +333,10 @@ __global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathSt
 // Shadow rays of a path: the IBL ray (flag 1), then the point-light ray
 // (flag 2); the unoccluded contributions are summed in that order and added
 // as L += (L_ibl + L_point) * path_weight (misptdl.glsl:72-74).
-__global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, PathState ps,
-                                                              const int32_t* __restrict__ queue,
-                                                              const int32_t* __restrict__ count_ptr, int32_t* __restrict__ next_ptr,
-                                                              int* __restrict__ spill, unsigned long long* __restrict__ cnt) {
+template <bool INST>
+__global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, PathState ps, const int32_t* __restrict__ queue,
+                                            const int32_t* __restrict__ count_ptr, int32_t* __restrict__ next_ptr,
+                                            int* __restrict__ spill, unsigned long long* __restrict__ cnt) {
   __shared__ int smem[kLdsStack * kTraceBlock];
   const int count = *count_ptr;
   LaneStack st;
@@ -335,57 +344,21 @@ __global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, Pat
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   st.spill = spill + gtid * (kStackSize - kLdsStack);
   unsigned nv = 0, nt = 0, nr = 0;
-#if !DSPT_TRACE_DYNAMIC
-  {
-    const int64_t nb = gridDim.x, lb = xcd_block(blockIdx.x, nb);
-    for (int64_t q = lb * blockDim.x + threadIdx.x; q < count; q += nb * blockDim.x) {
-      const int p = queue[q];
-      const Geo G = geo_of(jobs[ps.job[p]].S);
-      const float4 so = ps.sh_org[p];
-      const int flags = __float_as_int(so.w);
-      const f3 org = xyz(so);
-      f3 contrib = sp3(0.0f);
-      if (flags & 1) {
-        const float4 sd = ps.sh_dir[p];
-        const RayHit h = traverse<true>(G, org, xyz(sd), sd.w, st, nv, nt);
-        nr++;
-        if (h.tri < 0) contrib = xyz(ps.sh_c[p]);
-      }
-      if (flags & 2) {
-        const float4 pd = ps.sh_pdir[p];
-        const RayHit h = traverse<true>(G, org, xyz(pd), pd.w, st, nv, nt);
-        nr++;
-        if (h.tri < 0) contrib = contrib + xyz(ps.sh_pc[p]);
-      }
-      const float4 L = ps.L[p];
-      const f3 w = xyz(ps.sh_pw[p]);
-      const f3 nl = xyz(L) + contrib * w;
-      ps.L[p] = make_float4(nl.x, nl.y, nl.z, L.w);
-    }
-    wave_add_u64(&cnt[10], nv);
-    wave_add_u64(&cnt[11], nt);
-    wave_add_u64(&cnt[3], nr);
-    (void)next_ptr;
-    return;
-  }
-#endif
   bool active = false, exhausted = false;
   int p = 0, flags = 0, phase = 0;
-  Geo G{};
-  RayState r;
+  Walker<INST> w;
   f3 org = sp3(0.0f), contrib = sp3(0.0f);
   for (;;) {
     const int q = refill(active, count, next_ptr, exhausted);
     if (q >= 0) {
       p = queue[q];
-      G = geo_of(jobs[ps.job[p]].S);
       const float4 so = ps.sh_org[p];
       flags = __float_as_int(so.w);
       org = xyz(so);
       contrib = sp3(0.0f);
       phase = (flags & 1) ? 0 : 1;
       const float4 sd = phase == 0 ? ps.sh_dir[p] : ps.sh_pdir[p];
-      ray_init(r, st, org, xyz(sd), sd.w);
+      w.start(jobs[ps.job[p]].S, st, org, xyz(sd), sd.w);
       active = true;
     }
     if (__ballot(active) == 0) {
@@ -397,27 +370,27 @@ __global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, Pat
       const bool inner = active && !leaf && !fin;
       if (__ballot(inner) == 0) break;
       if (inner) {
-        const int res = trav_inner(G, r, st, nv);
+        const int res = w.inner(st, nv);
         if (res == 1) leaf = true;
         else if (res == 2) fin = true;
       }
     }
-    if (leaf) fin = trav_leaf<true>(G, r, nt) || ray_done(r);
+    if (leaf) fin = w.template leaf<true>(nt) || ray_done(w.r);
     if (active && fin) {
       nr++;
       if (phase == 0) {
-        if (r.h.tri < 0) contrib = xyz(ps.sh_c[p]);
+        if (w.r.h.tri < 0) contrib = xyz(ps.sh_c[p]);
       } else {
-        if (r.h.tri < 0) contrib = contrib + xyz(ps.sh_pc[p]);
+        if (w.r.h.tri < 0) contrib = contrib + xyz(ps.sh_pc[p]);
       }
       if (phase == 0 && (flags & 2)) {
         phase = 1;
         const float4 pd = ps.sh_pdir[p];
-        ray_init(r, st, org, xyz(pd), pd.w);
+        w.restart(st, org, xyz(pd), pd.w);
       } else {
         const float4 L = ps.L[p];
-        const f3 w = xyz(ps.sh_pw[p]);
-        const f3 nl = xyz(L) + contrib * w;
+        const f3 wt = xyz(ps.sh_pw[p]);
+        const f3 nl = xyz(L) + contrib * wt;
         ps.L[p] = make_float4(nl.x, nl.y, nl.z, L.w);
         active = false;
       }
@@ -427,6 +400,13 @@ __global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, Pat
   wave_add_u64(&cnt[11], nt);
   wave_add_u64(&cnt[3], nr);
 }
+
+template __global__ void k_trace_ext<false>(const DevJob*, PathState, const int32_t*, int32_t*, int*, unsigned long long*);
+template __global__ void k_trace_ext<true>(const DevJob*, PathState, const int32_t*, int32_t*, int*, unsigned long long*);
+template __global__ void k_trace_shadow<false>(const DevJob*, PathState, const int32_t*, const int32_t*, int32_t*, int*,
+                                               unsigned long long*);
+template __global__ void k_trace_shadow<true>(const DevJob*, PathState, const int32_t*, const int32_t*, int32_t*, int*,
+                                              unsigned long long*);
 
 DI void sig_mix(uint32_t& h, uint32_t v) { h = (h ^ v) * 16777619u; }
 
@@ -513,7 +493,7 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
         break;
       }
       n_hits++;
-      Hitpoint hp = fetch_hitpoint(S, org, dir, tri, hit.y, hit.z, hit.w);
+      Hitpoint hp = fetch_hitpoint(S, org, dir, tri, hit.y, hit.z, hit.w, S.tlas ? ps.hinst[p] : -1);
       f3 wi = -dir;
       Shading sh = configure(S, hp, wi);
       // Beer-Lambert absorption inside thick media (misptdl.glsl:60-63)
@@ -695,8 +675,8 @@ __global__ void k_advance(const DevJob* __restrict__ jobs, const int32_t* __rest
 // Standalone traversal on caller rays {org.xyz, tfar, dir.xyz, pad}.
 __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneView S, const float4* __restrict__ rays, int64_t n, int any,
                                                             int32_t* __restrict__ tri, float* __restrict__ t, float* __restrict__ u,
-                                                            float* __restrict__ v, int* __restrict__ spill,
-                                                            unsigned long long* __restrict__ cnt) {
+                                                            float* __restrict__ v, int32_t* __restrict__ inst,
+                                                            int* __restrict__ spill, unsigned long long* __restrict__ cnt) {
   __shared__ int smem[kLdsStack * kTraceBlock];
   LaneStack st;
   st.lds = smem + threadIdx.x;
@@ -705,9 +685,17 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneView S, const f
   unsigned nv = 0, nt = 0;
   for (int64_t i = gtid; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     float4 a = rays[i * 2], b = rays[i * 2 + 1];
-    const Geo G = geo_of(S);
-    RayHit h = any ? traverse<true>(G, xyz(a), xyz(b), a.w, st, nv, nt) : traverse<false>(G, xyz(a), xyz(b), a.w, st, nv, nt);
+    RayHit h;
+    int k = -1;
+    if (S.tlas) {
+      const Geo2 G = geo2_of(S);
+      h = any ? traverse2<true>(G, xyz(a), xyz(b), a.w, st, nv, nt, k) : traverse2<false>(G, xyz(a), xyz(b), a.w, st, nv, nt, k);
+    } else {
+      const Geo G = geo_of(S);
+      h = any ? traverse<true>(G, xyz(a), xyz(b), a.w, st, nv, nt) : traverse<false>(G, xyz(a), xyz(b), a.w, st, nv, nt);
+    }
     tri[i] = h.tri;
+    if (inst) inst[i] = k;
     if (t) { t[i] = h.t; u[i] = h.u; v[i] = h.v; }
   }
   wave_add_u64(&cnt[4], nv);

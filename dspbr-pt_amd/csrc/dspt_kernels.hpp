@@ -60,6 +60,7 @@ struct PathState {
   float4* org;    // extension ray origin
   float4* dir;    // extension ray direction
   float4* hit;    // (tri id bits, t, u, v) of the last extension ray
+  int32_t* hinst; // instance of that hit (two-level scenes; -1 otherwise)
   float4* pw;     // path weight xyz, last BSDF pdf in w
   float4* L;      // radiance xyz (+alpha)
   uint2* rng;     // generator state
@@ -88,8 +89,10 @@ __global__ void k_stream_init(StreamCtl* ctl);
 __global__ void k_plan(StreamCtl* ctl, const StreamSched* sch, int a, int32_t P);
 __global__ void k_regen(PathState ps, const DevJob* jobs, const PassTask* tasks, const StreamSched* sch, const StreamCtl* ctl,
                         unsigned long long* cnt);
+template <bool INST>
 __global__ void k_trace_ext(const DevJob* jobs, PathState ps, const int32_t* count_ptr, int32_t* next_ptr, int* spill,
                             unsigned long long* cnt);
+template <bool INST>
 __global__ void k_trace_shadow(const DevJob* jobs, PathState ps, const int32_t* queue, const int32_t* count_ptr,
                                int32_t* next_ptr, int* spill, unsigned long long* cnt);
 __global__ void k_shade(const DevJob* jobs, PathState cur, PathState nxt, const int32_t* count_in, int32_t* count_out,
@@ -97,4 +100,4 @@ __global__ void k_shade(const DevJob* jobs, PathState cur, PathState nxt, const 
 __global__ void k_accum_ready(const DevJob* jobs, const int32_t* job_list, const int32_t* prog);
 __global__ void k_advance(const DevJob* jobs, const int32_t* job_list, int n, int32_t* prog);
 __global__ void k_trace_rays(SceneView S, const float4* rays, int64_t n, int any, int32_t* tri, float* t, float* u, float* v,
-                             int* spill, unsigned long long* cnt);
+                             int32_t* inst, int* spill, unsigned long long* cnt);

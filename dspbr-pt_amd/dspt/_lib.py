@@ -55,6 +55,11 @@ SIGNATURES = {
     "dspt_get_kernel_stats": (_c_int, [_vp, _dp]),
     "dspt_scene_create": (_c_int, [_vp, _fp, _c_i64, _u32p, _c_i64, _fp, _c_i64, _i32p, _c_i64, _fp, _c_i32, _i32p]),
     "dspt_scene_destroy": (_c_int, [_vp, _c_i32]),
+    "dspt_scene_create_instanced": (_c_int, [_vp, _fp, _c_i64, _u32p, _c_i64, _i64p, _i64p, _c_i32, _fp, _i32p, _c_i32, _fp,
+                                             _c_i32, _i32p]),
+    "dspt_upload_scene_instanced": (_c_int, [_vp, _fp, _c_i64, _u32p, _c_i64, _i64p, _i64p, _c_i32, _fp, _i32p, _c_i32, _fp,
+                                             _c_i32]),
+    "dspt_trace_closest_instanced": (_c_int, [_vp, _fp, _c_i64, _i32p, _i32p, _fp, _fp, _fp]),
     "dspt_ibl_create": (_c_int, [_vp, _fp, _c_i32, _c_i32, _dp, _i32p]),
     "dspt_ibl_destroy": (_c_int, [_vp, _c_i32]),
     "dspt_job_create": (_c_int, [_vp, _c_i32, _c_i32, _fp, _c_i32, _c_i32, _c_i32, _c_i32, _i32p]),
@@ -266,6 +271,47 @@ class Engine:
                                         ptr(nodes), nodes.size // 16, ptr(bvh_idx, ctypes.c_int32), bvh_idx.size,
                                         ptr(mats), mats.size // 68, ctypes.byref(sid)))
         return sid.value
+
+    @staticmethod
+    def _instanced_args(verts, tris, mesh_first, mesh_count, o2w, inst_mesh, mats):
+        verts, mats = f32(verts), f32(mats)
+        tris = np.ascontiguousarray(tris, dtype=np.uint32)
+        mf = np.ascontiguousarray(mesh_first, np.int64)
+        mc = np.ascontiguousarray(mesh_count, np.int64)
+        o2w = f32(o2w).reshape(-1)
+        im = np.ascontiguousarray(inst_mesh, np.int32)
+        if mf.size != mc.size or o2w.size != 12 * im.size:
+            raise ValueError("mesh ranges or instance arrays have mismatched lengths")
+        keep = (verts, tris, mf, mc, o2w, im, mats)
+        args = (ptr(verts), verts.size // 20, ptr(tris, ctypes.c_uint32), tris.size // 3, ptr(mf, ctypes.c_int64),
+                ptr(mc, ctypes.c_int64), mf.size, ptr(o2w), ptr(im, ctypes.c_int32), im.size, ptr(mats), mats.size // 68)
+        return keep, args
+
+    def scene_create_instanced(self, verts, tris, mesh_first, mesh_count, o2w, inst_mesh, mats):
+        """Two-level scene: object-space meshes (triangle ranges of `tris`)
+        placed by instances (o2w f32[I, 3, 4], inst_mesh i32[I])."""
+        keep, args = self._instanced_args(verts, tris, mesh_first, mesh_count, o2w, inst_mesh, mats)
+        sid = ctypes.c_int32(-1)
+        check(self._L.dspt_scene_create_instanced(self.h, *args, ctypes.byref(sid)))
+        del keep
+        return sid.value
+
+    def upload_scene_instanced(self, verts, tris, mesh_first, mesh_count, o2w, inst_mesh, mats):
+        keep, args = self._instanced_args(verts, tris, mesh_first, mesh_count, o2w, inst_mesh, mats)
+        check(self._L.dspt_upload_scene_instanced(self.h, *args))
+        del keep
+
+    def trace_closest_instanced(self, rays):
+        rays = f32(rays).reshape(-1, 8)
+        n = rays.shape[0]
+        inst = np.zeros(n, np.int32)
+        tri = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        check(self._L.dspt_trace_closest_instanced(self.h, ptr(rays), n, ptr(inst, ctypes.c_int32), ptr(tri, ctypes.c_int32),
+                                                   ptr(t), ptr(u), ptr(v)))
+        return inst, tri, t, u, v
 
     def scene_destroy(self, sid):
         check(self._L.dspt_scene_destroy(self.h, int(sid)))

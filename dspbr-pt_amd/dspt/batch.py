@@ -45,6 +45,11 @@ class BatchRenderer:
 
     def add_scene(self, key, scene):
         from ._lib import build_bvh
+        from .scenes import InstancedScene
+        if isinstance(scene, InstancedScene):
+            sid = self.eng.scene_create_instanced(*scene.engine_args())
+            self.scenes[key] = (sid, (scene.scene.num_triangles, 0, 0))
+            return sid
         nodes, bidx = scene.bvhNodes, scene.bvhTriangleIndices
         if nodes is None:
             nodes, bidx = build_bvh(scene.bvhPositionBuffer, scene.bvhIndexBuffer)

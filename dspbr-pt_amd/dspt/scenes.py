@@ -475,3 +475,160 @@ def textured_spheres(seed=6, n_spheres=16, width=256, height=256, spp=16, max_bo
     mb.finish(scene)
     cam = ViewportCamera(position=(0, 3, 12), target=(0, 1, 0), fov=45, near=0.01)
     return Workload("textured_spheres", scene, sky_ibl(*ibl_size), cam, width, height, spp, max_bounces, profile)
+
+
+# ----------------------------------------------------------------- instancing
+def _rock(n_tris, radius, rng):
+    """Displaced UV sphere with about n_tris triangles (closed surface)."""
+    hs = max(4, int(round(math.sqrt(n_tris / 4.0))))
+    p, n, uv, f = uv_sphere(1.0, 2 * hs, hs)
+    ph = rng.uniform(0, 2 * math.pi, 3)
+    amp = rng.uniform(0.05, 0.2)
+    disp = 1.0 + amp * np.sin(3 * p[:, 0] + ph[0]) * np.sin(3 * p[:, 1] + ph[1]) * np.sin(3 * p[:, 2] + ph[2])
+    p = p * (radius * disp)[:, None]
+    return p, vertex_normals(p, f), uv, f
+
+
+def _torus(n_tris, radius, rng):
+    """Torus (major radius `radius`, minor U[0.15, 0.4] of it) with about n_tris triangles."""
+    v = max(4, int(round(math.sqrt(n_tris / 4.0))))
+    u = 2 * v
+    rr = radius * rng.uniform(0.15, 0.4)
+    a = np.arange(u) / u * 2 * math.pi
+    b = np.arange(v) / v * 2 * math.pi
+    A, B = np.meshgrid(a, b, indexing="ij")
+    ring = np.stack([np.cos(A), np.zeros_like(A), np.sin(A)], -1)
+    n = ring * np.cos(B)[..., None] + np.array([0.0, 1.0, 0.0]) * np.sin(B)[..., None]
+    p = ring * radius + n * rr
+    ii = np.arange(u * v).reshape(u, v)
+    i0, i1 = ii, np.roll(ii, -1, 0)
+    j0, j1 = i0, np.roll(i0, -1, 1)
+    k0, k1 = i1, np.roll(i1, -1, 1)
+    f = np.concatenate([np.stack([j0, j1, k1], -1).reshape(-1, 3), np.stack([j0, k1, k0], -1).reshape(-1, 3)])
+    uv = np.stack([A / (2 * math.pi), B / (2 * math.pi)], -1).reshape(-1, 2)
+    return p.reshape(-1, 3), n.reshape(-1, 3), uv, f
+
+
+def random_rotation(rng):
+    """Uniform rotation matrix from a normalised Gaussian quaternion."""
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+class InstancedScene:
+    """Object-space meshes placed by rigid instances (engine extension).
+
+    scene         PathtracingSceneData: object-space vertex stream, u32
+                  indices, materials (one per mesh here)
+    mesh_first / mesh_count   i64 triangle range of each mesh
+    mesh_vfirst / mesh_vcount vertex range of each mesh (for flatten)
+    o2w           f32[I, 3, 4] object -> world affine of each instance
+    inst_mesh     i32[I] mesh of each instance
+
+    flatten() bakes it into one world-space scene the way the reference's
+    loader bakes a glTF node hierarchy (scene_loader.ts:173-222, 244-345):
+    positions by the world matrix, normals and tangents by its normal
+    matrix (the inverse transpose) and renormalised, instance by instance;
+    that is the oracle's view of an instanced scene.
+    """
+
+    def __init__(self):
+        self.scene = PathtracingSceneData()
+        self.mesh_first, self.mesh_count, self.mesh_vfirst, self.mesh_vcount = [], [], [], []
+        self.o2w = np.zeros((0, 3, 4), np.float32)
+        self.inst_mesh = np.zeros(0, np.int32)
+
+    @property
+    def num_instances(self):
+        return len(self.inst_mesh)
+
+    @property
+    def instanced_triangles(self):
+        return int(np.asarray(self.mesh_count, np.int64)[self.inst_mesh].sum())
+
+    def engine_args(self):
+        s = self.scene
+        return (s.vertexBuffer, s.triangleIndexBuffer, np.asarray(self.mesh_first, np.int64),
+                np.asarray(self.mesh_count, np.int64), self.o2w, self.inst_mesh, s.getFlatMaterialBuffer())
+
+    def flatten(self):
+        """(PathtracingSceneData in world space, inst_of_tri i32[T'], mesh_tri_of_tri i32[T'])."""
+        s = self.scene
+        vb = s.vertexBuffer.reshape(-1, VERTEX_STRIDE)
+        tri = s.triangleIndexBuffer.reshape(-1, 3).astype(np.int64)
+        out_v, out_t, inst_of, mtri_of = [], [], [], []
+        nv = 0
+        for i, m in enumerate(self.inst_mesh):
+            M = self.o2w[i].astype(np.float64)
+            R, tr = M[:, :3], M[:, 3]
+            Nm = np.linalg.inv(R).T
+            v0, vc = self.mesh_vfirst[m], self.mesh_vcount[m]
+            t0, tc = self.mesh_first[m], self.mesh_count[m]
+            v = vb[v0:v0 + vc].astype(np.float64).copy()
+            v[:, 0:3] = v[:, 0:3] @ R.T + tr
+            for a, b in ((4, 7), (12, 15)):
+                d = v[:, a:b] @ Nm.T
+                ln = np.linalg.norm(d, axis=1, keepdims=True)
+                v[:, a:b] = np.where(ln > 1e-8, d / np.maximum(ln, 1e-30), d)
+            out_v.append(v.astype(np.float32))
+            out_t.append(tri[t0:t0 + tc] - v0 + nv)
+            inst_of.append(np.full(tc, i, np.int32))
+            mtri_of.append(np.arange(t0, t0 + tc, dtype=np.int32))
+            nv += vc
+        f = PathtracingSceneData()
+        f.materials = list(s.materials)
+        V = np.concatenate(out_v)
+        f.vertexBuffer = V.reshape(-1)
+        f.triangleIndexBuffer = np.concatenate(out_t).astype(np.uint32).reshape(-1)
+        f.bvhPositionBuffer = np.ascontiguousarray(V[:, 0:3]).reshape(-1)
+        f.bvhIndexBuffer = f.triangleIndexBuffer
+        return f, np.concatenate(inst_of), np.concatenate(mtri_of)
+
+
+def instanced_scene(seed=5, n_meshes=16, n_instances=4096, tri_range=(4_000, 64_000), extent=40.0, height=6.0,
+                    width=3840, height_px=2160, spp=256, max_bounces=16, ibl_size=(8192, 4096),
+                    families=("transmission", "dispersion")):
+    """Large instanced scene: n_meshes base meshes (displaced spheres and
+    tori, log-uniform triangle counts in tri_range, radius U[0.4, 1.6]) each
+    with one transmission+volume or dispersion material, placed by
+    n_instances seeded rigid transforms (uniform rotation, translation in
+    [-extent, extent] x [0, height] x [-extent, extent]); log-normal IBL +
+    sun. Draw order: per mesh (n_tris, kind, radius, shape draws, family,
+    material), then per instance (mesh, rotation, translation), then the
+    IBL seed."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = InstancedScene()
+    s = out.scene
+    mb = MeshBuilder()
+    nt = 0
+    for m in range(n_meshes):
+        n_tris = int(round(math.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1])))))
+        kind = int(rng.integers(0, 2))
+        radius = rng.uniform(0.4, 1.6)
+        p, n, uv, f = (_rock if kind == 0 else _torus)(n_tris, radius, rng)
+        fam = families[int(rng.integers(0, len(families)))]
+        mi = s.addMaterial(family_material(rng, fam))
+        out.mesh_vfirst.append(mb.nverts)
+        out.mesh_vcount.append(len(p))
+        out.mesh_first.append(nt)
+        out.mesh_count.append(len(f))
+        nt += len(f)
+        mb.add(p, n, uv, f, mi)
+    mb.finish(s)
+    I = n_instances
+    o2w = np.zeros((I, 3, 4), np.float64)
+    im = np.zeros(I, np.int32)
+    for i in range(I):
+        im[i] = int(rng.integers(0, n_meshes))
+        o2w[i, :, :3] = random_rotation(rng)
+        o2w[i, :, 3] = [rng.uniform(-extent, extent), rng.uniform(0, height), rng.uniform(-extent, extent)]
+    out.o2w = o2w.astype(np.float32)
+    out.inst_mesh = im
+    iseed = int(rng.integers(0, 2**31 - 1))
+    cam = ViewportCamera(position=(0, 14, extent * 1.15), target=(0, 0, 0), fov=45, near=0.01)
+    ibl = lognormal_ibl(*ibl_size, seed=iseed)
+    return Workload("instanced", out, ibl, cam, width, height_px, spp, max_bounces, "webgl-full")

@@ -154,6 +154,27 @@ int dspt_scene_create(dspt_ctx* ctx, const float* vertices, int64_t num_vertices
                       int64_t num_triangles, const float* bvh_nodes, int64_t num_nodes, const int32_t* bvh_indices,
                       int64_t num_bvh_indices, const float* materials, int32_t num_materials, int32_t* scene_id);
 int dspt_scene_destroy(dspt_ctx* ctx, int32_t scene_id);
+
+/* Two-level (instanced) scene. No counterpart in the reference, which
+ * flattens instances into world space (packages/lib/scene_loader.ts:178-222,
+ * 288-305); this keeps one copy per mesh. vertices f32[20*V] in object space,
+ * tri_indices u32[3*T]; mesh m owns triangles [mesh_first_tri[m],
+ * + mesh_num_tris[m]); instance i places mesh instance_mesh[i] with the
+ * object->world affine instance_o2w[12*i .. +12] (rows of a 3x4 matrix).
+ * The engine builds a bottom-level BVH per mesh and a top-level BVH over the
+ * instances. Hits report the mesh triangle id and the instance. */
+int dspt_scene_create_instanced(dspt_ctx* ctx, const float* vertices, int64_t num_vertices, const uint32_t* tri_indices,
+                                int64_t num_triangles, const int64_t* mesh_first_tri, const int64_t* mesh_num_tris,
+                                int32_t num_meshes, const float* instance_o2w, const int32_t* instance_mesh,
+                                int32_t num_instances, const float* materials, int32_t num_materials, int32_t* scene_id);
+/* The same as the single-image scene (replaces scene 0). */
+int dspt_upload_scene_instanced(dspt_ctx* ctx, const float* vertices, int64_t num_vertices, const uint32_t* tri_indices,
+                                int64_t num_triangles, const int64_t* mesh_first_tri, const int64_t* mesh_num_tris,
+                                int32_t num_meshes, const float* instance_o2w, const int32_t* instance_mesh,
+                                int32_t num_instances, const float* materials, int32_t num_materials);
+/* Closest hit on scene 0 reporting the instance too (-1 on a one-level scene). */
+int dspt_trace_closest_instanced(dspt_ctx* ctx, const float* rays, int64_t n, int32_t* inst, int32_t* tri, float* t, float* u,
+                                 float* v);
 int dspt_ibl_create(dspt_ctx* ctx, const float* rgba, int32_t width, int32_t height, double* total_sum, int32_t* ibl_id);
 int dspt_ibl_destroy(dspt_ctx* ctx, int32_t ibl_id);
 
