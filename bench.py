@@ -19,7 +19,11 @@ cores over a bounded sample of the same workload). Two more objects:
            reference's render loop): Mpaths/s and p50/p99 time-to-N-spp;
   serving  Poisson arrival streams over the full-material scene with
            resolution/spp mixes, admission by a device-memory budget,
-           p50/p99 time-to-N-spp and time-to-first-frame against Mpaths/s.
+           p50/p99 time-to-N-spp and time-to-first-frame against Mpaths/s;
+  instanced  a top-level BVH over 4096 rigid instances of 16 meshes
+           (~80M instanced triangles), transmission/volume/dispersion
+           materials, 8192x4096 IBL, 3840x2160 at 256 spp, depth 16: alone
+           and as one long job beside small-job traffic.
 """
 import argparse
 import json
@@ -265,6 +269,81 @@ def serving(wl, args):
             "resolutions": resolutions, "spp": spps, "budget_bytes_for_jobs": 6 << 30, "sweep": out}
 
 
+def instanced(args):
+    """Large instanced scene: a top-level BVH over 4096 seeded rigid instances
+    of 16 base meshes (4k-64k triangles each), transmission+volume and
+    dispersion materials, 8192x4096 log-normal IBL, 3840x2160, depth 16.
+    Rendered alone as one long job, then again as a long job while small
+    jobs from a batched mix arrive (Poisson) and share the pool."""
+    from dspt import Engine, PROFILE_FULL, POLICY_SRPT, build_bvh, scenes
+    from dspt.batch import BatchRenderer, JobSpec, Server, latency_stats
+    t0 = time.time()
+    wl = scenes.instanced_scene(spp=args.inst_spp)
+    s = wl.scene
+    log(f"[bench] instanced: {s.num_instances} instances of {len(s.mesh_count)} meshes, {s.scene.num_triangles} mesh tris, "
+        f"{s.instanced_triangles} instanced tris, generated in {time.time() - t0:.1f}s")
+    eng = Engine(PROFILE_FULL)
+    b = BatchRenderer(eng)
+    t1 = time.time()
+    b.add_scene("inst", s)
+    build_s = time.time() - t1
+    b.add_ibl("inst", wl.ibl)
+    w0 = b.create_job(JobSpec("inst", "inst", wl.camera, 256, 256, 2, max_bounces=wl.max_bounces))
+    eng.jobs_run([w0])
+    eng.job_destroy(w0)
+    long_spec = JobSpec("inst", "inst", wl.camera, wl.width, wl.height, wl.spp, max_bounces=wl.max_bounces)
+
+    eng.reset_counters()
+    eng.set_kernel_timing(True)
+    eng.synchronize()
+    t1 = time.perf_counter()
+    j = b.create_job(long_spec)
+    eng.jobs_run([j])
+    eng.synchronize()
+    el = time.perf_counter() - t1
+    c = eng.counters()
+    ks = eng.kernel_stats()
+    eng.set_kernel_timing(False)
+    eng.job_destroy(j)
+    rays = c["rays_primary"] + c["rays_extension"] + c["rays_shadow"]
+    alone = {"mpaths_per_s": round(long_spec.paths / el / 1e6, 2), "mrays_per_s": round(rays / el / 1e6, 2),
+             "seconds": round(el, 2), "kernel_ms": {k: round(v["total_ms"], 1) for k, v in ks.items()},
+             "node_visits_per_ray": round(c["node_visits"] / max(1, rays), 1),
+             "tri_tests_per_ray": round(c["tri_tests"] / max(1, rays), 1)}
+    log(f"[bench] instanced alone: {alone}")
+
+    # the same long job beside small batched-mix traffic, shortest-remaining-first
+    mix = scenes.batch_mix(seed=4, n_jobs=8, tri_range=(10_000, 200_000), side_range=(256, 512), spp_range=(16, 64))
+    for k, w in enumerate(mix):
+        w.scene.bvhNodes, w.scene.bvhTriangleIndices = build_bvh(w.scene.bvhPositionBuffer, w.scene.bvhIndexBuffer)
+        b.add_scene(k, w.scene)
+        b.add_ibl(k, w.ibl)
+    rng = np.random.default_rng(9)
+    reqs, t = [JobSpec("inst", "inst", wl.camera, wl.width, wl.height, wl.spp, max_bounces=wl.max_bounces)], 0.0
+    while True:
+        t += rng.exponential(1.0 / args.inst_traffic_rate)
+        if t > 0.8 * el:
+            break
+        k = int(rng.integers(0, len(mix)))
+        w = mix[k]
+        reqs.append(JobSpec(k, k, w.camera, w.width, w.height, w.spp, max_bounces=w.max_bounces, arrival=t))
+    eng.set_policy(POLICY_SRPT)
+    t1 = time.perf_counter()
+    rec = Server(b).run(reqs)
+    el2 = time.perf_counter() - t1
+    paths = sum(r.paths for r in reqs)
+    small = latency_stats(rec[1:]) if len(rec) > 1 else None
+    mixed = {"mpaths_per_s": round(paths / el2 / 1e6, 2), "seconds": round(el2, 2), "small_jobs": len(reqs) - 1,
+             "small_rate_jobs_per_s": args.inst_traffic_rate, "policy": "srpt",
+             "long_job_latency_ms": round((rec[0]["done"] - rec[0]["arrival"]) * 1e3, 1),
+             "small_latency": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in small.items()} if small else None}
+    log(f"[bench] instanced + traffic: {mixed}")
+    return {"scene": f"{s.num_instances} instances of {len(s.mesh_count)} meshes, {s.instanced_triangles} instanced tris "
+                     f"({s.scene.num_triangles} stored), {wl.width}x{wl.height} {wl.spp}spp depth{wl.max_bounces}, "
+                     f"IBL {wl.ibl['image']['width']}x{wl.ibl['image']['height']}",
+            "bvh_build_s": round(build_s, 2), "alone": alone, "with_traffic": mixed}
+
+
 def calibrate_copy(mib):
     """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
     per dispatch give the scale between the counters and real HBM bytes."""
@@ -292,6 +371,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[5.0, 25.0, 100.0, 200.0])
     ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--no-instanced", action="store_true", help="skip the large instanced scene")
+    ap.add_argument("--inst-spp", type=int, default=256)
+    ap.add_argument("--inst-traffic-rate", type=float, default=10.0, help="small jobs per second beside the long job")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -381,6 +463,7 @@ def main():
     }
     out["batched"] = None if args.no_batched else batched_vs_plain(args)
     out["serving"] = None if args.no_serving else serving(wl, args)
+    out["instanced"] = None if args.no_instanced else instanced(args)
     if not args.no_cpu_baseline:
         t0 = time.time()
         out["cpu_baseline"] = cpu_baseline(wl, block, args.cpu_budget)

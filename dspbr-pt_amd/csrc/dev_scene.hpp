@@ -17,13 +17,17 @@ struct TexArrayDesc {
   int32_t w, h, layers, pad;
 };
 
-// One instance of a two-level scene: affine transforms (rows of 3x4) and the
-// root of its mesh's bottom-level BVH in SceneView::nodes.
+// One instance of a two-level scene, stored in SceneView::nodes as two
+// 64-byte node slots: the world -> object rows and the root of its mesh's
+// bottom-level BVH first (what the walk reads on entering the instance, one
+// node-sized fetch), the object -> world rows after (read by shading).
 struct InstanceDesc {
-  float4 w2o[3];  // world -> object
-  float4 o2w[3];  // object -> world
+  float4 w2o[3];  // rows of the 3x4 affine
   int32_t root, mesh, pad0, pad1;
+  float4 o2w[3];
+  float4 pad2;
 };
+static_assert(sizeof(InstanceDesc) == 128, "an instance record is two node slots");
 
 struct SceneView {
   const float4* __restrict__ nodes;
@@ -46,12 +50,13 @@ struct SceneView {
   const TexArrayDesc* __restrict__ texarr;
   const uint32_t* __restrict__ texels;
   int32_t num_tex, num_texarr;
-  // two-level scenes (engine extension: the reference flattens instances):
-  // top-level bvh_gpu nodes over instance bounds, one instance per leaf
-  const float4* __restrict__ tlas;
-  const int32_t* __restrict__ tlas_idx;
-  const InstanceDesc* __restrict__ inst;
-  int32_t TN, TM, I, pad;
+  // two-level scenes (engine extension: the reference flattens instances).
+  // `nodes` then holds the meshes' bottom-level BVHs, the top-level BVH over
+  // the instances (root `top`) and the InstanceDesc records from node slot
+  // `inst_base` on; a top-level child index >= inst_base is an instance
+  // record (instance k at inst_base + 2k), so the top level has no leaf
+  // nodes. -1: one level.
+  int32_t top, inst_base;
 };
 
 // The 52-float uniform block (renderer.frag:10-31).
@@ -154,15 +159,13 @@ struct Geo {
 };
 DI Geo geo_of(const SceneView& S) { return Geo{S.nodes, S.tri, S.N, S.M}; }
 
-// Geometry of a two-level traversal.
+// Geometry of a two-level traversal: every level's nodes and the instance
+// records share one array, so a lane carries two more indices, not pointers.
 struct Geo2 {
   Geo g;
-  const float4* __restrict__ tlas;
-  const int32_t* __restrict__ tlas_idx;
-  const InstanceDesc* __restrict__ inst;
-  int32_t TN, TM, I;
+  int32_t top, inst_base;
 };
-DI Geo2 geo2_of(const SceneView& S) { return Geo2{geo_of(S), S.tlas, S.tlas_idx, S.inst, S.TN, S.TM, S.I}; }
+DI Geo2 geo2_of(const SceneView& S) { return Geo2{geo_of(S), S.top, S.inst_base}; }
 
 // Resumable traversal state of one ray (the loop variables of
 // SH/bvh.glsl:149-212). The entry on top of the reference's stack is held in
@@ -329,19 +332,21 @@ DI bool trav_leaf(const Geo& G, RayState& r, unsigned& nt) {
 // triangles are tested by trav_leaf as for one level; `inst` records the
 // instance of the closest hit.
 struct InstState {
-  f3 worg, wdir, winv;
-  int level, base, cur, hit;  // level 1 = inside an instance; cur = its id; hit = instance of the closest hit
+  f3 worg, wdir;       // the world ray while inside an instance
+  int base, cur, hit;  // base >= 0: inside instance `cur`, whose walk sits above stack depth base; hit = instance of the closest hit
 };
 
 DI void inst_init(InstState& q) {
-  q.level = 0; q.base = 0; q.cur = -1; q.hit = -1;
-  q.worg = q.wdir = q.winv = f3{0.0f, 0.0f, 0.0f};
+  q.base = -1; q.cur = -1; q.hit = -1;
+  q.worg = q.wdir = f3{0.0f, 0.0f, 0.0f};
 }
 
 DI int trav_inner2(const Geo2& G, RayState& r, InstState& q, LaneStack& st, unsigned& nv) {
-  if (q.level == 1 && !r.has_next && r.top == q.base) {
-    q.level = 0;
-    r.org = q.worg; r.dir = q.wdir; r.inv = q.winv;
+  if (q.base >= 0 && !r.has_next && r.top == q.base) {
+    // back at the top level: the world ray (its inverse recomputed exactly as ray_init does)
+    q.base = -1;
+    r.org = q.worg; r.dir = q.wdir;
+    r.inv = f3{1.0f / r.dir.x, 1.0f / r.dir.y, 1.0f / r.dir.z};
   }
   if (ray_done(r)) return 2;
   r.step++;
@@ -352,36 +357,35 @@ DI int trav_inner2(const Geo2& G, RayState& r, InstState& q, LaneStack& st, unsi
   } else {
     ni = st.get(--r.top);
   }
-  const bool blas = q.level == 1;
-  const float4* nodes = blas ? G.g.nodes : G.tlas;
-  const int N = blas ? G.g.N : G.TN;
+  const int N = G.g.N;
   if (ni < 0 || ni >= N) return ray_done(r) ? 2 : 0;
   nv++;
-  const float4* nd = nodes + (size_t)ni * 4;
-  const float4 a = ldg(nd), b = ldg(nd + 1), c = ldg(nd + 2), d = ldg(nd + 3);
-  const int cnt = (int)c.w;
-  if (cnt > 0) {
-    const int first = (int)d.w;
-    if (blas) {
-      r.leaf_first = first;
-      r.leaf_end = min(first + min(cnt, kMaxLeafTris), G.g.M);
-      return 1;
-    }
-    // top-level leaf: enter its instance
-    const int k = (first >= 0 && first < G.TM) ? G.tlas_idx[first] : -1;
-    if (k < 0 || k >= G.I || r.top >= kStackSize) return ray_done(r) ? 2 : 0;
-    const InstanceDesc* D = G.inst + k;
-    const float4 w2o[3] = {ldg(&D->w2o[0]), ldg(&D->w2o[1]), ldg(&D->w2o[2])};
-    q.worg = r.org; q.wdir = r.dir; q.winv = r.inv;
-    q.level = 1;
+  if (q.base < 0 && ni >= G.inst_base) {
+    // a top-level child that is an instance record (its world box was the
+    // parent's slab test): enter the instance
+    if (ni + 1 >= N) return ray_done(r) ? 2 : 0;
+    const float4* D = G.g.nodes + (size_t)ni * 4;
+    const float4 w2o[3] = {ldg(D), ldg(D + 1), ldg(D + 2)};
+    const float4 meta = ldg(D + 3);
+    q.worg = r.org; q.wdir = r.dir;
     q.base = r.top;
-    q.cur = k;
+    q.cur = (ni - G.inst_base) >> 1;
     r.org = xf_point(w2o, q.worg);
     r.dir = xf_vec(w2o, q.wdir);
     r.inv = f3{1.0f / r.dir.x, 1.0f / r.dir.y, 1.0f / r.dir.z};
-    r.next = D->root;
+    r.next = __float_as_int(meta.x);
     r.has_next = true;
     return 0;
+  }
+  const float4* nd = G.g.nodes + (size_t)ni * 4;
+  const float4 a = ldg(nd), b = ldg(nd + 1), c = ldg(nd + 2), d = ldg(nd + 3);
+  const int cnt = (int)c.w;
+  if (cnt > 0) {
+    if (q.base < 0) return ray_done(r) ? 2 : 0;  // the top level has no leaves
+    const int first = (int)d.w;
+    r.leaf_first = first;
+    r.leaf_end = min(first + min(cnt, kMaxLeafTris), G.g.M);
+    return 1;
   }
   const int left = (int)a.w, right = (int)b.w;
   const float d0 = slab(xyz(a), xyz(b), r.org, r.inv, r.h.t);
@@ -418,6 +422,7 @@ DI RayHit traverse2(const Geo2& G, f3 org, f3 dir, float tfar, LaneStack& st, un
   RayState r;
   InstState q;
   ray_init(r, st, org, dir, tfar);
+  r.next = G.top;
   inst_init(q);
   for (;;) {
     const int res = trav_inner2(G, r, q, st, nv);

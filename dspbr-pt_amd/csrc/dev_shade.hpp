@@ -28,7 +28,9 @@ DI Hitpoint fetch_hitpoint(const SceneView& S, f3 org, f3 dir, int tri, float t,
   const float4* v1 = S.verts + (size_t)vi[1] * 5;
   const float4* v2 = S.verts + (size_t)vi[2] * 5;
   float4 p0 = v0[0], p1 = v1[0], p2 = v2[0];
-  const InstanceDesc* D = (inst >= 0 && S.inst) ? S.inst + inst : nullptr;
+  const InstanceDesc* D =
+      (inst >= 0 && S.inst_base >= 0) ? reinterpret_cast<const InstanceDesc*>(S.nodes + (size_t)(S.inst_base + 2 * inst) * 4)
+                                      : nullptr;
   float4 o2w[3], w2o[3];
   if (D) {
     for (int k = 0; k < 3; k++) { o2w[k] = D->o2w[k]; w2o[k] = D->w2o[k]; }

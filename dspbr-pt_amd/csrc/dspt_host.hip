@@ -93,11 +93,8 @@ struct SceneRes {
   TexArrayDesc* texarr = nullptr;
   uint32_t* texels = nullptr;
   int32_t num_tex = 0, num_texarr = 0;
-  // two-level scenes
-  float4* tlas = nullptr;
-  int32_t* tlas_idx = nullptr;
-  InstanceDesc* inst = nullptr;
-  int32_t TN = 0, TM = 0, I = 0;
+  // two-level scenes: root of the top level and first instance record in `nodes`
+  int32_t top = -1, inst_base = -1, I = 0;
 };
 
 struct IblRes {
@@ -272,8 +269,7 @@ SceneView view_of(dspt_ctx* c, const JobRes* j) {
   S.N = s->N; S.M = s->M; S.T = s->T; S.K = s->K;
   S.texinfo = s->texinfo; S.texarr = s->texarr; S.texels = s->texels;
   S.num_tex = s->num_tex; S.num_texarr = s->num_texarr;
-  S.tlas = s->tlas; S.tlas_idx = s->tlas_idx; S.inst = s->inst;
-  S.TN = s->TN; S.TM = s->TM; S.I = s->I;
+  S.top = s->top; S.inst_base = s->inst_base;
   const IblRes* ib = (j->ibl >= 0 && j->ibl < (int)c->ibls.size()) ? c->ibls[j->ibl] : nullptr;
   if (ib) {
     S.env = ib->env; S.pdf = ib->pdf; S.cdf = ib->cdf; S.ypdf = ib->ypdf; S.ycdf = ib->ycdf;
@@ -499,7 +495,7 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
   // two-level traversal only while a job on a two-level scene is live
   bool inst = false;
   for (const JobRes* j : c->jobs)
-    if (j && j->alive && c->scenes[j->scene] && c->scenes[j->scene]->tlas) inst = true;
+    if (j && j->alive && c->scenes[j->scene] && c->scenes[j->scene]->top >= 0) inst = true;
   for (int k = 0; k < c->chunk_iters; k++) {
     const int a = c->qsel;
     const PathState cur = c->ps[a], nxt = c->ps[a ^ 1];
@@ -615,7 +611,7 @@ int validate_nodes(const float* nodes, int64_t n) {
   return 0;
 }
 int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, const float* nodes, int64_t N,
-               const int32_t* bvh_idx, int64_t M, const float* mats, int32_t K, SceneRes** out) {
+               const int32_t* bvh_idx, int64_t M, const float* mats, int32_t K, SceneRes** out, bool own_nodes = false) {
   if (!vertices || !tri_idx || !nodes || !bvh_idx || !mats) return fail(DSPT_ERR_ARG, "NULL scene buffer");
   if (V <= 0 || T <= 0 || N <= 0 || M <= 0 || K <= 0) return fail(DSPT_ERR_ARG, "empty scene");
   if (T >= (1LL << 24) || N >= (1LL << 30) || M >= (1LL << 30))
@@ -634,7 +630,8 @@ int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tr
         return fail(DSPT_ERR_ARG, "material " + std::to_string(k) + ": texture id is not a finite value below 2^24");
       if (id >= 0.0f) max_tex = std::max(max_tex, (int32_t)id);
     }
-  if (int r = validate_nodes(nodes, N)) return r;
+  if (!own_nodes)
+    if (int r = validate_nodes(nodes, N)) return r;
   HIPCHK(hipSetDevice(c->device));
   SceneRes* s = new SceneRes();
   Allocs& A = s->mem;
@@ -677,11 +674,13 @@ int make_scene_instanced(dspt_ctx* c, const float* vertices, int64_t V, const ui
   for (int32_t m = 0; m < num_meshes; m++)
     if (mesh_first[m] < 0 || mesh_count[m] <= 0 || mesh_first[m] + mesh_count[m] > T)
       return fail(DSPT_ERR_ARG, "mesh " + std::to_string(m) + ": triangle range out of bounds");
+  for (int64_t i = 0; i < 3 * T; i++)
+    if ((int64_t)tri_idx[i] >= V) return fail(DSPT_ERR_ARG, "triangle index " + std::to_string(i) + " out of range");
   // bottom-level BVHs, concatenated
   std::vector<float> nodes;
   std::vector<int32_t> bidx;
   std::vector<int32_t> root(num_meshes);
-  std::vector<std::array<float, 6>> mbox(num_meshes);
+  std::vector<std::vector<uint32_t>> mverts(num_meshes);  // vertices each mesh uses
   std::vector<float> pos(3 * (size_t)V);
   for (int64_t v = 0; v < V; v++)
     for (int k = 0; k < 3; k++) pos[3 * v + k] = vertices[v * 20 + k];
@@ -703,13 +702,10 @@ int make_scene_instanced(dspt_ctx* c, const float* vertices, int64_t V, const ui
     for (int32_t& t : mi) t += (int32_t)mesh_first[m];
     nodes.insert(nodes.end(), mn.begin(), mn.end());
     bidx.insert(bidx.end(), mi.begin(), mi.end());
-    std::array<float, 6> b = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    for (int64_t t = mesh_first[m]; t < mesh_first[m] + mesh_count[m]; t++)
-      for (int k = 0; k < 3; k++) {
-        const float* p = &pos[3 * (size_t)tri_idx[3 * t + k]];
-        for (int d = 0; d < 3; d++) { b[d] = std::min(b[d], p[d]); b[3 + d] = std::max(b[3 + d], p[d]); }
-      }
-    mbox[m] = b;
+    std::vector<uint32_t>& mv = mverts[m];
+    mv.assign(tri_idx + 3 * mesh_first[m], tri_idx + 3 * (mesh_first[m] + mesh_count[m]));
+    std::sort(mv.begin(), mv.end());
+    mv.erase(std::unique(mv.begin(), mv.end()), mv.end());
   }
   // instances: world -> object inverse, world bounds (slightly padded so the
   // top level never culls what the object-space test would hit)
@@ -747,9 +743,12 @@ int make_scene_instanced(dspt_ctx* c, const float* vertices, int64_t V, const ui
     D.root = root[m];
     D.mesh = m;
     D.pad0 = D.pad1 = 0;
+    D.pad2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // world box of the transformed vertices (tighter than the transformed
+    // object box for rotated instances)
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int corner = 0; corner < 8; corner++) {
-      const double p[3] = {mbox[m][(corner & 1) ? 3 : 0], mbox[m][(corner & 2) ? 4 : 1], mbox[m][(corner & 4) ? 5 : 2]};
+    for (uint32_t vi : mverts[m]) {
+      const float* p = &pos[3 * (size_t)vi];
       for (int r = 0; r < 3; r++) {
         const double w = M[r][0] * p[0] + M[r][1] * p[1] + M[r][2] * p[2] + M[r][3];
         lo[r] = std::min(lo[r], w);
@@ -766,23 +765,45 @@ int make_scene_instanced(dspt_ctx* c, const float* vertices, int64_t V, const ui
   std::vector<int32_t> ti;
   std::string err;
   if (dspt_bvh_build_boxes_impl(boxes.data(), num_inst, 1, tn, ti, err)) return fail(DSPT_ERR_ARG, "top level: " + err);
-  SceneRes* s = nullptr;
-  if (int r = make_scene(c, vertices, V, tri_idx, T, nodes.data(), (int64_t)nodes.size() / 16, bidx.data(), (int64_t)bidx.size(),
-                         mats, K, &s))
-    return r;
-  Allocs& A = s->mem;
-  int r = 0;
-  if ((r = A.get(&s->tlas, tn.size() / 4)) || (r = A.get(&s->tlas_idx, ti.size())) || (r = A.get(&s->inst, (size_t)num_inst))) {
-    A.release();
-    s->tex_mem.release();
-    delete s;
-    return r;
+  // one node array: bottom levels, the top level's inner nodes (children
+  // offset; a child that is a one-instance leaf points at the instance
+  // record instead, so entering an instance costs no leaf fetch), instance
+  // records
+  const int64_t nb = (int64_t)nodes.size() / 16, TN = (int64_t)tn.size() / 16;
+  const int64_t N = nb + TN + 2 * (int64_t)num_inst;
+  if (N >= (1LL << 24)) return fail(DSPT_ERR_UNSUPPORTED, "two-level node array exceeds float-encoded index range");
+  const int64_t inst_base = nb + TN;
+  auto remap = [&](int64_t child, int64_t* out) {
+    if (child < 0 || child >= TN) return false;
+    const float* o = &tn[child * 16];
+    if (o[11] > 0.0f) {
+      const int first = (int)o[15];
+      if (o[11] != 1.0f || first < 0 || first >= (int)ti.size()) return false;
+      *out = inst_base + 2 * (int64_t)ti[first];
+    } else {
+      *out = nb + child;
+    }
+    return true;
+  };
+  for (int64_t i = 0; i < TN; i++) {
+    float* o = &tn[i * 16];
+    if (o[11] > 0.0f) continue;
+    int64_t l, r;
+    if (!remap((int64_t)o[3], &l) || !remap((int64_t)o[7], &r)) return fail(DSPT_ERR_STATE, "top level: malformed node");
+    o[3] = (float)l;
+    o[7] = (float)r;
   }
-  HIPCHK(hipMemcpy(s->tlas, tn.data(), tn.size() * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(s->tlas_idx, ti.data(), ti.size() * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(s->inst, inst.data(), inst.size() * sizeof(InstanceDesc), hipMemcpyHostToDevice));
-  s->TN = (int32_t)(tn.size() / 16);
-  s->TM = (int32_t)ti.size();
+  int64_t top = 0;
+  if (!remap(0, &top)) return fail(DSPT_ERR_STATE, "top level: malformed root");
+  nodes.insert(nodes.end(), tn.begin(), tn.end());
+  const size_t ib = nodes.size();
+  nodes.resize(ib + (size_t)num_inst * 32);
+  std::memcpy(nodes.data() + ib, inst.data(), (size_t)num_inst * sizeof(InstanceDesc));
+  SceneRes* s = nullptr;
+  if (int r = make_scene(c, vertices, V, tri_idx, T, nodes.data(), N, bidx.data(), (int64_t)bidx.size(), mats, K, &s, true))
+    return r;
+  s->top = (int32_t)top;
+  s->inst_base = (int32_t)inst_base;
   s->I = num_inst;
   *out = s;
   return 0;

@@ -32,11 +32,16 @@
 #define SHADE_BOUNDS __launch_bounds__(256)
 #endif
 // Traversal kernels: full occupancy (8 waves per SIMD, <= 64 VGPRs) keeps
-// enough node fetches in flight.
+// enough node fetches in flight. The two-level walk carries the world ray
+// and the instance pointers on top (per lane: lanes may serve different
+// scenes), which does not fit 64 VGPRs without scratch spills.
 #ifndef DSPT_TRACE_WAVES
 #define DSPT_TRACE_WAVES 8
 #endif
-#define TRACE_BOUNDS __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(DSPT_TRACE_WAVES)))
+#ifndef DSPT_TRACE_WAVES_INST
+#define DSPT_TRACE_WAVES_INST 6
+#endif
+#define TRACE_BOUNDS __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(INST ? DSPT_TRACE_WAVES_INST : DSPT_TRACE_WAVES)))
 
 // --------------------------------------------------------------------------
 // Scene preparation: per BVH index slot, the triangle's three positions in
@@ -264,13 +269,13 @@ template <> struct Walker<true> {
   InstState q;
   DD void start(const SceneView& S, LaneStack& st, f3 o, f3 d, float tfar) {
     G = geo2_of(S);
-    ray_init(r, st, o, d, tfar);
-    inst_init(q);
+    restart(st, o, d, tfar);
   }
-  DD int inner(LaneStack& st, unsigned& nv) { return G.tlas ? trav_inner2(G, r, q, st, nv) : trav_inner(G.g, r, st, nv); }
-  template <bool ANY> DD bool leaf(unsigned& nt) { return G.tlas ? trav_leaf2<ANY>(G, r, q, nt) : trav_leaf<ANY>(G.g, r, nt); }
+  DD int inner(LaneStack& st, unsigned& nv) { return G.top >= 0 ? trav_inner2(G, r, q, st, nv) : trav_inner(G.g, r, st, nv); }
+  template <bool ANY> DD bool leaf(unsigned& nt) { return G.top >= 0 ? trav_leaf2<ANY>(G, r, q, nt) : trav_leaf<ANY>(G.g, r, nt); }
   DD void restart(LaneStack& st, f3 o, f3 d, float tfar) {
     ray_init(r, st, o, d, tfar);
+    if (G.top >= 0) r.next = G.top;
     inst_init(q);
   }
   DD int inst() const { return q.hit; }
@@ -493,7 +498,7 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
         break;
       }
       n_hits++;
-      Hitpoint hp = fetch_hitpoint(S, org, dir, tri, hit.y, hit.z, hit.w, S.tlas ? ps.hinst[p] : -1);
+      Hitpoint hp = fetch_hitpoint(S, org, dir, tri, hit.y, hit.z, hit.w, S.top >= 0 ? ps.hinst[p] : -1);
       f3 wi = -dir;
       Shading sh = configure(S, hp, wi);
       // Beer-Lambert absorption inside thick media (misptdl.glsl:60-63)
@@ -687,7 +692,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneView S, const f
     float4 a = rays[i * 2], b = rays[i * 2 + 1];
     RayHit h;
     int k = -1;
-    if (S.tlas) {
+    if (S.top >= 0) {
       const Geo2 G = geo2_of(S);
       h = any ? traverse2<true>(G, xyz(a), xyz(b), a.w, st, nv, nt, k) : traverse2<false>(G, xyz(a), xyz(b), a.w, st, nv, nt, k);
     } else {
