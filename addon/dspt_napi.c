@@ -297,6 +297,50 @@ static napi_value js_scene_create(napi_env env, napi_callback_info info) {
   return r;
 }
 
+/* sceneCreateInstanced(h, verts f32, tris u32, meshFirst i32, meshCount i32, o2w f32[12I], instMesh i32[I], mats f32)
+ * -> scene id (two-level scene: object-space meshes placed by instances) */
+static napi_value js_scene_create_instanced(napi_env env, napi_callback_info info) {
+  size_t argc = 8;
+  napi_value argv[8];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 8) { napi_throw_error(env, NULL, "sceneCreateInstanced expects 8 arguments"); return NULL; }
+  dspt_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return NULL;
+  void *v, *t, *mf, *mc, *x, *im, *m;
+  size_t lv, lt, lmf, lmc, lx, lim, lm;
+  if (get_typed(env, argv[1], napi_float32_array, &v, &lv) || get_typed(env, argv[2], napi_uint32_array, &t, &lt) ||
+      get_typed(env, argv[3], napi_int32_array, &mf, &lmf) || get_typed(env, argv[4], napi_int32_array, &mc, &lmc) ||
+      get_typed(env, argv[5], napi_float32_array, &x, &lx) || get_typed(env, argv[6], napi_int32_array, &im, &lim) ||
+      get_typed(env, argv[7], napi_float32_array, &m, &lm))
+    return NULL;
+  if (lmf != lmc || lx != 12 * lim) {
+    napi_throw_range_error(env, NULL, "mesh ranges or instance arrays have mismatched lengths");
+    return NULL;
+  }
+  int64_t* first = (int64_t*)malloc((lmf + 1) * sizeof(int64_t));
+  int64_t* count = (int64_t*)malloc((lmf + 1) * sizeof(int64_t));
+  if (!first || !count) {
+    free(first);
+    free(count);
+    napi_throw_error(env, NULL, "out of host memory");
+    return NULL;
+  }
+  for (size_t i = 0; i < lmf; i++) {
+    first[i] = ((const int32_t*)mf)[i];
+    count[i] = ((const int32_t*)mc)[i];
+  }
+  int32_t id = -1;
+  int rc = dspt_scene_create_instanced(c, (const float*)v, (int64_t)(lv / 20), (const uint32_t*)t, (int64_t)(lt / 3), first,
+                                       count, (int32_t)lmf, (const float*)x, (const int32_t*)im, (int32_t)lim,
+                                       (const float*)m, (int32_t)(lm / 68), &id);
+  free(first);
+  free(count);
+  if (rc) return throw_dspt(env, rc);
+  napi_value r;
+  CALL(env, napi_create_int32(env, id, &r));
+  return r;
+}
+
 /* sceneSetTextures(h, sceneId, texinfos f32[8X], dims i32[3A], texels u8) */
 static napi_value js_scene_set_textures(napi_env env, napi_callback_info info) {
   size_t argc = 5;
@@ -530,6 +574,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"sceneCreate", NULL, js_scene_create, NULL, NULL, NULL, napi_enumerable, NULL},
       {"sceneDestroy", NULL, js_scene_destroy, NULL, NULL, NULL, napi_enumerable, NULL},
       {"sceneSetTextures", NULL, js_scene_set_textures, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"sceneCreateInstanced", NULL, js_scene_create_instanced, NULL, NULL, NULL, napi_enumerable, NULL},
       {"iblCreate", NULL, js_ibl_create, NULL, NULL, NULL, napi_enumerable, NULL},
       {"iblDestroy", NULL, js_ibl_destroy, NULL, NULL, NULL, napi_enumerable, NULL},
       {"jobCreate", NULL, js_job_create, NULL, NULL, NULL, napi_enumerable, NULL},

@@ -17,6 +17,10 @@ export interface SceneBuffers {
   // material textures: TexInfo records (8 f32 each, material.ts:483-523), RGBA8 texture
   // arrays (width, height, layers per array) and their texels, row 0 first
   textures?: { texInfos: Float32Array; arrayDims: Int32Array; texels: Uint8Array };
+  // two-level scene (RenderServer.addScene only): the vertex stream and indices are in
+  // object space; mesh m = triangles [meshFirst[m], +meshCount[m]); instance i places mesh
+  // mesh[i] with the row-major 3x4 object->world affine objectToWorld[12i .. 12i+12)
+  instances?: { meshFirst: Int32Array; meshCount: Int32Array; objectToWorld: Float32Array; mesh: Int32Array };
 }
 export declare class PathtracingRenderer {
   constructor(parameters?: { materialProfile?: 'webgl-lean' | 'webgl-full'; device?: number; width?: number; height?: number });

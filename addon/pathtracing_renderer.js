@@ -166,6 +166,17 @@ class RenderServer {
   }
 
   addScene(scene) {
+    if (scene.instances) {
+      // two-level scene: object-space meshes placed by instances (engine extension)
+      const mats = new Float32Array(scene.materials.length * 68);
+      scene.materials.forEach((m, i) => mats.set(m.data, i * 68));
+      const inst = scene.instances;
+      const id = native.sceneCreateInstanced(this._h, scene.vertexBuffer, scene.triangleIndexBuffer, inst.meshFirst,
+                                             inst.meshCount, inst.objectToWorld, inst.mesh, mats);
+      if (scene.textures) native.sceneSetTextures(this._h, id, scene.textures.texInfos, scene.textures.arrayDims, scene.textures.texels);
+      this._scenes.set(id, { counts: [scene.triangleIndexBuffer.length / 3, 0, 0], lights: scene.lights || [] });
+      return id;
+    }
     let nodes = scene.bvhNodes, idx = scene.bvhTriangleIndices;
     if (!nodes) {
       const r = native.buildBvh(scene.bvhPositionBuffer, scene.bvhIndexBuffer, 4);

@@ -15,14 +15,21 @@ function load(dir, name, T) {
   const meta = JSON.parse(fs.readFileSync(path.join(dir, 'meta.json'), 'utf8'));
   const server = new RenderServer({ materialProfile: meta.profile, policy: meta.policy });
   const mats = load(dir, 'mats.f32', Float32Array);
-  const scene = server.addScene({
+  const sceneData = {
     vertexBuffer: load(dir, 'verts.f32', Float32Array),
     triangleIndexBuffer: load(dir, 'tris.u32', Uint32Array),
     bvhPositionBuffer: load(dir, 'bvhpos.f32', Float32Array),
     bvhIndexBuffer: load(dir, 'tris.u32', Uint32Array),
     materials: Array.from({ length: mats.length / 68 }, (_, i) => ({ data: mats.subarray(i * 68, (i + 1) * 68) })),
     lights: [],
-  });
+  };
+  if (meta.instanced) {
+    sceneData.instances = {
+      meshFirst: load(dir, 'meshfirst.i32', Int32Array), meshCount: load(dir, 'meshcount.i32', Int32Array),
+      objectToWorld: load(dir, 'o2w.f32', Float32Array), mesh: load(dir, 'instmesh.i32', Int32Array),
+    };
+  }
+  const scene = server.addScene(sceneData);
   const ibl = server.addIbl({ image: { data: load(dir, 'ibl.f32', Float32Array), width: meta.iblWidth, height: meta.iblHeight } });
   const camera = { fov: meta.fov, near: meta.near, matrixWorld: { elements: meta.matrixWorld }, position: meta.position };
   const progress = meta.requests.map(() => 0);

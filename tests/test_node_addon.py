@@ -86,12 +86,24 @@ def test_node_render_matches_python_engine(addon_built, tmp_path):
 
 
 @pytest.mark.gpu
-def test_node_render_server_matches_python_jobs(addon_built, tmp_path):
+@pytest.mark.parametrize("kind", ["plain", "instanced"])
+def test_node_render_server_matches_python_jobs(addon_built, tmp_path, kind):
     """Requests served by the Node RenderServer (continuous batching, staggered
-    arrivals) produce the same images as the same jobs run from Python."""
+    arrivals) produce the same images as the same jobs run from Python, for a
+    one-level and a two-level (instanced) scene."""
     from dspt import Engine, PROFILE_LEAN, build_bvh, pack_frame_block, scenes
-    wl = scenes.sphere_field(n_spheres=16)
-    s = wl.scene
+    inst = None
+    if kind == "plain":
+        wl = scenes.sphere_field(n_spheres=16)
+        s = wl.scene
+    else:
+        wl = scenes.instanced_scene(seed=7, n_meshes=3, n_instances=40, tri_range=(300, 2000), extent=6.0, height=3.0,
+                                    ibl_size=(128, 64))
+        inst, s = wl.scene, wl.scene.scene
+        np.asarray(inst.mesh_first, np.int32).tofile(tmp_path / "meshfirst.i32")
+        np.asarray(inst.mesh_count, np.int32).tofile(tmp_path / "meshcount.i32")
+        np.asarray(inst.o2w, np.float32).tofile(tmp_path / "o2w.f32")
+        np.asarray(inst.inst_mesh, np.int32).tofile(tmp_path / "instmesh.i32")
     img = wl.ibl["image"]
     np.asarray(s.vertexBuffer, np.float32).tofile(tmp_path / "verts.f32")
     np.asarray(s.triangleIndexBuffer, np.uint32).tofile(tmp_path / "tris.u32")
@@ -102,20 +114,25 @@ def test_node_render_server_matches_python_jobs(addon_built, tmp_path):
             {"width": 96, "height": 32, "spp": 4, "delayMs": 10}]
     meta = {"profile": "webgl-lean", "policy": "rr", "maxBounces": 6, "iblWidth": img["width"], "iblHeight": img["height"],
             "fov": wl.camera.fov, "near": wl.camera.near, "matrixWorld": [float(x) for x in wl.camera.matrixWorld],
-            "position": dict(zip("xyz", wl.camera.position)), "requests": reqs}
+            "position": dict(zip("xyz", wl.camera.position)), "requests": reqs, "instanced": inst is not None}
     (tmp_path / "meta.json").write_text(json.dumps(meta))
     r = subprocess.run(["node", os.path.join(ROOT, "tests", "node_serve.js"), str(tmp_path)], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     info = json.loads(r.stdout.strip().splitlines()[-1])
     assert info["frames"] == [q["spp"] for q in reqs] and info["progress"] == info["frames"]
-    nodes, bidx = build_bvh(s.bvhPositionBuffer, s.bvhIndexBuffer)
     eng = Engine(PROFILE_LEAN)
-    sid = eng.scene_create(s.vertexBuffer, s.triangleIndexBuffer, nodes, bidx, s.getFlatMaterialBuffer())
+    if inst is None:
+        nodes, bidx = build_bvh(s.bvhPositionBuffer, s.bvhIndexBuffer)
+        sid = eng.scene_create(s.vertexBuffer, s.triangleIndexBuffer, nodes, bidx, s.getFlatMaterialBuffer())
+        counts = (s.num_triangles, len(nodes) // 16, len(bidx), 0)
+    else:
+        sid = eng.scene_create_instanced(*inst.engine_args())
+        counts = (s.num_triangles, 0, 0, 0)
     iid, tot = eng.ibl_create(img["data"], img["width"], img["height"])
     for i, q in enumerate(reqs):
         blk = pack_frame_block(wl.camera, q["width"], q["height"], max_bounces=6, ibl_size=(img["width"], img["height"]),
-                               ibl_total_sum=tot, scene_counts=(s.num_triangles, len(nodes) // 16, len(bidx), 0))
+                               ibl_total_sum=tot, scene_counts=counts)
         j = eng.job_create(sid, iid, blk, q["width"], q["height"], q["spp"])
         eng.jobs_run([j])
         ref = eng.job_read_accum(j)
