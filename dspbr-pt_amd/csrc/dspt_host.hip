@@ -197,6 +197,9 @@ struct dspt_ctx {
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
   std::vector<hipEvent_t> ev_pool;
   double kstat[DSPT_K_COUNT][2] = {};
+  // canvas output of the display chain, grown on demand
+  char* post_buf = nullptr;
+  size_t post_cap = 0;
 };
 
 namespace {
@@ -1022,7 +1025,7 @@ int dspt_destroy(dspt_ctx* c) {
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->ev_chunk) if (e) (void)hipEventDestroy(e);
   for (void* d : {(void*)c->cnt, (void*)c->d_jobs, (void*)c->d_tasks, (void*)c->d_ctl, (void*)c->d_sched, (void*)c->d_prog,
-                  (void*)c->d_joblist})
+                  (void*)c->d_joblist, (void*)c->post_buf})
     if (d) (void)hipFree(d);
   for (void* h : {(void*)c->h_jobs, (void*)c->h_rb, (void*)c->h_sched, (void*)c->h_tasks, (void*)c->h_joblist})
     if (h) (void)hipHostFree(h);
@@ -1357,6 +1360,47 @@ int dspt_job_read_accum(dspt_ctx* c, int32_t id, float* rgba) {
   HIPCHK(hipMemcpyAsync(rgba, j->accum, (size_t)j->npix() * 16, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
+}
+
+int dspt_job_postprocess(dspt_ctx* c, int32_t id, int32_t mode, float exposure, int32_t gamma, int32_t fxaa, int32_t ow,
+                         int32_t oh, uint8_t* rgba8, float* rgba32) {
+  if (!c || !job_ok(c, id)) return fail(DSPT_ERR_ARG, "unknown job id");
+  if (ow <= 0 || oh <= 0 || (int64_t)ow * oh > (1LL << 28)) return fail(DSPT_ERR_ARG, "invalid canvas size");
+  if (!(exposure == exposure)) return fail(DSPT_ERR_ARG, "exposure is NaN");
+  if (!rgba8 && !rgba32) return 0;
+  JobRes* j = c->jobs[id];
+  HIPCHK(hipSetDevice(c->device));
+  const int64_t n = (int64_t)ow * oh;
+  const size_t need = (size_t)n * ((rgba8 ? 4 : 0) + (rgba32 ? 16 : 0));
+  if (need > c->post_cap) {
+    if (c->post_buf) HIPCHK(hipFree(c->post_buf));
+    c->post_buf = nullptr;
+    c->post_cap = 0;
+    if (hipMalloc(&c->post_buf, need) != hipSuccess) return fail(DSPT_ERR_MEMORY, "display buffer");
+    c->post_cap = need;
+  }
+  float4* d32 = rgba32 ? reinterpret_cast<float4*>(c->post_buf) : nullptr;
+  uint32_t* d8 = rgba8 ? reinterpret_cast<uint32_t*>(c->post_buf + (rgba32 ? (size_t)n * 16 : 0)) : nullptr;
+  post::Source S{j->accum, j->W, j->H, post::Params{mode, gamma ? 1 : 0, exposure}};
+  // fxaa.frag's u_inv_res: 1 / renderRes in JS doubles, rounded by uniform2f
+  const float ix = (float)(1.0 / (double)j->W), iy = (float)(1.0 / (double)j->H);
+  timed(c, DSPT_K_POST, [&] {
+    k_postprocess<<<grid_for(n, 256), 256, 0, c->stream>>>(S, fxaa ? 1 : 0, ix, iy, ow, oh, d8, d32);
+  });
+  HIPCHK(hipGetLastError());
+  if (rgba32) HIPCHK(hipMemcpyAsync(rgba32, d32, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream));
+  if (rgba8) HIPCHK(hipMemcpyAsync(rgba8, d8, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drain_events(c);
+  return 0;
+}
+
+int dspt_postprocess(dspt_ctx* c, int32_t mode, float exposure, int32_t gamma, int32_t fxaa, int32_t ow, int32_t oh,
+                     uint8_t* rgba8, float* rgba32) {
+  if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
+  int id;
+  if (int r = legacy_job(c, &id)) return r;
+  return dspt_job_postprocess(c, id, mode, exposure, gamma, fxaa, ow, oh, rgba8, rgba32);
 }
 
 int dspt_job_read_samples(dspt_ctx* c, int32_t id, float* rgba, uint32_t* sigs, int32_t* prim) {

@@ -18,6 +18,7 @@
 // items of different frames and jobs were interleaved in the pool.
 #include "dspt_kernels.hpp"
 #include "dev_shade.hpp"
+#include "dev_post.hpp"
 
 // Occupancy target of the shading kernel (waves per SIMD; 0 = compiler's
 // choice). Left alone the compiler gives the full-profile shading kernel
@@ -705,4 +706,23 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneView S, const f
   }
   wave_add_u64(&cnt[4], nv);
   wave_add_u64(&cnt[5], nt);
+}
+
+// --------------------------------------------------------------------------
+// Display chain (renderer.ts:665-700), one thread per canvas pixel: the
+// display resample (display.frag) picks a texel of the display buffer; FXAA
+// (fxaa.frag) or the tonemap pass alone (tonemap.frag) produces it from the
+// accumulation; the 8-bit canvas value and/or the float display value are
+// written. ix, iy = fxaa.frag's u_inv_res.
+__global__ __launch_bounds__(256) void k_postprocess(post::Source S, int fxaa_on, float ix, float iy, int ow, int oh,
+                                                     uint32_t* __restrict__ out8, float4* __restrict__ out32) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)ow * oh) return;
+  const int ox = (int)(i % ow), oy = (int)(i / ow);
+  const int sx = post::nearest((post::vuv(ox, ow) + 1.0f) * 0.5f, S.W);
+  const int sy = post::nearest((post::vuv(oy, oh) + 1.0f) * 0.5f, S.H);
+  const float4 c = fxaa_on ? post::fxaa(S, sx, sy, ix, iy) : S.texel(sx, sy);
+  if (out32) out32[i] = c;
+  if (out8)
+    out8[i] = post::unorm8(c.x) | (post::unorm8(c.y) << 8) | (post::unorm8(c.z) << 16) | (post::unorm8(c.w) << 24);
 }

@@ -29,6 +29,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
 
 SIGNATURES = {
     "dspt_last_error": (ctypes.c_char_p, []),
@@ -60,6 +61,8 @@ SIGNATURES = {
     "dspt_upload_scene_instanced": (_c_int, [_vp, _fp, _c_i64, _u32p, _c_i64, _i64p, _i64p, _c_i32, _fp, _i32p, _c_i32, _fp,
                                              _c_i32]),
     "dspt_trace_closest_instanced": (_c_int, [_vp, _fp, _c_i64, _i32p, _i32p, _fp, _fp, _fp]),
+    "dspt_postprocess": (_c_int, [_vp, _c_i32, ctypes.c_float, _c_i32, _c_i32, _c_i32, _c_i32, _u8p, _fp]),
+    "dspt_job_postprocess": (_c_int, [_vp, _c_i32, _c_i32, ctypes.c_float, _c_i32, _c_i32, _c_i32, _c_i32, _u8p, _fp]),
     "dspt_ibl_create": (_c_int, [_vp, _fp, _c_i32, _c_i32, _dp, _i32p]),
     "dspt_ibl_destroy": (_c_int, [_vp, _c_i32]),
     "dspt_job_create": (_c_int, [_vp, _c_i32, _c_i32, _fp, _c_i32, _c_i32, _c_i32, _c_i32, _i32p]),
@@ -82,7 +85,7 @@ SCHED_PASS = 1
 POLICY_RR = 0
 POLICY_FIFO = 1
 POLICY_SRPT = 2
-KERNEL_CLASSES = ["raygen", "trace_ext", "shade", "trace_shadow", "accumulate"]
+KERNEL_CLASSES = ["raygen", "trace_ext", "shade", "trace_shadow", "accumulate", "post"]
 
 
 class DsptError(RuntimeError):
@@ -312,6 +315,25 @@ class Engine:
         check(self._L.dspt_trace_closest_instanced(self.h, ptr(rays), n, ptr(inst, ctypes.c_int32), ptr(tri, ctypes.c_int32),
                                                    ptr(t), ptr(u), ptr(v)))
         return inst, tri, t, u, v
+
+    def _post(self, fn, args, size, tonemap_mode, exposure, gamma, fxaa, out_size, want_float):
+        ow, oh = out_size or size
+        u8 = np.zeros((oh, ow, 4), np.uint8)
+        f = np.zeros((oh, ow, 4), np.float32) if want_float else None
+        check(fn(self.h, *args, int(tonemap_mode), float(exposure), 1 if gamma else 0, 1 if fxaa else 0, int(ow), int(oh),
+                 ptr(u8, ctypes.c_uint8), ptr(f)))
+        return (u8, f) if want_float else u8
+
+    def postprocess(self, tonemap_mode=0, exposure=1.0, gamma=True, fxaa=False, out_size=None, want_float=False):
+        """Display chain (renderer.ts:665-700) on the single-image
+        accumulation: RGBA8 canvas u8[oh, ow, 4] (row 0 = bottom), and the
+        float display buffer when want_float."""
+        return self._post(self._L.dspt_postprocess, (), (self.W, self.H), tonemap_mode, exposure, gamma, fxaa, out_size, want_float)
+
+    def job_postprocess(self, jid, tonemap_mode=0, exposure=1.0, gamma=True, fxaa=False, out_size=None, want_float=False):
+        w, h, _ = self._job_dims[jid]
+        return self._post(self._L.dspt_job_postprocess, (int(jid),), (w, h), tonemap_mode, exposure, gamma, fxaa, out_size,
+                          want_float)
 
     def scene_destroy(self, sid):
         check(self._L.dspt_scene_destroy(self.h, int(sid)))

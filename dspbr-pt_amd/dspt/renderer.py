@@ -239,6 +239,21 @@ class PathtracingRenderer:
         """Linear RGBA f32 accumulation buffer, shape (H, W, 4), row 0 = bottom."""
         return self._engine.read_accum()
 
+    def readDisplay(self, want_float=False):
+        """What the reference's display chain puts on the canvas
+        (renderer.ts:665-700): tonemapping (an unknown name applies none, as
+        indexOf() == -1 does), exposure, gamma, FXAA, resampled to the display
+        resolution. RGBA8 u8[displayH, displayW, 4], row 0 = bottom; with
+        want_float also the float display buffer."""
+        mode = self.tonemappingModes.index(self._tonemapping) if self._tonemapping in self.tonemappingModes else -1
+        return self._engine.postprocess(mode, self._exposure, bool(self._enableGamma), bool(self._enableFxaa),
+                                        (self.displayRes[0], self.displayRes[1]), want_float)
+
+    def saveImage(self, path):
+        """The displayed canvas as a PNG file (top row first)."""
+        from .image import write_png
+        write_png(path, self.readDisplay()[::-1])
+
     @property
     def engine(self):
         return self._engine
@@ -329,3 +344,9 @@ class PathtracingViewport:
 
     def readAccumulation(self):
         return self.renderer.readAccumulation()
+
+    def readDisplay(self, want_float=False):
+        return self.renderer.readDisplay(want_float)
+
+    def saveImage(self, path):
+        self.renderer.saveImage(path)

@@ -121,6 +121,7 @@ enum {
   DSPT_K_SHADE = 2,
   DSPT_K_TRACE_SHADOW = 3, /* any-hit traversal of NEE shadow rays */
   DSPT_K_ACCUM = 4,        /* running mean of complete frames (accum_ready + advance) */
+  DSPT_K_POST = 5,         /* display chain (dspt_postprocess / dspt_job_postprocess) */
   DSPT_K_COUNT = 8
 };
 /* Stats per class, double[DSPT_K_COUNT * 4]: {launches, total_ms,
@@ -128,6 +129,25 @@ enum {
  * Reset together with dspt_reset_counters(). */
 int dspt_set_kernel_timing(dspt_ctx* ctx, int enable);
 int dspt_get_kernel_stats(dspt_ctx* ctx, double* out);
+
+/* Display chain of renderer.ts:665-700 on the accumulation (the implicit
+ * job of the single-image API, or job `job_id`): the tonemap pass
+ * (SH/tonemap.frag; tonemap_mode = index into the renderer's
+ * ["None", "Reinhard", "Cineon", "AcesFilm"] (renderer.ts:192): 0 exposure
+ * only, 1 Reinhard, 2 Cineon, 3 ACES; 4 is the shader's Uncharted2; any other
+ * value applies no operator, as indexOf() == -1 does), gamma 1/2.2 when
+ * gamma != 0, the FXAA pass (SH/fxaa.frag) when fxaa != 0, and the display
+ * pass (SH/display.frag) resampling to an out_w x out_h canvas (the
+ * pixelRatio scaling; pass the render size for none). All buffers behave as
+ * the reference's RGBA32F NEAREST + CLAMP_TO_EDGE textures
+ * (gl_utils.ts:22-33). Outputs, either may be NULL: rgba8 u8[4*out_w*out_h],
+ * the canvas value floor(clamp(c, 0, 1) * 255 + 0.5); rgba32
+ * f32[4*out_w*out_h], the display buffer. Row 0 = bottom (glReadPixels
+ * order). Synchronous. */
+int dspt_postprocess(dspt_ctx* ctx, int32_t tonemap_mode, float exposure, int32_t gamma, int32_t fxaa, int32_t out_w,
+                     int32_t out_h, uint8_t* rgba8, float* rgba32);
+int dspt_job_postprocess(dspt_ctx* ctx, int32_t job_id, int32_t tonemap_mode, float exposure, int32_t gamma, int32_t fxaa,
+                         int32_t out_w, int32_t out_h, uint8_t* rgba8, float* rgba32);
 
 /* Traversal entry points on caller rays {org.xyz, tfar, dir.xyz, pad}
  * (f32[8*n]): closest hit (SH/bvh.glsl:214-216) -> tri id (-1 = miss), t,

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "integrator_oracle.hpp"
+#include "post_oracle.hpp"
 
 using namespace orc;
 
@@ -227,5 +228,22 @@ int oracle_tex_value(const Scene* s, float id, const float* uv, int n, float* ou
   return 0;
 }
 int oracle_frame_struct_size() { return (int)sizeof(Frame); }
+
+// renderer.ts:665-700 over an ow x oh canvas: tonemap (+ gamma), optional
+// FXAA, display resample, 8-bit conversion. rgba32 / rgba8 may be NULL.
+void oracle_postprocess(const float* accum, int W, int H, int mode, float exposure, int gamma, int fxaa, int ow, int oh,
+                        float* rgba32, uint8_t* rgba8) {
+  orc_post::Post P{accum, W, H, mode, gamma, exposure};
+  for (int y = 0; y < oh; y++)
+    for (int x = 0; x < ow; x++) {
+      float o[4];
+      orc_post::display(P, fxaa, x, y, ow, oh, o);
+      const size_t k = 4 * ((size_t)y * ow + x);
+      for (int c = 0; c < 4; c++) {
+        if (rgba32) rgba32[k + c] = o[c];
+        if (rgba8) rgba8[k + c] = orc_post::unorm8(o[c]);
+      }
+    }
+}
 
 }  // extern "C"

@@ -1,0 +1,168 @@
+// CPU restatement of the reference's display chain (test infrastructure only:
+// the checker for the engine's post-process kernel; nothing in the product
+// links it).
+//
+//   renderer.ts:665-700   tonemap pass -> optional FXAA pass -> display pass
+//   tonemap.frag:20-93    exposure / Reinhard / Cineon / ACES / Uncharted2,
+//                         gamma 1/2.2
+//   fxaa.frag:22-64       mattdesl FXAA on the tonemapped buffer
+//   display.frag:9-12     resample of the display buffer to the canvas size
+//   gl_utils.ts:22-33     every render buffer is RGBA32F, NEAREST, CLAMP_TO_EDGE
+//
+// Pinned definitions (GL leaves them to the implementation; the engine uses
+// the same ones):
+//   NEAREST fetch at normalised u: texel floor(u * W) clamped to [0, W-1]
+//   fragment centre (x + 0.5, y + 0.5); v_uv = (2(x+0.5))/W - 1
+//   GLSL min/max/clamp select forms (pt_oracle.hpp), no FMA contraction
+//   `mediump vec2 dir` (fxaa.frag:42) evaluated in f32
+//   float -> 8-bit framebuffer: floor(clamp(c, 0, 1) * 255 + 0.5), NaN -> 0
+// Constant expressions fold in f32 (1.0/2.2, 1.0/3.0 - 0.5, 0.25 * (1/8)).
+#pragma once
+#include <cmath>
+#include <cstdint>
+
+namespace orc_post {
+
+struct C3 { float x, y, z; };
+static inline float gmin(float x, float y) { return y < x ? y : x; }
+static inline float gmax(float x, float y) { return x < y ? y : x; }
+static inline float sat(float a) { return gmin(gmax(a, 0.0f), 1.0f); }
+static inline C3 mul(C3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+static inline C3 add(C3 a, C3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+
+// tonemap.frag:20-23
+static inline C3 linear_tm(C3 c, float e) { return mul(c, e); }
+// tonemap.frag:25-28
+static inline C3 reinhard_tm(C3 c, float e) {
+  c = mul(c, e);
+  return {sat(c.x / (1.0f + c.x)), sat(c.y / (1.0f + c.y)), sat(c.z / (1.0f + c.z))};
+}
+// tonemap.frag:30-37 (Hable's operator; not reachable from the renderer's mode list)
+static inline float u2_helper(float x) {
+  const float n = x * (0.15f * x + 0.10f * 0.50f) + 0.20f * 0.02f;
+  const float d = x * (0.15f * x + 0.50f) + 0.20f * 0.30f;
+  return gmax(n / d - 0.02f / 0.30f, 0.0f);
+}
+static inline C3 uncharted2_tm(C3 c, float e) {
+  c = mul(c, e);
+  const float w = u2_helper(1.0f);
+  return {sat(u2_helper(c.x) / w), sat(u2_helper(c.y) / w), sat(u2_helper(c.z) / w)};
+}
+// tonemap.frag:39-44 (Hejl / Burgess-Dawson)
+static inline float cineon1(float x) {
+  x = gmax(0.0f, x - 0.004f);
+  return powf((x * (6.2f * x + 0.5f)) / (x * (6.2f * x + 1.7f) + 0.06f), 2.2f);
+}
+static inline C3 cineon_tm(C3 c, float e) {
+  c = mul(c, e);
+  return {cineon1(c.x), cineon1(c.y), cineon1(c.z)};
+}
+// tonemap.frag:46-71 (Hill's ACES fit; mat3 constructors are column-major)
+static inline float rrt_odt(float v) {
+  const float a = v * (v + 0.0245786f) - 0.000090537f;
+  const float b = v * (0.983729f * v + 0.4329510f) + 0.238081f;
+  return a / b;
+}
+static inline C3 aces_tm(C3 c) {
+  const float in[3][3] = {{0.59719f, 0.07600f, 0.02840f}, {0.35458f, 0.90834f, 0.13383f}, {0.04823f, 0.01566f, 0.83777f}};
+  const float out[3][3] = {{1.60475f, -0.10208f, -0.00327f}, {-0.53108f, 1.10813f, -0.07276f}, {-0.07367f, -0.00605f, 1.07602f}};
+  const float v[3] = {c.x, c.y, c.z};
+  float a[3], b[3];
+  for (int r = 0; r < 3; r++) a[r] = in[0][r] * v[0] + in[1][r] * v[1] + in[2][r] * v[2];
+  for (int r = 0; r < 3; r++) a[r] = rrt_odt(a[r]);
+  for (int r = 0; r < 3; r++) b[r] = out[0][r] * a[0] + out[1][r] * a[1] + out[2][r] * a[2];
+  return {sat(b[0]), sat(b[1]), sat(b[2])};
+}
+
+// tonemap.frag:73-93 on one texel. mode = the renderer's index into
+// ["None", "Reinhard", "Cineon", "AcesFilm"] (renderer.ts:192, 675); 4 is
+// the shader's Uncharted2; any other value leaves the colour unchanged.
+static inline void tonemap(const float* px, int mode, float exposure, int gamma, float* out) {
+  C3 c{px[0], px[1], px[2]};
+  if (mode == 0) c = linear_tm(c, exposure);
+  if (mode == 1) c = reinhard_tm(c, exposure);
+  if (mode == 2) c = cineon_tm(c, exposure);
+  if (mode == 3) c = aces_tm(mul(c, exposure / 0.6f));
+  if (mode == 4) c = uncharted2_tm(c, exposure);
+  if (gamma) {
+    const float g = 1.0f / 2.2f;
+    c = {powf(c.x, g), powf(c.y, g), powf(c.z, g)};
+  }
+  out[0] = c.x; out[1] = c.y; out[2] = c.z; out[3] = px[3];
+}
+
+static inline int nearest(float u, int n) {
+  int i = (int)floorf(u * (float)n);
+  return i < 0 ? 0 : (i > n - 1 ? n - 1 : i);
+}
+static inline float vuv(int p, int n) { return (2.0f * (float(p) + 0.5f)) / float(n) - 1.0f; }
+
+struct Post {
+  const float* accum;  // RGBA32F, row 0 = bottom
+  int W, H, mode, gamma;
+  float exposure;
+  // texel (i, j) of the tonemapped buffer: fragment (i, j) of the tonemap
+  // pass, which fetches the accumulation at its own uv (tonemap.frag:74-76)
+  void texel(int i, int j, float* o) const {
+    const int ti = nearest((vuv(i, W) + 1.0f) * 0.5f, W), tj = nearest((vuv(j, H) + 1.0f) * 0.5f, H);
+    tonemap(accum + 4 * ((size_t)tj * W + ti), mode, exposure, gamma, o);
+  }
+  // texture(postprocess, uv) with NEAREST + CLAMP_TO_EDGE
+  C3 fetch(float u, float v) const {
+    float o[4];
+    texel(nearest(u, W), nearest(v, H), o);
+    return {o[0], o[1], o[2]};
+  }
+};
+
+static inline float luma(C3 c) { return (c.x * 0.299f + c.y * 0.587f) + c.z * 0.114f; }
+
+// fxaa.frag:22-59 at fragment (x, y) of the tonemapped buffer
+static inline void fxaa(const Post& P, int x, int y, float* out) {
+  const float ix = (float)(1.0 / (double)P.W), iy = (float)(1.0 / (double)P.H);  // uniform2f of JS doubles
+  const float fx = float(x) + 0.5f, fy = float(y) + 0.5f;
+  const C3 nw = P.fetch((fx + -1.0f) * ix, (fy + -1.0f) * iy);
+  const C3 ne = P.fetch((fx + 1.0f) * ix, (fy + -1.0f) * iy);
+  const C3 sw = P.fetch((fx + -1.0f) * ix, (fy + 1.0f) * iy);
+  const C3 se = P.fetch((fx + 1.0f) * ix, (fy + 1.0f) * iy);
+  float m[4];
+  P.texel(nearest(fx * ix, P.W), nearest(fy * iy, P.H), m);
+  const C3 rgbM{m[0], m[1], m[2]};
+  const float lNW = luma(nw), lNE = luma(ne), lSW = luma(sw), lSE = luma(se), lM = luma(rgbM);
+  const float lmin = gmin(lM, gmin(gmin(lNW, lNE), gmin(lSW, lSE)));
+  const float lmax = gmax(lM, gmax(gmax(lNW, lNE), gmax(lSW, lSE)));
+  float dx = -((lNW + lNE) - (lSW + lSE));
+  float dy = (lNW + lSW) - (lNE + lSE);
+  const float reduce = gmax((((lNW + lNE) + lSW) + lSE) * (0.25f * (1.0f / 8.0f)), 1.0f / 128.0f);
+  const float rcp = 1.0f / (gmin(fabsf(dx), fabsf(dy)) + reduce);
+  dx = gmin(8.0f, gmax(-8.0f, dx * rcp)) * ix;
+  dy = gmin(8.0f, gmax(-8.0f, dy * rcp)) * iy;
+  const float bx = fx * ix, by = fy * iy;
+  const float k1 = 1.0f / 3.0f - 0.5f, k2 = 2.0f / 3.0f - 0.5f;
+  const C3 a1 = P.fetch(bx + dx * k1, by + dy * k1), a2 = P.fetch(bx + dx * k2, by + dy * k2);
+  const C3 rgbA = mul(add(a1, a2), 0.5f);
+  const C3 b1 = P.fetch(bx + dx * -0.5f, by + dy * -0.5f), b2 = P.fetch(bx + dx * 0.5f, by + dy * 0.5f);
+  const C3 rgbB = add(mul(rgbA, 0.5f), mul(add(b1, b2), 0.25f));
+  const float lB = luma(rgbB);
+  const C3 c = (lB < lmin || lB > lmax) ? rgbA : rgbB;
+  out[0] = c.x; out[1] = c.y; out[2] = c.z; out[3] = m[3];
+}
+
+static inline uint8_t unorm8(float c) {
+  if (!(c > 0.0f)) return 0;
+  if (c >= 1.0f) return 255;
+  return (uint8_t)floorf(c * 255.0f + 0.5f);
+}
+
+// renderer.ts:665-700: output pixel (ox, oy) of an ow x oh canvas
+static inline void display(const Post& P, int fxaa_on, int ox, int oy, int ow, int oh, float* out) {
+  const float u = (vuv(ox, ow) + 1.0f) * 0.5f, v = (vuv(oy, oh) + 1.0f) * 0.5f;
+  const int sx = nearest(u, P.W), sy = nearest(v, P.H);
+  if (fxaa_on) {
+    fxaa(P, sx, sy, out);
+  } else {
+    P.texel(sx, sy, out);
+  }
+}
+
+}  // namespace orc_post

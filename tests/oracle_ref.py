@@ -40,7 +40,8 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(ORACLE_LIB):
+        srcs = [os.path.join(ORACLE_DIR, f) for f in os.listdir(ORACLE_DIR) if f.endswith((".hpp", ".cpp"))]
+        if not os.path.exists(ORACLE_LIB) or max(os.path.getmtime(f) for f in srcs) > os.path.getmtime(ORACLE_LIB):
             build_oracle()
         L = ctypes.CDLL(ORACLE_LIB)
         L.oracle_ibl_cdf.restype = ctypes.c_double
@@ -56,6 +57,8 @@ def lib():
         L.oracle_ibl_sample.argtypes = [ctypes.POINTER(CScene), _f, _f, ctypes.c_int, _f]
         L.oracle_ibl_lookup.argtypes = [ctypes.POINTER(CScene), _f, _f, ctypes.c_int, _f]
         L.oracle_tex_value.argtypes = [ctypes.POINTER(CScene), ctypes.c_float, _f, ctypes.c_int, _f]
+        L.oracle_postprocess.argtypes = [_f, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, _f, ctypes.POINTER(ctypes.c_uint8)]
         assert L.oracle_scene_struct_size() == ctypes.sizeof(CScene)
         assert L.oracle_frame_struct_size() == 52 * 4
         _lib = L
@@ -197,3 +200,16 @@ def bsdf_sample(mat, geo, rnd, profile=0, use_double=True):
     lib().oracle_bsdf_sample(_p(np.ascontiguousarray(mat, np.float32)), profile, _p(np.ascontiguousarray(geo, np.float32)),
                              _p(rnd), rnd.shape[0], 1 if use_double else 0, _p(out, ctypes.c_double))
     return out
+
+
+def postprocess(accum, mode, exposure=1.0, gamma=True, fxaa=False, out_size=None):
+    """renderer.ts:665-700 on an RGBA32F accumulation f32[H, W, 4] (row 0 =
+    bottom): (display buffer f32[oh, ow, 4], 8-bit canvas u8[oh, ow, 4])."""
+    a = np.ascontiguousarray(accum, np.float32)
+    H, W = a.shape[0], a.shape[1]
+    ow, oh = out_size or (W, H)
+    f = np.zeros((oh, ow, 4), np.float32)
+    u8 = np.zeros((oh, ow, 4), np.uint8)
+    lib().oracle_postprocess(_p(a), W, H, int(mode), float(exposure), int(bool(gamma)), int(bool(fxaa)), ow, oh, _p(f),
+                             _p(u8, ctypes.c_uint8))
+    return f, u8
