@@ -506,6 +506,34 @@ static napi_value js_read_accum(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* postprocess(h, jobId (-1: the single-image accumulation), tonemapMode, exposure, gamma, fxaa, outW, outH)
+ * -> Uint8Array(4*outW*outH), the display chain's RGBA8 canvas, row 0 = bottom */
+static napi_value js_postprocess(napi_env env, napi_callback_info info) {
+  size_t argc = 8;
+  napi_value argv[8];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 8) { napi_throw_error(env, NULL, "postprocess expects 8 arguments"); return NULL; }
+  dspt_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return NULL;
+  const int32_t job = (int32_t)get_i64(env, argv[1]), mode = (int32_t)get_i64(env, argv[2]);
+  double exposure = 1.0;
+  CALL(env, napi_get_value_double(env, argv[3], &exposure));
+  bool gamma = false, fxaa = false;
+  CALL(env, napi_get_value_bool(env, argv[4], &gamma));
+  CALL(env, napi_get_value_bool(env, argv[5], &fxaa));
+  const int64_t ow = get_i64(env, argv[6]), oh = get_i64(env, argv[7]);
+  if (ow <= 0 || oh <= 0 || ow * oh > (1LL << 28)) { napi_throw_range_error(env, NULL, "invalid canvas size"); return NULL; }
+  void* data = NULL;
+  napi_value ab, out;
+  CALL(env, napi_create_arraybuffer(env, (size_t)(4 * ow * oh), &data, &ab));
+  int rc = job < 0 ? dspt_postprocess(c, mode, (float)exposure, gamma, fxaa, (int32_t)ow, (int32_t)oh, (uint8_t*)data, NULL)
+                   : dspt_job_postprocess(c, job, mode, (float)exposure, gamma, fxaa, (int32_t)ow, (int32_t)oh, (uint8_t*)data,
+                                          NULL);
+  if (rc) return throw_dspt(env, rc);
+  CALL(env, napi_create_typedarray(env, napi_uint8_array, (size_t)(4 * ow * oh), ab, 0, &out));
+  return out;
+}
+
 /* counters(h) -> Float64Array(16) */
 static napi_value js_counters(napi_env env, napi_callback_info info) {
   size_t argc = 1;
@@ -568,6 +596,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"setFrame", NULL, js_set_frame, NULL, NULL, NULL, napi_enumerable, NULL},
       {"render", NULL, js_render, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readAccum", NULL, js_read_accum, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"postprocess", NULL, js_postprocess, NULL, NULL, NULL, napi_enumerable, NULL},
       {"counters", NULL, js_counters, NULL, NULL, NULL, napi_enumerable, NULL},
       {"buildBvh", NULL, js_build_bvh, NULL, NULL, NULL, napi_enumerable, NULL},
       {"version", NULL, js_version, NULL, NULL, NULL, napi_enumerable, NULL},

@@ -38,6 +38,8 @@ export declare class PathtracingRenderer {
          frameFinishedCB?: (frameCount: number) => void, renderingFinishedCB?: () => void,
          framesPerCall?: number): Promise<void>;
   readAccumulation(): Float32Array;
+  readDisplay(): Uint8Array;          // RGBA8 canvas of the display chain, row 0 = bottom
+  saveImage(file: string): void;      // that canvas as a PNG
   counters(): Float64Array;
 }
 export declare function packFrameBlock(r: PathtracingRenderer, camera: PathtracingCamera, frameCount: number): Float32Array;

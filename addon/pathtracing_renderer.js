@@ -4,8 +4,10 @@
 // N-API addon. Same method names, argument meaning and error behaviour:
 // setScene(scene) / setIBL(texture) / resize(w, h) / render(camera, numSamples,
 // tileCB, frameCB, finishedCB) / stopRendering() / resetAccumulation() and the
-// settings properties. The GL post chain (tonemap, FXAA, display) is not part
-// of the engine; readAccumulation() returns the linear RGBA32F buffer.
+// settings properties. readAccumulation() returns the linear RGBA32F buffer;
+// readDisplay() runs the display chain (tonemap, gamma, FXAA, display resample,
+// renderer.ts:665-700) on the GPU and returns the RGBA8 canvas; saveImage(path)
+// writes that canvas as a PNG.
 const path = require('path');
 const native = require(path.join(__dirname, 'build', 'dspt.node'));
 
@@ -134,7 +136,55 @@ class PathtracingRenderer {
     return out;
   }
 
+  // renderer.ts:192 lists the operators; an unknown name gives indexOf() == -1: no operator
+  readDisplay() {
+    const mode = TONEMAPPING_MODES.indexOf(this._tonemapping);
+    return native.postprocess(this._h, -1, mode, this._exposure, !!this._enableGamma, !!this._enableFxaa,
+                              this.displayRes[0], this.displayRes[1]);
+  }
+
+  saveImage(file) {
+    require('fs').writeFileSync(file, encodePng(this.readDisplay(), this.displayRes[0], this.displayRes[1]));
+  }
+
   counters() { return native.counters(this._h); }
+}
+
+const TONEMAPPING_MODES = ['None', 'Reinhard', 'Cineon', 'AcesFilm'];
+
+// PNG of an RGBA8 canvas given bottom row first (GL order): rows are written top first
+const CRC_TABLE = (() => {
+  const t = new Uint32Array(256);
+  for (let n = 0; n < 256; n++) {
+    let c = n;
+    for (let k = 0; k < 8; k++) c = c & 1 ? 0xedb88320 ^ (c >>> 1) : c >>> 1;
+    t[n] = c >>> 0;
+  }
+  return t;
+})();
+function crc32(buf) {
+  let c = 0xffffffff;
+  for (let i = 0; i < buf.length; i++) c = CRC_TABLE[(c ^ buf[i]) & 0xff] ^ (c >>> 8);
+  return (c ^ 0xffffffff) >>> 0;
+}
+function pngChunk(tag, data) {
+  const len = Buffer.alloc(4); len.writeUInt32BE(data.length, 0);
+  const td = Buffer.concat([Buffer.from(tag, 'ascii'), data]);
+  const crc = Buffer.alloc(4); crc.writeUInt32BE(crc32(td), 0);
+  return Buffer.concat([len, td, crc]);
+}
+function encodePng(rgba, w, h) {
+  const raw = Buffer.alloc((w * 4 + 1) * h);
+  for (let y = 0; y < h; y++) {
+    const src = (h - 1 - y) * w * 4;
+    raw[y * (w * 4 + 1)] = 0;
+    Buffer.from(rgba.buffer, rgba.byteOffset + src, w * 4).copy(raw, y * (w * 4 + 1) + 1);
+  }
+  const ihdr = Buffer.alloc(13);
+  ihdr.writeUInt32BE(w, 0); ihdr.writeUInt32BE(h, 4);
+  ihdr[8] = 8; ihdr[9] = 6; ihdr[10] = 0; ihdr[11] = 0; ihdr[12] = 0;
+  return Buffer.concat([Buffer.from([0x89, 0x50, 0x4e, 0x47, 0x0d, 0x0a, 0x1a, 0x0a]), pngChunk('IHDR', ihdr),
+                        pngChunk('IDAT', require('zlib').deflateSync(raw)), pngChunk('IEND', Buffer.alloc(0))]);
 }
 
 const POLICIES = { rr: 0, fifo: 1, srpt: 2 };

@@ -461,6 +461,19 @@ def main():
                      "kernel": "k_trace_ext", "avg_launch_ms": round(avg_ms, 4),
                      "alg_bytes_per_launch": round(bytes_per_launch)},
     }
+    # display chain (ACES + gamma + FXAA) on the rendered frame; algorithmic
+    # bytes per launch = 16 B accumulation read + 4 B canvas written per pixel
+    eng.reset_counters()
+    eng.set_kernel_timing(True)
+    for _ in range(20):
+        eng.postprocess(3, 1.0, True, True)
+    kp = eng.kernel_stats()["post"]
+    eng.set_kernel_timing(False)
+    post_ms = kp["total_ms"] / max(1, kp["launches"])
+    out["post"] = {"kernel": "k_postprocess", "ops": "AcesFilm + gamma + FXAA, 8-bit canvas",
+                   "avg_launch_ms": round(post_ms, 4),
+                   "achieved_gbs": round(20.0 * wl.width * wl.height / (post_ms * 1e-3) / 1e9, 1) if post_ms > 0 else None}
+    log(f"[bench] post: {out['post']}")
     out["batched"] = None if args.no_batched else batched_vs_plain(args)
     out["serving"] = None if args.no_serving else serving(wl, args)
     out["instanced"] = None if args.no_instanced else instanced(args)

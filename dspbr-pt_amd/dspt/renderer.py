@@ -145,6 +145,22 @@ class PathtracingRenderer:
     def setScene(self, scene: PathtracingSceneData, progress=None):
         t0 = time.perf_counter()
         self.stopRendering()
+        from .scenes import InstancedScene
+        if isinstance(scene, InstancedScene):
+            # engine extension: object-space meshes + instances, both BVH levels built by the engine
+            self._engine.upload_scene_instanced(*scene.engine_args())
+            inner = scene.scene
+            if getattr(inner, "num_textures", 0):
+                dims, texels = inner.getTextureArrays()
+                self._engine.scene_set_textures(0, inner.getFlatTextureInfoBuffer(), dims, texels)
+            self._scene = inner
+            self._bvh_counts = (0, 0)
+            self.diagnostics["scene"] = {"triangles": inner.num_triangles, "instances": scene.num_instances,
+                                         "instancedTriangles": scene.instanced_triangles,
+                                         "materials": inner.num_materials, "lights": len(inner.lights)}
+            self.diagnostics["profiling"]["rendererSetupMs"] = (time.perf_counter() - t0) * 1e3
+            self.resetAccumulation()
+            return
         if scene.bvhNodes is not None:
             nodes, bidx = scene.bvhNodes, scene.bvhTriangleIndices
         else:
@@ -307,6 +323,29 @@ class PathtracingViewport:
     def setIBL(self, ibl):
         self.renderer.setIBL(ibl)
         self._emit("iblLoaded", {"ibl": ibl})
+
+    # pathtracing_viewport.ts:172-234 over local files (URLs are not fetched)
+    def loadSceneFromFile(self, path, progress=None):
+        from .loaders import loadSceneFromFile
+        try:
+            scene, profile = loadSceneFromFile(path, progress=progress)
+            self.renderer.setScene(scene, progress)
+        except Exception as e:
+            self._emit("sceneLoadFailed", {"error": e})
+            raise
+        self.renderer.diagnostics["profiling"] = {**profile, **self.renderer.diagnostics.get("profiling", {})}
+        self._emit("sceneLoaded", {"scene": scene, "loaded": {"scene": scene, "profile": profile}})
+        return {"scene": scene, "profile": profile}
+
+    def loadIblFromFile(self, path):
+        from .loaders import loadIblFromFile
+        try:
+            ibl = loadIblFromFile(path)
+            self.setIBL(ibl)
+        except Exception as e:
+            self._emit("iblLoadFailed", {"error": e})
+            raise
+        return ibl
 
     def setCamera(self, camera):
         self.camera = camera

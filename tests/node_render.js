@@ -30,5 +30,9 @@ function load(dir, name, T) {
   let frames = 0;
   await r.render(camera, meta.numSamples, null, (n) => { frames = n; }, null, meta.framesPerCall);
   fs.writeFileSync(path.join(dir, 'accum.f32'), Buffer.from(r.readAccumulation().buffer));
+  // the display chain with the reference's settings names
+  r.tonemapping = 'AcesFilm'; r.enableFxaa = true;
+  fs.writeFileSync(path.join(dir, 'display.u8'), Buffer.from(r.readDisplay().buffer));
+  r.saveImage(path.join(dir, 'display.png'));
   console.log(JSON.stringify({ frames, sampleCount: r.diagnostics.sampleCount, bvh: r.diagnostics.bvh }));
 })().catch((e) => { console.error(e.message); process.exit(1); });

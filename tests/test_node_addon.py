@@ -83,6 +83,11 @@ def test_node_render_matches_python_engine(addon_built, tmp_path):
                                    ibl_total_sum=tot, scene_counts=(s.num_triangles, len(nodes) // 16, len(bidx), 0)))
     eng.render(1, F)
     np.testing.assert_array_equal(eng.read_accum().view(np.uint32), node_acc.view(np.uint32))
+    # readDisplay: ACES + gamma + FXAA at the display size; saveImage: the same canvas, top row first
+    from dspt.image import read_png
+    disp = np.fromfile(tmp_path / "display.u8", np.uint8).reshape(H, W, 4)
+    np.testing.assert_array_equal(disp, eng.postprocess(3, 1.0, True, True))
+    np.testing.assert_array_equal(read_png(str(tmp_path / "display.png")), disp[::-1])
 
 
 @pytest.mark.gpu
