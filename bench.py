@@ -344,6 +344,40 @@ def instanced(args):
             "bvh_build_s": round(build_s, 2), "alone": alone, "with_traffic": mixed}
 
 
+def bvh_builds(wl, spp):
+    from dspt import Engine, PROFILE_FULL, build_bvh
+    s = wl.scene
+    t0 = time.perf_counter()
+    build_bvh(s.bvhPositionBuffer, s.bvhIndexBuffer)
+    host_ms = (time.perf_counter() - t0) * 1e3
+    eng = Engine(PROFILE_FULL)
+    eng.build_bvh_gpu(s.vertexBuffer, s.triangleIndexBuffer, 4)  # warm (code objects, allocator)
+    gms = []
+    for _ in range(3):
+        nodes, bidx, ms = eng.build_bvh_gpu(s.vertexBuffer, s.triangleIndexBuffer, 4)
+        gms.append(ms)
+    img = wl.ibl["image"]
+    rate = {}
+    for name, (nd, bi) in (("host_sah", (s.bvhNodes, s.bvhTriangleIndices)), ("gpu_lbvh", (nodes, bidx))):
+        e = Engine(PROFILE_FULL)
+        e.upload_scene(s.vertexBuffer, s.triangleIndexBuffer, nd, bi, s.getFlatMaterialBuffer())
+        tot = e.upload_ibl(img["data"], img["width"], img["height"])
+        e.resize(wl.width, wl.height)
+        e.set_frame(frame_block(wl, tot))
+        e.render(1, spp)  # warm: pool, rings, code objects
+        e.synchronize()
+        t0 = time.perf_counter()
+        e.render(1, spp)
+        e.synchronize()
+        rate[name] = round(wl.width * wl.height * spp / (time.perf_counter() - t0) / 1e6, 1)
+        del e
+    res = {"triangles": s.num_triangles, "host_sah_ms": round(host_ms, 1), "gpu_lbvh_ms": round(min(gms), 2),
+           "host_sah_nodes": len(s.bvhNodes) // 16, "gpu_lbvh_nodes": len(nodes) // 16,
+           "mpaths_per_s": rate}
+    log(f"[bench] bvh builds: {res}")
+    return res
+
+
 def calibrate_copy(mib):
     """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
     per dispatch give the scale between the counters and real HBM bytes."""
@@ -372,6 +406,7 @@ def main():
     ap.add_argument("--serve-rates", type=float, nargs="*", default=[5.0, 25.0, 100.0, 200.0])
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--no-instanced", action="store_true", help="skip the large instanced scene")
+    ap.add_argument("--no-bvh-build", action="store_true", help="skip the host vs GPU BVH build comparison")
     ap.add_argument("--inst-spp", type=int, default=256)
     ap.add_argument("--inst-traffic-rate", type=float, default=10.0, help="small jobs per second beside the long job")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -474,6 +509,9 @@ def main():
                    "avg_launch_ms": round(post_ms, 4),
                    "achieved_gbs": round(20.0 * wl.width * wl.height / (post_ms * 1e-3) / 1e9, 1) if post_ms > 0 else None}
     log(f"[bench] post: {out['post']}")
+    # BVH builds of the default scene: host binned SAH vs the GPU linear BVH,
+    # and the render rate over each tree
+    out["bvh_build"] = None if args.no_bvh_build else bvh_builds(wl, spp)
     out["batched"] = None if args.no_batched else batched_vs_plain(args)
     out["serving"] = None if args.no_serving else serving(wl, args)
     out["instanced"] = None if args.no_instanced else instanced(args)

@@ -18,6 +18,7 @@
 // dspt_set_frame / dspt_render) drives an implicit job on scene 0 / IBL 0.
 // Kernels are in dspt_kernels.hip (same translation unit).
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <array>
@@ -205,6 +206,8 @@ struct dspt_ctx {
 namespace {
 
 int grid_for(int64_t n, int block) { return (int)std::max<int64_t>(1, (n + block - 1) / block); }
+
+#include "bvh_gpu.hip"
 
 hipEvent_t take_event(dspt_ctx* c) {
   if (!c->ev_pool.empty()) {
@@ -613,6 +616,84 @@ int validate_nodes(const float* nodes, int64_t n) {
   }
   return 0;
 }
+// Checks shared by every scene constructor; returns the largest texture id
+// a material references in *max_tex.
+int validate_geometry(const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, const float* mats, int32_t K,
+                      int32_t* max_tex_out) {
+  if (!vertices || !tri_idx || !mats) return fail(DSPT_ERR_ARG, "NULL scene buffer");
+  if (V <= 0 || T <= 0 || K <= 0) return fail(DSPT_ERR_ARG, "empty scene");
+  if (T >= (1LL << 24)) return fail(DSPT_ERR_UNSUPPORTED, "scene exceeds float-encoded index range");
+  for (int64_t i = 0; i < 3 * T; i++)
+    if ((int64_t)tri_idx[i] >= V) return fail(DSPT_ERR_ARG, "triangle index " + std::to_string(i) + " out of range");
+  for (int64_t v = 0; v < V; v++) {
+    float mi = vertices[v * 20 + 3];
+    if (!(mi >= 0.0f) || mi >= (float)K) return fail(DSPT_ERR_ARG, "vertex " + std::to_string(v) + ": material index out of range");
+  }
+  int32_t max_tex = -1;
+  for (int32_t k = 0; k < K; k++)
+    for (int i : {39, 44, 45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 55, 56, 57, 58, 59, 63}) {
+      const float id = mats[(size_t)k * 68 + i];
+      if (!std::isfinite(id) || id >= 16777216.0f)
+        return fail(DSPT_ERR_ARG, "material " + std::to_string(k) + ": texture id is not a finite value below 2^24");
+      if (id >= 0.0f) max_tex = std::max(max_tex, (int32_t)id);
+    }
+  *max_tex_out = max_tex;
+  return 0;
+}
+
+// Scene whose BVH is built on the GPU (gpu_bvh_build) from the uploaded
+// vertex stream; no host-side node array.
+int make_scene_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, const float* mats,
+                       int32_t K, int32_t max_leaf, SceneRes** out, double* build_ms) {
+  int32_t max_tex = -1;
+  if (int r = validate_geometry(vertices, V, tri_idx, T, mats, K, &max_tex)) return r;
+  HIPCHK(hipSetDevice(c->device));
+  SceneRes* s = new SceneRes();
+  Allocs& A = s->mem;
+  int r = 0;
+  float4* big_nodes = nullptr;
+  int32_t* d_bvh = nullptr;
+  auto bail = [&](int code) {
+    if (big_nodes) (void)hipFree(big_nodes);
+    if (d_bvh) (void)hipFree(d_bvh);
+    A.release();
+    delete s;
+    return code;
+  };
+  if ((r = A.get(&s->verts, (size_t)V * 5)) || (r = A.get(&s->tris, (size_t)T * 3)) || (r = A.get(&s->mats, (size_t)K * 17)) ||
+      (r = A.get(&s->tri, (size_t)T * 3)))
+    return bail(r);
+  if (hipMalloc(&big_nodes, (size_t)(2 * T - 1) * 64) != hipSuccess || hipMalloc(&d_bvh, (size_t)T * 4) != hipSuccess)
+    return bail(fail(DSPT_ERR_MEMORY, "bvh build buffers"));
+  HIPCHK(hipMemcpyAsync(s->verts, vertices, (size_t)V * 80, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(s->tris, tri_idx, (size_t)T * 12, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(s->mats, mats, (size_t)K * 272, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  hipEvent_t e0 = take_event(c), e1 = take_event(c);
+  (void)hipEventRecord(e0, c->stream);
+  int64_t N = 0;
+  std::string err;
+  if (int rc = gpu_bvh_build(c->stream, s->verts, s->tris, (int32_t)T, max_leaf, big_nodes, d_bvh, &N, err))
+    return bail(fail(rc, err));
+  (void)hipEventRecord(e1, c->stream);
+  if ((r = A.get(&s->nodes, (size_t)N * 4))) return bail(r);
+  HIPCHK(hipMemcpyAsync(s->nodes, big_nodes, (size_t)N * 64, hipMemcpyDeviceToDevice, c->stream));
+  k_prep_tris<<<grid_for(T, 256), 256, 0, c->stream>>>(d_bvh, s->tris, s->verts, (int32_t)T, (int32_t)T, s->tri);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  c->ev_pool.push_back(e0);
+  c->ev_pool.push_back(e1);
+  if (build_ms) *build_ms = ms;
+  (void)hipFree(big_nodes);
+  (void)hipFree(d_bvh);
+  s->N = (int32_t)N; s->M = (int32_t)T; s->T = (int32_t)T; s->K = K; s->V = V;
+  s->max_tex_id = max_tex;
+  *out = s;
+  return 0;
+}
+
 int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, const float* nodes, int64_t N,
                const int32_t* bvh_idx, int64_t M, const float* mats, int32_t K, SceneRes** out, bool own_nodes = false) {
   if (!vertices || !tri_idx || !nodes || !bvh_idx || !mats) return fail(DSPT_ERR_ARG, "NULL scene buffer");
@@ -1057,6 +1138,38 @@ int dspt_scene_create_instanced(dspt_ctx* c, const float* vertices, int64_t V, c
   return 0;
 }
 
+int dspt_scene_create_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T,
+                              const float* mats, int32_t K, int32_t max_leaf, int32_t* scene_id, double* build_ms) {
+  if (!c || !scene_id) return fail(DSPT_ERR_ARG, "NULL argument");
+  SceneRes* s = nullptr;
+  if (int r = make_scene_gpu_bvh(c, vertices, V, tri_idx, T, mats, K, max_leaf, &s, build_ms)) return r;
+  *scene_id = put_slot(c->scenes, s);
+  return 0;
+}
+
+int dspt_build_bvh_gpu(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, int32_t max_leaf,
+                       float* nodes, int64_t* num_nodes, int32_t* bvh_indices, double* build_ms) {
+  if (!c || !nodes || !num_nodes || !bvh_indices) return fail(DSPT_ERR_ARG, "NULL argument");
+  SceneRes* s = nullptr;
+  std::vector<float> mat(68, 0.0f);
+  for (int i : {39, 44, 45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 55, 56, 57, 58, 59, 63}) mat[i] = -1.0f;
+  // material indices of the vertex stream are not needed here: validate against a large K
+  std::vector<float> mats;
+  int32_t K = 1;
+  for (int64_t v = 0; v < V; v++) K = std::max<int32_t>(K, (int32_t)std::min(vertices[v * 20 + 3] + 1.0f, 1e6f));
+  mats.resize((size_t)K * 68);
+  for (int32_t k = 0; k < K; k++) std::memcpy(&mats[(size_t)k * 68], mat.data(), 68 * 4);
+  if (int r = make_scene_gpu_bvh(c, vertices, V, tri_idx, T, mats.data(), K, max_leaf, &s, build_ms)) return r;
+  *num_nodes = s->N;
+  std::vector<float4> tri((size_t)T * 3);
+  HIPCHK(hipMemcpy(nodes, s->nodes, (size_t)s->N * 64, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(tri.data(), s->tri, (size_t)T * 48, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < T; i++) std::memcpy(&bvh_indices[i], &tri[(size_t)i * 3].w, 4);  // id bits in .w
+  s->mem.release();
+  delete s;
+  return 0;
+}
+
 int dspt_scene_destroy(dspt_ctx* c, int32_t id) {
   if (!c || id < 0 || id >= (int)c->scenes.size() || !c->scenes[id]) return fail(DSPT_ERR_ARG, "unknown scene id");
   if (c->scenes[id]->users > 0) return fail(DSPT_ERR_STATE, "scene is used by live jobs");
@@ -1179,6 +1292,26 @@ int dspt_upload_scene_instanced(dspt_ctx* c, const float* vertices, int64_t V, c
   if (int r = make_scene_instanced(c, vertices, V, tri_idx, T, mesh_first_tri, mesh_num_tris, num_meshes, instance_o2w,
                                    instance_mesh, num_instances, mats, K, &s))
     return r;
+  if (int r = drain_all(c)) return r;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int users = 0;
+  if (!c->scenes.empty() && c->scenes[0]) {
+    users = c->scenes[0]->users;
+    c->scenes[0]->mem.release();
+    c->scenes[0]->tex_mem.release();
+    delete c->scenes[0];
+  }
+  s->users = users;
+  put_slot(c->scenes, s, 0);
+  c->jobs_dirty = true;
+  return 0;
+}
+
+int dspt_upload_scene_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T,
+                              const float* mats, int32_t K, int32_t max_leaf, double* build_ms) {
+  if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
+  SceneRes* s = nullptr;
+  if (int r = make_scene_gpu_bvh(c, vertices, V, tri_idx, T, mats, K, max_leaf, &s, build_ms)) return r;
   if (int r = drain_all(c)) return r;
   HIPCHK(hipStreamSynchronize(c->stream));
   int users = 0;
