@@ -351,14 +351,17 @@ def bvh_builds(wl, spp):
     build_bvh(s.bvhPositionBuffer, s.bvhIndexBuffer)
     host_ms = (time.perf_counter() - t0) * 1e3
     eng = Engine(PROFILE_FULL)
-    eng.build_bvh_gpu(s.vertexBuffer, s.triangleIndexBuffer, 4)  # warm (code objects, allocator)
-    gms = []
-    for _ in range(3):
-        nodes, bidx, ms = eng.build_bvh_gpu(s.vertexBuffer, s.triangleIndexBuffer, 4)
-        gms.append(ms)
+    trees, gms = {}, {}
+    for name, method in (("gpu_lbvh", 0), ("gpu_ploc", 1)):
+        eng.build_bvh_gpu(s.vertexBuffer, s.triangleIndexBuffer, 4, method)  # warm (code objects, allocator)
+        best = None
+        for _ in range(3):
+            nodes, bidx, ms = eng.build_bvh_gpu(s.vertexBuffer, s.triangleIndexBuffer, 4, method)
+            best = ms if best is None else min(best, ms)
+        trees[name], gms[name] = (nodes, bidx), best
     img = wl.ibl["image"]
     rate = {}
-    for name, (nd, bi) in (("host_sah", (s.bvhNodes, s.bvhTriangleIndices)), ("gpu_lbvh", (nodes, bidx))):
+    for name, (nd, bi) in [("host_sah", (s.bvhNodes, s.bvhTriangleIndices))] + list(trees.items()):
         e = Engine(PROFILE_FULL)
         e.upload_scene(s.vertexBuffer, s.triangleIndexBuffer, nd, bi, s.getFlatMaterialBuffer())
         tot = e.upload_ibl(img["data"], img["width"], img["height"])
@@ -371,8 +374,9 @@ def bvh_builds(wl, spp):
         e.synchronize()
         rate[name] = round(wl.width * wl.height * spp / (time.perf_counter() - t0) / 1e6, 1)
         del e
-    res = {"triangles": s.num_triangles, "host_sah_ms": round(host_ms, 1), "gpu_lbvh_ms": round(min(gms), 2),
-           "host_sah_nodes": len(s.bvhNodes) // 16, "gpu_lbvh_nodes": len(nodes) // 16,
+    res = {"triangles": s.num_triangles, "build_ms": {"host_sah": round(host_ms, 1),
+                                                      **{k: round(v, 2) for k, v in gms.items()}},
+           "nodes": {"host_sah": len(s.bvhNodes) // 16, **{k: len(v[0]) // 16 for k, v in trees.items()}},
            "mpaths_per_s": rate}
     log(f"[bench] bvh builds: {res}")
     return res

@@ -644,7 +644,7 @@ int validate_geometry(const float* vertices, int64_t V, const uint32_t* tri_idx,
 // Scene whose BVH is built on the GPU (gpu_bvh_build) from the uploaded
 // vertex stream; no host-side node array.
 int make_scene_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, const float* mats,
-                       int32_t K, int32_t max_leaf, SceneRes** out, double* build_ms) {
+                       int32_t K, int32_t max_leaf, int32_t method, SceneRes** out, double* build_ms) {
   int32_t max_tex = -1;
   if (int r = validate_geometry(vertices, V, tri_idx, T, mats, K, &max_tex)) return r;
   HIPCHK(hipSetDevice(c->device));
@@ -673,7 +673,7 @@ int make_scene_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint
   (void)hipEventRecord(e0, c->stream);
   int64_t N = 0;
   std::string err;
-  if (int rc = gpu_bvh_build(c->stream, s->verts, s->tris, (int32_t)T, max_leaf, big_nodes, d_bvh, &N, err))
+  if (int rc = gpu_bvh_build(c->stream, s->verts, s->tris, (int32_t)T, max_leaf, method, big_nodes, d_bvh, &N, err))
     return bail(fail(rc, err));
   (void)hipEventRecord(e1, c->stream);
   if ((r = A.get(&s->nodes, (size_t)N * 4))) return bail(r);
@@ -1139,16 +1139,17 @@ int dspt_scene_create_instanced(dspt_ctx* c, const float* vertices, int64_t V, c
 }
 
 int dspt_scene_create_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T,
-                              const float* mats, int32_t K, int32_t max_leaf, int32_t* scene_id, double* build_ms) {
+                              const float* mats, int32_t K, int32_t max_leaf, int32_t method, int32_t* scene_id,
+                              double* build_ms) {
   if (!c || !scene_id) return fail(DSPT_ERR_ARG, "NULL argument");
   SceneRes* s = nullptr;
-  if (int r = make_scene_gpu_bvh(c, vertices, V, tri_idx, T, mats, K, max_leaf, &s, build_ms)) return r;
+  if (int r = make_scene_gpu_bvh(c, vertices, V, tri_idx, T, mats, K, max_leaf, method, &s, build_ms)) return r;
   *scene_id = put_slot(c->scenes, s);
   return 0;
 }
 
 int dspt_build_bvh_gpu(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, int32_t max_leaf,
-                       float* nodes, int64_t* num_nodes, int32_t* bvh_indices, double* build_ms) {
+                       int32_t method, float* nodes, int64_t* num_nodes, int32_t* bvh_indices, double* build_ms) {
   if (!c || !nodes || !num_nodes || !bvh_indices) return fail(DSPT_ERR_ARG, "NULL argument");
   SceneRes* s = nullptr;
   std::vector<float> mat(68, 0.0f);
@@ -1159,7 +1160,7 @@ int dspt_build_bvh_gpu(dspt_ctx* c, const float* vertices, int64_t V, const uint
   for (int64_t v = 0; v < V; v++) K = std::max<int32_t>(K, (int32_t)std::min(vertices[v * 20 + 3] + 1.0f, 1e6f));
   mats.resize((size_t)K * 68);
   for (int32_t k = 0; k < K; k++) std::memcpy(&mats[(size_t)k * 68], mat.data(), 68 * 4);
-  if (int r = make_scene_gpu_bvh(c, vertices, V, tri_idx, T, mats.data(), K, max_leaf, &s, build_ms)) return r;
+  if (int r = make_scene_gpu_bvh(c, vertices, V, tri_idx, T, mats.data(), K, max_leaf, method, &s, build_ms)) return r;
   *num_nodes = s->N;
   std::vector<float4> tri((size_t)T * 3);
   HIPCHK(hipMemcpy(nodes, s->nodes, (size_t)s->N * 64, hipMemcpyDeviceToHost));
@@ -1308,10 +1309,10 @@ int dspt_upload_scene_instanced(dspt_ctx* c, const float* vertices, int64_t V, c
 }
 
 int dspt_upload_scene_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T,
-                              const float* mats, int32_t K, int32_t max_leaf, double* build_ms) {
+                              const float* mats, int32_t K, int32_t max_leaf, int32_t method, double* build_ms) {
   if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
   SceneRes* s = nullptr;
-  if (int r = make_scene_gpu_bvh(c, vertices, V, tri_idx, T, mats, K, max_leaf, &s, build_ms)) return r;
+  if (int r = make_scene_gpu_bvh(c, vertices, V, tri_idx, T, mats, K, max_leaf, method, &s, build_ms)) return r;
   if (int r = drain_all(c)) return r;
   HIPCHK(hipStreamSynchronize(c->stream));
   int users = 0;

@@ -61,9 +61,9 @@ SIGNATURES = {
     "dspt_upload_scene_instanced": (_c_int, [_vp, _fp, _c_i64, _u32p, _c_i64, _i64p, _i64p, _c_i32, _fp, _i32p, _c_i32, _fp,
                                              _c_i32]),
     "dspt_trace_closest_instanced": (_c_int, [_vp, _fp, _c_i64, _i32p, _i32p, _fp, _fp, _fp]),
-    "dspt_scene_create_gpu_bvh": (_c_int, [_vp, _fp, _c_i64, _u32p, _c_i64, _fp, _c_i32, _c_i32, _i32p, _dp]),
-    "dspt_upload_scene_gpu_bvh": (_c_int, [_vp, _fp, _c_i64, _u32p, _c_i64, _fp, _c_i32, _c_i32, _dp]),
-    "dspt_build_bvh_gpu": (_c_int, [_vp, _fp, _c_i64, _u32p, _c_i64, _c_i32, _fp, _i64p, _i32p, _dp]),
+    "dspt_scene_create_gpu_bvh": (_c_int, [_vp, _fp, _c_i64, _u32p, _c_i64, _fp, _c_i32, _c_i32, _c_i32, _i32p, _dp]),
+    "dspt_upload_scene_gpu_bvh": (_c_int, [_vp, _fp, _c_i64, _u32p, _c_i64, _fp, _c_i32, _c_i32, _c_i32, _dp]),
+    "dspt_build_bvh_gpu": (_c_int, [_vp, _fp, _c_i64, _u32p, _c_i64, _c_i32, _c_i32, _fp, _i64p, _i32p, _dp]),
     "dspt_postprocess": (_c_int, [_vp, _c_i32, ctypes.c_float, _c_i32, _c_i32, _c_i32, _c_i32, _u8p, _fp]),
     "dspt_job_postprocess": (_c_int, [_vp, _c_i32, _c_i32, ctypes.c_float, _c_i32, _c_i32, _c_i32, _c_i32, _u8p, _fp]),
     "dspt_ibl_create": (_c_int, [_vp, _fp, _c_i32, _c_i32, _dp, _i32p]),
@@ -88,6 +88,7 @@ SCHED_PASS = 1
 POLICY_RR = 0
 POLICY_FIFO = 1
 POLICY_SRPT = 2
+BVH_LBVH, BVH_PLOC = 0, 1
 KERNEL_CLASSES = ["raygen", "trace_ext", "shade", "trace_shadow", "accumulate", "post"]
 
 
@@ -338,26 +339,28 @@ class Engine:
         return self._post(self._L.dspt_job_postprocess, (int(jid),), (w, h), tonemap_mode, exposure, gamma, fxaa, out_size,
                           want_float)
 
-    def scene_create_gpu_bvh(self, verts, tris, mats, max_leaf=4):
-        """Resident scene whose BVH the GPU builds; returns (scene id, build ms)."""
+    def scene_create_gpu_bvh(self, verts, tris, mats, max_leaf=4, method=1):
+        """Resident scene whose BVH the GPU builds (method BVH_PLOC or
+        BVH_LBVH); returns (scene id, build ms)."""
         verts, mats = f32(verts), f32(mats)
         tris = np.ascontiguousarray(tris, dtype=np.uint32)
         sid, ms = ctypes.c_int32(-1), ctypes.c_double(0)
         check(self._L.dspt_scene_create_gpu_bvh(self.h, ptr(verts), verts.size // 20, ptr(tris, ctypes.c_uint32),
-                                                tris.size // 3, ptr(mats), mats.size // 68, int(max_leaf),
+                                                tris.size // 3, ptr(mats), mats.size // 68, int(max_leaf), int(method),
                                                 ctypes.byref(sid), ctypes.byref(ms)))
         return sid.value, ms.value
 
-    def upload_scene_gpu_bvh(self, verts, tris, mats, max_leaf=4):
+    def upload_scene_gpu_bvh(self, verts, tris, mats, max_leaf=4, method=1):
         verts, mats = f32(verts), f32(mats)
         tris = np.ascontiguousarray(tris, dtype=np.uint32)
         ms = ctypes.c_double(0)
         check(self._L.dspt_upload_scene_gpu_bvh(self.h, ptr(verts), verts.size // 20, ptr(tris, ctypes.c_uint32),
-                                                tris.size // 3, ptr(mats), mats.size // 68, int(max_leaf), ctypes.byref(ms)))
+                                                tris.size // 3, ptr(mats), mats.size // 68, int(max_leaf), int(method),
+                                                ctypes.byref(ms)))
         return ms.value
 
-    def build_bvh_gpu(self, verts, tris, max_leaf=4):
-        """GPU LBVH of a vertex stream: (nodes f32[16 N], bvh indices i32[T], build ms)."""
+    def build_bvh_gpu(self, verts, tris, max_leaf=4, method=1):
+        """GPU BVH of a vertex stream (method BVH_PLOC or BVH_LBVH): (nodes f32[16 N], bvh indices i32[T], build ms)."""
         verts = f32(verts)
         tris = np.ascontiguousarray(tris, dtype=np.uint32)
         T = tris.size // 3
@@ -365,7 +368,7 @@ class Engine:
         bidx = np.zeros(max(1, T), np.int32)
         nn, ms = ctypes.c_int64(0), ctypes.c_double(0)
         check(self._L.dspt_build_bvh_gpu(self.h, ptr(verts), verts.size // 20, ptr(tris, ctypes.c_uint32), T, int(max_leaf),
-                                         ptr(nodes), ctypes.byref(nn), ptr(bidx, ctypes.c_int32), ctypes.byref(ms)))
+                                         int(method), ptr(nodes), ctypes.byref(nn), ptr(bidx, ctypes.c_int32), ctypes.byref(ms)))
         return nodes[:16 * nn.value].copy(), bidx[:T], ms.value
 
     def scene_destroy(self, sid):

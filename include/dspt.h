@@ -165,27 +165,30 @@ int dspt_build_bvh(const float* positions, int64_t num_vertices, const uint32_t*
                    int32_t max_leaf, float* nodes, int64_t* num_nodes, int32_t* bvh_indices, int64_t* num_bvh_indices);
 
 /* GPU BVH build (engine extension; the reference builds on the host,
- * renderer.ts:872-912 -> tinybvh_builder.ts:142-215): a linear BVH over
- * 63-bit Morton codes of the triangle centroids (Karras 2012), subtrees of at
- * most max_leaf triangles collapsed into leaves, emitted in the same bvh_gpu
- * layout. Milliseconds instead of seconds for millions of triangles, at a
- * lower tree quality than the binned-SAH host build; meant for admitting
- * scenes while serving. Inputs are the scene buffers themselves (vertex
- * stream f32[20*V], u32[3*T] indices).
+ * renderer.ts:872-912 -> tinybvh_builder.ts:142-215), emitted in the same
+ * bvh_gpu layout, for admitting scenes while serving. Both methods start
+ * from 63-bit Morton codes of the triangle centroids:
+ *   DSPT_BVH_LBVH  Karras' linear BVH (fastest build, lower tree quality)
+ *   DSPT_BVH_PLOC  parallel locally-ordered clustering (Meister & Bittner):
+ *                  nearest-neighbour merges by surface area, trees close to
+ *                  a SAH build
+ * and collapse subtrees of at most max_leaf triangles into leaves. Inputs
+ * are the scene buffers themselves (vertex stream f32[20*V], u32[3*T]).
  * dspt_scene_create_gpu_bvh / dspt_upload_scene_gpu_bvh: a resident scene
  * (or scene 0) whose BVH is built on the device; build_ms (may be NULL)
  * receives the build time measured with HIP events.
  * dspt_build_bvh_gpu: the same build returned to the host: nodes f32[16 *
  * (2T-1)] capacity, *num_nodes = nodes written, bvh_indices i32[T]. */
+enum { DSPT_BVH_LBVH = 0, DSPT_BVH_PLOC = 1 };
 int dspt_scene_create_gpu_bvh(dspt_ctx* ctx, const float* vertices, int64_t num_vertices, const uint32_t* tri_indices,
                               int64_t num_triangles, const float* materials, int32_t num_materials, int32_t max_leaf,
-                              int32_t* scene_id, double* build_ms);
+                              int32_t method, int32_t* scene_id, double* build_ms);
 int dspt_upload_scene_gpu_bvh(dspt_ctx* ctx, const float* vertices, int64_t num_vertices, const uint32_t* tri_indices,
                               int64_t num_triangles, const float* materials, int32_t num_materials, int32_t max_leaf,
-                              double* build_ms);
+                              int32_t method, double* build_ms);
 int dspt_build_bvh_gpu(dspt_ctx* ctx, const float* vertices, int64_t num_vertices, const uint32_t* tri_indices,
-                       int64_t num_triangles, int32_t max_leaf, float* nodes, int64_t* num_nodes, int32_t* bvh_indices,
-                       double* build_ms);
+                       int64_t num_triangles, int32_t max_leaf, int32_t method, float* nodes, int64_t* num_nodes,
+                       int32_t* bvh_indices, double* build_ms);
 
 /* Device memory currently held by the context, bytes. */
 int dspt_memory_usage(dspt_ctx* ctx, int64_t* bytes);

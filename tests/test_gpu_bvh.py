@@ -1,5 +1,5 @@
-"""GPU BVH build (linear BVH over Morton codes, dspt_build_bvh_gpu) against
-what a bvh_gpu tree must satisfy, and traversal over it.
+"""GPU BVH builds (linear BVH and PLOC over Morton codes, dspt_build_bvh_gpu)
+against what a bvh_gpu tree must satisfy, and traversal over them.
 
 Builder parity with the reference's TinyBVH is unpinned by design (the tree
 is an input both traversals share); pinned here: the node array is a
@@ -67,8 +67,9 @@ def _soup(n, seed, dup=0):
     return v.reshape(-1), np.arange(3 * n, dtype=np.uint32)
 
 
+@pytest.mark.parametrize("method", [0, 1])
 @pytest.mark.parametrize("case", ["sphere_field", "soup", "dups", "n1", "n2", "n7"])
-def test_gpu_tree_is_well_formed(case, sphere_field_small):
+def test_gpu_tree_is_well_formed(case, method, sphere_field_small):
     from dspt import Engine
     eng = Engine(0)
     if case == "sphere_field":
@@ -79,19 +80,50 @@ def test_gpu_tree_is_well_formed(case, sphere_field_small):
         verts, tris = _soup(n, 3, dup=(case == "dups"))
     T = tris.size // 3
     for max_leaf in (1, 4):
-        nodes, bidx, ms = eng.build_bvh_gpu(verts, tris, max_leaf)
+        nodes, bidx, ms = eng.build_bvh_gpu(verts, tris, max_leaf, method)
         N, reached = _check_tree(nodes, bidx, verts, tris, T, max_leaf)
         assert reached == N  # no unreferenced slots
         assert ms > 0
 
 
-def test_traversal_over_gpu_tree(sphere_field_small):
+def _sah_cost(nodes, ct=1.0, ci=1.0):
+    """Surface-area cost of a bvh_gpu tree relative to its root box."""
+    nd = nodes.reshape(-1, 16).astype(np.float64)
+
+    def area(lo, hi):
+        d = np.maximum(hi - lo, 0)
+        return 2 * (d[..., 0] * d[..., 1] + d[..., 1] * d[..., 2] + d[..., 2] * d[..., 0])
+    inner = nd[:, 11] == 0
+    # every child box is stored in its parent: sum child areas over inner nodes
+    la, ra = area(nd[:, 0:3], nd[:, 4:7]), area(nd[:, 8:11], nd[:, 12:15])
+    cnt = nd[:, 11]
+    leaf_area = area(nd[:, 0:3], nd[:, 4:7])
+    root = area(np.minimum(nd[0, 0:3], nd[0, 8:11]), np.maximum(nd[0, 4:7], nd[0, 12:15])) if inner[0] else leaf_area[0]
+    return (ct * (la[inner] + ra[inner]).sum() + ci * (leaf_area[~inner] * cnt[~inner]).sum()) / root
+
+
+def test_ploc_tree_beats_linear_tree(sphere_field_small):
+    """Clustering by surface area gives a lower SAH cost than the Morton
+    split of the linear BVH (Meister & Bittner's result), close to the host
+    binned SAH build."""
+    from dspt import Engine
+    s = sphere_field_small.scene
+    eng = Engine(0)
+    lb, _, _ = eng.build_bvh_gpu(s.vertexBuffer, s.triangleIndexBuffer, 4, 0)
+    pl, _, _ = eng.build_bvh_gpu(s.vertexBuffer, s.triangleIndexBuffer, 4, 1)
+    c_lb, c_pl, c_host = _sah_cost(lb), _sah_cost(pl), _sah_cost(s.bvhNodes)
+    print(f"SAH cost: linear {c_lb:.1f}, ploc {c_pl:.1f}, host {c_host:.1f}")
+    assert c_pl < c_lb and c_pl < 1.25 * c_host
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_traversal_over_gpu_tree(sphere_field_small, method):
     from dspt import Engine, build_bvh
     from test_gpu_parity import _camera_rays, _surface_rays
     wl = sphere_field_small
     s = wl.scene
     eng = Engine(0)
-    nodes, bidx, _ = eng.build_bvh_gpu(s.vertexBuffer, s.triangleIndexBuffer, 4)
+    nodes, bidx, _ = eng.build_bvh_gpu(s.vertexBuffer, s.triangleIndexBuffer, 4, method)
     eng.upload_scene(s.vertexBuffer, s.triangleIndexBuffer, nodes, bidx, s.getFlatMaterialBuffer())
     rays = np.concatenate([_camera_rays(wl, 20000, 1), _surface_rays(wl, 20000, 2)])
     tri, t, u, v = eng.trace_closest(rays)
@@ -108,10 +140,10 @@ def test_traversal_over_gpu_tree(sphere_field_small):
     np.testing.assert_array_equal(t[same], t2[same])
     # scene 0 built on the device: the build is deterministic, so the same tree and the same hits
     eng3 = Engine(0)
-    ms = eng3.upload_scene_gpu_bvh(s.vertexBuffer, s.triangleIndexBuffer, s.getFlatMaterialBuffer())
+    ms = eng3.upload_scene_gpu_bvh(s.vertexBuffer, s.triangleIndexBuffer, s.getFlatMaterialBuffer(), 4, method)
     assert ms > 0
     tri3, t3, _, _ = eng3.trace_closest(rays)
     np.testing.assert_array_equal(tri3, tri)
     np.testing.assert_array_equal(t3.view(np.uint32), t.view(np.uint32))
-    sid, ms = eng3.scene_create_gpu_bvh(s.vertexBuffer, s.triangleIndexBuffer, s.getFlatMaterialBuffer())
+    sid, ms = eng3.scene_create_gpu_bvh(s.vertexBuffer, s.triangleIndexBuffer, s.getFlatMaterialBuffer(), 4, method)
     assert sid >= 1 and ms > 0
