@@ -341,6 +341,34 @@ static napi_value js_scene_create_instanced(napi_env env, napi_callback_info inf
   return r;
 }
 
+/* sceneCreateGpuBvh(h, verts f32, tris u32, mats f32, maxLeaf, method) -> {id, buildMs}: the BVH is built on the GPU */
+static napi_value js_scene_create_gpu_bvh(napi_env env, napi_callback_info info) {
+  size_t argc = 6;
+  napi_value argv[6];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 6) { napi_throw_error(env, NULL, "sceneCreateGpuBvh expects 6 arguments"); return NULL; }
+  dspt_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return NULL;
+  void *v, *t, *m;
+  size_t lv, lt, lm;
+  if (get_typed(env, argv[1], napi_float32_array, &v, &lv) || get_typed(env, argv[2], napi_uint32_array, &t, &lt) ||
+      get_typed(env, argv[3], napi_float32_array, &m, &lm))
+    return NULL;
+  int32_t id = -1;
+  double ms = 0.0;
+  int rc = dspt_scene_create_gpu_bvh(c, (const float*)v, (int64_t)(lv / 20), (const uint32_t*)t, (int64_t)(lt / 3),
+                                     (const float*)m, (int32_t)(lm / 68), (int32_t)get_i64(env, argv[4]),
+                                     (int32_t)get_i64(env, argv[5]), &id, &ms);
+  if (rc) return throw_dspt(env, rc);
+  napi_value r, vid, vms;
+  CALL(env, napi_create_object(env, &r));
+  CALL(env, napi_create_int32(env, id, &vid));
+  CALL(env, napi_create_double(env, ms, &vms));
+  CALL(env, napi_set_named_property(env, r, "id", vid));
+  CALL(env, napi_set_named_property(env, r, "buildMs", vms));
+  return r;
+}
+
 /* sceneSetTextures(h, sceneId, texinfos f32[8X], dims i32[3A], texels u8) */
 static napi_value js_scene_set_textures(napi_env env, napi_callback_info info) {
   size_t argc = 5;
@@ -604,6 +632,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"sceneDestroy", NULL, js_scene_destroy, NULL, NULL, NULL, napi_enumerable, NULL},
       {"sceneSetTextures", NULL, js_scene_set_textures, NULL, NULL, NULL, napi_enumerable, NULL},
       {"sceneCreateInstanced", NULL, js_scene_create_instanced, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"sceneCreateGpuBvh", NULL, js_scene_create_gpu_bvh, NULL, NULL, NULL, napi_enumerable, NULL},
       {"iblCreate", NULL, js_ibl_create, NULL, NULL, NULL, napi_enumerable, NULL},
       {"iblDestroy", NULL, js_ibl_destroy, NULL, NULL, NULL, napi_enumerable, NULL},
       {"jobCreate", NULL, js_job_create, NULL, NULL, NULL, napi_enumerable, NULL},

@@ -53,7 +53,7 @@ export interface RenderRequest {
 export interface RenderResult { accumulation: Float32Array; width: number; height: number; frames: number; latencyMs: number; }
 export declare class RenderServer {
   constructor(options?: { materialProfile?: 'webgl-lean' | 'webgl-full'; device?: number; memoryBudget?: number;
-                          policy?: 'rr' | 'fifo' | 'srpt' });
+                          policy?: 'rr' | 'fifo' | 'srpt'; gpuBvh?: boolean });
   addScene(scene: SceneBuffers): number;
   addIbl(texture: { image: { data: Float32Array; width: number; height: number } }): number;
   submit(request: RenderRequest): Promise<RenderResult>;

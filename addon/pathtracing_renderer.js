@@ -203,8 +203,9 @@ function errorCode(e) {
 // linear RGBA32F accumulation of frames 1..spp. One loop drives the engine;
 // the JS thread stays free while the GPU works (jobsStep is asynchronous).
 class RenderServer {
-  constructor({ materialProfile = 'webgl-full', device = 0, memoryBudget = 0, policy = 'rr' } = {}) {
+  constructor({ materialProfile = 'webgl-full', device = 0, memoryBudget = 0, policy = 'rr', gpuBvh = false } = {}) {
     this._h = native.create(materialProfile === 'webgl-full' ? 1 : 0, device);
+    this._gpuBvh = !!gpuBvh;  // build scene BVHs on the GPU instead of the host
     if (memoryBudget) native.setMemoryBudget(this._h, memoryBudget);
     if (!(policy in POLICIES)) throw new Error(`unknown policy ${policy}`);
     native.setPolicy(this._h, POLICIES[policy]);
@@ -228,12 +229,20 @@ class RenderServer {
       return id;
     }
     let nodes = scene.bvhNodes, idx = scene.bvhTriangleIndices;
+    const mats = new Float32Array(scene.materials.length * 68);
+    scene.materials.forEach((m, i) => mats.set(m.data, i * 68));
+    if (!nodes && this._gpuBvh) {
+      // BVH built on the GPU (PLOC): milliseconds, for admitting scenes while serving
+      const r = native.sceneCreateGpuBvh(this._h, scene.vertexBuffer, scene.triangleIndexBuffer, mats, 4, 1);
+      if (scene.textures) native.sceneSetTextures(this._h, r.id, scene.textures.texInfos, scene.textures.arrayDims, scene.textures.texels);
+      const nt = scene.triangleIndexBuffer.length / 3;
+      this._scenes.set(r.id, { counts: [nt, 0, nt], lights: scene.lights || [] });
+      return r.id;
+    }
     if (!nodes) {
       const r = native.buildBvh(scene.bvhPositionBuffer, scene.bvhIndexBuffer, 4);
       nodes = r.nodeData; idx = r.triangleIndices;
     }
-    const mats = new Float32Array(scene.materials.length * 68);
-    scene.materials.forEach((m, i) => mats.set(m.data, i * 68));
     const id = native.sceneCreate(this._h, scene.vertexBuffer, scene.triangleIndexBuffer, nodes, idx, mats);
     if (scene.textures) native.sceneSetTextures(this._h, id, scene.textures.texInfos, scene.textures.arrayDims, scene.textures.texels);
     this._scenes.set(id, { counts: [scene.triangleIndexBuffer.length / 3, nodes.length / 16, idx.length],

@@ -43,12 +43,23 @@ class BatchRenderer:
         self.scenes = {}  # key -> (id, counts)
         self.ibls = {}    # key -> (id, (w, h), total)
 
-    def add_scene(self, key, scene):
+    def add_scene(self, key, scene, gpu_bvh=False):
+        """Make a scene resident. Without a prebuilt tree the BVH is built on
+        the host (binned SAH), or on the GPU (PLOC, milliseconds) with
+        gpu_bvh=True."""
         from ._lib import build_bvh
         from .scenes import InstancedScene
         if isinstance(scene, InstancedScene):
             sid = self.eng.scene_create_instanced(*scene.engine_args())
             self.scenes[key] = (sid, (scene.scene.num_triangles, 0, 0))
+            return sid
+        if gpu_bvh and scene.bvhNodes is None:
+            sid, _ = self.eng.scene_create_gpu_bvh(scene.vertexBuffer, scene.triangleIndexBuffer,
+                                                   scene.getFlatMaterialBuffer())
+            if getattr(scene, "num_textures", 0):
+                dims, texels = scene.getTextureArrays()
+                self.eng.scene_set_textures(sid, scene.getFlatTextureInfoBuffer(), dims, texels)
+            self.scenes[key] = (sid, (scene.num_triangles, 0, scene.num_triangles))
             return sid
         nodes, bidx = scene.bvhNodes, scene.bvhTriangleIndices
         if nodes is None:

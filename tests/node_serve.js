@@ -13,7 +13,7 @@ function load(dir, name, T) {
 (async () => {
   const dir = process.argv[2];
   const meta = JSON.parse(fs.readFileSync(path.join(dir, 'meta.json'), 'utf8'));
-  const server = new RenderServer({ materialProfile: meta.profile, policy: meta.policy });
+  const server = new RenderServer({ materialProfile: meta.profile, policy: meta.policy, gpuBvh: !!meta.gpuBvh });
   const mats = load(dir, 'mats.f32', Float32Array);
   const sceneData = {
     vertexBuffer: load(dir, 'verts.f32', Float32Array),

@@ -91,14 +91,14 @@ def test_node_render_matches_python_engine(addon_built, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["plain", "instanced"])
+@pytest.mark.parametrize("kind", ["plain", "instanced", "gpu_bvh"])
 def test_node_render_server_matches_python_jobs(addon_built, tmp_path, kind):
     """Requests served by the Node RenderServer (continuous batching, staggered
     arrivals) produce the same images as the same jobs run from Python, for a
     one-level and a two-level (instanced) scene."""
     from dspt import Engine, PROFILE_LEAN, build_bvh, pack_frame_block, scenes
     inst = None
-    if kind == "plain":
+    if kind in ("plain", "gpu_bvh"):
         wl = scenes.sphere_field(n_spheres=16)
         s = wl.scene
     else:
@@ -119,7 +119,8 @@ def test_node_render_server_matches_python_jobs(addon_built, tmp_path, kind):
             {"width": 96, "height": 32, "spp": 4, "delayMs": 10}]
     meta = {"profile": "webgl-lean", "policy": "rr", "maxBounces": 6, "iblWidth": img["width"], "iblHeight": img["height"],
             "fov": wl.camera.fov, "near": wl.camera.near, "matrixWorld": [float(x) for x in wl.camera.matrixWorld],
-            "position": dict(zip("xyz", wl.camera.position)), "requests": reqs, "instanced": inst is not None}
+            "position": dict(zip("xyz", wl.camera.position)), "requests": reqs, "instanced": inst is not None,
+            "gpuBvh": kind == "gpu_bvh"}
     (tmp_path / "meta.json").write_text(json.dumps(meta))
     r = subprocess.run(["node", os.path.join(ROOT, "tests", "node_serve.js"), str(tmp_path)], capture_output=True,
                        text=True, timeout=120)
@@ -127,7 +128,10 @@ def test_node_render_server_matches_python_jobs(addon_built, tmp_path, kind):
     info = json.loads(r.stdout.strip().splitlines()[-1])
     assert info["frames"] == [q["spp"] for q in reqs] and info["progress"] == info["frames"]
     eng = Engine(PROFILE_LEAN)
-    if inst is None:
+    if kind == "gpu_bvh":
+        sid, _ = eng.scene_create_gpu_bvh(s.vertexBuffer, s.triangleIndexBuffer, s.getFlatMaterialBuffer())
+        counts = (s.num_triangles, 0, s.num_triangles, 0)
+    elif inst is None:
         nodes, bidx = build_bvh(s.bvhPositionBuffer, s.bvhIndexBuffer)
         sid = eng.scene_create(s.vertexBuffer, s.triangleIndexBuffer, nodes, bidx, s.getFlatMaterialBuffer())
         counts = (s.num_triangles, len(nodes) // 16, len(bidx), 0)
