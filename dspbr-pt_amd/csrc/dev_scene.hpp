@@ -74,6 +74,12 @@ struct FrameParams {
   float light_emission[4];
 };
 
+#ifndef DSPT_SLAB_SELECT
+#define DSPT_SLAB_SELECT 1
+#endif
+#ifndef DSPT_TRI_SELECT
+#define DSPT_TRI_SELECT 0
+#endif
 constexpr int kStackSize = 64;       // BVH_STACK_SIZE, SH/bvh.glsl:8
 constexpr int kMaxSteps = 4096;      // MAX_BVH_TRAVERSAL_STEPS
 constexpr int kMaxLeafTris = 64;     // MAX_BVH_LEAF_TRIANGLES
@@ -128,12 +134,39 @@ DI float slab(f3 bmin, f3 bmax, f3 org, f3 inv, float tfar) {
   f3 lo = vmin(t0, t1), hi = vmax(t0, t1);
   float tmin = gmax(gmax(lo.x, lo.y), lo.z);
   float tmax = gmin(gmin(hi.x, hi.y), hi.z);
+#if DSPT_SLAB_SELECT
+  // the reference's test `isnan(tmin) || isnan(tmax) || tmin > tmax ||
+  // tmax < 0 || tmin >= tfar` as one predicate without short-circuit
+  // branches: (tmin <= tmax) is false exactly when either is NaN or
+  // tmin > tmax, and tfar is never NaN
+  const bool ok = (tmin <= tmax) & (tmax >= 0.0f) & (tmin < tfar);
+  return ok ? tmin : kBVH_FAR;
+#else
   if (isnan(tmin) || isnan(tmax) || tmin > tmax || tmax < 0.0f || tmin >= tfar) return kBVH_FAR;
   return tmin;
+#endif
 }
 
 // Moeller-Trumbore, SH/bvh.glsl:44-62.
 DI bool tri_test(f3 org, f3 dir, f3 p0, f3 p1, f3 p2, float tfar, float& t, float& u, float& v) {
+#if DSPT_TRI_SELECT
+  // the same arithmetic and the same rejections without early exits: each
+  // reference test `if (c) return false` becomes `!c` in one predicate, so
+  // NaN comparisons behave as in the branching form
+  f3 e0 = p1 - p0, e1 = p2 - p0;
+  f3 pv = cross3(dir, e1);
+  float det = dot3(e0, pv);
+  float f = 1.0f / det;
+  f3 s = org - p0;
+  float uu = f * dot3(s, pv);
+  f3 q = cross3(s, e0);
+  float vv = f * dot3(dir, q);
+  float tt = f * dot3(e1, q);
+  const bool ok = !(fabsf(det) < kEPS) & !(uu < 0.0f) & !(uu > 1.0f) & !(vv < 0.0f) & !(uu + vv > 1.0f) & !(tt < kEPS) &
+                  !(tt >= tfar);
+  if (ok) { t = tt; u = uu; v = vv; }
+  return ok;
+#else
   f3 e0 = p1 - p0, e1 = p2 - p0;
   f3 pv = cross3(dir, e1);
   float det = dot3(e0, pv);
@@ -149,6 +182,7 @@ DI bool tri_test(f3 org, f3 dir, f3 p0, f3 p1, f3 p2, float tfar, float& t, floa
   if (tt < kEPS || tt >= tfar) return false;
   t = tt; u = uu; v = vv;
   return true;
+#endif
 }
 
 // Geometry a traversal reads, copied into registers once per ray.
