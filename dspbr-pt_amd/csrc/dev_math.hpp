@@ -34,6 +34,22 @@ DI float4 ldg(const float4* p) {
 #endif
 }
 
+// issued(): an empty asm that consumes the loaded values, placed after all
+// loads of one record. Without it the compiler sinks loads that only one
+// side of a later branch uses into that branch (e.g. a node's child boxes
+// below the leaf test on its count field), which turns one memory round trip
+// per record into two dependent ones. The asm loads nothing.
+#ifndef DSPT_ISSUE_ALL
+#define DSPT_ISSUE_ALL 1
+#endif
+DI void issued(const float4& a) {
+#if defined(__HIP_DEVICE_COMPILE__) && DSPT_ISSUE_ALL
+  asm volatile("" ::"v"(a.w));
+#else
+  (void)a;
+#endif
+}
+
 struct f3 {
   float x, y, z;
 };

@@ -250,6 +250,7 @@ DI bool trav_step(const Geo& G, RayState& r, LaneStack& st, unsigned& nv, unsign
   nv++;
   const float4* nd = G.nodes + (size_t)ni * 4;
   const float4 a = ldg(nd), b = ldg(nd + 1), c = ldg(nd + 2), d = ldg(nd + 3);
+  issued(a); issued(b); issued(c); issued(d);
   const int cnt = (int)c.w;
   const int first = (int)d.w;
   if (cnt > 0) {
@@ -258,6 +259,7 @@ DI bool trav_step(const Geo& G, RayState& r, LaneStack& st, unsigned& nv, unsign
       if (i < 0) continue;
       const float4* tp = G.tri + (size_t)i * 3;
       const float4 q0 = ldg(tp), q1 = ldg(tp + 1), q2 = ldg(tp + 2);
+    issued(q0); issued(q1); issued(q2);
       const int id = __float_as_int(q0.w);
       if (id < 0) continue;
       nt++;
@@ -311,6 +313,7 @@ DI int trav_inner(const Geo& G, RayState& r, LaneStack& st, unsigned& nv) {
   nv++;
   const float4* nd = G.nodes + (size_t)ni * 4;
   const float4 a = ldg(nd), b = ldg(nd + 1), c = ldg(nd + 2), d = ldg(nd + 3);
+  issued(a); issued(b); issued(c); issued(d);
   const int cnt = (int)c.w;
   if (cnt > 0) {
     const int first = (int)d.w;
@@ -346,6 +349,7 @@ DI bool trav_leaf(const Geo& G, RayState& r, unsigned& nt) {
     if (i < 0) continue;
     const float4* tp = G.tri + (size_t)i * 3;
     const float4 q0 = ldg(tp), q1 = ldg(tp + 1), q2 = ldg(tp + 2);
+    issued(q0); issued(q1); issued(q2);
     const int id = __float_as_int(q0.w);
     if (id < 0) continue;
     nt++;
@@ -413,6 +417,7 @@ DI int trav_inner2(const Geo2& G, RayState& r, InstState& q, LaneStack& st, unsi
   }
   const float4* nd = G.g.nodes + (size_t)ni * 4;
   const float4 a = ldg(nd), b = ldg(nd + 1), c = ldg(nd + 2), d = ldg(nd + 3);
+  issued(a); issued(b); issued(c); issued(d);
   const int cnt = (int)c.w;
   if (cnt > 0) {
     if (q.base < 0) return ray_done(r) ? 2 : 0;  // the top level has no leaves
