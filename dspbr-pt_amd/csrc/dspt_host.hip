@@ -642,34 +642,52 @@ int validate_geometry(const float* vertices, int64_t V, const uint32_t* tri_idx,
 }
 
 // Scene whose BVH is built on the GPU (gpu_bvh_build) from the uploaded
-// vertex stream; no host-side node array.
+// vertex stream; no host-side node array. With mats == NULL (K = 0) only the
+// geometry is checked and uploaded (dspt_build_bvh_gpu).
 int make_scene_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, const float* mats,
                        int32_t K, int32_t max_leaf, int32_t method, SceneRes** out, double* build_ms) {
   int32_t max_tex = -1;
-  if (int r = validate_geometry(vertices, V, tri_idx, T, mats, K, &max_tex)) return r;
+  if (mats) {
+    if (int r = validate_geometry(vertices, V, tri_idx, T, mats, K, &max_tex)) return r;
+  } else {
+    if (!vertices || !tri_idx) return fail(DSPT_ERR_ARG, "NULL scene buffer");
+    if (V <= 0 || T <= 0) return fail(DSPT_ERR_ARG, "empty scene");
+    if (T >= (1LL << 24)) return fail(DSPT_ERR_UNSUPPORTED, "scene exceeds float-encoded index range");
+    for (int64_t i = 0; i < 3 * T; i++)
+      if ((int64_t)tri_idx[i] >= V) return fail(DSPT_ERR_ARG, "triangle index " + std::to_string(i) + " out of range");
+    K = 0;
+  }
   HIPCHK(hipSetDevice(c->device));
   SceneRes* s = new SceneRes();
   Allocs& A = s->mem;
   int r = 0;
   float4* big_nodes = nullptr;
   int32_t* d_bvh = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   auto bail = [&](int code) {
     if (big_nodes) (void)hipFree(big_nodes);
     if (d_bvh) (void)hipFree(d_bvh);
+    if (e0) c->ev_pool.push_back(e0);
+    if (e1) c->ev_pool.push_back(e1);
     A.release();
     delete s;
     return code;
   };
-  if ((r = A.get(&s->verts, (size_t)V * 5)) || (r = A.get(&s->tris, (size_t)T * 3)) || (r = A.get(&s->mats, (size_t)K * 17)) ||
-      (r = A.get(&s->tri, (size_t)T * 3)))
+  auto chk = [&](hipError_t e, const char* what) {
+    return e == hipSuccess ? 0 : bail(fail(DSPT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e)));
+  };
+  if ((r = A.get(&s->verts, (size_t)V * 5)) || (r = A.get(&s->tris, (size_t)T * 3)) ||
+      (K > 0 && (r = A.get(&s->mats, (size_t)K * 17))) || (r = A.get(&s->tri, (size_t)T * 3)))
     return bail(r);
   if (hipMalloc(&big_nodes, (size_t)(2 * T - 1) * 64) != hipSuccess || hipMalloc(&d_bvh, (size_t)T * 4) != hipSuccess)
     return bail(fail(DSPT_ERR_MEMORY, "bvh build buffers"));
-  HIPCHK(hipMemcpyAsync(s->verts, vertices, (size_t)V * 80, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(s->tris, tri_idx, (size_t)T * 12, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(s->mats, mats, (size_t)K * 272, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  hipEvent_t e0 = take_event(c), e1 = take_event(c);
+  if ((r = chk(hipMemcpyAsync(s->verts, vertices, (size_t)V * 80, hipMemcpyHostToDevice, c->stream), "vertex upload")) ||
+      (r = chk(hipMemcpyAsync(s->tris, tri_idx, (size_t)T * 12, hipMemcpyHostToDevice, c->stream), "index upload")) ||
+      (K > 0 && (r = chk(hipMemcpyAsync(s->mats, mats, (size_t)K * 272, hipMemcpyHostToDevice, c->stream), "material upload"))) ||
+      (r = chk(hipStreamSynchronize(c->stream), "upload")))
+    return r;
+  e0 = take_event(c);
+  e1 = take_event(c);
   (void)hipEventRecord(e0, c->stream);
   int64_t N = 0;
   std::string err;
@@ -677,10 +695,10 @@ int make_scene_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint
     return bail(fail(rc, err));
   (void)hipEventRecord(e1, c->stream);
   if ((r = A.get(&s->nodes, (size_t)N * 4))) return bail(r);
-  HIPCHK(hipMemcpyAsync(s->nodes, big_nodes, (size_t)N * 64, hipMemcpyDeviceToDevice, c->stream));
+  if ((r = chk(hipMemcpyAsync(s->nodes, big_nodes, (size_t)N * 64, hipMemcpyDeviceToDevice, c->stream), "node copy")))
+    return r;
   k_prep_tris<<<grid_for(T, 256), 256, 0, c->stream>>>(d_bvh, s->tris, s->verts, (int32_t)T, (int32_t)T, s->tri);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(c->stream));
+  if ((r = chk(hipGetLastError(), "k_prep_tris")) || (r = chk(hipStreamSynchronize(c->stream), "scene build"))) return r;
   float ms = 0.0f;
   (void)hipEventElapsedTime(&ms, e0, e1);
   c->ev_pool.push_back(e0);
@@ -697,47 +715,42 @@ int make_scene_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint
 int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, const float* nodes, int64_t N,
                const int32_t* bvh_idx, int64_t M, const float* mats, int32_t K, SceneRes** out, bool own_nodes = false) {
   if (!vertices || !tri_idx || !nodes || !bvh_idx || !mats) return fail(DSPT_ERR_ARG, "NULL scene buffer");
-  if (V <= 0 || T <= 0 || N <= 0 || M <= 0 || K <= 0) return fail(DSPT_ERR_ARG, "empty scene");
-  if (T >= (1LL << 24) || N >= (1LL << 30) || M >= (1LL << 30))
-    return fail(DSPT_ERR_UNSUPPORTED, "scene exceeds float-encoded index range");
-  for (int64_t i = 0; i < 3 * T; i++)
-    if ((int64_t)tri_idx[i] >= V) return fail(DSPT_ERR_ARG, "triangle index " + std::to_string(i) + " out of range");
-  for (int64_t v = 0; v < V; v++) {
-    float mi = vertices[v * 20 + 3];
-    if (!(mi >= 0.0f) || mi >= (float)K) return fail(DSPT_ERR_ARG, "vertex " + std::to_string(v) + ": material index out of range");
-  }
+  if (N <= 0 || M <= 0) return fail(DSPT_ERR_ARG, "empty BVH");
+  if (N >= (1LL << 30) || M >= (1LL << 30)) return fail(DSPT_ERR_UNSUPPORTED, "scene exceeds float-encoded index range");
   int32_t max_tex = -1;
-  for (int32_t k = 0; k < K; k++)
-    for (int i : {39, 44, 45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 55, 56, 57, 58, 59, 63}) {
-      const float id = mats[(size_t)k * 68 + i];
-      if (!std::isfinite(id) || id >= 16777216.0f)
-        return fail(DSPT_ERR_ARG, "material " + std::to_string(k) + ": texture id is not a finite value below 2^24");
-      if (id >= 0.0f) max_tex = std::max(max_tex, (int32_t)id);
-    }
+  if (int r = validate_geometry(vertices, V, tri_idx, T, mats, K, &max_tex)) return r;
   if (!own_nodes)
     if (int r = validate_nodes(nodes, N)) return r;
   HIPCHK(hipSetDevice(c->device));
   SceneRes* s = new SceneRes();
   Allocs& A = s->mem;
   int r = 0;
-  if ((r = A.get(&s->nodes, (size_t)N * 4)) || (r = A.get(&s->tri, (size_t)M * 3)) || (r = A.get(&s->verts, (size_t)V * 5)) ||
-      (r = A.get(&s->tris, (size_t)T * 3)) || (r = A.get(&s->mats, (size_t)K * 17))) {
+  int32_t* d_bvh = nullptr;
+  auto bail = [&](int code) {
+    if (d_bvh) (void)hipFree(d_bvh);
     A.release();
     delete s;
-    return r;
+    return code;
+  };
+  auto chk = [&](hipError_t e, const char* what) {
+    return e == hipSuccess ? 0 : bail(fail(DSPT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e)));
+  };
+  if ((r = A.get(&s->nodes, (size_t)N * 4)) || (r = A.get(&s->tri, (size_t)M * 3)) || (r = A.get(&s->verts, (size_t)V * 5)) ||
+      (r = A.get(&s->tris, (size_t)T * 3)) || (r = A.get(&s->mats, (size_t)K * 17)))
+    return bail(r);
+  if (hipMalloc(&d_bvh, (size_t)M * sizeof(int32_t)) != hipSuccess) {
+    d_bvh = nullptr;
+    return bail(fail(DSPT_ERR_MEMORY, "bvh index staging"));
   }
-  int32_t* d_bvh = nullptr;
-  hipError_t e = hipMalloc(&d_bvh, (size_t)M * sizeof(int32_t));
-  if (e != hipSuccess) { A.release(); delete s; return fail(DSPT_ERR_MEMORY, "bvh index staging"); }
-  HIPCHK(hipMemcpyAsync(s->nodes, nodes, (size_t)N * 64, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(s->verts, vertices, (size_t)V * 80, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(s->tris, tri_idx, (size_t)T * 12, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(s->mats, mats, (size_t)K * 272, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(d_bvh, bvh_idx, (size_t)M * 4, hipMemcpyHostToDevice, c->stream));
+  if ((r = chk(hipMemcpyAsync(s->nodes, nodes, (size_t)N * 64, hipMemcpyHostToDevice, c->stream), "node upload")) ||
+      (r = chk(hipMemcpyAsync(s->verts, vertices, (size_t)V * 80, hipMemcpyHostToDevice, c->stream), "vertex upload")) ||
+      (r = chk(hipMemcpyAsync(s->tris, tri_idx, (size_t)T * 12, hipMemcpyHostToDevice, c->stream), "index upload")) ||
+      (r = chk(hipMemcpyAsync(s->mats, mats, (size_t)K * 272, hipMemcpyHostToDevice, c->stream), "material upload")) ||
+      (r = chk(hipMemcpyAsync(d_bvh, bvh_idx, (size_t)M * 4, hipMemcpyHostToDevice, c->stream), "bvh index upload")))
+    return r;
   k_prep_tris<<<grid_for(M, 256), 256, 0, c->stream>>>(d_bvh, s->tris, s->verts, (int32_t)M, (int32_t)T, s->tri);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipFree(d_bvh));
+  if ((r = chk(hipGetLastError(), "k_prep_tris")) || (r = chk(hipStreamSynchronize(c->stream), "scene upload"))) return r;
+  (void)hipFree(d_bvh);
   s->N = (int32_t)N; s->M = (int32_t)M; s->T = (int32_t)T; s->K = K; s->V = V;
   s->max_tex_id = max_tex;
   *out = s;
@@ -1152,22 +1165,15 @@ int dspt_build_bvh_gpu(dspt_ctx* c, const float* vertices, int64_t V, const uint
                        int32_t method, float* nodes, int64_t* num_nodes, int32_t* bvh_indices, double* build_ms) {
   if (!c || !nodes || !num_nodes || !bvh_indices) return fail(DSPT_ERR_ARG, "NULL argument");
   SceneRes* s = nullptr;
-  std::vector<float> mat(68, 0.0f);
-  for (int i : {39, 44, 45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 55, 56, 57, 58, 59, 63}) mat[i] = -1.0f;
-  // material indices of the vertex stream are not needed here: validate against a large K
-  std::vector<float> mats;
-  int32_t K = 1;
-  for (int64_t v = 0; v < V; v++) K = std::max<int32_t>(K, (int32_t)std::min(vertices[v * 20 + 3] + 1.0f, 1e6f));
-  mats.resize((size_t)K * 68);
-  for (int32_t k = 0; k < K; k++) std::memcpy(&mats[(size_t)k * 68], mat.data(), 68 * 4);
-  if (int r = make_scene_gpu_bvh(c, vertices, V, tri_idx, T, mats.data(), K, max_leaf, method, &s, build_ms)) return r;
-  *num_nodes = s->N;
+  if (int r = make_scene_gpu_bvh(c, vertices, V, tri_idx, T, nullptr, 0, max_leaf, method, &s, build_ms)) return r;
   std::vector<float4> tri((size_t)T * 3);
-  HIPCHK(hipMemcpy(nodes, s->nodes, (size_t)s->N * 64, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(tri.data(), s->tri, (size_t)T * 48, hipMemcpyDeviceToHost));
-  for (int64_t i = 0; i < T; i++) std::memcpy(&bvh_indices[i], &tri[(size_t)i * 3].w, 4);  // id bits in .w
+  hipError_t e = hipMemcpy(nodes, s->nodes, (size_t)s->N * 64, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(tri.data(), s->tri, (size_t)T * 48, hipMemcpyDeviceToHost);
+  *num_nodes = s->N;
   s->mem.release();
   delete s;
+  if (e != hipSuccess) return fail(DSPT_ERR_DEVICE, std::string("bvh readback: ") + hipGetErrorString(e));
+  for (int64_t i = 0; i < T; i++) std::memcpy(&bvh_indices[i], &tri[(size_t)i * 3].w, 4);  // id bits in .w
   return 0;
 }
 
