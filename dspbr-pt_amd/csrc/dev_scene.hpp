@@ -1,6 +1,7 @@
 // Device-side scene/IBL views and the traversal, IBL and shading-point
 // routines shared by the wavefront kernels.
 #pragma once
+#include <climits>
 #include "dev_bsdf.hpp"
 
 // Device view of one uploaded scene + IBL. Layouts in HBM:
@@ -201,6 +202,26 @@ struct Geo2 {
 };
 DI Geo2 geo2_of(const SceneView& S) { return Geo2{geo_of(S), S.top, S.inst_base}; }
 
+// Child references inside the engine's copy of the node array. At upload
+// each inner node's child fields (a.w, b.w) are rewritten from float-encoded
+// node indices to int codes (k_encode_children): code >= 0 is the child's
+// node index; a child that is a well-formed leaf becomes
+// -(1 + first * 64 + min(count, 64) - 1), so the walk reaches its triangles
+// without fetching the leaf node (whose box its parent already tested);
+// kChildInvalid marks an index the reference would not follow (out of
+// range or the node itself). Leaf nodes keep their float fields, so a leaf
+// reached by index (the root, or one with an out-of-range range) is read as
+// before. Visit order, step counts and the node/triangle counters are those
+// of the original array.
+constexpr int kChildInvalid = INT_MIN;
+DI int child_code(float w) { return __float_as_int(w); }
+DI bool leaf_code(int code) { return code < 0; }  // kChildInvalid is never pushed
+DI void leaf_range(int code, int M, int& first, int& end) {
+  const int v = -code - 1;
+  first = v >> 6;
+  end = min(first + (v & 63) + 1, M);
+}
+
 // Resumable traversal state of one ray (the loop variables of
 // SH/bvh.glsl:149-212). The entry on top of the reference's stack is held in
 // `next` when it was pushed by the previous step (has_next); the stack then
@@ -246,20 +267,31 @@ DI bool trav_step(const Geo& G, RayState& r, LaneStack& st, unsigned& nv, unsign
   } else {
     ni = st.get(--r.top);
   }
-  if (ni < 0 || ni >= G.N) return ray_done(r);
+  int first = 0, end = 0;
+  bool leaf;
+  float4 a, b, c, d;
+  if (leaf_code(ni)) {
+    leaf = true;
+    leaf_range(ni, G.M, first, end);
+  } else {
+    if (ni >= G.N) return ray_done(r);
+    const float4* nd = G.nodes + (size_t)ni * 4;
+    a = ldg(nd); b = ldg(nd + 1); c = ldg(nd + 2); d = ldg(nd + 3);
+    issued(a); issued(b); issued(c); issued(d);
+    const int cnt = (int)c.w;
+    leaf = cnt > 0;
+    if (leaf) {
+      first = (int)d.w;
+      end = min(first + min(cnt, kMaxLeafTris), G.M);
+    }
+  }
   nv++;
-  const float4* nd = G.nodes + (size_t)ni * 4;
-  const float4 a = ldg(nd), b = ldg(nd + 1), c = ldg(nd + 2), d = ldg(nd + 3);
-  issued(a); issued(b); issued(c); issued(d);
-  const int cnt = (int)c.w;
-  const int first = (int)d.w;
-  if (cnt > 0) {
-    const int end = min(first + min(cnt, kMaxLeafTris), G.M);
+  if (leaf) {
     for (int i = first; i < end; i++) {
       if (i < 0) continue;
       const float4* tp = G.tri + (size_t)i * 3;
       const float4 q0 = ldg(tp), q1 = ldg(tp + 1), q2 = ldg(tp + 2);
-    issued(q0); issued(q1); issued(q2);
+      issued(q0); issued(q1); issued(q2);
       const int id = __float_as_int(q0.w);
       if (id < 0) continue;
       nt++;
@@ -270,7 +302,7 @@ DI bool trav_step(const Geo& G, RayState& r, LaneStack& st, unsigned& nv, unsign
       }
     }
   } else {
-    const int left = (int)a.w, right = (int)b.w;
+    const int left = child_code(a.w), right = child_code(b.w);
     const float d0 = slab(xyz(a), xyz(b), r.org, r.inv, r.h.t);
     const float d1 = slab(xyz(c), xyz(d), r.org, r.inv, r.h.t);
     const bool h0 = d0 < kBVH_FAR, h1 = d1 < kBVH_FAR;
@@ -281,11 +313,10 @@ DI bool trav_step(const Geo& G, RayState& r, LaneStack& st, unsigned& nv, unsign
     // next node visited, so it stays in a register
     const int c1 = both ? (near_left ? right : left) : (h0 ? left : right);
     const int c2 = near_left ? left : right;
-    auto ok = [&](int child) { return child >= 0 && child < G.N && child != ni; };
-    const bool p1 = (h0 || h1) && r.top < kStackSize && ok(c1);
+    const bool p1 = (h0 || h1) && r.top < kStackSize && c1 != kChildInvalid;
     if (both) {
       if (p1) st.put(r.top++, c1);
-      if (r.top < kStackSize && ok(c2)) { r.next = c2; r.has_next = true; }
+      if (r.top < kStackSize && c2 != kChildInvalid) { r.next = c2; r.has_next = true; }
     } else if (p1) {
       r.next = c1;
       r.has_next = true;
@@ -309,7 +340,12 @@ DI int trav_inner(const Geo& G, RayState& r, LaneStack& st, unsigned& nv) {
   } else {
     ni = st.get(--r.top);
   }
-  if (ni < 0 || ni >= G.N) return ray_done(r) ? 2 : 0;
+  if (leaf_code(ni)) {  // a leaf child: its range is in the code, no fetch
+    nv++;
+    leaf_range(ni, G.M, r.leaf_first, r.leaf_end);
+    return 1;
+  }
+  if (ni >= G.N) return ray_done(r) ? 2 : 0;
   nv++;
   const float4* nd = G.nodes + (size_t)ni * 4;
   const float4 a = ldg(nd), b = ldg(nd + 1), c = ldg(nd + 2), d = ldg(nd + 3);
@@ -321,7 +357,7 @@ DI int trav_inner(const Geo& G, RayState& r, LaneStack& st, unsigned& nv) {
     r.leaf_end = min(first + min(cnt, kMaxLeafTris), G.M);
     return 1;
   }
-  const int left = (int)a.w, right = (int)b.w;
+  const int left = child_code(a.w), right = child_code(b.w);
   const float d0 = slab(xyz(a), xyz(b), r.org, r.inv, r.h.t);
   const float d1 = slab(xyz(c), xyz(d), r.org, r.inv, r.h.t);
   const bool h0 = d0 < kBVH_FAR, h1 = d1 < kBVH_FAR;
@@ -329,11 +365,10 @@ DI int trav_inner(const Geo& G, RayState& r, LaneStack& st, unsigned& nv) {
   const bool near_left = d0 < d1;
   const int c1 = both ? (near_left ? right : left) : (h0 ? left : right);
   const int c2 = near_left ? left : right;
-  auto ok = [&](int child) { return child >= 0 && child < G.N && child != ni; };
-  const bool p1 = (h0 || h1) && r.top < kStackSize && ok(c1);
+  const bool p1 = (h0 || h1) && r.top < kStackSize && c1 != kChildInvalid;
   if (both) {
     if (p1) st.put(r.top++, c1);
-    if (r.top < kStackSize && ok(c2)) { r.next = c2; r.has_next = true; }
+    if (r.top < kStackSize && c2 != kChildInvalid) { r.next = c2; r.has_next = true; }
   } else if (p1) {
     r.next = c1;
     r.has_next = true;
@@ -396,7 +431,13 @@ DI int trav_inner2(const Geo2& G, RayState& r, InstState& q, LaneStack& st, unsi
     ni = st.get(--r.top);
   }
   const int N = G.g.N;
-  if (ni < 0 || ni >= N) return ray_done(r) ? 2 : 0;
+  if (leaf_code(ni)) {  // a bottom-level leaf child (the top level has none)
+    nv++;
+    if (q.base < 0) return ray_done(r) ? 2 : 0;
+    leaf_range(ni, G.g.M, r.leaf_first, r.leaf_end);
+    return 1;
+  }
+  if (ni >= N) return ray_done(r) ? 2 : 0;
   nv++;
   if (q.base < 0 && ni >= G.inst_base) {
     // a top-level child that is an instance record (its world box was the
@@ -426,7 +467,7 @@ DI int trav_inner2(const Geo2& G, RayState& r, InstState& q, LaneStack& st, unsi
     r.leaf_end = min(first + min(cnt, kMaxLeafTris), G.g.M);
     return 1;
   }
-  const int left = (int)a.w, right = (int)b.w;
+  const int left = child_code(a.w), right = child_code(b.w);
   const float d0 = slab(xyz(a), xyz(b), r.org, r.inv, r.h.t);
   const float d1 = slab(xyz(c), xyz(d), r.org, r.inv, r.h.t);
   const bool h0 = d0 < kBVH_FAR, h1 = d1 < kBVH_FAR;
@@ -434,11 +475,10 @@ DI int trav_inner2(const Geo2& G, RayState& r, InstState& q, LaneStack& st, unsi
   const bool near_left = d0 < d1;
   const int c1 = both ? (near_left ? right : left) : (h0 ? left : right);
   const int c2 = near_left ? left : right;
-  auto ok = [&](int child) { return child >= 0 && child < N && child != ni; };
-  const bool p1 = (h0 || h1) && r.top < kStackSize && ok(c1);
+  const bool p1 = (h0 || h1) && r.top < kStackSize && c1 != kChildInvalid;
   if (both) {
     if (p1) st.put(r.top++, c1);
-    if (r.top < kStackSize && ok(c2)) { r.next = c2; r.has_next = true; }
+    if (r.top < kStackSize && c2 != kChildInvalid) { r.next = c2; r.has_next = true; }
   } else if (p1) {
     r.next = c1;
     r.has_next = true;

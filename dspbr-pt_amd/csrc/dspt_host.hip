@@ -698,7 +698,9 @@ int make_scene_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint
   if ((r = chk(hipMemcpyAsync(s->nodes, big_nodes, (size_t)N * 64, hipMemcpyDeviceToDevice, c->stream), "node copy")))
     return r;
   k_prep_tris<<<grid_for(T, 256), 256, 0, c->stream>>>(d_bvh, s->tris, s->verts, (int32_t)T, (int32_t)T, s->tri);
-  if ((r = chk(hipGetLastError(), "k_prep_tris")) || (r = chk(hipStreamSynchronize(c->stream), "scene build"))) return r;
+  if (mats)  // a resident scene: child codes (the host-readback build keeps the bvh_gpu fields)
+    k_encode_children<<<grid_for(N, 256), 256, 0, c->stream>>>(s->nodes, (int32_t)N, (int32_t)N);
+  if ((r = chk(hipGetLastError(), "scene kernels")) || (r = chk(hipStreamSynchronize(c->stream), "scene build"))) return r;
   float ms = 0.0f;
   (void)hipEventElapsedTime(&ms, e0, e1);
   c->ev_pool.push_back(e0);
@@ -712,8 +714,11 @@ int make_scene_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint
   return 0;
 }
 
+// encode_limit: nodes below it get child codes (k_encode_children); -1 = all
+// N (a two-level scene passes the first instance-record slot).
 int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, const float* nodes, int64_t N,
-               const int32_t* bvh_idx, int64_t M, const float* mats, int32_t K, SceneRes** out, bool own_nodes = false) {
+               const int32_t* bvh_idx, int64_t M, const float* mats, int32_t K, SceneRes** out, bool own_nodes = false,
+               int64_t encode_limit = -1) {
   if (!vertices || !tri_idx || !nodes || !bvh_idx || !mats) return fail(DSPT_ERR_ARG, "NULL scene buffer");
   if (N <= 0 || M <= 0) return fail(DSPT_ERR_ARG, "empty BVH");
   if (N >= (1LL << 30) || M >= (1LL << 30)) return fail(DSPT_ERR_UNSUPPORTED, "scene exceeds float-encoded index range");
@@ -749,7 +754,9 @@ int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tr
       (r = chk(hipMemcpyAsync(d_bvh, bvh_idx, (size_t)M * 4, hipMemcpyHostToDevice, c->stream), "bvh index upload")))
     return r;
   k_prep_tris<<<grid_for(M, 256), 256, 0, c->stream>>>(d_bvh, s->tris, s->verts, (int32_t)M, (int32_t)T, s->tri);
-  if ((r = chk(hipGetLastError(), "k_prep_tris")) || (r = chk(hipStreamSynchronize(c->stream), "scene upload"))) return r;
+  const int64_t lim = encode_limit < 0 ? N : std::min(encode_limit, N);
+  k_encode_children<<<grid_for(lim, 256), 256, 0, c->stream>>>(s->nodes, (int32_t)N, (int32_t)lim);
+  if ((r = chk(hipGetLastError(), "scene kernels")) || (r = chk(hipStreamSynchronize(c->stream), "scene upload"))) return r;
   (void)hipFree(d_bvh);
   s->N = (int32_t)N; s->M = (int32_t)M; s->T = (int32_t)T; s->K = K; s->V = V;
   s->max_tex_id = max_tex;
@@ -897,7 +904,8 @@ int make_scene_instanced(dspt_ctx* c, const float* vertices, int64_t V, const ui
   nodes.resize(ib + (size_t)num_inst * 32);
   std::memcpy(nodes.data() + ib, inst.data(), (size_t)num_inst * sizeof(InstanceDesc));
   SceneRes* s = nullptr;
-  if (int r = make_scene(c, vertices, V, tri_idx, T, nodes.data(), N, bidx.data(), (int64_t)bidx.size(), mats, K, &s, true))
+  if (int r = make_scene(c, vertices, V, tri_idx, T, nodes.data(), N, bidx.data(), (int64_t)bidx.size(), mats, K, &s, true,
+                         inst_base))
     return r;
   s->top = (int32_t)top;
   s->inst_base = (int32_t)inst_base;

@@ -63,6 +63,31 @@ __global__ void k_prep_tris(const int32_t* __restrict__ bvh_idx, const uint32_t*
   out[i * 3 + 2] = make_float4(p[2].x, p[2].y, p[2].z, 0.0f);
 }
 
+// Child codes of the engine's node array (see child_code in dev_scene.hpp):
+// one thread per node below `limit` (the instance records of a two-level
+// scene follow the nodes and are not nodes). Only a.w / b.w of inner nodes
+// are written; the count/first fields read here are never written.
+__global__ void k_encode_children(float4* __restrict__ nodes, int32_t N, int32_t limit) {
+  const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= limit) return;
+  if ((int)nodes[(size_t)n * 4 + 2].w > 0) return;  // a leaf node keeps its fields
+  for (int side = 0; side < 2; side++) {
+    float4 e = nodes[(size_t)n * 4 + side];
+    const int child = (int)e.w;
+    int code;
+    if (child < 0 || child >= N || child == n) {
+      code = kChildInvalid;
+    } else if (child < limit) {
+      const int cnt = (int)nodes[(size_t)child * 4 + 2].w, first = (int)nodes[(size_t)child * 4 + 3].w;
+      code = (cnt > 0 && first >= 0 && first < (1 << 24)) ? -(1 + (first << 6) + (min(cnt, kMaxLeafTris) - 1)) : child;
+    } else {
+      code = child;
+    }
+    e.w = __int_as_float(code);
+    nodes[(size_t)n * 4 + side] = e;
+  }
+}
+
 // --------------------------------------------------------------------------
 // IBL importance-sampling tables, renderer.ts:777-838, bit-identical: the
 // luminance and the per-row sums are double (JS numbers), pdf is rounded to
