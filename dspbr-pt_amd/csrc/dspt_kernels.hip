@@ -229,20 +229,6 @@ __global__ void k_regen(PathState ps, const DevJob* __restrict__ jobs, const Pas
   wave_add_u64(&cnt[12], issued);
 }
 
-// XCD-aware block order: consecutive queue ranges go to the same XCD (the
-// hardware deals workgroups round-robin over the 8 XCDs), so coherent rays
-// share an L2.
-DI int64_t xcd_block(int64_t b, int64_t nb) {
-  const int64_t X = 8;
-  if (nb < X) return b;
-  int64_t per = nb / X;
-  int64_t rem = nb - per * X;
-  int64_t x = b % X, k = b / X;
-  if (k >= per) return per * X + x;  // tail blocks beyond a full round
-  (void)rem;
-  return x * per + k;
-}
-
 // Lanes of a traversal wave take new rays from the queue whenever at least
 // kRefill of them are idle (one atomic per wave), so a wave is not held by
 // its slowest ray; every ray still runs the same step sequence.

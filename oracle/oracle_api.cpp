@@ -229,6 +229,47 @@ int oracle_tex_value(const Scene* s, float id, const float* uv, int n, float* ou
 }
 int oracle_frame_struct_size() { return (int)sizeof(Frame); }
 
+// Thin-film reflectance probes (KF:789-915): in f32[5n] = {outsideIor,
+// filmIor, baseF0 (one value for all channels, or < 0 to use in[5n + 3i..]),
+// thicknessNm, cosTheta1}; out f64[3n]. base3 (f32[3n]) holds per-channel
+// base F0 for the rows whose baseF0 is negative (may be NULL otherwise).
+int oracle_thin_film(const float* in, const float* base3, int n, int use_double, double* out) {
+  for (int i = 0; i < n; i++) {
+    const float* r = in + (size_t)5 * i;
+    float b[3] = {r[2], r[2], r[2]};
+    if (r[2] < 0.0f) {
+      if (!base3) return -1;
+      for (int k = 0; k < 3; k++) b[k] = base3[(size_t)3 * i + k];
+    }
+    if (use_double) {
+      V3<double> f = thinFilmReflectance<double>(r[0], r[1], V3<double>(b[0], b[1], b[2]), r[3], r[4]);
+      out[3 * i] = f.x; out[3 * i + 1] = f.y; out[3 * i + 2] = f.z;
+    } else {
+      V3<float> f = thinFilmReflectance<float>(r[0], r[1], V3<float>(b[0], b[1], b[2]), r[3], r[4]);
+      out[3 * i] = f.x; out[3 * i + 1] = f.y; out[3 * i + 2] = f.z;
+    }
+  }
+  return 0;
+}
+
+// Dispersion channel IORs (KF:1289-1298) of (ior, dispersion) pairs: out f64[3n].
+int oracle_dispersion_iors(const float* ior_disp, int n, int use_double, double* out) {
+  for (int i = 0; i < n; i++) {
+    if (use_double) {
+      Closure<double> c{};
+      c.ior = ior_disp[2 * i]; c.dispersion = ior_disp[2 * i + 1];
+      V3<double> f = dispersionIors(c);
+      out[3 * i] = f.x; out[3 * i + 1] = f.y; out[3 * i + 2] = f.z;
+    } else {
+      Closure<float> c{};
+      c.ior = ior_disp[2 * i]; c.dispersion = ior_disp[2 * i + 1];
+      V3<float> f = dispersionIors(c);
+      out[3 * i] = f.x; out[3 * i + 1] = f.y; out[3 * i + 2] = f.z;
+    }
+  }
+  return 0;
+}
+
 // renderer.ts:665-700 over an ow x oh canvas: tonemap (+ gamma), optional
 // FXAA, display resample, 8-bit conversion. rgba32 / rgba8 may be NULL.
 void oracle_postprocess(const float* accum, int W, int H, int mode, float exposure, int gamma, int fxaa, int ow, int oh,

@@ -59,6 +59,8 @@ def lib():
         L.oracle_tex_value.argtypes = [ctypes.POINTER(CScene), ctypes.c_float, _f, ctypes.c_int, _f]
         L.oracle_postprocess.argtypes = [_f, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, _f, ctypes.POINTER(ctypes.c_uint8)]
+        L.oracle_thin_film.argtypes = [_f, _f, ctypes.c_int, ctypes.c_int, _d]
+        L.oracle_dispersion_iors.argtypes = [_f, ctypes.c_int, ctypes.c_int, _d]
         assert L.oracle_scene_struct_size() == ctypes.sizeof(CScene)
         assert L.oracle_frame_struct_size() == 52 * 4
         _lib = L
@@ -185,6 +187,25 @@ def rng_step(x, y, n):
     states = np.zeros(2 * n, np.uint32)
     lib().oracle_rng_step(int(x), int(y), n, _p(states, ctypes.c_uint32))
     return states.reshape(n, 2)
+
+
+def thin_film(rows, base3=None, use_double=True):
+    """rows: (n, 5) {outsideIor, filmIor, baseF0 (< 0: per channel from base3),
+    thicknessNm, cosTheta1} -> (n, 3) thin-film reflectance (KF:789-915)."""
+    rows = np.ascontiguousarray(rows, np.float32).reshape(-1, 5)
+    b3 = None if base3 is None else np.ascontiguousarray(base3, np.float32).reshape(-1, 3)
+    out = np.zeros((rows.shape[0], 3), np.float64)
+    rc = lib().oracle_thin_film(_p(rows), _p(b3), rows.shape[0], 1 if use_double else 0, _p(out, ctypes.c_double))
+    assert rc == 0
+    return out
+
+
+def dispersion_iors(ior_disp, use_double=True):
+    """(n, 2) {ior, dispersion} -> (n, 3) channel IORs (KF:1289-1298)."""
+    a = np.ascontiguousarray(ior_disp, np.float32).reshape(-1, 2)
+    out = np.zeros((a.shape[0], 3), np.float64)
+    lib().oracle_dispersion_iors(_p(a), a.shape[0], 1 if use_double else 0, _p(out, ctypes.c_double))
+    return out
 
 
 def bsdf_eval(mat, geo, wo, profile=0, use_double=True):

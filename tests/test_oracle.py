@@ -24,9 +24,19 @@ wrong sign/index, swapped operand) fails:
   integrator (misptdl.glsl)     emissive furnace closed form including the
                                 (1/0.9)^k Russian-roulette factor; constant-environment
                                 NEE+MIS furnace; background; running mean
-Parity unpinned (no check above distinguishes a plausible slip): the thin-film
-Airy sum (iridescence) and the dispersion channel IORs; they are checked only for
-invariants (finite, in range, ordering) and GPU-vs-oracle agreement.
+  thin film (KF:741-936)        the Airy reflectance |r|^2 integrated by quadrature over
+                                the spectrum against the Gaussian fits of the CIE 1931
+                                matching functions (Belcour & Barla 2017), within the
+                                bound of the harmonics the closed form drops; fit areas,
+                                fit peaks and the D65 white of the XYZ->sRGB matrix;
+                                per-channel conductor substrate == per-channel scalar
+                                film; total internal reflection; thick-film incoherent
+                                limit R12 + T^2 R23 / (1 - R12 R23)
+  dispersion IORs (KF:1289-98)  Abbe spread n_F - n_C = (n_d - 1) dispersion / 20 and
+                                Cauchy's A + B/l^2 at the Fraunhofer C, d, F lines
+Parity unpinned: a conductor substrate whose channels lie on both sides of the
+film index (each channel's phase then enters one XYZ component, a trait of the
+closed form with no physical counterpart); checked for invariants only.
 """
 import math
 from fractions import Fraction
@@ -538,6 +548,158 @@ def test_film_and_dispersion_invariants():
     w = s[refr, 4:7]
     # a refracted dispersion sample carries exactly one colour channel
     assert ((w > 0).sum(1) <= 1).all()
+
+
+# Thin film: the published spectral method (Belcour & Barla 2017, "A Practical
+# Extension to Microfacet Theory for the Modeling of Varying Iridescence")
+# fits the CIE 1931 matching functions by Gaussians in wavenumber nu (1/m) and
+# sums the Fourier series of the Airy reflectance in closed form, truncated
+# after the second harmonic. The pin integrates the Airy reflectance itself,
+# |r|^2 with r = (r12 + r23 e^{i delta}) / (1 + r12 r23 e^{i delta}) and signed
+# Fresnel amplitudes, against those Gaussians by quadrature over nu. The mean
+# (DC) reflectance is rendered white (the reference's convention); the
+# oscillating part goes through the spectral weights and XYZ -> linear sRGB.
+_CMF = [  # (weight, centre nu, variance of the Fourier-domain fit) per XYZ channel
+    [(5.4856e-13, 1.6810e6, 4.3278e9), (9.7470e-14, 2.2399e6, 4.5282e9)],
+    [(4.4201e-13, 1.7953e6, 9.3046e9)],
+    [(5.2481e-13, 2.2084e6, 6.6121e9)],
+]
+_CMF_NORM = 1.0685e-7  # integral of the y fit (luminance normalisation)
+_XYZ_TO_RGB = np.array([[3.2404542, -1.5371385, -0.4985314],
+                        [-0.9692660, 1.8760108, 0.0415560],
+                        [0.0556434, -0.2040259, 1.0572252]])
+
+
+def _schlick(f0, c):
+    return f0 + (1.0 - f0) * (1.0 - c) ** 5
+
+
+def _airy_spectral_rgb(n1, n2, base_f0, d_nm, c1):
+    """Reflectance of a film (n2, thickness d) between n1 and a substrate of
+    normal-incidence reflectance base_f0 (same for all channels), integrated
+    over the spectrum. Returns (rgb, C, x): C, x give the tail bound."""
+    c2 = math.sqrt(1.0 - (n1 / n2) ** 2 * (1.0 - c1 * c1))
+    n3 = (1.0 + math.sqrt(base_f0)) / (1.0 - math.sqrt(base_f0))
+    R12 = _schlick(((n2 - n1) / (n2 + n1)) ** 2, c1)
+    R23 = _schlick(((n3 - n2) / (n3 + n2)) ** 2, c2)
+    r12 = (1.0 if n1 > n2 else -1.0) * math.sqrt(R12)  # (n_i - n_t) / (n_i + n_t) sign
+    r23 = (1.0 if n2 > n3 else -1.0) * math.sqrt(R23)
+    opd = 2.0 * n2 * d_nm * 1e-9 * c2  # metres
+
+    def refl(delta):
+        e = np.exp(1j * delta)
+        return np.abs((r12 + r23 * e) / (1.0 + r12 * r23 * e)) ** 2
+
+    # DC term: the period average of |r|^2, by quadrature (exact for this smooth periodic integrand)
+    C0 = refl(np.linspace(0.0, 2 * math.pi, 4096, endpoint=False)).mean()
+    nu = np.linspace(0.02e6, 4.2e6, 400001)
+    osc = refl(2 * math.pi * nu * opd) - C0
+    dnu = nu[1] - nu[0]
+    xyz = np.zeros(3)
+    for k, lobes in enumerate(_CMF):
+        for val, mu, var in lobes:
+            # weight whose cosine transform is val*sqrt(2 pi var)*cos(mu k + s)*exp(-k^2 var)
+            w = val / math.sqrt(2.0) * np.exp(-(nu - mu) ** 2 / (4.0 * var))
+            xyz[k] += (osc * w).sum() * dnu
+    # out-of-gamut (negative) linear sRGB clamps to zero, as any reflectance
+    rgb = np.maximum(C0 + _XYZ_TO_RGB @ (xyz / _CMF_NORM), 0.0)
+    x = math.sqrt(R12 * R23)
+    C = (1 - R12) * (1 - R23) / (1 - x * x)
+    return rgb, C, x
+
+
+def test_thin_film_cmf_fit_and_matrix():
+    # the y fit integrates to the normalisation, and the x, z fits to the same
+    # area within 0.2 % (CIE 1931 x, y, z have equal areas); the matrix maps
+    # the D65 white point to RGB (1, 1, 1); the fitted peaks sit at the
+    # matching functions' peaks (x 595 / 446 nm, y 557 nm, z 453 nm)
+    area = [sum(v * math.sqrt(2 * math.pi * var) for v, _, var in lobes) / _CMF_NORM for lobes in _CMF]
+    assert abs(area[1] - 1) < 5e-4 and abs(area[0] - 1) < 2e-3 and abs(area[2] - 1) < 2e-3, area
+    np.testing.assert_allclose(_XYZ_TO_RGB @ np.array([0.95047, 1.0, 1.08883]), 1.0, atol=2e-4)
+    peaks = [1e9 / mu for lobes in _CMF for _, mu, _ in lobes]
+    np.testing.assert_allclose(peaks, [594.9, 446.4, 557.0, 452.8], atol=0.5)
+
+
+def test_thin_film_matches_spectral_airy_integral():
+    rng = np.random.default_rng(41)
+    rows, truth, tol = [], [], []
+    while len(rows) < 60:
+        n1 = float(rng.choice([1.0, 1.0, 1.33]))
+        n2 = float(rng.uniform(1.05, 2.4))
+        f0 = float(rng.uniform(0.0, 0.25))
+        n3 = (1 + math.sqrt(f0)) / (1 - math.sqrt(f0))
+        if abs(n2 - n1) < 0.03 or abs(n3 - n2) < 0.03:
+            continue
+        d = float(rng.choice([0.0, rng.uniform(50, 1500), rng.uniform(1500, 4000)]))
+        c1 = float(rng.uniform(0.3, 1.0))
+        if 1.0 - (n1 / n2) ** 2 * (1 - c1 * c1) <= 0:
+            continue
+        rgb, C, x = _airy_spectral_rgb(n1, n2, f0, d, c1)
+        # neglected harmonics m >= 3: |2 C x^m S_m| with |S_m| <= row-abs-sum of the matrix x max fit area
+        bound = 2 * C * x ** 3 / (1 - x) * np.abs(_XYZ_TO_RGB).sum(1) * 1.002
+        if x * x < 1e-5 or bound.max() > 3e-4:
+            continue
+        rows.append([n1, n2, f0, d, c1])
+        truth.append(rgb)
+        tol.append(bound + 2e-6)
+    rows = np.array(rows, np.float32)
+    # the spectral integral runs on the f32-rounded inputs the oracle sees
+    truth = np.array([_airy_spectral_rgb(*map(float, r))[0] for r in rows])
+    tol = np.array(tol)
+    got = O.thin_film(rows)
+    err = np.abs(got - truth)
+    assert (err <= tol).all(), (err / tol).max()
+    # the cases are coloured well beyond the tolerance (a film dropped or
+    # mis-phased would not pass), and thin enough that the harmonics matter
+    assert np.abs(truth - truth.mean(1, keepdims=True)).max() > 100 * tol.max()
+    assert (rows[:, 3] == 0).any() and ((rows[:, 3] > 0) & (rows[:, 3] < 1500)).sum() >= 10
+
+
+def test_thin_film_per_channel_substrate_and_limits():
+    # a conductor substrate (per-channel F0, all channels on one side of the
+    # film index): each channel is the scalar film over that channel's F0
+    base = np.array([[0.9, 0.5, 0.3], [0.6, 0.62, 0.64]], np.float32)
+    for b in base:
+        rows = np.array([[1.0, 1.4, -1.0, 600.0, 0.7]], np.float32)
+        got = O.thin_film(rows, base3=b[None])[0]
+        for k in range(3):
+            one = O.thin_film(np.array([[1.0, 1.4, b[k], 600.0, 0.7]], np.float32))[0]
+            assert abs(got[k] - one[k]) <= 1e-12 * max(1.0, abs(one[k]))
+    # total internal reflection into the film gives 1
+    np.testing.assert_array_equal(O.thin_film(np.array([[1.6, 1.2, 0.04, 300.0, 0.2]], np.float32)), 1.0)
+    # a thick film averages out to the incoherent sum R12 + T^2 R23 / (1 - R12 R23)
+    rows = np.array([[1.0, 1.5, 0.2, 2e5, 0.9]], np.float32)
+    n1, n2, f0, _, c1 = map(float, rows[0])
+    c2 = math.sqrt(1.0 - (n1 / n2) ** 2 * (1.0 - c1 * c1))
+    n3 = (1 + math.sqrt(f0)) / (1 - math.sqrt(f0))
+    R12 = _schlick(((n2 - n1) / (n2 + n1)) ** 2, c1)
+    R23 = _schlick(((n3 - n2) / (n3 + n2)) ** 2, c2)
+    np.testing.assert_allclose(O.thin_film(rows)[0], R12 + (1 - R12) ** 2 * R23 / (1 - R12 * R23), rtol=1e-6)
+
+
+def test_dispersion_ior_is_cauchy_at_fraunhofer_lines():
+    # KHR_materials_dispersion: dispersion = 20 / V_d with the Abbe number
+    # V_d = (n_d - 1) / (n_F - n_C). With Cauchy's n(l) = A + B / l^2 through
+    # n_d, the red / green / blue channel indices are n_C, n_d, n_F at the
+    # Fraunhofer C (656.2725 nm), d (587.5618 nm) and F (486.1327 nm) lines.
+    lC, ld, lF = 656.2725, 587.5618, 486.1327
+    kr = (1 / lC ** 2 - 1 / ld ** 2) / (1 / lF ** 2 - 1 / lC ** 2)
+    kb = (1 / lF ** 2 - 1 / ld ** 2) / (1 / lF ** 2 - 1 / lC ** 2)
+    rng = np.random.default_rng(43)
+    a = np.stack([rng.uniform(1.05, 2.4, 200), rng.uniform(0.0, 10.0, 200)], 1).astype(np.float32)
+    got = O.dispersion_iors(a)
+    n_d = a[:, 0].astype(np.float64)
+    spread = (n_d - 1) / (20.0 / np.maximum(a[:, 1].astype(np.float64), 1e-30))  # n_F - n_C
+    np.testing.assert_array_equal(got[:, 1], n_d)
+    # blue - red is the Abbe spread (f32 constants: 1e-6 relative)
+    np.testing.assert_allclose(got[:, 2] - got[:, 0], spread, rtol=2e-6, atol=1e-12)
+    # each channel sits at its line (the line wavelengths pin the split to 1e-4 of the spread)
+    assert (np.abs(got[:, 0] - (n_d + kr * spread)) <= 1e-4 * spread + 1e-12).all()
+    assert (np.abs(got[:, 2] - (n_d + kb * spread)) <= 1e-4 * spread + 1e-12).all()
+    # normal dispersion: red < green < blue; no dispersion, no split; indices stay >= 1
+    assert (got[a[:, 1] > 0, 0] < got[a[:, 1] > 0, 1]).all() and (got[a[:, 1] > 0, 2] > got[a[:, 1] > 0, 1]).all()
+    np.testing.assert_array_equal(O.dispersion_iors(np.array([[1.5, 0.0]], np.float32))[0], 1.5)
+    assert (O.dispersion_iors(np.array([[1.01, 60.0], [0.5, 3.0]], np.float32)) >= 1.0).all()
 
 
 # --------------------------------------------------------------------- integrator
