@@ -417,8 +417,10 @@ int gpu_bvh_build(hipStream_t stream, const float4* verts, const uint32_t* tris,
   k_bvh_boxes<<<grid_for(n, 256), 256, 0, stream>>>(verts, tris, n, tlo, thi, cb);
   k_bvh_morton<<<grid_for(n, 256), 256, 0, stream>>>(tlo, thi, cb, n, k0, v0);
   size_t sort_bytes = 0, scan_bytes = 0;
-  hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k0, k1, v0, v1, n, 0, 63, stream);
-  hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flag, slot, (int)m, stream);
+  if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k0, k1, v0, v1, n, 0, 63, stream)) != hipSuccess)
+    return fail_hip(e, "sort sizing");
+  if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flag, slot, (int)m, stream)) != hipSuccess)
+    return fail_hip(e, "scan sizing");
   size_t tmp_bytes = std::max(sort_bytes, scan_bytes);
   if ((e = hipMalloc(&tmp, tmp_bytes)) != hipSuccess) return fail_hip(e, "sort storage");
   if ((e = hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, k0, k1, v0, v1, n, 0, 63, stream)) != hipSuccess)

@@ -201,11 +201,26 @@ struct dspt_ctx {
   // canvas output of the display chain, grown on demand
   char* post_buf = nullptr;
   size_t post_cap = 0;
+  // the bounce loop of a chunk captured as a HIP graph, one per (state
+  // buffer holding the live paths at chunk start, two-level kernels);
+  // rebuilt when the pool, the grids or the chunk length change
+  bool use_graphs = true;
+  hipGraphExec_t bounce_graph[2][2] = {};
+  uint64_t graph_key = 0;
 };
 
 namespace {
 
 int grid_for(int64_t n, int block) { return (int)std::max<int64_t>(1, (n + block - 1) / block); }
+
+void drop_graphs(dspt_ctx* c) {
+  for (auto& row : c->bounce_graph)
+    for (auto& g : row)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+}
 
 #include "bvh_gpu.hip"
 
@@ -323,6 +338,8 @@ int ensure_pool(dspt_ctx* c) {
   if (c->cap > 0 && (c->cap == c->max_paths || !pool_idle(c))) return 0;  // resize once idle
   HIPCHK(hipStreamSynchronize(c->stream));
   const int64_t cap = c->max_paths;
+  drop_graphs(c);  // they hold the old pool's addresses
+  c->graph_key = 0;
   c->state_mem.release();
   c->cap = 0;
   PathState& a = c->ps[0];
@@ -502,35 +519,70 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
   bool inst = false;
   for (const JobRes* j : c->jobs)
     if (j && j->alive && c->scenes[j->scene] && c->scenes[j->scene]->top >= 0) inst = true;
-  for (int k = 0; k < c->chunk_iters; k++) {
-    const int a = c->qsel;
-    const PathState cur = c->ps[a], nxt = c->ps[a ^ 1];
-    timed(c, DSPT_K_RAYGEN, [&] {
-      k_plan<<<1, 64, 0, c->stream>>>(ctl, c->d_sched, a, P32);
-      k_regen<<<shade_blocks, 256, 0, c->stream>>>(cur, c->d_jobs, c->d_tasks, c->d_sched, ctl, c->cnt);
-    });
-    timed(c, DSPT_K_TRACE_EXT, [&] {
-      if (inst)
-        k_trace_ext<true><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], &ctl->ext_next, c->spill,
-                                                                      c->cnt);
-      else
-        k_trace_ext<false><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], &ctl->ext_next, c->spill,
-                                                                       c->cnt);
-    });
-    timed(c, DSPT_K_SHADE, [&] {
-      k_shade<<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1], c->shq,
-                                                    &ctl->sh_count, c->cnt);
-    });
-    timed(c, DSPT_K_TRACE_SHADOW, [&] {
-      if (inst)
-        k_trace_shadow<true><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, nxt, c->shq, &ctl->sh_count, &ctl->sh_next,
+  // chunk_iters bounces; the state buffers alternate, starting at q0
+  auto bounces = [&](int q0) {
+    for (int k = 0; k < c->chunk_iters; k++) {
+      const int a = (q0 + k) & 1;
+      const PathState cur = c->ps[a], nxt = c->ps[a ^ 1];
+      timed(c, DSPT_K_RAYGEN, [&] {
+        k_plan<<<1, 64, 0, c->stream>>>(ctl, c->d_sched, a, P32);
+        k_regen<<<shade_blocks, 256, 0, c->stream>>>(cur, c->d_jobs, c->d_tasks, c->d_sched, ctl, c->cnt);
+      });
+      timed(c, DSPT_K_TRACE_EXT, [&] {
+        if (inst)
+          k_trace_ext<true><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], &ctl->ext_next,
+                                                                        c->spill, c->cnt);
+        else
+          k_trace_ext<false><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], &ctl->ext_next,
                                                                          c->spill, c->cnt);
-      else
-        k_trace_shadow<false><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, nxt, c->shq, &ctl->sh_count,
-                                                                          &ctl->sh_next, c->spill, c->cnt);
-    });
-    c->qsel ^= 1;
+      });
+      timed(c, DSPT_K_SHADE, [&] {
+        k_shade<<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1], c->shq,
+                                                      &ctl->sh_count, c->cnt);
+      });
+      timed(c, DSPT_K_TRACE_SHADOW, [&] {
+        if (inst)
+          k_trace_shadow<true><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, nxt, c->shq, &ctl->sh_count,
+                                                                           &ctl->sh_next, c->spill, c->cnt);
+        else
+          k_trace_shadow<false><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, nxt, c->shq, &ctl->sh_count,
+                                                                            &ctl->sh_next, c->spill, c->cnt);
+      });
+    }
+  };
+  const int q0 = c->qsel;
+  if (c->use_graphs && !c->timing) {
+    // every launch argument of the loop is fixed by (pool, grids, chunk
+    // length, q0, inst): the work sizes live in device counters
+    const uint64_t key = ((uint64_t)(uintptr_t)c->ps[0].org * 1000003u) ^ ((uint64_t)P << 20) ^
+                         ((uint64_t)trace_blocks << 8) ^ ((uint64_t)shade_blocks << 36) ^ (uint64_t)c->chunk_iters;
+    if (key != c->graph_key) {
+      HIPCHK(hipStreamSynchronize(c->stream));  // no launch of an old graph in flight
+      drop_graphs(c);
+      c->graph_key = key;
+    }
+    hipGraphExec_t& ge = c->bounce_graph[q0][inst ? 1 : 0];
+    if (!ge) {
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      bounces(q0);
+      const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+      if (ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(DSPT_ERR_DEVICE, std::string("bounce-loop capture: ") + hipGetErrorString(ce));
+      }
+      const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) {
+        ge = nullptr;
+        return fail(DSPT_ERR_DEVICE, std::string("bounce-loop graph: ") + hipGetErrorString(ie));
+      }
+    }
+    HIPCHK(hipGraphLaunch(ge, c->stream));
+  } else {
+    bounces(q0);
   }
+  c->qsel = (q0 + c->chunk_iters) & 1;
   if (nj > 0) {
     timed(c, DSPT_K_ACCUM, [&] {
       k_accum_ready<<<dim3(grid_for(max_npix, 256), (unsigned)nj), 256, 0, c->stream>>>(c->d_jobs, c->d_joblist, c->d_prog);
@@ -1079,6 +1131,7 @@ int dspt_create(int profile, int device, dspt_ctx** out) {
   c->device = device;
   c->profile = profile;
   if (const char* mp = std::getenv("DSPT_MAX_PATHS")) c->max_paths = std::max<int64_t>(65536, std::atoll(mp));
+  if (const char* g = std::getenv("DSPT_BOUNCE_GRAPHS")) c->use_graphs = std::atoi(g) != 0;
   bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
   void* p = nullptr;
   auto dev = [&](size_t bytes) -> void* {
@@ -1119,6 +1172,7 @@ int dspt_destroy(dspt_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drain_events(c);
+  drop_graphs(c);
   for (size_t i = 0; i < c->jobs.size(); i++)
     if (c->jobs[i]) { c->jobs[i]->mem.release(); delete c->jobs[i]; }
   for (auto* s : c->scenes) if (s) { s->mem.release(); s->tex_mem.release(); delete s; }
@@ -1663,6 +1717,16 @@ int dspt_reset_counters(dspt_ctx* c) {
   std::memset(c->kstat, 0, sizeof(c->kstat));
   HIPCHK(hipMemsetAsync(c->cnt, 0, kNumCounters * sizeof(unsigned long long), c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int dspt_set_bounce_graphs(dspt_ctx* c, int enable) {
+  if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->use_graphs = enable != 0;
+  drop_graphs(c);
+  c->graph_key = 0;
   return 0;
 }
 

@@ -53,6 +53,7 @@ SIGNATURES = {
     "dspt_build_bvh": (_c_int, [_fp, _c_i64, _u32p, _c_i64, _c_i32, _fp, _i64p, _i32p, _i64p]),
     "dspt_memory_usage": (_c_int, [_vp, _i64p]),
     "dspt_set_kernel_timing": (_c_int, [_vp, _c_int]),
+    "dspt_set_bounce_graphs": (_c_int, [_vp, _c_int]),
     "dspt_get_kernel_stats": (_c_int, [_vp, _dp]),
     "dspt_scene_create": (_c_int, [_vp, _fp, _c_i64, _u32p, _c_i64, _fp, _c_i64, _i32p, _c_i64, _fp, _c_i32, _i32p]),
     "dspt_scene_destroy": (_c_int, [_vp, _c_i32]),
@@ -257,6 +258,10 @@ class Engine:
         occ = np.zeros(n, np.int32)
         check(self._L.dspt_trace_occluded(self.h, ptr(rays), n, ptr(occ, ctypes.c_int32)))
         return occ
+
+    def set_bounce_graphs(self, on):
+        """Replay the per-chunk bounce loop as a captured HIP graph (default) or launch it eagerly."""
+        check(self._L.dspt_set_bounce_graphs(self.h, 1 if on else 0))
 
     def set_kernel_timing(self, on):
         check(self._L.dspt_set_kernel_timing(self.h, 1 if on else 0))

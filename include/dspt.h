@@ -130,6 +130,16 @@ enum {
 int dspt_set_kernel_timing(dspt_ctx* ctx, int enable);
 int dspt_get_kernel_stats(dspt_ctx* ctx, double* out);
 
+/* The bounce loop of a chunk (per bounce: plan + regen, closest-hit trace,
+ * shade, shadow trace) is captured once as a HIP graph per pool layout and
+ * replayed; the work sizes live in device counters, so every replay is the
+ * same launch sequence. On by default (the environment variable
+ * DSPT_BOUNCE_GRAPHS=0 turns it off at dspt_create); kernel timing launches
+ * eagerly. Results are identical either way. No counterpart in the
+ * reference, whose loop is one WebGL draw per tile and frame
+ * (renderer.ts:595-714). */
+int dspt_set_bounce_graphs(dspt_ctx* ctx, int enable);
+
 /* Display chain of renderer.ts:665-700 on the accumulation (the implicit
  * job of the single-image API, or job `job_id`): the tonemap pass
  * (SH/tonemap.frag; tonemap_mode = index into the renderer's

@@ -159,3 +159,40 @@ def test_server_loop_completes_every_request(mix):
     assert all(r["done"] is not None and r["done"] >= r["arrival"] for r in rec)
     st = latency_stats(rec)
     assert st["jobs"] == len(reqs) and st["p99_ms"] >= st["p50_ms"] > 0
+
+
+def test_bounce_graph_replays_equal_eager_launches(mix):
+    """The bounce loop captured as a HIP graph, replayed chunk after chunk
+    (both state-buffer parities with an odd chunk length, one- and two-level
+    kernels, recaptured after a pool reallocation), gives the results of
+    eager launches bit for bit."""
+    from dspt import Engine, PROFILE_FULL, SCHED_STREAM, scenes
+    from dspt.batch import JobSpec
+    inst = scenes.instanced_scene(seed=9, n_meshes=3, n_instances=40, tri_range=(300, 1500), extent=5.0, height=2.0,
+                                  width=40, height_px=32, spp=5, max_bounces=6, ibl_size=(128, 64))
+    results = {}
+    for graphs in (False, True):
+        eng = Engine(PROFILE_FULL)
+        eng.set_bounce_graphs(graphs)
+        b, specs = _batch(eng, mix)
+        b.add_scene("inst", inst.scene)
+        b.add_ibl("inst", inst.ibl)
+        specs.append(JobSpec("inst", "inst", inst.camera, inst.width, inst.height, inst.spp,
+                             max_bounces=inst.max_bounces, record=True))
+        out = []
+        ids = [b.create_job(s) for s in specs[:-1]]  # one-level kernels only
+        eng.jobs_run(ids)
+        out += [_result(eng, j) for j in ids]
+        eng.set_schedule(SCHED_STREAM, chunk_iters=3)  # chunks start on either state buffer
+        ids = [b.create_job(s) for s in specs]  # with the two-level job
+        eng.jobs_run(ids)
+        out += [_result(eng, j) for j in ids]
+        eng.set_launch_config(max_paths=65536)  # a new pool: graphs captured again
+        ids = [b.create_job(s) for s in specs]
+        eng.jobs_run(ids)
+        out += [_result(eng, j) for j in ids]
+        results[graphs] = out
+        eng.close()
+    assert len(results[True]) == len(results[False])
+    for a, g in zip(results[False], results[True]):
+        _same(a, g)
