@@ -207,6 +207,11 @@ struct dspt_ctx {
   bool use_graphs = true;
   hipGraphExec_t bounce_graph[2][2] = {};
   uint64_t graph_key = 0;
+  // ray order of the extension queue (allocated with the pool)
+  int ray_order = DSPT_ORDER_BUCKETS;
+  uint16_t* order_keys = nullptr;
+  int32_t* order_perm = nullptr;
+  uint32_t* order_bins = nullptr;
 };
 
 namespace {
@@ -360,6 +365,8 @@ int ensure_pool(dspt_ctx* c) {
   b.sh_pc = a.sh_pc;
   c->trace_blocks = (int)std::min<int64_t>(c->trace_blocks_max, grid_for(cap, kTraceBlock));
   if ((r = A.get(&c->spill, (size_t)c->trace_blocks * kTraceBlock * (kStackSize - kLdsStack)))) return r;
+  if ((r = A.get(&c->order_keys, cap)) || (r = A.get(&c->order_perm, cap)) || (r = A.get(&c->order_bins, kOrderBins)))
+    return r;
   k_stream_init<<<1, 64, 0, c->stream>>>(c->d_ctl);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -519,26 +526,35 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
   bool inst = false;
   for (const JobRes* j : c->jobs)
     if (j && j->alive && c->scenes[j->scene] && c->scenes[j->scene]->top >= 0) inst = true;
+  const bool order = c->ray_order == DSPT_ORDER_BUCKETS;
+  const int32_t* perm = order ? c->order_perm : nullptr;
+  const int order_blocks = (int)std::min<int64_t>(1024, grid_for(P, 256));
   // chunk_iters bounces; the state buffers alternate, starting at q0
   auto bounces = [&](int q0) {
     for (int k = 0; k < c->chunk_iters; k++) {
       const int a = (q0 + k) & 1;
       const PathState cur = c->ps[a], nxt = c->ps[a ^ 1];
       timed(c, DSPT_K_RAYGEN, [&] {
-        k_plan<<<1, 64, 0, c->stream>>>(ctl, c->d_sched, a, P32);
+        k_plan<<<1, 64, 0, c->stream>>>(ctl, c->d_sched, a, P32, order ? c->order_bins : nullptr);
         k_regen<<<shade_blocks, 256, 0, c->stream>>>(cur, c->d_jobs, c->d_tasks, c->d_sched, ctl, c->cnt);
       });
+      if (order)
+        timed(c, DSPT_K_ORDER, [&] {
+          k_ray_keys<<<order_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, ctl, c->order_keys, c->order_bins);
+          k_bin_scan<<<1, 1024, 0, c->stream>>>(c->order_bins);
+          k_bin_scatter<<<order_blocks, 256, 0, c->stream>>>(c->order_keys, ctl, c->order_bins, c->order_perm);
+        });
       timed(c, DSPT_K_TRACE_EXT, [&] {
         if (inst)
           k_trace_ext<true><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], &ctl->ext_next,
-                                                                        c->spill, c->cnt);
+                                                                        c->spill, c->cnt, perm, &ctl->q_base);
         else
           k_trace_ext<false><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], &ctl->ext_next,
-                                                                         c->spill, c->cnt);
+                                                                         c->spill, c->cnt, perm, &ctl->q_base);
       });
       timed(c, DSPT_K_SHADE, [&] {
         k_shade<<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1], c->shq,
-                                                      &ctl->sh_count, c->cnt);
+                                                      &ctl->sh_count, c->cnt, perm, &ctl->q_base);
       });
       timed(c, DSPT_K_TRACE_SHADOW, [&] {
         if (inst)
@@ -555,7 +571,8 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
     // every launch argument of the loop is fixed by (pool, grids, chunk
     // length, q0, inst): the work sizes live in device counters
     const uint64_t key = ((uint64_t)(uintptr_t)c->ps[0].org * 1000003u) ^ ((uint64_t)P << 20) ^
-                         ((uint64_t)trace_blocks << 8) ^ ((uint64_t)shade_blocks << 36) ^ (uint64_t)c->chunk_iters;
+                         ((uint64_t)trace_blocks << 8) ^ ((uint64_t)shade_blocks << 36) ^ (uint64_t)c->chunk_iters ^
+                         ((uint64_t)c->ray_order << 62);
     if (key != c->graph_key) {
       HIPCHK(hipStreamSynchronize(c->stream));  // no launch of an old graph in flight
       drop_graphs(c);
@@ -1132,6 +1149,7 @@ int dspt_create(int profile, int device, dspt_ctx** out) {
   c->profile = profile;
   if (const char* mp = std::getenv("DSPT_MAX_PATHS")) c->max_paths = std::max<int64_t>(65536, std::atoll(mp));
   if (const char* g = std::getenv("DSPT_BOUNCE_GRAPHS")) c->use_graphs = std::atoi(g) != 0;
+  if (const char* o = std::getenv("DSPT_RAY_ORDER")) c->ray_order = std::atoi(o) == DSPT_ORDER_BUCKETS ? DSPT_ORDER_BUCKETS : DSPT_ORDER_NONE;
   bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
   void* p = nullptr;
   auto dev = [&](size_t bytes) -> void* {
@@ -1717,6 +1735,13 @@ int dspt_reset_counters(dspt_ctx* c) {
   std::memset(c->kstat, 0, sizeof(c->kstat));
   HIPCHK(hipMemsetAsync(c->cnt, 0, kNumCounters * sizeof(unsigned long long), c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int dspt_set_ray_order(dspt_ctx* c, int mode) {
+  if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
+  if (mode != DSPT_ORDER_NONE && mode != DSPT_ORDER_BUCKETS) return fail(DSPT_ERR_ARG, "unknown ray order mode");
+  c->ray_order = mode;  // takes effect with the next chunk (the graph key includes it)
   return 0;
 }
 

@@ -86,17 +86,26 @@ __global__ void k_ibl_guides(const float* cdf, const float* ycdf, int w, int h, 
 __global__ void k_fill_i32(int32_t* p, int n, int32_t v);
 __global__ void k_fill_f4(float4* p, int64_t n, float4 v);
 __global__ void k_stream_init(StreamCtl* ctl);
-__global__ void k_plan(StreamCtl* ctl, const StreamSched* sch, int a, int32_t P);
+__global__ void k_plan(StreamCtl* ctl, const StreamSched* sch, int a, int32_t P, uint32_t* order_bins);
 __global__ void k_regen(PathState ps, const DevJob* jobs, const PassTask* tasks, const StreamSched* sch, const StreamCtl* ctl,
                         unsigned long long* cnt);
+// Ray order of the extension queue (engine extension; the reference traces
+// one pixel per fragment): the continuing paths [0, q_base) are bucketed by
+// (origin cell of a 16x16x16 grid over the scene's root box, up or down)
+// into perm; camera rays appended behind them keep pixel order.
+constexpr int kOrderBins = 8192;
+__global__ void k_ray_keys(const DevJob* jobs, PathState ps, const StreamCtl* ctl, uint16_t* keys, uint32_t* bins);
+__global__ void k_bin_scan(uint32_t* bins);
+__global__ void k_bin_scatter(const uint16_t* keys, const StreamCtl* ctl, uint32_t* bins, int32_t* perm);
 template <bool INST>
 __global__ void k_trace_ext(const DevJob* jobs, PathState ps, const int32_t* count_ptr, int32_t* next_ptr, int* spill,
-                            unsigned long long* cnt);
+                            unsigned long long* cnt, const int32_t* perm, const int32_t* nsorted_ptr);
 template <bool INST>
 __global__ void k_trace_shadow(const DevJob* jobs, PathState ps, const int32_t* queue, const int32_t* count_ptr,
                                int32_t* next_ptr, int* spill, unsigned long long* cnt);
 __global__ void k_shade(const DevJob* jobs, PathState cur, PathState nxt, const int32_t* count_in, int32_t* count_out,
-                        int32_t* shq, int32_t* count_sh, unsigned long long* cnt);
+                        int32_t* shq, int32_t* count_sh, unsigned long long* cnt, const int32_t* perm,
+                        const int32_t* nsorted_ptr);
 __global__ void k_accum_ready(const DevJob* jobs, const int32_t* job_list, const int32_t* prog);
 __global__ void k_advance(const DevJob* jobs, const int32_t* job_list, int n, int32_t* prog);
 __global__ void k_trace_rays(SceneView S, const float4* rays, int64_t n, int any, int32_t* tri, float* t, float* u, float* v,

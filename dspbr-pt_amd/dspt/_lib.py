@@ -54,6 +54,7 @@ SIGNATURES = {
     "dspt_memory_usage": (_c_int, [_vp, _i64p]),
     "dspt_set_kernel_timing": (_c_int, [_vp, _c_int]),
     "dspt_set_bounce_graphs": (_c_int, [_vp, _c_int]),
+    "dspt_set_ray_order": (_c_int, [_vp, _c_int]),
     "dspt_get_kernel_stats": (_c_int, [_vp, _dp]),
     "dspt_scene_create": (_c_int, [_vp, _fp, _c_i64, _u32p, _c_i64, _fp, _c_i64, _i32p, _c_i64, _fp, _c_i32, _i32p]),
     "dspt_scene_destroy": (_c_int, [_vp, _c_i32]),
@@ -90,7 +91,8 @@ POLICY_RR = 0
 POLICY_FIFO = 1
 POLICY_SRPT = 2
 BVH_LBVH, BVH_PLOC = 0, 1
-KERNEL_CLASSES = ["raygen", "trace_ext", "shade", "trace_shadow", "accumulate", "post"]
+ORDER_NONE, ORDER_BUCKETS = 0, 1
+KERNEL_CLASSES = ["raygen", "trace_ext", "shade", "trace_shadow", "accumulate", "post", "order"]
 
 
 class DsptError(RuntimeError):
@@ -262,6 +264,10 @@ class Engine:
     def set_bounce_graphs(self, on):
         """Replay the per-chunk bounce loop as a captured HIP graph (default) or launch it eagerly."""
         check(self._L.dspt_set_bounce_graphs(self.h, 1 if on else 0))
+
+    def set_ray_order(self, mode):
+        """ORDER_NONE (compaction order) or ORDER_BUCKETS (octant + origin-cell buckets)."""
+        check(self._L.dspt_set_ray_order(self.h, int(mode)))
 
     def set_kernel_timing(self, on):
         check(self._L.dspt_set_kernel_timing(self.h, 1 if on else 0))

@@ -4,7 +4,7 @@ BatchRenderer  many scenes / cameras / resolutions / frame counts rendered
                together through one path pool (`dspt_jobs_run` with the
                STREAM schedule: finished paths are refilled with samples of
                any job on the next bounce), with the pool size, traversal
-               grid and chunk length chosen by a small autotuner.
+               grid, chunk length and ray order chosen by a small autotuner.
                `run_plain` is the one-job-at-a-time loop with
                frame-synchronous passes (PASS schedule), optionally one frame
                per pass as in the reference's render loop.
@@ -113,13 +113,15 @@ class BatchRenderer:
         `chunks` chunks after `warm_chunks`. Keeps the fastest; returns
         (best, table)."""
         if candidates is None:
-            candidates = [dict(max_paths=mp, trace_blocks=tb, chunk_iters=ci)
-                          for mp in (4 << 20, 8 << 20, 16 << 20) for tb in (1024, 2048) for ci in (2, 4)]
+            candidates = [dict(max_paths=mp, trace_blocks=tb, chunk_iters=ci, ray_order=ro)
+                          for mp in (4 << 20, 8 << 20, 16 << 20) for tb in (1024, 2048) for ci in (2, 4)
+                          for ro in (_lib.ORDER_BUCKETS, _lib.ORDER_NONE)]
         table = []
         self.eng.set_schedule(_lib.SCHED_STREAM)
         for cand in candidates:
             self.eng.set_launch_config(cand["max_paths"], cand["trace_blocks"], 0)
             self.eng.set_schedule(_lib.SCHED_STREAM, 0, cand["chunk_iters"])
+            self.eng.set_ray_order(cand.get("ray_order", _lib.ORDER_BUCKETS))
             ids = make_jobs()
             for _ in range(warm_chunks):
                 self.eng.jobs_step(ids)
@@ -138,6 +140,7 @@ class BatchRenderer:
         best = max(table, key=lambda x: x[0])[1]
         self.eng.set_launch_config(best["max_paths"], best["trace_blocks"], 0)
         self.eng.set_schedule(_lib.SCHED_STREAM, 0, best["chunk_iters"])
+        self.eng.set_ray_order(best.get("ray_order", _lib.ORDER_BUCKETS))
         return best, table
 
 

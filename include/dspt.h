@@ -122,6 +122,7 @@ enum {
   DSPT_K_TRACE_SHADOW = 3, /* any-hit traversal of NEE shadow rays */
   DSPT_K_ACCUM = 4,        /* running mean of complete frames (accum_ready + advance) */
   DSPT_K_POST = 5,         /* display chain (dspt_postprocess / dspt_job_postprocess) */
+  DSPT_K_ORDER = 6,        /* ray order of the extension queue (keys, bucket scan, scatter) */
   DSPT_K_COUNT = 8
 };
 /* Stats per class, double[DSPT_K_COUNT * 4]: {launches, total_ms,
@@ -139,6 +140,18 @@ int dspt_get_kernel_stats(dspt_ctx* ctx, double* out);
  * reference, whose loop is one WebGL draw per tile and frame
  * (renderer.ts:595-714). */
 int dspt_set_bounce_graphs(dspt_ctx* ctx, int enable);
+
+/* Ray order of the extension queue before each closest-hit trace:
+ * DSPT_ORDER_BUCKETS (default) sorts the continuing paths into 8192 buckets
+ * by (Morton code of the origin's cell in a 16x16x16 grid over the scene's
+ * root box, sign of dir.y); camera rays keep pixel order, and shading
+ * follows the same order. DSPT_ORDER_NONE keeps the compaction order. Results
+ * are identical either way (every path is computed from its own state); only
+ * speed changes. DSPT_RAY_ORDER=0 in the environment selects NONE at
+ * dspt_create; the batched renderer's autotuner tries both. No counterpart in
+ * the reference (one path per fragment). */
+enum { DSPT_ORDER_NONE = 0, DSPT_ORDER_BUCKETS = 1 };
+int dspt_set_ray_order(dspt_ctx* ctx, int mode);
 
 /* Display chain of renderer.ts:665-700 on the accumulation (the implicit
  * job of the single-image API, or job `job_id`): the tonemap pass

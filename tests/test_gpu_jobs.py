@@ -196,3 +196,39 @@ def test_bounce_graph_replays_equal_eager_launches(mix):
     assert len(results[True]) == len(results[False])
     for a, g in zip(results[False], results[True]):
         _same(a, g)
+
+
+def test_ray_order_keeps_results(mix):
+    """Bucketing the extension queue by (octant, origin cell) changes which
+    lane walks which ray, not any path's result: accumulations, samples and
+    the work counters equal those of the compaction order, with and without
+    the captured graphs, one- and two-level."""
+    from dspt import Engine, ORDER_BUCKETS, ORDER_NONE, PROFILE_FULL, SCHED_STREAM, scenes
+    from dspt.batch import JobSpec
+    inst = scenes.instanced_scene(seed=9, n_meshes=3, n_instances=40, tri_range=(300, 1500), extent=5.0, height=2.0,
+                                  width=40, height_px=32, spp=5, max_bounces=6, ibl_size=(128, 64))
+    runs = {}
+    for order, graphs in ((ORDER_NONE, True), (ORDER_BUCKETS, True), (ORDER_BUCKETS, False)):
+        eng = Engine(PROFILE_FULL)
+        eng.set_ray_order(order)
+        eng.set_bounce_graphs(graphs)
+        eng.set_launch_config(max_paths=65536)  # several refills per frame: sorted and unsorted entries mix
+        eng.set_schedule(SCHED_STREAM, chunk_iters=3)
+        b, specs = _batch(eng, mix)
+        b.add_scene("inst", inst.scene)
+        b.add_ibl("inst", inst.ibl)
+        specs.append(JobSpec("inst", "inst", inst.camera, inst.width, inst.height, inst.spp,
+                             max_bounces=inst.max_bounces, record=True))
+        eng.reset_counters()
+        ids = [b.create_job(s) for s in specs]
+        eng.jobs_run(ids)
+        c = eng.counters()
+        runs[(order, graphs)] = ([_result(eng, j) for j in ids], {k: c[k] for k in ("paths", "rays_extension",
+                                                                                   "rays_shadow", "node_visits",
+                                                                                   "tri_tests", "shade_hits")})
+        eng.close()
+    ref, ref_cnt = runs[(ORDER_NONE, True)]
+    for key, (res, cnt) in runs.items():
+        assert cnt == ref_cnt, key
+        for a, g in zip(ref, res):
+            _same(a, g)
