@@ -54,7 +54,6 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(DSPT_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
   } while (0)
 
-constexpr int kNumCounters = 32;
 constexpr int kMaxJobs = 4096;
 constexpr int kMaxTasks = 8192;
 constexpr int64_t kStallChunks = 4096;
@@ -217,6 +216,16 @@ struct dspt_ctx {
 namespace {
 
 int grid_for(int64_t n, int block) { return (int)std::max<int64_t>(1, (n + block - 1) / block); }
+
+// Sums the kCntSlots rows of the work counters into h[kNumCounters].
+int read_counters(dspt_ctx* c, unsigned long long* h) {
+  std::vector<unsigned long long> rows((size_t)kCntSlots * kNumCounters);
+  HIPCHK(hipMemcpy(rows.data(), c->cnt, rows.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  for (int i = 0; i < kNumCounters; i++) h[i] = 0;
+  for (int r = 0; r < kCntSlots; r++)
+    for (int i = 0; i < kNumCounters; i++) h[i] += rows[(size_t)r * kNumCounters + i];
+  return 0;
+}
 
 void drop_graphs(dspt_ctx* c) {
   for (auto& row : c->bounce_graph)
@@ -1162,8 +1171,8 @@ int dspt_create(int profile, int device, dspt_ctx** out) {
     if (ok && hipHostMalloc(&q, bytes, hipHostMallocDefault) != hipSuccess) { ok = false; q = nullptr; }
     return q;
   };
-  c->cnt = (unsigned long long*)dev(kNumCounters * sizeof(unsigned long long));
-  if (ok) ok = hipMemset(c->cnt, 0, kNumCounters * sizeof(unsigned long long)) == hipSuccess;
+  c->cnt = (unsigned long long*)dev((size_t)kCntSlots * kNumCounters * sizeof(unsigned long long));
+  if (ok) ok = hipMemset(c->cnt, 0, (size_t)kCntSlots * kNumCounters * sizeof(unsigned long long)) == hipSuccess;
   c->d_jobs = (DevJob*)dev(kMaxJobs * sizeof(DevJob));
   c->d_tasks = (PassTask*)dev(kMaxTasks * sizeof(PassTask));
   c->d_ctl = (StreamCtl*)dev(sizeof(StreamCtl));
@@ -1721,7 +1730,7 @@ int dspt_get_counters(dspt_ctx* c, int64_t* out) {
   if (!c || !out) return fail(DSPT_ERR_ARG, "NULL argument");
   unsigned long long h[kNumCounters];
   HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipMemcpy(h, c->cnt, sizeof(h), hipMemcpyDeviceToHost));
+  if (int r = read_counters(c, h)) return r;
   for (int i = 0; i < DSPT_CNT_COUNT; i++) out[i] = (int64_t)h[i];
   out[DSPT_CNT_NODE_VISITS] = (int64_t)(h[4] + h[10]);
   out[DSPT_CNT_TRI_TESTS] = (int64_t)(h[5] + h[11]);
@@ -1733,7 +1742,7 @@ int dspt_reset_counters(dspt_ctx* c) {
   if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
   drain_events(c);
   std::memset(c->kstat, 0, sizeof(c->kstat));
-  HIPCHK(hipMemsetAsync(c->cnt, 0, kNumCounters * sizeof(unsigned long long), c->stream));
+  HIPCHK(hipMemsetAsync(c->cnt, 0, (size_t)kCntSlots * kNumCounters * sizeof(unsigned long long), c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1766,7 +1775,7 @@ int dspt_get_kernel_stats(dspt_ctx* c, double* out) {
   if (!c || !out) return fail(DSPT_ERR_ARG, "NULL argument");
   drain_events(c);
   unsigned long long h[kNumCounters];
-  HIPCHK(hipMemcpy(h, c->cnt, sizeof(h), hipMemcpyDeviceToHost));
+  if (int r = read_counters(c, h)) return r;
   for (int k = 0; k < DSPT_K_COUNT; k++) {
     out[k * 4 + 0] = c->kstat[k][0];
     out[k * 4 + 1] = c->kstat[k][1];

@@ -217,9 +217,10 @@ static_assert(kOrderBins == 8192, "the key has 13 bits");
 __global__ void k_ray_keys(const DevJob* __restrict__ jobs, PathState ps, const StreamCtl* __restrict__ ctl,
                            uint16_t* __restrict__ keys, uint32_t* __restrict__ bins) {
   __shared__ uint32_t h[kOrderBins];
+  const int n = ctl->q_base;
+  if ((int64_t)blockIdx.x * blockDim.x >= n) return;  // small queues leave most blocks idle
   for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x) h[i] = 0u;
   __syncthreads();
-  const int n = ctl->q_base;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
     const int p = (int)q;
     const uint32_t k = ray_key(jobs[ps.job[p]].S, ps.org[p], ps.dir[p]);
@@ -256,11 +257,12 @@ __global__ void k_bin_scan(uint32_t* __restrict__ bins) {
 __global__ void k_bin_scatter(const uint16_t* __restrict__ keys, const StreamCtl* __restrict__ ctl,
                               uint32_t* __restrict__ bins, int32_t* __restrict__ perm) {
   __shared__ uint32_t h[kOrderBins];
-  for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x) h[i] = 0u;
-  __syncthreads();
   const int n = ctl->q_base;
   const int per = (n + gridDim.x - 1) / gridDim.x;
   const int lo = (int)min((int64_t)blockIdx.x * per, (int64_t)n), hi = min(lo + per, n);
+  if (lo >= hi) return;  // block-uniform: no work for this block
+  for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x) h[i] = 0u;
+  __syncthreads();
   for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) atomicAdd(&h[keys[q]], 1u);
   __syncthreads();
   for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x)
@@ -318,7 +320,7 @@ __global__ void k_regen(PathState ps, const DevJob* __restrict__ jobs, const Pas
     if (J->rec_L) ps.sig[p] = 2166136261u;
     issued++;
   }
-  wave_add_u64(&cnt[12], issued);
+  stat_add(cnt, 12, issued);
 }
 
 // Lanes of a traversal wave take new rays from the queue whenever at least
@@ -436,9 +438,9 @@ __global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathSt
       if (ray_done(w.r)) finish_ray();
     }
   }
-  wave_add_u64(&cnt[4], nv);
-  wave_add_u64(&cnt[5], nt);
-  wave_add_u64(&cnt[9], nr);
+  stat_add(cnt, 4, nv);
+  stat_add(cnt, 5, nt);
+  stat_add(cnt, 9, nr);
 }
 
 // Shadow rays of a path: the IBL ray (flag 1), then the point-light ray
@@ -507,9 +509,9 @@ __global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, Pat
       }
     }
   }
-  wave_add_u64(&cnt[10], nv);
-  wave_add_u64(&cnt[11], nt);
-  wave_add_u64(&cnt[3], nr);
+  stat_add(cnt, 10, nv);
+  stat_add(cnt, 11, nt);
+  stat_add(cnt, 3, nr);
 }
 
 template __global__ void k_trace_ext<false>(const DevJob*, PathState, const int32_t*, int32_t*, int*, unsigned long long*,
@@ -738,11 +740,11 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
       atomic_inc_grouped(J->done + rs);
     }
   }
-  wave_add_u64(&cnt[0], n_paths);
-  wave_add_u64(&cnt[6], n_hits);
-  wave_add_u64(&cnt[7], n_miss);
-  wave_add_u64(&cnt[8], n_nee);
-  wave_add_u64(&cnt[2], n_ext);
+  stat_add(cnt, 0, n_paths);
+  stat_add(cnt, 6, n_hits);
+  stat_add(cnt, 7, n_miss);
+  stat_add(cnt, 8, n_nee);
+  stat_add(cnt, 2, n_ext);
 }
 
 // Running mean (renderer.frag:153-156) over every frame that is complete,
@@ -813,8 +815,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(SceneView S, const f
     if (inst) inst[i] = k;
     if (t) { t[i] = h.t; u[i] = h.u; v[i] = h.v; }
   }
-  wave_add_u64(&cnt[4], nv);
-  wave_add_u64(&cnt[5], nt);
+  stat_add(cnt, 4, nv);
+  stat_add(cnt, 5, nt);
 }
 
 // --------------------------------------------------------------------------

@@ -77,6 +77,16 @@ struct PathState {
   float4* sh_pc;    // point-light radiance before path weight
 };
 
+// Work counters: kCntSlots copies of kNumCounters u64 (a 256-byte row each);
+// a block adds into row blockIdx.x % kCntSlots, so the per-wave atomics of a
+// launch spread over 256 addresses instead of queueing on one; readers sum
+// the rows.
+constexpr int kNumCounters = 32;
+constexpr int kCntSlots = 256;
+__device__ inline void stat_add(unsigned long long* cnt, int i, unsigned long long v) {
+  wave_add_u64(cnt + (size_t)(blockIdx.x & (kCntSlots - 1)) * kNumCounters + i, v);
+}
+
 __global__ void k_prep_tris(const int32_t* bvh_idx, const uint32_t* tris, const float4* verts, int32_t M, int32_t T,
                             float4* out);
 __global__ void k_ibl_luminance(const float4* env, int64_t n, float* f);
