@@ -41,7 +41,7 @@ HBM_PEAK_GBS = 8000.0
 # Memory traffic of one k_trace_ext launch on the default workload (1080p,
 # 64 spp, full-material scene), from rocprofv3 FETCH_SIZE + WRITE_SIZE passes
 # scaled by a plain-copy calibration (profiles/k_trace_ext_counters_*.md).
-TRACE_EXT_TRAFFIC_BYTES = 7.297e9
+TRACE_EXT_TRAFFIC_BYTES = 7.281e9
 METRIC = "Mpaths/s + Mrays/s at 1080p; p50/p99 job latency, batched vs plain, 1 GPU"
 
 
