@@ -58,6 +58,10 @@ struct SceneView {
   // record (instance k at inst_base + 2k), so the top level has no leaf
   // nodes. -1: one level.
   int32_t top, inst_base;
+  // 1 when every inner node's child boxes are finite (checked at upload):
+  // then a ray with finite origin and finite, non-zero 1/dir can take the
+  // slab test on hardware min/max (slab_fast)
+  int32_t fast_slab, pad_fs;
 };
 
 // The 52-float uniform block (renderer.frag:10-31).
@@ -146,6 +150,28 @@ DI float slab(f3 bmin, f3 bmax, f3 org, f3 inv, float tfar) {
   if (isnan(tmin) || isnan(tmax) || tmin > tmax || tmax < 0.0f || tmin >= tfar) return kBVH_FAR;
   return tmin;
 #endif
+}
+
+// The slab test on hardware min/max (v_min_f32 / v_max_f32): one
+// instruction per min/max instead of a compare, a select and a hazard wait.
+// Equal to slab() whenever no t value can be NaN: with finite box corners,
+// a finite origin and finite, non-zero 1/dir, every (b - o) * inv is finite
+// or infinite, never 0 * inf. GLSL's select min/max and the hardware's then
+// pick the same value up to the sign of a zero, which no comparison below
+// (nor the near/far order of the caller) distinguishes.
+DI float slab_fast(f3 bmin, f3 bmax, f3 org, f3 inv, float tfar) {
+  const f3 t0 = (bmin - org) * inv;
+  const f3 t1 = (bmax - org) * inv;
+  const float tmin = fmaxf(fmaxf(fminf(t0.x, t1.x), fminf(t0.y, t1.y)), fminf(t0.z, t1.z));
+  const float tmax = fminf(fminf(fmaxf(t0.x, t1.x), fmaxf(t0.y, t1.y)), fmaxf(t0.z, t1.z));
+  const bool ok = (tmin <= tmax) & (tmax >= 0.0f) & (tmin < tfar);
+  return ok ? tmin : kBVH_FAR;
+}
+DI bool finite_nz(float x) { return isfinite(x) && x != 0.0f; }
+// Whether a ray may use slab_fast in a scene (see SceneView::fast_slab).
+DI bool slab_fast_ok(int32_t scene_ok, f3 org, f3 inv) {
+  return scene_ok != 0 && isfinite(org.x) && isfinite(org.y) && isfinite(org.z) && finite_nz(inv.x) && finite_nz(inv.y) &&
+         finite_nz(inv.z);
 }
 
 // Moeller-Trumbore, SH/bvh.glsl:44-62.
@@ -330,6 +356,7 @@ DI bool trav_step(const Geo& G, RayState& r, LaneStack& st, unsigned& nv, unsign
 // to trav_leaf, which tests its triangles before the ray pops again, so the
 // per-ray sequence of node visits, triangle tests and steps is exactly that
 // of trav_step. Returns 0 = continue, 1 = leaf pending, 2 = ray finished.
+template <bool FAST = false>
 DI int trav_inner(const Geo& G, RayState& r, LaneStack& st, unsigned& nv) {
   if (ray_done(r)) return 2;
   r.step++;
@@ -358,8 +385,8 @@ DI int trav_inner(const Geo& G, RayState& r, LaneStack& st, unsigned& nv) {
     return 1;
   }
   const int left = child_code(a.w), right = child_code(b.w);
-  const float d0 = slab(xyz(a), xyz(b), r.org, r.inv, r.h.t);
-  const float d1 = slab(xyz(c), xyz(d), r.org, r.inv, r.h.t);
+  const float d0 = FAST ? slab_fast(xyz(a), xyz(b), r.org, r.inv, r.h.t) : slab(xyz(a), xyz(b), r.org, r.inv, r.h.t);
+  const float d1 = FAST ? slab_fast(xyz(c), xyz(d), r.org, r.inv, r.h.t) : slab(xyz(c), xyz(d), r.org, r.inv, r.h.t);
   const bool h0 = d0 < kBVH_FAR, h1 = d1 < kBVH_FAR;
   const bool both = h0 && h1;
   const bool near_left = d0 < d1;

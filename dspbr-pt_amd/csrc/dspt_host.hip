@@ -95,6 +95,8 @@ struct SceneRes {
   int32_t num_tex = 0, num_texarr = 0;
   // two-level scenes: root of the top level and first instance record in `nodes`
   int32_t top = -1, inst_base = -1, I = 0;
+  int32_t fast_slab = 0;  // every inner node's child boxes are finite
+  int32_t* d_flag = nullptr;
 };
 
 struct IblRes {
@@ -305,6 +307,7 @@ SceneView view_of(dspt_ctx* c, const JobRes* j) {
   S.texinfo = s->texinfo; S.texarr = s->texarr; S.texels = s->texels;
   S.num_tex = s->num_tex; S.num_texarr = s->num_texarr;
   S.top = s->top; S.inst_base = s->inst_base;
+  S.fast_slab = s->fast_slab;
   const IblRes* ib = (j->ibl >= 0 && j->ibl < (int)c->ibls.size()) ? c->ibls[j->ibl] : nullptr;
   if (ib) {
     S.env = ib->env; S.pdf = ib->pdf; S.cdf = ib->cdf; S.ypdf = ib->ypdf; S.ycdf = ib->ycdf;
@@ -776,9 +779,14 @@ int make_scene_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint
   if ((r = chk(hipMemcpyAsync(s->nodes, big_nodes, (size_t)N * 64, hipMemcpyDeviceToDevice, c->stream), "node copy")))
     return r;
   k_prep_tris<<<grid_for(T, 256), 256, 0, c->stream>>>(d_bvh, s->tris, s->verts, (int32_t)T, (int32_t)T, s->tri);
-  if (mats)  // a resident scene: child codes (the host-readback build keeps the bvh_gpu fields)
+  if (mats) {  // a resident scene: child codes (the host-readback build keeps the bvh_gpu fields)
+    if ((r = A.get(&s->d_flag, 1))) return bail(r);
     k_encode_children<<<grid_for(N, 256), 256, 0, c->stream>>>(s->nodes, (int32_t)N, (int32_t)N);
+    k_fill_i32<<<1, 64, 0, c->stream>>>(s->d_flag, 1, 1);
+    k_boxes_finite<<<grid_for(N, 256), 256, 0, c->stream>>>(s->nodes, (int32_t)N, s->d_flag);
+  }
   if ((r = chk(hipGetLastError(), "scene kernels")) || (r = chk(hipStreamSynchronize(c->stream), "scene build"))) return r;
+  if (mats && (r = chk(hipMemcpy(&s->fast_slab, s->d_flag, 4, hipMemcpyDeviceToHost), "scene flag"))) return r;
   float ms = 0.0f;
   (void)hipEventElapsedTime(&ms, e0, e1);
   c->ev_pool.push_back(e0);
@@ -834,7 +842,11 @@ int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tr
   k_prep_tris<<<grid_for(M, 256), 256, 0, c->stream>>>(d_bvh, s->tris, s->verts, (int32_t)M, (int32_t)T, s->tri);
   const int64_t lim = encode_limit < 0 ? N : std::min(encode_limit, N);
   k_encode_children<<<grid_for(lim, 256), 256, 0, c->stream>>>(s->nodes, (int32_t)N, (int32_t)lim);
+  if ((r = A.get(&s->d_flag, 1))) return bail(r);
+  k_fill_i32<<<1, 64, 0, c->stream>>>(s->d_flag, 1, 1);
+  k_boxes_finite<<<grid_for(lim, 256), 256, 0, c->stream>>>(s->nodes, (int32_t)lim, s->d_flag);
   if ((r = chk(hipGetLastError(), "scene kernels")) || (r = chk(hipStreamSynchronize(c->stream), "scene upload"))) return r;
+  if ((r = chk(hipMemcpy(&s->fast_slab, s->d_flag, 4, hipMemcpyDeviceToHost), "scene flag"))) return r;
   (void)hipFree(d_bvh);
   s->N = (int32_t)N; s->M = (int32_t)M; s->T = (int32_t)T; s->K = K; s->V = V;
   s->max_tex_id = max_tex;

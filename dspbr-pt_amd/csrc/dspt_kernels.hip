@@ -88,6 +88,21 @@ __global__ void k_encode_children(float4* __restrict__ nodes, int32_t N, int32_t
   }
 }
 
+// Clears *ok when an inner node below `limit` has a non-finite child box
+// corner (the walk's slab tests read a.xyz, b.xyz, c.xyz, d.xyz of inner
+// nodes only); *ok starts at 1.
+__global__ void k_boxes_finite(const float4* __restrict__ nodes, int32_t limit, int32_t* __restrict__ ok) {
+  const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= limit) return;
+  const float4* nd = nodes + (size_t)n * 4;
+  const float4 c = nd[2];
+  if ((int)c.w > 0) return;  // a leaf
+  const float4 a = nd[0], b = nd[1], d = nd[3];
+  const bool fin = isfinite(a.x) && isfinite(a.y) && isfinite(a.z) && isfinite(b.x) && isfinite(b.y) && isfinite(b.z) &&
+                   isfinite(c.x) && isfinite(c.y) && isfinite(c.z) && isfinite(d.x) && isfinite(d.y) && isfinite(d.z);
+  if (!fin) atomicAnd(ok, 0);
+}
+
 // --------------------------------------------------------------------------
 // IBL importance-sampling tables, renderer.ts:777-838, bit-identical: the
 // luminance and the per-row sums are double (JS numbers), pdf is rounded to
@@ -360,13 +375,19 @@ template <bool INST> struct Walker;
 template <> struct Walker<false> {
   Geo G;
   RayState r;
+  bool fast;  // this ray may take slab_fast (a lane mask, no vector register)
   DD void start(const SceneView& S, LaneStack& st, f3 o, f3 d, float tfar) {
     G = geo_of(S);
     ray_init(r, st, o, d, tfar);
+    fast = slab_fast_ok(S.fast_slab, r.org, r.inv);
   }
-  DD int inner(LaneStack& st, unsigned& nv) { return trav_inner(G, r, st, nv); }
+  // FAST: the caller checked that every busy lane of the wave has `fast`
+  template <bool FAST = false> DD int inner(LaneStack& st, unsigned& nv) { return trav_inner<FAST>(G, r, st, nv); }
   template <bool ANY> DD bool leaf(unsigned& nt) { return trav_leaf<ANY>(G, r, nt); }
-  DD void restart(LaneStack& st, f3 o, f3 d, float tfar) { ray_init(r, st, o, d, tfar); }
+  DD void restart(LaneStack& st, f3 o, f3 d, float tfar) {
+    ray_init(r, st, o, d, tfar);
+    fast = false;  // the scene's flag is not kept here: the exact test
+  }
   DD int inst() const { return -1; }
 };
 template <> struct Walker<true> {
@@ -377,7 +398,10 @@ template <> struct Walker<true> {
     G = geo2_of(S);
     restart(st, o, d, tfar);
   }
-  DD int inner(LaneStack& st, unsigned& nv) { return G.top >= 0 ? trav_inner2(G, r, q, st, nv) : trav_inner(G.g, r, st, nv); }
+  static constexpr bool fast = false;  // object-space rays may have an infinite 1/dir: the exact slab test
+  template <bool FAST = false> DD int inner(LaneStack& st, unsigned& nv) {
+    return G.top >= 0 ? trav_inner2(G, r, q, st, nv) : trav_inner(G.g, r, st, nv);
+  }
   template <bool ANY> DD bool leaf(unsigned& nt) { return G.top >= 0 ? trav_leaf2<ANY>(G, r, q, nt) : trav_leaf<ANY>(G.g, r, nt); }
   DD void restart(LaneStack& st, f3 o, f3 d, float tfar) {
     ray_init(r, st, o, d, tfar);
@@ -422,13 +446,15 @@ __global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathSt
       continue;
     }
     // inner nodes until every busy lane has reached its next leaf (or at
-    // most kInnerRun steps), then the leaves together
+    // most kInnerRun steps), then the leaves together; the slab test runs on
+    // hardware min/max when every busy lane allows it (a uniform branch)
+    const bool wfast = !INST && __ballot(active && !w.fast) == 0;
     bool leaf = false;
     for (int k = 0; k < kInnerRun; k++) {
       const bool inner = active && !leaf;
       if (__ballot(inner) == 0) break;
       if (inner) {
-        const int res = w.inner(st, nv);
+        const int res = wfast ? w.template inner<true>(st, nv) : w.template inner<false>(st, nv);
         if (res == 1) leaf = true;
         else if (res == 2) finish_ray();
       }
@@ -478,6 +504,7 @@ __global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, Pat
       if (exhausted) break;
       continue;
     }
+    // the exact slab test only: the fast one measured no gain here and cost a spill
     bool leaf = false, fin = false;
     for (int k = 0; k < kInnerRun; k++) {
       const bool inner = active && !leaf && !fin;

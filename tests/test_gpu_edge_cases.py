@@ -266,3 +266,41 @@ def test_sphere_field_256_whole_frame():
                                  wl.camera, wl.width, wl.height, 2, 0, wl.max_bounces)
     rel = np.abs(g_acc - o["accum"]) / np.maximum(np.abs(o["accum"]), 1e-3)
     assert np.median(rel) < 1e-6
+
+
+def test_fast_slab_test_equals_exact_form(sphere_field_small):
+    """Scenes whose inner-node boxes are all finite run the closest-hit
+    kernel's slab test on hardware min/max; one non-finite box anywhere in
+    the array selects the exact GLSL select form for the whole scene. An
+    unreachable inner node with a NaN corner switches the same scene to the
+    exact form: samples, accumulation and counters must not change."""
+    from dspt import Engine, pack_frame_block
+    wl = sphere_field_small
+    s = wl.scene
+    nodes = np.asarray(s.bvhNodes, np.float32).reshape(-1, 16)
+    bad = np.zeros((1, 16), np.float32)
+    bad[0, 0] = np.nan
+    bad[0, 3], bad[0, 7] = -1, -1  # an inner node without children, referenced by none
+    img = wl.ibl["image"]
+    out = []
+    for nd in (nodes, np.concatenate([nodes, bad])):
+        eng = Engine(0)
+        eng.upload_scene(s.vertexBuffer, s.triangleIndexBuffer, nd.reshape(-1), s.bvhTriangleIndices,
+                         s.getFlatMaterialBuffer())
+        tot = eng.upload_ibl(img["data"], img["width"], img["height"])
+        blk = pack_frame_block(wl.camera, 96, 64, max_bounces=wl.max_bounces, ibl_size=(img["width"], img["height"]),
+                               ibl_total_sum=tot)
+        eng.resize(96, 64)
+        eng.set_frame(blk)
+        eng.set_record_samples(True)
+        eng.reset_counters()
+        eng.render(1, 3)
+        c = eng.counters()
+        out.append((eng.read_accum(), *eng.read_samples(3), c["node_visits"], c["tri_tests"], c["rays_extension"]))
+        eng.close()
+    for a, b in zip(out[0], out[1]):
+        if isinstance(a, np.ndarray):
+            np.testing.assert_array_equal(a.view(np.uint32) if a.dtype == np.float32 else a,
+                                          b.view(np.uint32) if b.dtype == np.float32 else b)
+        else:
+            assert a == b
