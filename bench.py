@@ -131,8 +131,8 @@ def batched_vs_plain(args):
     sub = specs[: min(8, len(specs))]
     best, table = b.autotune(lambda: [b.create_job(s) for s in sub],
                              candidates=[dict(max_paths=mp, trace_blocks=tb, chunk_iters=2, ray_order=ORDER_BUCKETS)
-                                         for mp in (4 << 20, 8 << 20, 16 << 20) for tb in (1024, 2048)]
-                             + [dict(max_paths=8 << 20, trace_blocks=2048, chunk_iters=2, ray_order=ORDER_NONE)],
+                                         for mp in (8 << 20, 16 << 20, 32 << 20) for tb in (1024, 2048)]
+                             + [dict(max_paths=16 << 20, trace_blocks=2048, chunk_iters=2, ray_order=ORDER_NONE)],
                              chunks=16, warm_chunks=4)
     log(f"[bench] autotune: best {best}; " + ", ".join(f"{r:.0f}" for r, _ in table))
 
@@ -404,6 +404,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="full_materials")
     ap.add_argument("--spp", type=int, default=None, help="override frames per job")
+    ap.add_argument("--max-paths", type=int, default=32 << 20,
+                    help="path pool of the single-job leg (32 Mi paths, about 10 GB of the 288 GB HBM)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain comparison")
@@ -438,7 +440,8 @@ def main():
     eng.resize(wl.width, wl.height)
     block = frame_block(wl, total)
     eng.set_frame(block)
-    log(f"[bench] uploaded; device memory {eng.memory_usage() / 1e9:.2f} GB")
+    eng.set_launch_config(max_paths=args.max_paths)
+    log(f"[bench] uploaded; device memory {eng.memory_usage() / 1e9:.2f} GB; path pool {args.max_paths}")
 
     for i in range(args.warmup):
         t0 = time.time()

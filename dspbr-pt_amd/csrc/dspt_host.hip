@@ -167,7 +167,7 @@ struct dspt_ctx {
   int trace_blocks = 0;
   int trace_blocks_max = 2048;
   int shade_blocks_max = 8192;
-  int64_t max_paths = 8 << 20;
+  int64_t max_paths = 16 << 20;  // about 5 GB of path state
   unsigned long long* cnt = nullptr;
   // scheduler
   StreamCtl* d_ctl = nullptr;
@@ -1505,11 +1505,15 @@ int dspt_render(dspt_ctx* c, int32_t first_frame, int32_t num_frames) {
   HIPCHK(hipSetDevice(c->device));
   int ibl = (!c->ibls.empty() && c->ibls[0]) ? 0 : -1;
   if (int r = check_job_frame(c, c->F, 0, ibl)) return r;
-  // the implicit job is rebuilt when the recording request changes
+  // the implicit job is rebuilt when the recording request changes, or when
+  // a call that restarts the accumulation (first_frame 1) could keep more
+  // frames in flight than its ring holds (the ring was sized by an earlier,
+  // shorter call); a continuing call keeps the job and its running mean
   if (c->legacy_job >= 0 && job_ok(c, c->legacy_job)) {
     JobRes* j = c->jobs[c->legacy_job];
     bool rec_fit = !c->record || (j->record && j->rec_count >= num_frames);
-    if (!rec_fit || j->record != c->record || j->ibl != ibl) {
+    bool ring_fit = first_frame > 1 || j->ring_frames >= ring_frames_for(c, j->npix(), num_frames);
+    if (!rec_fit || !ring_fit || j->record != c->record || j->ibl != ibl) {
       if (int r = destroy_job(c, c->legacy_job)) return r;
       c->legacy_job = -1;
     }

@@ -114,7 +114,7 @@ class BatchRenderer:
         (best, table)."""
         if candidates is None:
             candidates = [dict(max_paths=mp, trace_blocks=tb, chunk_iters=ci, ray_order=ro)
-                          for mp in (4 << 20, 8 << 20, 16 << 20) for tb in (1024, 2048) for ci in (2, 4)
+                          for mp in (8 << 20, 16 << 20, 32 << 20) for tb in (1024, 2048) for ci in (2, 4)
                           for ro in (_lib.ORDER_BUCKETS, _lib.ORDER_NONE)]
         table = []
         self.eng.set_schedule(_lib.SCHED_STREAM)

@@ -232,3 +232,38 @@ def test_ray_order_keeps_results(mix):
         assert cnt == ref_cnt, key
         for a, g in zip(ref, res):
             _same(a, g)
+
+
+def test_render_calls_resize_ring_and_continue(mix):
+    """The single-image API: a short call, then a longer restart (the ring
+    grows to keep more frames in flight), then a continuation; each matches
+    the same frames rendered in one call, bit for bit."""
+    from dspt import Engine, PROFILE_FULL, pack_frame_block
+    w = mix[0]
+    s = w.scene
+    img = w.ibl["image"]
+
+    def engine():
+        eng = Engine(PROFILE_FULL)
+        eng.upload_scene(s.vertexBuffer, s.triangleIndexBuffer, s.bvhNodes, s.bvhTriangleIndices,
+                         s.getFlatMaterialBuffer())
+        tot = eng.upload_ibl(img["data"], img["width"], img["height"])
+        eng.resize(w.width, w.height)
+        eng.set_frame(pack_frame_block(w.camera, w.width, w.height, max_bounces=w.max_bounces,
+                                       ibl_size=(img["width"], img["height"]), ibl_total_sum=tot))
+        eng.set_launch_config(max_paths=65536)  # a small pool: the ring size matters
+        return eng
+
+    one = engine()
+    one.render(1, 9)
+    ref = one.read_accum()
+    one.render(1, 5)
+    ref5 = one.read_accum()
+    one.close()
+    eng = engine()
+    eng.render(1, 2)  # sizes the implicit job's ring for 2 frames
+    eng.render(1, 5)  # restart: a larger ring
+    np.testing.assert_array_equal(eng.read_accum(), ref5)
+    eng.render(6, 4)  # continue with frames 6..9 on the running mean
+    np.testing.assert_array_equal(eng.read_accum(), ref)
+    eng.close()
