@@ -39,9 +39,10 @@ sys.path.insert(0, os.path.join(ROOT, "dspbr-pt_amd"))
 
 HBM_PEAK_GBS = 8000.0
 # Memory traffic of one k_trace_ext launch on the default workload (1080p,
-# 64 spp, full-material scene), from rocprofv3 FETCH_SIZE + WRITE_SIZE passes
-# scaled by a plain-copy calibration (profiles/k_trace_ext_counters_*.md).
-TRACE_EXT_TRAFFIC_BYTES = 7.281e9
+# 64 spp, full-material scene, 32 Mi-path pool), from rocprofv3 FETCH_SIZE +
+# WRITE_SIZE passes scaled by a plain-copy calibration
+# (profiles/k_trace_ext_counters_*.md).
+TRACE_EXT_TRAFFIC_BYTES = 12.55e9
 METRIC = "Mpaths/s + Mrays/s at 1080p; p50/p99 job latency, batched vs plain, 1 GPU"
 
 
@@ -501,7 +502,7 @@ def main():
         "kernel_ms": kernel_ms,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": TRACE_EXT_TRAFFIC_BYTES if default_workload else None,
+                     "traffic": TRACE_EXT_TRAFFIC_BYTES if default_workload and args.max_paths == 32 << 20 else None,
                      "kernel": "k_trace_ext", "avg_launch_ms": round(avg_ms, 4),
                      "alg_bytes_per_launch": round(bytes_per_launch)},
     }
