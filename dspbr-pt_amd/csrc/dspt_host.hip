@@ -207,7 +207,7 @@ struct dspt_ctx {
   // rebuilt when the pool, the grids or the chunk length change
   bool use_graphs = true;
   hipGraphExec_t bounce_graph[2][2] = {};
-  uint64_t graph_key = 0;
+  std::array<int64_t, 6> graph_key{};  // (pool, pool address, grids, chunk length, ray order) of the graphs
   // ray order of the extension queue (allocated with the pool)
   int ray_order = DSPT_ORDER_BUCKETS;
   uint16_t* order_keys = nullptr;
@@ -356,7 +356,7 @@ int ensure_pool(dspt_ctx* c) {
   HIPCHK(hipStreamSynchronize(c->stream));
   const int64_t cap = c->max_paths;
   drop_graphs(c);  // they hold the old pool's addresses
-  c->graph_key = 0;
+  c->graph_key = {};
   c->state_mem.release();
   c->cap = 0;
   PathState& a = c->ps[0];
@@ -582,9 +582,8 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
   if (c->use_graphs && !c->timing) {
     // every launch argument of the loop is fixed by (pool, grids, chunk
     // length, q0, inst): the work sizes live in device counters
-    const uint64_t key = ((uint64_t)(uintptr_t)c->ps[0].org * 1000003u) ^ ((uint64_t)P << 20) ^
-                         ((uint64_t)trace_blocks << 8) ^ ((uint64_t)shade_blocks << 36) ^ (uint64_t)c->chunk_iters ^
-                         ((uint64_t)c->ray_order << 62);
+    const std::array<int64_t, 6> key{P, (int64_t)(uintptr_t)c->ps[0].org, trace_blocks, shade_blocks, c->chunk_iters,
+                                     c->ray_order};
     if (key != c->graph_key) {
       HIPCHK(hipStreamSynchronize(c->stream));  // no launch of an old graph in flight
       drop_graphs(c);
@@ -1776,7 +1775,7 @@ int dspt_set_bounce_graphs(dspt_ctx* c, int enable) {
   HIPCHK(hipStreamSynchronize(c->stream));
   c->use_graphs = enable != 0;
   drop_graphs(c);
-  c->graph_key = 0;
+  c->graph_key = {};
   return 0;
 }
 
