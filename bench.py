@@ -109,8 +109,8 @@ def cpu_baseline(wl, block, budget_s=20.0):
 def batched_vs_plain(args):
     """Config-3 style comparison; every job is rendered by both schedules and
     one job's accumulation is compared bit for bit."""
-    from dspt import (Engine, ORDER_BUCKETS, ORDER_NONE, PROFILE_FULL, SCHED_PASS, SCHED_STREAM, POLICY_FIFO, POLICY_RR,
-                      build_bvh, scenes)
+    from dspt import (Engine, ORDER_BUCKETS, ORDER_NONE, ORDER_SHADE, PROFILE_FULL, SCHED_PASS, SCHED_STREAM, POLICY_FIFO,
+                      POLICY_RR, build_bvh, scenes)
     from dspt.batch import BatchRenderer, JobSpec
     t0 = time.time()
     mix = scenes.batch_mix(seed=3, n_jobs=args.batch_jobs)
@@ -131,9 +131,10 @@ def batched_vs_plain(args):
     # autotune the pool on the first jobs' early chunks
     sub = specs[: min(8, len(specs))]
     best, table = b.autotune(lambda: [b.create_job(s) for s in sub],
-                             candidates=[dict(max_paths=mp, trace_blocks=tb, chunk_iters=2, ray_order=ORDER_BUCKETS)
+                             candidates=[dict(max_paths=mp, trace_blocks=tb, chunk_iters=2, ray_order=ORDER_SHADE)
                                          for mp in (8 << 20, 16 << 20, 32 << 20) for tb in (1024, 2048)]
-                             + [dict(max_paths=16 << 20, trace_blocks=2048, chunk_iters=2, ray_order=ORDER_NONE)],
+                             + [dict(max_paths=16 << 20, trace_blocks=2048, chunk_iters=2, ray_order=ro)
+                                for ro in (ORDER_BUCKETS, ORDER_NONE)],
                              chunks=16, warm_chunks=4)
     log(f"[bench] autotune: best {best}; " + ", ".join(f"{r:.0f}" for r, _ in table))
 

@@ -209,10 +209,12 @@ struct dspt_ctx {
   hipGraphExec_t bounce_graph[2][2] = {};
   std::array<int64_t, 6> graph_key{};  // (pool, pool address, grids, chunk length, ray order) of the graphs
   // ray order of the extension queue (allocated with the pool)
-  int ray_order = DSPT_ORDER_BUCKETS;
+  int ray_order = DSPT_ORDER_SHADE;
   uint16_t* order_keys = nullptr;
   int32_t* order_perm = nullptr;
   uint32_t* order_bins = nullptr;
+  int32_t* shade_perm = nullptr;
+  uint32_t* shade_bins = nullptr;
 };
 
 namespace {
@@ -377,7 +379,8 @@ int ensure_pool(dspt_ctx* c) {
   b.sh_pc = a.sh_pc;
   c->trace_blocks = (int)std::min<int64_t>(c->trace_blocks_max, grid_for(cap, kTraceBlock));
   if ((r = A.get(&c->spill, (size_t)c->trace_blocks * kTraceBlock * (kStackSize - kLdsStack)))) return r;
-  if ((r = A.get(&c->order_keys, cap)) || (r = A.get(&c->order_perm, cap)) || (r = A.get(&c->order_bins, kOrderBins)))
+  if ((r = A.get(&c->order_keys, cap)) || (r = A.get(&c->order_perm, cap)) || (r = A.get(&c->order_bins, kOrderBins)) ||
+      (r = A.get(&c->shade_perm, cap)) || (r = A.get(&c->shade_bins, kOrderBins)))
     return r;
   k_stream_init<<<1, 64, 0, c->stream>>>(c->d_ctl);
   HIPCHK(hipGetLastError());
@@ -538,7 +541,8 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
   bool inst = false;
   for (const JobRes* j : c->jobs)
     if (j && j->alive && c->scenes[j->scene] && c->scenes[j->scene]->top >= 0) inst = true;
-  const bool order = c->ray_order == DSPT_ORDER_BUCKETS;
+  const bool order = c->ray_order == DSPT_ORDER_BUCKETS || c->ray_order == DSPT_ORDER_SHADE;
+  const bool shade_order = c->ray_order == DSPT_ORDER_SHADE;
   const int32_t* perm = order ? c->order_perm : nullptr;
   const int order_blocks = (int)std::min<int64_t>(1024, grid_for(P, 256));
   // chunk_iters bounces; the state buffers alternate, starting at q0
@@ -547,14 +551,15 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
       const int a = (q0 + k) & 1;
       const PathState cur = c->ps[a], nxt = c->ps[a ^ 1];
       timed(c, DSPT_K_RAYGEN, [&] {
-        k_plan<<<1, 64, 0, c->stream>>>(ctl, c->d_sched, a, P32, order ? c->order_bins : nullptr);
+        k_plan<<<1, 64, 0, c->stream>>>(ctl, c->d_sched, a, P32, order ? c->order_bins : nullptr,
+                                        shade_order ? c->shade_bins : nullptr);
         k_regen<<<shade_blocks, 256, 0, c->stream>>>(cur, c->d_jobs, c->d_tasks, c->d_sched, ctl, c->cnt);
       });
       if (order)
         timed(c, DSPT_K_ORDER, [&] {
           k_ray_keys<<<order_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, ctl, c->order_keys, c->order_bins);
           k_bin_scan<<<1, 1024, 0, c->stream>>>(c->order_bins);
-          k_bin_scatter<<<order_blocks, 256, 0, c->stream>>>(c->order_keys, ctl, c->order_bins, c->order_perm);
+          k_bin_scatter<<<order_blocks, 256, 0, c->stream>>>(c->order_keys, &ctl->q_base, c->order_bins, c->order_perm);
         });
       timed(c, DSPT_K_TRACE_EXT, [&] {
         if (inst)
@@ -564,9 +569,20 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
           k_trace_ext<false><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], &ctl->ext_next,
                                                                          c->spill, c->cnt, perm, &ctl->q_base);
       });
+      if (shade_order)
+        timed(c, DSPT_K_ORDER, [&] {
+          k_hit_keys<<<order_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], c->order_keys, c->shade_bins);
+          k_bin_scan<<<1, 1024, 0, c->stream>>>(c->shade_bins);
+          k_bin_scatter<<<order_blocks, 256, 0, c->stream>>>(c->order_keys, &ctl->qcount[a], c->shade_bins,
+                                                             c->shade_perm);
+        });
       timed(c, DSPT_K_SHADE, [&] {
-        k_shade<<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1], c->shq,
-                                                      &ctl->sh_count, c->cnt, perm, &ctl->q_base);
+        if (shade_order)
+          k_shade<<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1], c->shq,
+                                                        &ctl->sh_count, c->cnt, c->shade_perm, &ctl->qcount[a]);
+        else
+          k_shade<<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1], c->shq,
+                                                        &ctl->sh_count, c->cnt, perm, &ctl->q_base);
       });
       timed(c, DSPT_K_TRACE_SHADOW, [&] {
         if (inst)
@@ -1169,7 +1185,10 @@ int dspt_create(int profile, int device, dspt_ctx** out) {
   c->profile = profile;
   if (const char* mp = std::getenv("DSPT_MAX_PATHS")) c->max_paths = std::max<int64_t>(65536, std::atoll(mp));
   if (const char* g = std::getenv("DSPT_BOUNCE_GRAPHS")) c->use_graphs = std::atoi(g) != 0;
-  if (const char* o = std::getenv("DSPT_RAY_ORDER")) c->ray_order = std::atoi(o) == DSPT_ORDER_BUCKETS ? DSPT_ORDER_BUCKETS : DSPT_ORDER_NONE;
+  if (const char* o = std::getenv("DSPT_RAY_ORDER")) {
+    const int m = std::atoi(o);
+    c->ray_order = (m == DSPT_ORDER_BUCKETS || m == DSPT_ORDER_SHADE) ? m : DSPT_ORDER_NONE;
+  }
   bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
   void* p = nullptr;
   auto dev = [&](size_t bytes) -> void* {
@@ -1764,7 +1783,8 @@ int dspt_reset_counters(dspt_ctx* c) {
 
 int dspt_set_ray_order(dspt_ctx* c, int mode) {
   if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
-  if (mode != DSPT_ORDER_NONE && mode != DSPT_ORDER_BUCKETS) return fail(DSPT_ERR_ARG, "unknown ray order mode");
+  if (mode != DSPT_ORDER_NONE && mode != DSPT_ORDER_BUCKETS && mode != DSPT_ORDER_SHADE)
+    return fail(DSPT_ERR_ARG, "unknown ray order mode");
   c->ray_order = mode;  // takes effect with the next chunk (the graph key includes it)
   return 0;
 }

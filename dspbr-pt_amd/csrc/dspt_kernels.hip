@@ -175,9 +175,11 @@ __global__ void k_stream_init(StreamCtl* __restrict__ ctl) {
 // One thread: sizes this bounce's refill (free positions vs scheduled
 // items) and resets the queues the bounce appends to.
 __global__ void k_plan(StreamCtl* __restrict__ ctl, const StreamSched* __restrict__ sch, int a, int32_t P,
-                       uint32_t* __restrict__ order_bins) {
+                       uint32_t* __restrict__ order_bins, uint32_t* __restrict__ shade_bins) {
   if (order_bins)  // cleared here rather than by a memset, which a captured graph must not hold
     for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x) order_bins[i] = 0u;
+  if (shade_bins)
+    for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x) shade_bins[i] = 0u;
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int Q = ctl->qcount[a];
   const int F = P - Q;
@@ -266,13 +268,13 @@ __global__ void k_bin_scan(uint32_t* __restrict__ bins) {
   for (int k = 0; k < per; k++) { bins[t * per + k] = run; run += v[k]; }
 }
 
-// perm[start of bucket + rank] = position, for [0, q_base): each block takes
+// perm[start of bucket + rank] = position, for [0, *n_ptr): each block takes
 // a contiguous range, counts it per bucket in LDS, reserves its slots with
 // one global add per non-empty bucket, then places its positions.
-__global__ void k_bin_scatter(const uint16_t* __restrict__ keys, const StreamCtl* __restrict__ ctl,
+__global__ void k_bin_scatter(const uint16_t* __restrict__ keys, const int32_t* __restrict__ n_ptr,
                               uint32_t* __restrict__ bins, int32_t* __restrict__ perm) {
   __shared__ uint32_t h[kOrderBins];
-  const int n = ctl->q_base;
+  const int n = *n_ptr;
   const int per = (n + gridDim.x - 1) / gridDim.x;
   const int lo = (int)min((int64_t)blockIdx.x * per, (int64_t)n), hi = min(lo + per, n);
   if (lo >= hi) return;  // block-uniform: no work for this block
@@ -284,6 +286,33 @@ __global__ void k_bin_scatter(const uint16_t* __restrict__ keys, const StreamCtl
     if (h[i]) h[i] = atomicAdd(&bins[i], h[i]);  // count -> this block's first slot in the bucket
   __syncthreads();
   for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) perm[atomicAdd(&h[keys[q]], 1u)] = q;
+}
+
+// Shading order (mode DSPT_ORDER_SHADE): the whole queue [0, qcount) keyed
+// by the material of each path's hit (misses in bucket 0), so that a wave
+// shades one material's lobes together.
+__global__ void k_hit_keys(const DevJob* __restrict__ jobs, PathState ps, const int32_t* __restrict__ n_ptr,
+                           uint16_t* __restrict__ keys, uint32_t* __restrict__ bins) {
+  __shared__ uint32_t h[kOrderBins];
+  const int n = *n_ptr;
+  if ((int64_t)blockIdx.x * blockDim.x >= n) return;
+  for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x) h[i] = 0u;
+  __syncthreads();
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    const int p = (int)q;
+    const int tri = __float_as_int(ps.hit[p].x);
+    uint32_t k = 0u;
+    if (tri >= 0) {
+      const SceneView& S = jobs[ps.job[p]].S;
+      const float mat = S.verts[(size_t)S.tris[(size_t)tri * 3] * 5].w;  // material index of the hit, as fetch_hitpoint reads it
+      k = 1u + (uint32_t)mat % (uint32_t)(kOrderBins - 1);
+    }
+    keys[p] = (uint16_t)k;
+    atomicAdd(&h[k], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x)
+    if (h[i]) atomicAdd(&bins[i], h[i]);
 }
 
 // Camera rays for the items of this refill, renderer.frag:129-142 +

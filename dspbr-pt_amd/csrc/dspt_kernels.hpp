@@ -97,7 +97,8 @@ __global__ void k_ibl_guides(const float* cdf, const float* ycdf, int w, int h, 
 __global__ void k_fill_i32(int32_t* p, int n, int32_t v);
 __global__ void k_fill_f4(float4* p, int64_t n, float4 v);
 __global__ void k_stream_init(StreamCtl* ctl);
-__global__ void k_plan(StreamCtl* ctl, const StreamSched* sch, int a, int32_t P, uint32_t* order_bins);
+__global__ void k_plan(StreamCtl* ctl, const StreamSched* sch, int a, int32_t P, uint32_t* order_bins,
+                       uint32_t* shade_bins);
 __global__ void k_regen(PathState ps, const DevJob* jobs, const PassTask* tasks, const StreamSched* sch, const StreamCtl* ctl,
                         unsigned long long* cnt);
 // Ray order of the extension queue (engine extension; the reference traces
@@ -107,7 +108,8 @@ __global__ void k_regen(PathState ps, const DevJob* jobs, const PassTask* tasks,
 constexpr int kOrderBins = 8192;
 __global__ void k_ray_keys(const DevJob* jobs, PathState ps, const StreamCtl* ctl, uint16_t* keys, uint32_t* bins);
 __global__ void k_bin_scan(uint32_t* bins);
-__global__ void k_bin_scatter(const uint16_t* keys, const StreamCtl* ctl, uint32_t* bins, int32_t* perm);
+__global__ void k_bin_scatter(const uint16_t* keys, const int32_t* n_ptr, uint32_t* bins, int32_t* perm);
+__global__ void k_hit_keys(const DevJob* jobs, PathState ps, const int32_t* n_ptr, uint16_t* keys, uint32_t* bins);
 template <bool INST>
 __global__ void k_trace_ext(const DevJob* jobs, PathState ps, const int32_t* count_ptr, int32_t* next_ptr, int* spill,
                             unsigned long long* cnt, const int32_t* perm, const int32_t* nsorted_ptr);

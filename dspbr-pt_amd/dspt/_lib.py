@@ -91,7 +91,7 @@ POLICY_RR = 0
 POLICY_FIFO = 1
 POLICY_SRPT = 2
 BVH_LBVH, BVH_PLOC = 0, 1
-ORDER_NONE, ORDER_BUCKETS = 0, 1
+ORDER_NONE, ORDER_BUCKETS, ORDER_SHADE = 0, 1, 2
 KERNEL_CLASSES = ["raygen", "trace_ext", "shade", "trace_shadow", "accumulate", "post", "order"]
 
 
@@ -266,7 +266,8 @@ class Engine:
         check(self._L.dspt_set_bounce_graphs(self.h, 1 if on else 0))
 
     def set_ray_order(self, mode):
-        """ORDER_NONE (compaction order) or ORDER_BUCKETS (octant + origin-cell buckets)."""
+        """ORDER_NONE (compaction order), ORDER_BUCKETS (origin-cell + up/down buckets before each trace) or
+        ORDER_SHADE (the default: BUCKETS, then shading grouped by the hit's material)."""
         check(self._L.dspt_set_ray_order(self.h, int(mode)))
 
     def set_kernel_timing(self, on):

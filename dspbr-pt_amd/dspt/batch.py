@@ -115,13 +115,13 @@ class BatchRenderer:
         if candidates is None:
             candidates = [dict(max_paths=mp, trace_blocks=tb, chunk_iters=ci, ray_order=ro)
                           for mp in (8 << 20, 16 << 20, 32 << 20) for tb in (1024, 2048) for ci in (2, 4)
-                          for ro in (_lib.ORDER_BUCKETS, _lib.ORDER_NONE)]
+                          for ro in (_lib.ORDER_SHADE, _lib.ORDER_BUCKETS, _lib.ORDER_NONE)]
         table = []
         self.eng.set_schedule(_lib.SCHED_STREAM)
         for cand in candidates:
             self.eng.set_launch_config(cand["max_paths"], cand["trace_blocks"], 0)
             self.eng.set_schedule(_lib.SCHED_STREAM, 0, cand["chunk_iters"])
-            self.eng.set_ray_order(cand.get("ray_order", _lib.ORDER_BUCKETS))
+            self.eng.set_ray_order(cand.get("ray_order", _lib.ORDER_SHADE))
             ids = make_jobs()
             for _ in range(warm_chunks):
                 self.eng.jobs_step(ids)
@@ -140,7 +140,7 @@ class BatchRenderer:
         best = max(table, key=lambda x: x[0])[1]
         self.eng.set_launch_config(best["max_paths"], best["trace_blocks"], 0)
         self.eng.set_schedule(_lib.SCHED_STREAM, 0, best["chunk_iters"])
-        self.eng.set_ray_order(best.get("ray_order", _lib.ORDER_BUCKETS))
+        self.eng.set_ray_order(best.get("ray_order", _lib.ORDER_SHADE))
         return best, table
 
 

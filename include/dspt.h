@@ -141,16 +141,20 @@ int dspt_get_kernel_stats(dspt_ctx* ctx, double* out);
  * (renderer.ts:595-714). */
 int dspt_set_bounce_graphs(dspt_ctx* ctx, int enable);
 
-/* Ray order of the extension queue before each closest-hit trace:
- * DSPT_ORDER_BUCKETS (default) sorts the continuing paths into 8192 buckets
- * by (Morton code of the origin's cell in a 16x16x16 grid over the scene's
- * root box, sign of dir.y); camera rays keep pixel order, and shading
- * follows the same order. DSPT_ORDER_NONE keeps the compaction order. Results
- * are identical either way (every path is computed from its own state); only
- * speed changes. DSPT_RAY_ORDER=0 in the environment selects NONE at
- * dspt_create; the batched renderer's autotuner tries both. No counterpart in
- * the reference (one path per fragment). */
-enum { DSPT_ORDER_NONE = 0, DSPT_ORDER_BUCKETS = 1 };
+/* Ray order of the path queue:
+ * DSPT_ORDER_BUCKETS sorts the continuing paths into 8192 buckets by (Morton
+ * code of the origin's cell in a 16x16x16 grid over the scene's root box,
+ * sign of dir.y) before each closest-hit trace; camera rays keep pixel
+ * order, and shading follows the same order.
+ * DSPT_ORDER_SHADE (default) does that and then, between the closest-hit
+ * trace and shading, sorts the whole queue by the material of each hit
+ * (misses together), so a wave shades one material's lobes.
+ * DSPT_ORDER_NONE keeps the compaction order.
+ * Results are identical in every mode (every path is computed from its own
+ * state); only speed changes. DSPT_RAY_ORDER=<mode> in the environment sets
+ * the mode at dspt_create; the batched renderer's autotuner tries the modes.
+ * No counterpart in the reference (one path per fragment). */
+enum { DSPT_ORDER_NONE = 0, DSPT_ORDER_BUCKETS = 1, DSPT_ORDER_SHADE = 2 };
 int dspt_set_ray_order(dspt_ctx* ctx, int mode);
 
 /* Display chain of renderer.ts:665-700 on the accumulation (the implicit

@@ -203,12 +203,12 @@ def test_ray_order_keeps_results(mix):
     lane walks which ray, not any path's result: accumulations, samples and
     the work counters equal those of the compaction order, with and without
     the captured graphs, one- and two-level."""
-    from dspt import Engine, ORDER_BUCKETS, ORDER_NONE, PROFILE_FULL, SCHED_STREAM, scenes
+    from dspt import Engine, ORDER_BUCKETS, ORDER_NONE, ORDER_SHADE, PROFILE_FULL, SCHED_STREAM, scenes
     from dspt.batch import JobSpec
     inst = scenes.instanced_scene(seed=9, n_meshes=3, n_instances=40, tri_range=(300, 1500), extent=5.0, height=2.0,
                                   width=40, height_px=32, spp=5, max_bounces=6, ibl_size=(128, 64))
     runs = {}
-    for order, graphs in ((ORDER_NONE, True), (ORDER_BUCKETS, True), (ORDER_BUCKETS, False)):
+    for order, graphs in ((ORDER_NONE, True), (ORDER_BUCKETS, True), (ORDER_BUCKETS, False), (ORDER_SHADE, True)):
         eng = Engine(PROFILE_FULL)
         eng.set_ray_order(order)
         eng.set_bounce_graphs(graphs)
