@@ -62,6 +62,7 @@ struct SceneView {
   // then a ray with finite origin and finite, non-zero 1/dir can take the
   // slab test on hardware min/max (slab_fast)
   int32_t fast_slab, pad_fs;
+  const uint32_t* __restrict__ tri_mat;  // material index of each triangle (its first vertex's pos.w)
 };
 
 // The 52-float uniform block (renderer.frag:10-31).

@@ -88,6 +88,15 @@ __global__ void k_encode_children(float4* __restrict__ nodes, int32_t N, int32_t
   }
 }
 
+// Material index of each triangle: the first vertex's pos.w, as
+// fetch_hitpoint reads it (the shading order's key).
+__global__ void k_tri_mat(const uint32_t* __restrict__ tris, const float4* __restrict__ verts, int32_t T,
+                          uint32_t* __restrict__ out) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  out[t] = (uint32_t)verts[(size_t)tris[(size_t)t * 3] * 5].w;
+}
+
 // Clears *ok when an inner node below `limit` has a non-finite child box
 // corner (the walk's slab tests read a.xyz, b.xyz, c.xyz, d.xyz of inner
 // nodes only); *ok starts at 1.
@@ -303,9 +312,7 @@ __global__ void k_hit_keys(const DevJob* __restrict__ jobs, PathState ps, const 
     const int tri = __float_as_int(ps.hit[p].x);
     uint32_t k = 0u;
     if (tri >= 0) {
-      const SceneView& S = jobs[ps.job[p]].S;
-      const float mat = S.verts[(size_t)S.tris[(size_t)tri * 3] * 5].w;  // material index of the hit, as fetch_hitpoint reads it
-      k = 1u + (uint32_t)mat % (uint32_t)(kOrderBins - 1);
+      k = 1u + jobs[ps.job[p]].S.tri_mat[tri] % (uint32_t)(kOrderBins - 1);
     }
     keys[p] = (uint16_t)k;
     atomicAdd(&h[k], 1u);
