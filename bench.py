@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # 64 spp, full-material scene, 32 Mi-path pool), from rocprofv3 FETCH_SIZE +
 # WRITE_SIZE passes scaled by a plain-copy calibration
 # (profiles/k_trace_ext_counters_*.md).
-TRACE_EXT_TRAFFIC_BYTES = 12.55e9
+TRACE_EXT_TRAFFIC_BYTES = 12.48e9
 METRIC = "Mpaths/s + Mrays/s at 1080p; p50/p99 job latency, batched vs plain, 1 GPU"
 
 
