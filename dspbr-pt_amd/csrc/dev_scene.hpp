@@ -63,6 +63,10 @@ struct SceneView {
   // slab test on hardware min/max (slab_fast)
   int32_t fast_slab, pad_fs;
   const uint32_t* __restrict__ tri_mat;  // material index of each triangle (its first vertex's pos.w)
+  // the three vertex records of each triangle (5 float4 each), copied at
+  // upload so a hit reaches its vertices without the index load; NULL: go
+  // through tris and verts
+  const float4* __restrict__ tri_verts;
 };
 
 // The 52-float uniform block (renderer.frag:10-31).

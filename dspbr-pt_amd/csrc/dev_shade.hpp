@@ -22,11 +22,18 @@ struct Hitpoint {
 DI Hitpoint fetch_hitpoint(const SceneView& S, f3 org, f3 dir, int tri, float t, float u, float v, int inst = -1) {
   Hitpoint hp;
   hp.pos = org + dir * t;
-  uint32_t vi[3];
-  for (int k = 0; k < 3; k++) vi[k] = S.tris[(size_t)tri * 3 + k];
-  const float4* v0 = S.verts + (size_t)vi[0] * 5;
-  const float4* v1 = S.verts + (size_t)vi[1] * 5;
-  const float4* v2 = S.verts + (size_t)vi[2] * 5;
+  const float4 *v0, *v1, *v2;
+  if (S.tri_verts) {  // the same records, stored per triangle
+    v0 = S.tri_verts + (size_t)tri * 15;
+    v1 = v0 + 5;
+    v2 = v0 + 10;
+  } else {
+    uint32_t vi[3];
+    for (int k = 0; k < 3; k++) vi[k] = S.tris[(size_t)tri * 3 + k];
+    v0 = S.verts + (size_t)vi[0] * 5;
+    v1 = S.verts + (size_t)vi[1] * 5;
+    v2 = S.verts + (size_t)vi[2] * 5;
+  }
   float4 p0 = v0[0], p1 = v1[0], p2 = v2[0];
   const InstanceDesc* D =
       (inst >= 0 && S.inst_base >= 0) ? reinterpret_cast<const InstanceDesc*>(S.nodes + (size_t)(S.inst_base + 2 * inst) * 4)

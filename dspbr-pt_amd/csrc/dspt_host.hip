@@ -98,6 +98,7 @@ struct SceneRes {
   int32_t fast_slab = 0;  // every inner node's child boxes are finite
   int32_t* d_flag = nullptr;
   uint32_t* tri_mat = nullptr;  // material of each triangle (resident scenes)
+  float4* tri_verts = nullptr;  // vertex records per triangle corner (resident scenes)
 };
 
 struct IblRes {
@@ -306,6 +307,7 @@ SceneView view_of(dspt_ctx* c, const JobRes* j) {
   SceneView S{};
   const SceneRes* s = c->scenes[j->scene];
   S.nodes = s->nodes; S.tri = s->tri; S.verts = s->verts; S.tris = s->tris; S.mats = s->mats; S.tri_mat = s->tri_mat;
+  S.tri_verts = s->tri_verts;
   S.N = s->N; S.M = s->M; S.T = s->T; S.K = s->K;
   S.texinfo = s->texinfo; S.texarr = s->texarr; S.texels = s->texels;
   S.num_tex = s->num_tex; S.num_texarr = s->num_texarr;
@@ -796,8 +798,10 @@ int make_scene_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint
     return r;
   k_prep_tris<<<grid_for(T, 256), 256, 0, c->stream>>>(d_bvh, s->tris, s->verts, (int32_t)T, (int32_t)T, s->tri);
   if (mats) {  // a resident scene: child codes (the host-readback build keeps the bvh_gpu fields)
-    if ((r = A.get(&s->d_flag, 1)) || (r = A.get(&s->tri_mat, T))) return bail(r);
+    if ((r = A.get(&s->d_flag, 1)) || (r = A.get(&s->tri_mat, T)) || (r = A.get(&s->tri_verts, (size_t)T * 15)))
+      return bail(r);
     k_tri_mat<<<grid_for(T, 256), 256, 0, c->stream>>>(s->tris, s->verts, (int32_t)T, s->tri_mat);
+    k_tri_verts<<<grid_for(T * 15, 256), 256, 0, c->stream>>>(s->tris, s->verts, (int32_t)T, s->tri_verts);
     k_encode_children<<<grid_for(N, 256), 256, 0, c->stream>>>(s->nodes, (int32_t)N, (int32_t)N);
     k_fill_i32<<<1, 64, 0, c->stream>>>(s->d_flag, 1, 1);
     k_boxes_finite<<<grid_for(N, 256), 256, 0, c->stream>>>(s->nodes, (int32_t)N, s->d_flag);
@@ -859,10 +863,12 @@ int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tr
   k_prep_tris<<<grid_for(M, 256), 256, 0, c->stream>>>(d_bvh, s->tris, s->verts, (int32_t)M, (int32_t)T, s->tri);
   const int64_t lim = encode_limit < 0 ? N : std::min(encode_limit, N);
   k_encode_children<<<grid_for(lim, 256), 256, 0, c->stream>>>(s->nodes, (int32_t)N, (int32_t)lim);
-  if ((r = A.get(&s->d_flag, 1)) || (r = A.get(&s->tri_mat, T))) return bail(r);
+  if ((r = A.get(&s->d_flag, 1)) || (r = A.get(&s->tri_mat, T)) || (r = A.get(&s->tri_verts, (size_t)T * 15)))
+    return bail(r);
   k_fill_i32<<<1, 64, 0, c->stream>>>(s->d_flag, 1, 1);
   k_boxes_finite<<<grid_for(lim, 256), 256, 0, c->stream>>>(s->nodes, (int32_t)lim, s->d_flag);
   k_tri_mat<<<grid_for(T, 256), 256, 0, c->stream>>>(s->tris, s->verts, (int32_t)T, s->tri_mat);
+  k_tri_verts<<<grid_for(T * 15, 256), 256, 0, c->stream>>>(s->tris, s->verts, (int32_t)T, s->tri_verts);
   if ((r = chk(hipGetLastError(), "scene kernels")) || (r = chk(hipStreamSynchronize(c->stream), "scene upload"))) return r;
   if ((r = chk(hipMemcpy(&s->fast_slab, s->d_flag, 4, hipMemcpyDeviceToHost), "scene flag"))) return r;
   (void)hipFree(d_bvh);

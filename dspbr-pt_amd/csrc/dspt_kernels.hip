@@ -97,6 +97,17 @@ __global__ void k_tri_mat(const uint32_t* __restrict__ tris, const float4* __res
   out[t] = (uint32_t)verts[(size_t)tris[(size_t)t * 3] * 5].w;
 }
 
+// The three vertex records of each triangle, in corner order (15 float4
+// per triangle), so shading reads them without the index indirection.
+__global__ void k_tri_verts(const uint32_t* __restrict__ tris, const float4* __restrict__ verts, int32_t T,
+                            float4* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // one float4 of the output
+  if (i >= (int64_t)T * 15) return;
+  const int64_t t = i / 15;
+  const int r = (int)(i - t * 15), corner = r / 5, field = r - corner * 5;
+  out[i] = verts[(size_t)tris[t * 3 + corner] * 5 + field];
+}
+
 // Clears *ok when an inner node below `limit` has a non-finite child box
 // corner (the walk's slab tests read a.xyz, b.xyz, c.xyz, d.xyz of inner
 // nodes only); *ok starts at 1.

@@ -91,6 +91,7 @@ __global__ void k_prep_tris(const int32_t* bvh_idx, const uint32_t* tris, const 
                             float4* out);
 __global__ void k_boxes_finite(const float4* nodes, int32_t limit, int32_t* ok);
 __global__ void k_tri_mat(const uint32_t* tris, const float4* verts, int32_t T, uint32_t* out);
+__global__ void k_tri_verts(const uint32_t* tris, const float4* verts, int32_t T, float4* out);
 __global__ void k_ibl_luminance(const float4* env, int64_t n, float* f);
 __global__ void k_ibl_rows(const float* f, int w, int h, float* pdf, float* cdf, float* rowsum);
 __global__ void k_ibl_marginal(const float* rowsum, int h, float* ypdf, float* ycdf, double* total);
