@@ -267,3 +267,17 @@ def test_render_calls_resize_ring_and_continue(mix):
     eng.render(6, 4)  # continue with frames 6..9 on the running mean
     np.testing.assert_array_equal(eng.read_accum(), ref)
     eng.close()
+
+
+def test_launch_switch_errors():
+    """The launch switches check their arguments."""
+    from dspt import DsptError, Engine, ORDER_SHADE
+    eng = Engine(0)
+    with pytest.raises(DsptError):
+        eng.set_ray_order(7)
+    eng.set_ray_order(ORDER_SHADE)
+    eng.set_bounce_graphs(False)
+    eng.set_bounce_graphs(True)
+    with pytest.raises(DsptError):
+        eng.set_launch_config(max_paths=(1 << 30) + 1)
+    eng.close()
