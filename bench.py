@@ -39,10 +39,12 @@ sys.path.insert(0, os.path.join(ROOT, "dspbr-pt_amd"))
 
 HBM_PEAK_GBS = 8000.0
 # Memory traffic of one k_trace_ext launch on the default workload (1080p,
-# 64 spp, full-material scene, 32 Mi-path pool), from rocprofv3 FETCH_SIZE +
+# 64 spp, full-material scene, 64 Mi-path pool), from rocprofv3 FETCH_SIZE +
 # WRITE_SIZE passes scaled by a plain-copy calibration
 # (profiles/k_trace_ext_counters_*.md).
-TRACE_EXT_TRAFFIC_BYTES = 12.48e9
+TRACE_EXT_TRAFFIC_BYTES = 14.81e9
+TRACE_EXT_TRAFFIC_POOL = 64 << 20  # the pool size that figure was measured at
+DEFAULT_MAX_PATHS = 64 << 20
 METRIC = "Mpaths/s + Mrays/s at 1080p; p50/p99 job latency, batched vs plain, 1 GPU"
 
 
@@ -132,7 +134,7 @@ def batched_vs_plain(args):
     sub = specs[: min(8, len(specs))]
     best, table = b.autotune(lambda: [b.create_job(s) for s in sub],
                              candidates=[dict(max_paths=mp, trace_blocks=tb, chunk_iters=2, ray_order=ORDER_SHADE)
-                                         for mp in (8 << 20, 16 << 20, 32 << 20) for tb in (1024, 2048)]
+                                         for mp in (16 << 20, 32 << 20, 64 << 20) for tb in (1024, 2048)]
                              + [dict(max_paths=16 << 20, trace_blocks=2048, chunk_iters=2, ray_order=ro)
                                 for ro in (ORDER_BUCKETS, ORDER_NONE)],
                              chunks=16, warm_chunks=4)
@@ -406,8 +408,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="full_materials")
     ap.add_argument("--spp", type=int, default=None, help="override frames per job")
-    ap.add_argument("--max-paths", type=int, default=32 << 20,
-                    help="path pool of the single-job leg (32 Mi paths, about 10 GB of the 288 GB HBM)")
+    ap.add_argument("--max-paths", type=int, default=DEFAULT_MAX_PATHS,
+                    help="path pool of the single-job leg (64 Mi paths, about 20 GB of the 288 GB HBM)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain comparison")
@@ -503,7 +505,8 @@ def main():
         "kernel_ms": kernel_ms,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": TRACE_EXT_TRAFFIC_BYTES if default_workload and args.max_paths == 32 << 20 else None,
+                     "traffic": (TRACE_EXT_TRAFFIC_BYTES
+                                 if default_workload and args.max_paths == TRACE_EXT_TRAFFIC_POOL else None),
                      "kernel": "k_trace_ext", "avg_launch_ms": round(avg_ms, 4),
                      "alg_bytes_per_launch": round(bytes_per_launch)},
     }
