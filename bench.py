@@ -289,6 +289,7 @@ def instanced(args):
     log(f"[bench] instanced: {s.num_instances} instances of {len(s.mesh_count)} meshes, {s.scene.num_triangles} mesh tris, "
         f"{s.instanced_triangles} instanced tris, generated in {time.time() - t0:.1f}s")
     eng = Engine(PROFILE_FULL)
+    eng.set_launch_config(max_paths=args.inst_max_paths)
     b = BatchRenderer(eng)
     t1 = time.time()
     b.add_scene("inst", s)
@@ -420,6 +421,7 @@ def main():
     ap.add_argument("--no-instanced", action="store_true", help="skip the large instanced scene")
     ap.add_argument("--no-bvh-build", action="store_true", help="skip the host vs GPU BVH build comparison")
     ap.add_argument("--inst-spp", type=int, default=256)
+    ap.add_argument("--inst-max-paths", type=int, default=16 << 20, help="path pool of the instanced leg")
     ap.add_argument("--inst-traffic-rate", type=float, default=10.0, help="small jobs per second beside the long job")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
