@@ -234,6 +234,31 @@ def test_ray_order_keeps_results(mix):
             _same(a, g)
 
 
+def test_large_pool_keeps_results(mix):
+    """A 67 M-path job (1024x1024, 64 frames) fills a 64 Mi-path pool, so
+    positions up to the top of the pool are used; its accumulation and work
+    counters equal those of the same job in a 16 Mi-path pool."""
+    from dspt import Engine, PROFILE_FULL
+    from dspt.batch import JobSpec
+    w = mix[0]
+    runs = []
+    for mp in (64 << 20, 16 << 20):
+        eng = Engine(PROFILE_FULL)
+        eng.set_launch_config(max_paths=mp)
+        b, _ = _batch(eng, [w])
+        eng.reset_counters()
+        j = b.create_job(JobSpec(0, 0, w.camera, 1024, 1024, 64, max_bounces=w.max_bounces))
+        eng.jobs_run([j])
+        c = eng.counters()
+        runs.append((eng.job_read_accum(j), {k: c[k] for k in ("paths", "rays_extension", "rays_shadow",
+                                                               "node_visits", "tri_tests", "shade_hits")}))
+        assert eng.launch_config()["max_paths"] == mp
+        eng.close()
+    assert runs[0][1]["paths"] == 1024 * 1024 * 64
+    assert runs[0][1] == runs[1][1]
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+
+
 def test_render_calls_resize_ring_and_continue(mix):
     """The single-image API: a short call, then a longer restart (the ring
     grows to keep more frames in flight), then a continuation; each matches
