@@ -547,7 +547,7 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
   const bool order = c->ray_order == DSPT_ORDER_BUCKETS || c->ray_order == DSPT_ORDER_SHADE;
   const bool shade_order = c->ray_order == DSPT_ORDER_SHADE;
   const int32_t* perm = order ? c->order_perm : nullptr;
-  const int order_blocks = (int)std::min<int64_t>(1024, grid_for(P, 256));
+  const int order_blocks = (int)std::min<int64_t>(DSPT_ORDER_BLOCKS, grid_for(P, DSPT_ORDER_THREADS));
   // chunk_iters bounces; the state buffers alternate, starting at q0
   auto bounces = [&](int q0) {
     for (int k = 0; k < c->chunk_iters; k++) {
@@ -560,9 +560,9 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
       });
       if (order)
         timed(c, DSPT_K_ORDER, [&] {
-          k_ray_keys<<<order_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, ctl, c->order_keys, c->order_bins);
+          k_ray_keys<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->d_jobs, cur, ctl, c->order_keys, c->order_bins);
           k_bin_scan<<<1, 1024, 0, c->stream>>>(c->order_bins);
-          k_bin_scatter<<<order_blocks, 256, 0, c->stream>>>(c->order_keys, &ctl->q_base, c->order_bins, c->order_perm);
+          k_bin_scatter<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->order_keys, &ctl->q_base, c->order_bins, c->order_perm);
         });
       timed(c, DSPT_K_TRACE_EXT, [&] {
         if (inst)
@@ -574,9 +574,9 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
       });
       if (shade_order)
         timed(c, DSPT_K_ORDER, [&] {
-          k_hit_keys<<<order_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], c->order_keys, c->shade_bins);
+          k_hit_keys<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], c->order_keys, c->shade_bins);
           k_bin_scan<<<1, 1024, 0, c->stream>>>(c->shade_bins);
-          k_bin_scatter<<<order_blocks, 256, 0, c->stream>>>(c->order_keys, &ctl->qcount[a], c->shade_bins,
+          k_bin_scatter<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->order_keys, &ctl->qcount[a], c->shade_bins,
                                                              c->shade_perm);
         });
       timed(c, DSPT_K_SHADE, [&] {

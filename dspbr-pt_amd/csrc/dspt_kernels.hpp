@@ -108,6 +108,16 @@ __global__ void k_regen(PathState ps, const DevJob* jobs, const PassTask* tasks,
 // (origin cell of a 16x16x16 grid over the scene's root box, up or down)
 // into perm; camera rays appended behind them keep pixel order.
 constexpr int kOrderBins = 8192;
+// launch shape of the key and scatter passes: blocks of 1024 threads (more
+// loads in flight per CU), at most 512 of them (each block adds its bucket
+// counts with one global atomic per non-empty bucket)
+#ifndef DSPT_ORDER_THREADS
+#define DSPT_ORDER_THREADS 1024
+#endif
+#ifndef DSPT_ORDER_BLOCKS
+#define DSPT_ORDER_BLOCKS 512
+#endif
+static_assert(DSPT_ORDER_THREADS % 64 == 0 && DSPT_ORDER_THREADS <= 1024, "order passes: whole waves, at most 1024 threads");
 __global__ void k_ray_keys(const DevJob* jobs, PathState ps, const StreamCtl* ctl, uint16_t* keys, uint32_t* bins);
 __global__ void k_bin_scan(uint32_t* bins);
 __global__ void k_bin_scatter(const uint16_t* keys, const int32_t* n_ptr, uint32_t* bins, int32_t* perm);
