@@ -411,6 +411,8 @@ def main():
     ap.add_argument("--spp", type=int, default=None, help="override frames per job")
     ap.add_argument("--max-paths", type=int, default=DEFAULT_MAX_PATHS,
                     help="path pool of the single-job leg (64 Mi paths, about 20 GB of the 288 GB HBM)")
+    ap.add_argument("--shade-blocks", type=int, default=0,
+                    help="grid of the shading and refill kernels in the single-job leg (0: the engine default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain comparison")
@@ -446,7 +448,7 @@ def main():
     eng.resize(wl.width, wl.height)
     block = frame_block(wl, total)
     eng.set_frame(block)
-    eng.set_launch_config(max_paths=args.max_paths)
+    eng.set_launch_config(max_paths=args.max_paths, shade_blocks=args.shade_blocks)
     log(f"[bench] uploaded; device memory {eng.memory_usage() / 1e9:.2f} GB; path pool {args.max_paths}")
 
     for i in range(args.warmup):

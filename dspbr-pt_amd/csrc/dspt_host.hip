@@ -168,7 +168,7 @@ struct dspt_ctx {
   int* spill = nullptr;
   int trace_blocks = 0;
   int trace_blocks_max = 2048;
-  int shade_blocks_max = 8192;
+  int shade_blocks_max = 16384;  // blocks of 256 threads, 64 per CU (8192 and 32768 measured slower)
   int64_t max_paths = 16 << 20;  // about 5 GB of path state
   unsigned long long* cnt = nullptr;
   // scheduler

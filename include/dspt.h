@@ -288,7 +288,8 @@ int dspt_set_memory_budget(dspt_ctx* ctx, int64_t bytes);
 
 /* Launch configuration: path-pool size (paths in flight; default 16 Mi,
  * about 5 GB of path state; larger pools amortise each launch's drain over
- * more rays), traversal and shading grid caps (<= 0 keeps a value). Chosen
+ * more rays), traversal and shading grid caps in blocks of 256 threads
+ * (defaults 2048 and 16384; <= 0 keeps a value). Chosen
  * by the autotuner in the host layer. A new pool size takes effect once the
  * work in flight drains. */
 int dspt_set_launch_config(dspt_ctx* ctx, int64_t max_paths, int32_t trace_blocks, int32_t shade_blocks);
