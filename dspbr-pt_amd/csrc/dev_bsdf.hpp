@@ -79,7 +79,7 @@ DI Closure build_closure(const GltfInputs& m, f3 anisoTangent, bool full_profile
   c.dielF90 = specular;
   c.dielectricWeight = dw;
   float s = gmax(iridIor, 1.0f);
-  float film = gclamp(irid, 0.0f, 1.0f) * powf((s - 1.0f) / gmax(s + 1.0f, E5), 2.0f);
+  float film = gclamp(irid, 0.0f, 1.0f) * pm::ppow((s - 1.0f) / gmax(s + 1.0f, E5), 2.0f);
   c.specularWeight = gmax(max3c(metalE), film) * metallic;
   c.ccRough = gclamp(gmax(ccRoughIn, 0.0020000000949949f), 0.0f, 1.0f);
   c.clearcoat = gclamp(m.clearcoat, 0.0f, 1.0f);
@@ -112,11 +112,11 @@ DI float avg_albedo_ms(float a, float e0) {
   return gclamp(gclamp(e0 + (1.0f - e0) * AVG_W, 0.0f, 1.0f) * avg_albedo(a), 0.0f, 1.0f);
 }
 DI float dir_albedo(float a, float ct) {
-  return gclamp(1.0f - sqrtf(gmax(a, 0.0f)) * (0.5f + 0.5f * powf(1.0f - gclamp(ct, 0.0f, 1.0f), 5.0f)), 0.0f, 1.0f);
+  return gclamp(1.0f - sqrtf(gmax(a, 0.0f)) * (0.5f + 0.5f * pm::ppow(1.0f - gclamp(ct, 0.0f, 1.0f), 5.0f)), 0.0f, 1.0f);
 }
 DI float dir_albedo_ms(float ct, float a, float e0) {
   float nv = gclamp(ct, 0.0f, 1.0f);
-  return gclamp(gclamp(e0 + (1.0f - e0) * powf(1.0f - nv, 5.0f), 0.0f, 1.0f) * gmix(1.0f, dir_albedo(a, nv), 0.64999997615814209f),
+  return gclamp(gclamp(e0 + (1.0f - e0) * pm::ppow(1.0f - nv, 5.0f), 0.0f, 1.0f) * gmix(1.0f, dir_albedo(a, nv), 0.64999997615814209f),
                 0.0f, 1.0f);
 }
 
@@ -199,7 +199,7 @@ DI f3 film_sensitivity(float opd, f3 shift) {
   const f3 var = f3{4.327799808e+09f, 9.304599552e+09f, 6.612100096e+09f};
   f3 xyzv = f3{5.48560003844900113e-13f, 4.42010008238832852e-13f, 5.24810013155518895e-13f} * vsqrt(6.28318548202514648f * var) *
             vcos(f3{1.681e+06f, 1.7953e+06f, 2.2084e+06f} * ph + shift) * vexp(-ph2 * var);
-  xyzv.x = xyzv.x + 9.74699989231449238e-14f * sqrtf(2.845152256e+10f) * cosf(2.2399e+06f * ph + shift.x) * expf(-4.528200192e+09f * ph2);
+  xyzv.x = xyzv.x + 9.74699989231449238e-14f * sqrtf(2.845152256e+10f) * pm::pcos(2.2399e+06f * ph + shift.x) * pm::pexp(-4.528200192e+09f * ph2);
   f3 q = xyzv / 1.06850002623559703e-07f;
   return f3{3.24045419692993164f * q.x - 1.53713846206665039f * q.y - 0.49853140115737915f * q.z,
             -0.96926599740982056f * q.x + 1.87601077556610107f * q.y + 0.04155600070953369f * q.z,
@@ -313,7 +313,7 @@ DI f3 thick_transmission(const Closure& c, const Frame3& F, f3 tc, f3 n, f3 v, f
   float d = D_aniso(a, lh);
   float g = G2_aniso(a, lv, ll);
   float avh = fabsf(vh);
-  float factor = eta * eta * avh * fabsf(lhd) / gmax(powf(vh + eta * lhd, 2.0f) * nv, E5);
+  float factor = eta * eta * avh * fabsf(lhd) / gmax(pm::ppow(vh + eta * lhd, 2.0f) * nv, E5);
   f3 etc = tc;
   if (c.irid > E5) {
     f3 rr = schlick_dielectric(avh, c.dielF0, sp3(c.dielF90), iorO, false);
@@ -357,8 +357,8 @@ DI float transmissive_diffuse_comp(const Closure& c, float nv, float nl) {
   const f3 O = sp3(1.0f);
   f3 energy = vmin(vmax(O - (schlick_avg(c.albedo, O) * c.metallic + schlick_avg(c.dielF0, sp3(c.dielF90)) * c.dielectricWeight), sp3(0.0f)), O);
   return coupled * gmix(1.0f, 1.0f / gmax(max3c(energy), E4),
-                        sqrtf(0.5f * (gclamp(f0 + d * powf(1.0f - gclamp(nv, 0.0f, 1.0f), 5.0f), 0.0f, 1.0f) +
-                                      gclamp(f0 + d * powf(1.0f - gclamp(nl, 0.0f, 1.0f), 5.0f), 0.0f, 1.0f))));
+                        sqrtf(0.5f * (gclamp(f0 + d * pm::ppow(1.0f - gclamp(nv, 0.0f, 1.0f), 5.0f), 0.0f, 1.0f) +
+                                      gclamp(f0 + d * pm::ppow(1.0f - gclamp(nl, 0.0f, 1.0f), 5.0f), 0.0f, 1.0f))));
 }
 DI f3 diffuse_reflection(const Closure& c, f3 n, f3 v, f3 l) {
   float nl = gmax(dot3(n, l), 0.0f);
@@ -410,12 +410,12 @@ DI float sheen_log(float x, float alpha) {
   float s = 1.0f - alpha;
   float o = s * s;
   return gmix(21.54730033874511719f, 25.32449913024902344f, o) /
-             (1.0f + gmix(3.82986998558044434f, 3.32435011863708496f, o) * powf(fabsf(x), gmix(0.19822999835014343f, 0.16800999641418457f, o))) +
+             (1.0f + gmix(3.82986998558044434f, 3.32435011863708496f, o) * pm::ppow(fabsf(x), gmix(0.19822999835014343f, 0.16800999641418457f, o))) +
          gmix(-1.97759997844696045f, -1.27392995357513428f, o) * x + gmix(-4.32053995132446289f, -4.85967016220092773f, o);
 }
 DI float sheen_lambda(float ct, float alpha) {
-  if (fabsf(ct) >= 0.5f) return expf(2.0f * sheen_log(0.5f, alpha) - sheen_log(1.0f - ct, alpha));
-  return expf(sheen_log(ct, alpha));
+  if (fabsf(ct) >= 0.5f) return pm::pexp(2.0f * sheen_log(0.5f, alpha) - sheen_log(1.0f - ct, alpha));
+  return pm::pexp(sheen_log(ct, alpha));
 }
 DI f3 sheen(const Closure& c, f3 n, f3 v, f3 l) {
   if (max3c(c.sheenColor) <= 0.0f) return sp3(0.0f);
@@ -425,7 +425,7 @@ DI f3 sheen(const Closure& c, f3 n, f3 v, f3 l) {
   float inv = 1.0f / gmax(alpha, E5);
   return c.sheenColor *
          (1.0f / (1.0f + sheen_lambda(nv, alpha) + sheen_lambda(nl, alpha)) *
-          ((2.0f + inv) * powf(fabsf(gmax(1.0f - nh * nh, 0.00100000004749745f)), 0.5f * inv) / 6.28318548202514648f) / gmax(4.0f * nv * nl, E5)) *
+          ((2.0f + inv) * pm::ppow(fabsf(gmax(1.0f - nh * nh, 0.00100000004749745f)), 0.5f * inv) / 6.28318548202514648f) / gmax(4.0f * nv * nl, E5)) *
          nl;
 }
 DI f3 reflected_base(const Closure& c, const Frame3& F, f3 n, f3 v, f3 l) {
@@ -478,7 +478,7 @@ DI float thick_trans_pdf(const Closure& c, const Frame3& F, f3 n, f3 v, f3 l) {
   float vh = dot3(v, h), lhd = dot3(l, h);
   if (fabsf(lh.z) <= E5 || vh * lhd >= 0.0f) return 0.0f;
   float2 a = alpha_xy(c.rough, c.aniso);
-  return D_aniso(a, lh) * G1_aniso(a, lv) * (eta * eta * fabsf(vh) * fabsf(lhd) / gmax(powf(vh + eta * lhd, 2.0f) * nv, E5));
+  return D_aniso(a, lh) * G1_aniso(a, lv) * (eta * eta * fabsf(vh) * fabsf(lhd) / gmax(pm::ppow(vh + eta * lhd, 2.0f) * nv, E5));
 }
 DI float dielectric_trans_pdf(const Closure& c, const Frame3& F, f3 n, f3 v, f3 l) {
   TransCoeffs co = trans_coeffs(c, n, v);
@@ -563,7 +563,9 @@ DI void branchless_onb(f3 n, f3& t, f3& b) {  // orthonormalBasis, KF:2631-2659
 DI f3 cosine_dir(f3 n, float u0, float u1) {
   float r = sqrtf(u0);
   float phi = TWO_PIf * u1;
-  float x = r * cosf(phi), y = r * sinf(phi);
+  float sphi, cphi;
+  pm::psincos(phi, sphi, cphi);
+  float x = r * cphi, y = r * sphi;
   float z = sqrtf(gmax(0.0f, 1.0f - u0));
   f3 t, b;
   branchless_onb(n, t, b);
@@ -572,7 +574,8 @@ DI f3 cosine_dir(f3 n, float u0, float u1) {
 DI f3 ggx_half(const Closure& c, const Frame3& F, float u0, float u1) {
   float2 a = alpha_xy(c.rough, c.aniso);
   float phi = TWO_PIf * u1;
-  float sp = sinf(phi), cp = cosf(phi);
+  float sp, cp;
+  pm::psincos(phi, sp, cp);
   float ct = 1.0f / sqrtf(1.0f + u0 / gmax(1.0f - u0, E6) / gmax(cp * cp / gmax(a.x * a.x, E12) + sp * sp / gmax(a.y * a.y, E12), E6));
   float st = sqrtf(gmax(0.0f, 1.0f - ct * ct));
   return nrm3(F.t * (st * cp) + F.b * (st * sp) + F.n * ct);
@@ -582,7 +585,9 @@ DI f3 ggx_half_iso(f3 n, float rough, float u0, float u1) {
   float phi = TWO_PIf * u1;
   float ct = sqrtf((1.0f - u0) / gmax(1.0f + (a * a - 1.0f) * u0, E6));
   float st = sqrtf(gmax(0.0f, 1.0f - ct * ct));
-  float x = st * cosf(phi), y = st * sinf(phi);
+  float sphi, cphi;
+  pm::psincos(phi, sphi, cphi);
+  float x = st * cphi, y = st * sphi;
   f3 t, b;
   branchless_onb(n, t, b);
   return nrm3(t * x + b * y + n * ct);
@@ -644,7 +649,7 @@ DI Sample sample_dielectric(const Closure& c, const Frame3& F, f3 n, f3 v, float
   }
   float vh = dot3(v, h);
   float rh = dot3(rf, h);
-  float lobe = hp * (tIor * tIor) * fabsf(rh) / gmax(powf(1.0f * vh + tIor * rh, 2.0f), E5);
+  float lobe = hp * (tIor * tIor) * fabsf(rh) / gmax(pm::ppow(1.0f * vh + tIor * rh, 2.0f), E5);
   float g1 = G1_aniso(alpha_xy(c.rough, c.aniso), to_local(F, rf));
   f3 etc = co.tc;
   if (c.irid > E5) {

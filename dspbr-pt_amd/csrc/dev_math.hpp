@@ -7,10 +7,14 @@
 // the implementation are pinned the same way as in the oracle:
 //   normalize(v) = v * (1/sqrt(dot(v,v))), dot = (x*x'+y*y')+z*z',
 //   min/max = the GLSL select forms (NaN-sensitive, unlike fminf/fmaxf),
-//   mix(x,y,a) = x*(1-a) + y*a.
+//   mix(x,y,a) = x*(1-a) + y*a,
+//   sin, cos, exp, log, pow, atan2, acos = the pinned implementations of
+//   pin_math.hpp (the oracle's float build runs the same code).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "pin_math.hpp"
 
 #define DI __host__ __device__ __forceinline__
 #define DD __device__ __forceinline__
@@ -82,9 +86,9 @@ DI f3 nrm3(f3 v) { return v * (1.0f / sqrtf(dot3(v, v))); }
 DI f3 reflect3(f3 i, f3 n) { return i - (2.0f * dot3(n, i)) * n; }
 DI float max3c(f3 v) { return gmax(v.x, gmax(v.y, v.z)); }
 DI f3 vsqrt(f3 v) { return f3{sqrtf(v.x), sqrtf(v.y), sqrtf(v.z)}; }
-DI f3 vexp(f3 v) { return f3{expf(v.x), expf(v.y), expf(v.z)}; }
-DI f3 vlog(f3 v) { return f3{logf(v.x), logf(v.y), logf(v.z)}; }
-DI f3 vcos(f3 v) { return f3{cosf(v.x), cosf(v.y), cosf(v.z)}; }
+DI f3 vexp(f3 v) { return f3{pm::pexp(v.x), pm::pexp(v.y), pm::pexp(v.z)}; }
+DI f3 vlog(f3 v) { return f3{pm::plog(v.x), pm::plog(v.y), pm::plog(v.z)}; }
+DI f3 vcos(f3 v) { return f3{pm::pcos(v.x), pm::pcos(v.y), pm::pcos(v.z)}; }
 DI f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
 DI f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
 

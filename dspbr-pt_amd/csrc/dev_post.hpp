@@ -29,7 +29,7 @@ DI float u2_helper(float x) {  // tonemap.frag:30-31
 
 DI float cineon(float x) {  // tonemap.frag:39-44, per channel after exposure
   x = gmax(0.0f, x - 0.004f);
-  return powf((x * (6.2f * x + 0.5f)) / (x * (6.2f * x + 1.7f) + 0.06f), 2.2f);
+  return pm::ppow((x * (6.2f * x + 0.5f)) / (x * (6.2f * x + 1.7f) + 0.06f), 2.2f);
 }
 
 DI float rrt_odt(float v) {  // tonemap.frag:46-51
@@ -71,7 +71,7 @@ DI f3 tonemap(f3 c, const Params& P) {
   }
   if (P.gamma) {
     const float g = 1.0f / 2.2f;
-    c = f3{powf(c.x, g), powf(c.y, g), powf(c.z, g)};
+    c = f3{pm::ppow(c.x, g), pm::ppow(c.y, g), pm::ppow(c.z, g)};
   }
   return c;
 }

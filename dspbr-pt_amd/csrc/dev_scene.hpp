@@ -594,16 +594,17 @@ DI int lower_bound_guided(const float* __restrict__ row, const int32_t* __restri
 }
 DI f3 rotate_phi(f3 d, float rot, bool inverse) {
   float a = inverse ? -rot : rot;
-  float c = cosf(a), s = sinf(a);
+  float c, s;
+  pm::psincos(a, s, c);
   return f3{c * d.x + 0.0f * d.y + (-s) * d.z, 0.0f * d.x + 1.0f * d.y + 0.0f * d.z, s * d.x + 0.0f * d.y + c * d.z};
 }
 DI float sin_pdf(float theta) {
-  return 1.0f / (2.0f * kPI * kPI * gmax(kEPS_COS, sinf(theta)));
+  return 1.0f / (2.0f * kPI * kPI * gmax(kEPS_COS, pm::psin(theta)));
 }
 // dir_to_uv, SH/utils.glsl:141-147
 DI float2 dir_uv(f3 d, float& theta) {
-  theta = acosf(gmax(-1.0f, gmin(1.0f, d.y)));
-  float ph = atan2f(d.z, d.x);
+  theta = pm::pacos(gmax(-1.0f, gmin(1.0f, d.y)));
+  float ph = pm::patan2(d.z, d.x);
   if (ph < 0.0f) ph = kTWO_PI + ph;
   return make_float2(ph / kTWO_PI, theta / kPI);
 }
@@ -692,8 +693,11 @@ DI f3 ibl_sample(const SceneView& S, const FrameParams& F, float r0, float r1, f
   float spdf = F.ibl_res[0] * F.ibl_res[1] * py * px;
   float u = float(x) / F.ibl_res[0], v = float(y) / F.ibl_res[1];
   float theta = v * kPI, phi = u * kTWO_PI;
-  float pw = sin_pdf(theta);
-  f3 d = f3{sinf(theta) * cosf(phi), cosf(theta), sinf(theta) * sinf(phi)};
+  float st, ct, sp, cp;
+  pm::psincos(theta, st, ct);
+  pm::psincos(phi, sp, cp);
+  float pw = 1.0f / (2.0f * kPI * kPI * gmax(kEPS_COS, st));  // sin_pdf(theta)
+  f3 d = f3{st * cp, ct, st * sp};
   pdf = spdf * pw;
   return rotate_phi(d, F.ibl_rotation, true);
 }

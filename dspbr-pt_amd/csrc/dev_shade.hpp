@@ -80,7 +80,7 @@ struct Shading {
   float attDist;
 };
 
-DI f3 to_linear(float4 t) { return f3{powf(t.x, 2.2f), powf(t.y, 2.2f), powf(t.z, 2.2f)}; }
+DI f3 to_linear(float4 t) { return f3{pm::ppow(t.x, 2.2f), pm::ppow(t.y, 2.2f), pm::ppow(t.z, 2.2f)}; }
 
 // Material texture slots in the order configure_gltf_material reads them.
 enum TexSlot {
@@ -201,11 +201,13 @@ DI Shading configure(const SceneView& S, const Hitpoint& hp, f3 wi) {
   float ax = m9.x, ay = m9.y;
   if (aniso_dir_tex) { ax = aniso_dir.x; ay = aniso_dir.y; }
   else if (aniso_tex) { ax = aniso_rg.x * 2.0f - 1.0f; ay = aniso_rg.y * 2.0f - 1.0f; }
-  float rot = atan2f(ay, ax);
+  float rot = pm::patan2(ay, ax);
   float angle = rot + kPI;
   if (angle > 0.0f) {
     f3 b = cross3(n, t) * hp.tw;
-    t = t * cosf(angle) + b * sinf(angle);
+    float sa, ca;
+    pm::psincos(angle, sa, ca);
+    t = t * ca + b * sa;
   }
   sh.tanA = t;
   return sh;

@@ -80,6 +80,14 @@ struct Allocs {
   }
 };
 
+// Allocations that live for one call: released on every return path.
+struct ScopedAllocs : Allocs {
+  ScopedAllocs() = default;
+  ScopedAllocs(const ScopedAllocs&) = delete;
+  ScopedAllocs& operator=(const ScopedAllocs&) = delete;
+  ~ScopedAllocs() { release(); }
+};
+
 struct SceneRes {
   Allocs mem;
   float4 *nodes = nullptr, *tri = nullptr, *verts = nullptr, *mats = nullptr;
@@ -1919,6 +1927,27 @@ int dspt_build_bvh(const float* positions, int64_t nv, const uint32_t* indices, 
   cache_nodes.clear(); cache_nodes.shrink_to_fit();
   cache_idx.clear(); cache_idx.shrink_to_fit();
   cache_pos = nullptr; cache_nt = -1;
+  return 0;
+}
+
+int dspt_selftest_pin_math(int device, int32_t fn, const float* x, const float* y, float* out, int64_t n) {
+  if (!x || !out || (fn == 4 || fn == 5) && !y) return fail(DSPT_ERR_ARG, "NULL argument");
+  if (fn < 0 || fn > 8) return fail(DSPT_ERR_ARG, "fn must be 0..8");
+  if (n <= 0) return 0;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(DSPT_ERR_DEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(DSPT_ERR_ARG, "device index out of range");
+  HIPCHK(hipSetDevice(device));
+  ScopedAllocs A;
+  float *dx = nullptr, *dy = nullptr, *dout = nullptr;
+  int r;
+  if ((r = A.get(&dx, (size_t)n)) || (r = A.get(&dout, (size_t)n))) return r;
+  if (y && (r = A.get(&dy, (size_t)n))) return r;
+  HIPCHK(hipMemcpy(dx, x, n * 4, hipMemcpyHostToDevice));
+  if (y) HIPCHK(hipMemcpy(dy, y, n * 4, hipMemcpyHostToDevice));
+  k_pin_math<<<grid_for(n, 256), 256>>>(fn, dx, dy, dout, n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost));
   return 0;
 }
 

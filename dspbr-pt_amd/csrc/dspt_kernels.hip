@@ -658,7 +658,7 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
         if (bounce == 0) {
           // primary miss: background, returned unclamped (misptdl.glsl:41-46)
           if (F.bg_from_ibl > 0.0f) { f3 e = ibl_eval(S, F, dir) * pw; out = make_float4(e.x, e.y, e.z, 1.0f); }
-          else out = make_float4(powf(F.bg[0], 2.2f), powf(F.bg[1], 2.2f), powf(F.bg[2], 2.2f), F.bg[3]);
+          else out = make_float4(pm::ppow(F.bg[0], 2.2f), pm::ppow(F.bg[1], 2.2f), pm::ppow(F.bg[2], 2.2f), F.bg[3]);
           if (rec) ps.prim[p] = -1;
         } else {
           n_miss++;
@@ -910,4 +910,27 @@ __global__ __launch_bounds__(256) void k_postprocess(post::Source S, int fxaa_on
   if (out32) out32[i] = c;
   if (out8)
     out8[i] = post::unorm8(c.x) | (post::unorm8(c.y) << 8) | (post::unorm8(c.z) << 16) | (post::unorm8(c.w) << 24);
+}
+
+// --------------------------------------------------------------------------
+// Self-test of the pinned transcendentals (pin_math.hpp) on the device: the
+// same function table as the CPU oracle's float build.
+__global__ __launch_bounds__(256) void k_pin_math(int fn, const float* __restrict__ x, const float* __restrict__ y,
+                                                  float* __restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float r = 0.0f, s, c;
+  switch (fn) {
+    case 0: r = pm::psin(x[i]); break;
+    case 1: r = pm::pcos(x[i]); break;
+    case 2: r = pm::pexp(x[i]); break;
+    case 3: r = pm::plog(x[i]); break;
+    case 4: r = pm::ppow(x[i], y[i]); break;
+    case 5: r = pm::patan2(x[i], y[i]); break;
+    case 6: r = pm::pacos(x[i]); break;
+    case 7: pm::psincos(x[i], s, c); r = s; break;
+    case 8: pm::psincos(x[i], s, c); r = c; break;
+    default: break;
+  }
+  out[i] = r;
 }

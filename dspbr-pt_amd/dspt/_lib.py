@@ -83,6 +83,7 @@ SIGNATURES = {
     "dspt_set_schedule": (_c_int, [_vp, _c_i32, _c_i32, _c_i32, ctypes.c_float]),
     "dspt_get_schedule": (_c_int, [_vp, _i32p, _i32p, _i32p, _fp]),
     "dspt_set_policy": (_c_int, [_vp, _c_i32]),
+    "dspt_selftest_pin_math": (_c_int, [_c_int, _c_i32, _fp, _fp, _fp, _c_i64]),
     "dspt_scene_set_textures": (_c_int, [_vp, _c_i32, _fp, _c_i32, _i32p, _c_i32, ctypes.POINTER(ctypes.c_uint8), _c_i64]),
 }
 SCHED_STREAM = 0
@@ -155,6 +156,16 @@ def build_bvh(positions, indices, max_leaf=4):
     check(L.dspt_build_bvh(ptr(pos), pos.size // 3, ptr(ind, ctypes.c_uint32), ind.size // 3, max_leaf, ptr(nodes),
                            ctypes.byref(nn), ptr(bidx, ctypes.c_int32), ctypes.byref(nm)))
     return nodes, bidx
+
+
+def selftest_pin_math(fn, x, y=None, device=0):
+    """Runs pinned transcendental `fn` (0 sin, 1 cos, 2 exp, 3 log, 4 pow,
+    5 atan2, 6 acos, 7/8 sincos) on the GPU over float32 arrays."""
+    x = f32(x).reshape(-1)
+    yy = None if y is None else f32(y).reshape(-1)
+    out = np.empty_like(x)
+    check(lib().dspt_selftest_pin_math(device, fn, ptr(x), ptr(yy), ptr(out), x.size))
+    return out
 
 
 class Engine:

@@ -320,6 +320,13 @@ int dspt_get_schedule(dspt_ctx* ctx, int32_t* mode, int32_t* pass_frames, int32_
 enum { DSPT_POLICY_RR = 0, DSPT_POLICY_FIFO = 1, DSPT_POLICY_SRPT = 2 };
 int dspt_set_policy(dspt_ctx* ctx, int32_t policy);
 
+/* Engine self-test (no counterpart in the reference): evaluates the pinned
+ * float transcendentals of pin_math.hpp on `device` over n arguments, the
+ * same code the kernels and the CPU oracle's float build run. fn: 0 sin,
+ * 1 cos, 2 exp, 3 log, 4 pow(x, y), 5 atan2(x, y) (x = the y-coordinate),
+ * 6 acos, 7 / 8 the sine / cosine of psincos. y is read for fn 4 and 5. */
+int dspt_selftest_pin_math(int device, int32_t fn, const float* x, const float* y, float* out, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

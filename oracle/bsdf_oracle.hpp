@@ -55,17 +55,17 @@ template <class R> R approxAvgAlbedoGGXMs(R alpha, R e0) {
   return gclamp(gclamp(e0 + (R(1) - e0) * R(0.0476190485060215), R(0), R(1)) * approxAvgAlbedoGGX(alpha), R(0), R(1));
 }
 template <class R> R approxDirAlbedoGGX(R alpha, R cosT) {
-  return gclamp(R(1) - std::sqrt(gmax(alpha, R(0))) * (R(0.5) + R(0.5) * std::pow(R(1) - gclamp(cosT, R(0), R(1)), R(5))), R(0), R(1));
+  return gclamp(R(1) - std::sqrt(gmax(alpha, R(0))) * (R(0.5) + R(0.5) * opow(R(1) - gclamp(cosT, R(0), R(1)), R(5))), R(0), R(1));
 }
 template <class R> R approxDirAlbedoGGXMs(R theta, R alpha, R e0) {
   R noV = gclamp(theta, R(0), R(1));
-  return gclamp(gclamp(e0 + (R(1) - e0) * std::pow(R(1) - noV, R(5)), R(0), R(1)) * gmix(R(1), approxDirAlbedoGGX(alpha, noV), R(0.64999997615814209)), R(0), R(1));
+  return gclamp(gclamp(e0 + (R(1) - e0) * opow(R(1) - noV, R(5)), R(0), R(1)) * gmix(R(1), approxDirAlbedoGGX(alpha, noV), R(0.64999997615814209)), R(0), R(1));
 }
 
 // ---- closure build, KF:209-261 buildEnterpriseSurfaceFromGltf + KF:357-422 buildSurfaceClosure
 template <class R> R iridescenceInterfaceWeight(R irid, R filmIor) {  // KF:307
   R s = gmax(filmIor, R(1));
-  return gclamp(irid, R(0), R(1)) * std::pow((s - R(1)) / gmax(s + R(1), F1E5), R(2));
+  return gclamp(irid, R(0), R(1)) * opow((s - R(1)) / gmax(s + R(1), F1E5), R(2));
 }
 template <class R> Closure<R> buildClosure(const PbrMat<R>& m, V3<R> anisotropyTangent, int profile) {
   const V3<R> Z(R(0)), O(R(1));
@@ -234,8 +234,8 @@ template <class R> V3<R> thinFilmSensitivity(R opd, V3<R> shift) {
   V3<R> xyz = V3<R>(R(5.48560003844900113e-13), R(4.42010008238832852e-13), R(5.24810013155518895e-13)) *
               vsqrt(R(6.28318548202514648) * var) *
               vcos(V3<R>(R(1.681e+06), R(1.7953e+06), R(2.2084e+06)) * phase + shift) * vexp(-phase2 * var);
-  xyz.x = xyz.x + R(9.74699989231449238e-14) * std::sqrt(R(2.845152256e+10)) * std::cos(R(2.2399e+06) * phase + shift.x) *
-                      std::exp(R(-4.528200192e+09) * phase2);
+  xyz.x = xyz.x + R(9.74699989231449238e-14) * std::sqrt(R(2.845152256e+10)) * ocos(R(2.2399e+06) * phase + shift.x) *
+                      oexp(R(-4.528200192e+09) * phase2);
   V3<R> s = xyz / R(1.06850002623559703e-07);
   R X = s.x, Y = s.y, Z = s.z;
   return V3<R>(R(3.24045419692993164) * X - R(1.53713846206665039) * Y - R(0.49853140115737915) * Z,
@@ -347,7 +347,7 @@ V3<R> evalThickTransmission(const Closure<R>& c, V3<R> tc, V3<R> n, V3<R> v, V3<
   R d = ggxDAniso(a, lh);
   R g = ggxG2Aniso(a, lv, ll);
   R avoh = std::fabs(voh);
-  R factor = eta * eta * avoh * std::fabs(loh) / gmax(std::pow(voh + eta * loh, R(2)) * noV, F1E5);
+  R factor = eta * eta * avoh * std::fabs(loh) / gmax(opow(voh + eta * loh, R(2)) * noV, F1E5);
   V3<R> rr = fresnelSchlickDielectric(avoh, c.dielectricSpecularColor, c.dielectricSpecularF90, iorI, iorO, false);
   V3<R> etc = (c.iridescence > F1E5 && iorI <= R(1.00010001659393311))
                   ? (V3<R>(R(1)) - thinFilmDielectricFresnel(c, avoh, rr, R(1))) * weightedTransmissionColor(c)
@@ -412,8 +412,8 @@ template <class R> R transmissiveDiffuseCompFactor(const Closure<R>& c, R noV, R
   R f0 = max3(sc);
   R d = max3(sf) - f0;
   return coupled * gmix(R(1), R(1) / gmax(max3(closureDiffuseEnergy(c)), F1E4),
-                        std::sqrt(R(0.5) * (gclamp(f0 + d * std::pow(R(1) - gclamp(noV, R(0), R(1)), R(5)), R(0), R(1)) +
-                                            gclamp(f0 + d * std::pow(R(1) - gclamp(noL, R(0), R(1)), R(5)), R(0), R(1)))));
+                        std::sqrt(R(0.5) * (gclamp(f0 + d * opow(R(1) - gclamp(noV, R(0), R(1)), R(5)), R(0), R(1)) +
+                                            gclamp(f0 + d * opow(R(1) - gclamp(noL, R(0), R(1)), R(5)), R(0), R(1)))));
 }
 template <class R> V3<R> evalDiffuseReflection(const Closure<R>& c, V3<R> n, V3<R> v, V3<R> l) {
   R noL = gmax(dot(n, l), R(0));
@@ -477,14 +477,14 @@ template <class R> R sheenLogApprox(R x, R alpha) {
   R o = s * s;
   return gmix(R(21.54730033874511719), R(25.32449913024902344), o) /
              (R(1) + gmix(R(3.82986998558044434), R(3.32435011863708496), o) *
-                         std::pow(std::fabs(x), gmix(R(0.19822999835014343), R(0.16800999641418457), o))) +
+                         opow(std::fabs(x), gmix(R(0.19822999835014343), R(0.16800999641418457), o))) +
          gmix(R(-1.97759997844696045), R(-1.27392995357513428), o) * x +
          gmix(R(-4.32053995132446289), R(-4.85967016220092773), o);
 }
 template <class R> R lambdaSheen(R cosT, R alpha) {
   if (std::fabs(cosT) >= R(0.5))
-    return std::exp(R(2) * sheenLogApprox(R(0.5), alpha) - sheenLogApprox(R(1) - cosT, alpha));
-  return std::exp(sheenLogApprox(cosT, alpha));
+    return oexp(R(2) * sheenLogApprox(R(0.5), alpha) - sheenLogApprox(R(1) - cosT, alpha));
+  return oexp(sheenLogApprox(cosT, alpha));
 }
 template <class R> V3<R> evalSheen(const Closure<R>& c, V3<R> n, V3<R> v, V3<R> l) {
   if (max3(c.sheenColor) <= R(0)) return V3<R>(R(0));
@@ -494,7 +494,7 @@ template <class R> V3<R> evalSheen(const Closure<R>& c, V3<R> n, V3<R> v, V3<R> 
   R inv = R(1) / gmax(alpha, F1E5);
   return c.sheenColor *
          (R(1) / (R(1) + lambdaSheen(noV, alpha) + lambdaSheen(noL, alpha)) *
-          ((R(2) + inv) * std::pow(std::fabs(gmax(R(1) - noH * noH, R(0.00100000004749745))), R(0.5) * inv) / R(6.28318548202514648)) /
+          ((R(2) + inv) * opow(std::fabs(gmax(R(1) - noH * noH, R(0.00100000004749745))), R(0.5) * inv) / R(6.28318548202514648)) /
           gmax(R(4) * noV * noL, F1E5)) *
          noL;
 }
@@ -571,7 +571,7 @@ template <class R> R thickTransmissionPdf(const Closure<R>& c, V3<R> n, V3<R> v,
   if (std::fabs(lh.z) <= F1E5 || voh * loh >= R(0)) return R(0);
   V2<R> a = anisoAlpha(c.roughness, c.anisotropy);
   return ggxDAniso(a, lh) * ggxG1Aniso(a, lv) *
-         (eta * eta * std::fabs(voh) * std::fabs(loh) / gmax(std::pow(voh + eta * loh, R(2)) * noV, F1E5));
+         (eta * eta * std::fabs(voh) * std::fabs(loh) / gmax(opow(voh + eta * loh, R(2)) * noV, F1E5));
 }
 template <class R> R pdfDielectricTransmission(const Closure<R>& c, R curIor, V3<R> n, V3<R> v, V3<R> l) {
   bool inside; R iorI, iorO;
@@ -658,7 +658,7 @@ template <class R> void orthonormalBasis(V3<R> n, V3<R>& t, V3<R>& b) {
 template <class R> V3<R> sampleCosineHemisphere(V3<R> n, R u0, R u1) {
   R r = std::sqrt(u0);
   R phi = R(6.28318548202514648) * u1;
-  R x = r * std::cos(phi), y = r * std::sin(phi);
+  R x = r * ocos(phi), y = r * osin(phi);
   R z = std::sqrt(gmax(R(0), R(1) - u0));
   V3<R> t, b;
   orthonormalBasis(n, t, b);
@@ -691,7 +691,7 @@ template <class R> void anisotropicBasis(V3<R> n, V3<R> ti, V3<R>& t, V3<R>& b) 
 template <class R> V3<R> sampleGGXHalf(const Closure<R>& c, V3<R> n, R rough, R u0, R u1) {
   V2<R> a = anisoAlpha(rough, c.anisotropy);
   R phi = R(6.28318548202514648) * u1;
-  R sp = std::sin(phi), cp = std::cos(phi);
+  R sp = osin(phi), cp = ocos(phi);
   R cosT = R(1) / std::sqrt(R(1) + u0 / gmax(R(1) - u0, R(9.99999997475242708e-07)) /
                                       gmax(cp * cp / gmax(a.x * a.x, R(9.999999960041972e-13)) + sp * sp / gmax(a.y * a.y, R(9.999999960041972e-13)),
                                            R(9.99999997475242708e-07)));
@@ -774,7 +774,7 @@ BSample<R> sampleDielectricTransmission(const Closure<R>& c, R cur, V3<R> tn, V3
   }
   R voh = dot(v, h);
   R roh = dot(refr, h);
-  R lobe = halfPdf * (sO * sO) * std::fabs(roh) / gmax(std::pow(sI * voh + sO * roh, R(2)), F1E5);
+  R lobe = halfPdf * (sO * sO) * std::fabs(roh) / gmax(opow(sI * voh + sO * roh, R(2)), F1E5);
   R g1 = ggxG1Aniso(anisoAlpha(c.roughness, c.anisotropy), projectAniso(n, c.anisotropyTangent, refr));
   R avoh = std::fabs(voh);
   V3<R> rr = fresnelSchlickDielectric(avoh, c.dielectricSpecularColor, c.dielectricSpecularF90, sI, sO, false);
@@ -800,7 +800,7 @@ template <class R> V3<R> sampleGGXHalfIso(V3<R> n, R rough, R u0, R u1) {
   R phi = R(6.28318548202514648) * u1;
   R cosT = std::sqrt((R(1) - u0) / gmax(R(1) + (a * a - R(1)) * u0, R(9.99999997475242708e-07)));
   R sinT = std::sqrt(gmax(R(0), R(1) - cosT * cosT));
-  R x = sinT * std::cos(phi), y = sinT * std::sin(phi);
+  R x = sinT * ocos(phi), y = sinT * osin(phi);
   V3<R> t, b;
   orthonormalBasis(n, t, b);
   return normalize(t * x + b * y + n * cosT);

@@ -134,13 +134,13 @@ template <class R> MClosure<R> configure_material(const Scene& s, uint32_t matId
   } else if (md.texIds[12] >= 0.0f) {
     adir = V3<R>(anis.x * R(2) - R(1), anis.y * R(2) - R(1), R(0));
   }
-  m.anisotropyRotation = std::atan2(adir.y, adir.x);
+  m.anisotropyRotation = oatan2(adir.y, adir.x);
   // rotation_to_tangent(angle, n, t), SH/utils.glsl:183-190
   R angle = m.anisotropyRotation + K<R>::PI;
   if (angle > R(0)) {
     V3<R> gt = c.t;
     V3<R> gb = cross(c.n, c.t) * c.tw;
-    c.t = gt * std::cos(angle) + gb * std::sin(angle);
+    c.t = gt * ocos(angle) + gb * osin(angle);
   }
   c.anisotropyTangent = c.t;
   m.emissive = V3<R>(R(md.emissive[0]), R(md.emissive[1]), R(md.emissive[2])) * vpow(xyz4(gtv(s, md.texIds[3], rs)), E22);
@@ -291,7 +291,7 @@ PathOut<R> trace_path(const Scene& s, const Frame& f, int W, int H, int px, int 
     out.sig = sig.h;
     if (f.bg_from_ibl > 0.0f) { out.L = ibl_eval(s, f, ray.dir) * pw; out.alpha = R(1); }
     else {
-      out.L = V3<R>(std::pow(R(f.bg[0]), R(2.2f)), std::pow(R(f.bg[1]), R(2.2f)), std::pow(R(f.bg[2]), R(2.2f)));
+      out.L = V3<R>(opow(R(f.bg[0]), R(2.2f)), opow(R(f.bg[1]), R(2.2f)), opow(R(f.bg[2]), R(2.2f)));
       out.alpha = R(f.bg[3]);
     }
     return out;

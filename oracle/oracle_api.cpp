@@ -287,4 +287,26 @@ void oracle_postprocess(const float* accum, int W, int H, int mode, float exposu
     }
 }
 
+// The pinned float transcendentals (pin_math.hpp) over n arguments:
+// fn 0 sin, 1 cos, 2 exp, 3 log, 4 pow(x, y), 5 atan2(x = y-argument, y = x-argument), 6 acos,
+// 7 sin and 8 cos through psincos.
+int oracle_pin_math(int fn, const float* x, const float* y, float* out, int64_t n) {
+  for (int64_t i = 0; i < n; i++) {
+    float s, c;
+    switch (fn) {
+      case 0: out[i] = pm::psin(x[i]); break;
+      case 1: out[i] = pm::pcos(x[i]); break;
+      case 2: out[i] = pm::pexp(x[i]); break;
+      case 3: out[i] = pm::plog(x[i]); break;
+      case 4: out[i] = pm::ppow(x[i], y[i]); break;
+      case 5: out[i] = pm::patan2(x[i], y[i]); break;
+      case 6: out[i] = pm::pacos(x[i]); break;
+      case 7: pm::psincos(x[i], s, c); out[i] = s; break;
+      case 8: pm::psincos(x[i], s, c); out[i] = c; break;
+      default: return -1;
+    }
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -21,6 +21,8 @@
 #include <cmath>
 #include <cstdint>
 
+#include "../dspbr-pt_amd/csrc/pin_math.hpp"
+
 namespace orc_post {
 
 struct C3 { float x, y, z; };
@@ -51,7 +53,7 @@ static inline C3 uncharted2_tm(C3 c, float e) {
 // tonemap.frag:39-44 (Hejl / Burgess-Dawson)
 static inline float cineon1(float x) {
   x = gmax(0.0f, x - 0.004f);
-  return powf((x * (6.2f * x + 0.5f)) / (x * (6.2f * x + 1.7f) + 0.06f), 2.2f);
+  return pm::ppow((x * (6.2f * x + 0.5f)) / (x * (6.2f * x + 1.7f) + 0.06f), 2.2f);
 }
 static inline C3 cineon_tm(C3 c, float e) {
   c = mul(c, e);
@@ -86,7 +88,7 @@ static inline void tonemap(const float* px, int mode, float exposure, int gamma,
   if (mode == 4) c = uncharted2_tm(c, exposure);
   if (gamma) {
     const float g = 1.0f / 2.2f;
-    c = {powf(c.x, g), powf(c.y, g), powf(c.z, g)};
+    c = {pm::ppow(c.x, g), pm::ppow(c.y, g), pm::ppow(c.z, g)};
   }
   out[0] = c.x; out[1] = c.y; out[2] = c.z; out[3] = px[3];
 }

@@ -41,10 +41,12 @@ def lib():
     global _lib
     if _lib is None:
         srcs = [os.path.join(ORACLE_DIR, f) for f in os.listdir(ORACLE_DIR) if f.endswith((".hpp", ".cpp"))]
+        srcs.append(os.path.join(ROOT, "dspbr-pt_amd", "csrc", "pin_math.hpp"))
         if not os.path.exists(ORACLE_LIB) or max(os.path.getmtime(f) for f in srcs) > os.path.getmtime(ORACLE_LIB):
             build_oracle()
         L = ctypes.CDLL(ORACLE_LIB)
         L.oracle_ibl_cdf.restype = ctypes.c_double
+        L.oracle_pin_math.argtypes = [ctypes.c_int, _f, _f, _f, ctypes.c_int64]
         L.oracle_ibl_cdf.argtypes = [_f, ctypes.c_int, ctypes.c_int, _f, _f, _f, _f]
         L.oracle_rng_seq.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _u32, _u32, _f, _u32]
         L.oracle_rng_step.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _u32]
@@ -234,3 +236,18 @@ def postprocess(accum, mode, exposure=1.0, gamma=True, fxaa=False, out_size=None
     lib().oracle_postprocess(_p(a), W, H, int(mode), float(exposure), int(bool(gamma)), int(bool(fxaa)), ow, oh, _p(f),
                              _p(u8, ctypes.c_uint8))
     return f, u8
+
+
+PIN_FNS = {"sin": 0, "cos": 1, "exp": 2, "log": 3, "pow": 4, "atan2": 5, "acos": 6, "sincos_s": 7, "sincos_c": 8}
+
+
+def pin_math(name, x, y=None):
+    """The oracle's pinned float transcendental `name` over float32 arrays
+    (atan2 takes (y-coordinate, x-coordinate) as (x, y))."""
+    x = np.ascontiguousarray(x, np.float32)
+    yy = None if y is None else np.ascontiguousarray(y, np.float32)
+    out = np.empty_like(x)
+    r = lib().oracle_pin_math(PIN_FNS[name], x.ctypes.data_as(_f), None if yy is None else yy.ctypes.data_as(_f),
+                              out.ctypes.data_as(_f), x.size)
+    assert r == 0
+    return out

@@ -99,14 +99,9 @@ def test_files_render_like_the_oracle(tmp_path):
     dims, texels = s.getTextureArrays()
     osc.set_textures(s.getFlatTextureInfoBuffer(), dims, texels)
     o = osc.render(blk, W, H, 1, F, threads=8)
-    np.testing.assert_array_equal(g_prim, o["prim"])
     assert (g_prim >= 0).mean() > 0.5
-    same = g_sig == o["sigs"]
-    assert same.mean() >= 0.98, f"branch agreement {same.mean():.5f}"
-    a = g_s[same][:, :3].astype(np.float64)
-    b = o["samples"][same][:, :3].astype(np.float64)
-    err = (np.abs(a - b) / np.maximum(np.abs(b), 1e-3)).max(axis=1)
-    assert (err <= 1e-4).mean() >= 0.99 and (err <= 1e-3).mean() >= 0.999, f"worst {err.max():.3e}"
+    from test_gpu_parity import _check_samples
+    _check_samples(g_s, g_sig, g_prim, o)
     # display chain -> PNG
     from dspt.image import read_png
     r.tonemapping = "AcesFilm"

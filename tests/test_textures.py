@@ -127,13 +127,8 @@ def test_textured_render_matches_oracle():
                         wl.ibl, 1)
     osc.set_textures(s.getFlatTextureInfoBuffer(), dims, texels)
     o = osc.render(blk, W, H, 1, F, threads=8)
-    np.testing.assert_array_equal(g_prim, o["prim"])
-    same = g_sig == o["sigs"]
-    assert same.mean() >= 0.98, f"branch agreement {same.mean():.5f}"
-    a = g_s[same][:, :3].astype(np.float64)
-    b = o["samples"][same][:, :3].astype(np.float64)
-    err = (np.abs(a - b) / np.maximum(np.abs(b), 1e-3)).max(axis=1)
-    assert (err <= 1e-4).mean() >= 0.99 and (err <= 1e-3).mean() >= 0.999, f"worst {err.max():.3e}"
+    from test_gpu_parity import _check_samples
+    _check_samples(g_s, g_sig, g_prim, o)
     # the textures change the image: the same scene without them is refused
     eng2 = Engine(PROFILE_FULL)
     _upload(eng2, wl)
