@@ -28,6 +28,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -38,6 +39,16 @@ int dspt_bvh_build_impl(const float* pos, int64_t nv, const uint32_t* ind, int64
                         std::vector<float>& out_nodes, std::vector<int32_t>& out_idx, std::string& err);
 int dspt_bvh_build_boxes_impl(const float* boxes, int64_t n, int max_leaf, std::vector<float>& out_nodes,
                               std::vector<int32_t>& out_idx, std::string& err);
+
+// Scratch memory of the allocation-free ray query (dspt_trace_closest_noalloc):
+// the traversal-stack spill (kStackSize - kLdsStack ints per lane) of at most
+// kNoAllocBlocks blocks of kTraceBlock lanes, 24 MiB. Every GPU the library
+// runs on holds its own copy until the process ends. Host code never uses the
+// array's name as an address: dspt_create looks up each device's address
+// with hipGetSymbolAddress and the entry point passes that to the kernel.
+constexpr int kNoAllocBlocks = 512;
+constexpr size_t kNoAllocScratchInts = (size_t)kNoAllocBlocks * kTraceBlock * (kStackSize - kLdsStack);
+static __device__ __align__(256) int scratch[kNoAllocScratchInts];
 
 namespace {
 
@@ -148,6 +159,24 @@ struct Readback {
   int32_t prog[kMaxJobs];
 };
 
+// Device address of `scratch` per device, looked up once per device with that
+// device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(int device, int** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(DSPT_ERR_UNSUPPORTED, "device index above the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = (int*)p;
+  }
+  *out = g_scratch_ptr[device];
+  return 0;
+}
+
 }  // namespace
 
 struct dspt_ctx {
@@ -220,6 +249,7 @@ struct dspt_ctx {
   std::array<int64_t, 6> graph_key{};  // (pool, pool address, grids, chunk length, ray order) of the graphs
   // ray order of the extension queue (allocated with the pool)
   int ray_order = DSPT_ORDER_SHADE;
+  int* scratch = nullptr;  // this device's address of the no-alloc scratch array
   uint16_t* order_keys = nullptr;
   int32_t* order_perm = nullptr;
   uint32_t* order_bins = nullptr;
@@ -1237,6 +1267,10 @@ int dspt_create(int profile, int device, dspt_ctx** out) {
     dspt_destroy(c);
     return fail(DSPT_ERR_MEMORY, "context allocation failed");
   }
+  if (int r = scratch_address(device, &c->scratch)) {  // hipSetDevice(device) is current
+    dspt_destroy(c);
+    return r;
+  }
   *out = c;
   return 0;
 }
@@ -1891,6 +1925,30 @@ int dspt_trace_closest_instanced(dspt_ctx* c, const float* rays, int64_t n, int3
                                  float* v) {
   if (!inst || !t || !u || !v) return fail(DSPT_ERR_ARG, "NULL output");
   return trace_common(c, rays, n, 0, tri, t, u, v, inst);
+}
+
+int dspt_trace_closest_noalloc(dspt_ctx* c, const float* d_rays, int64_t n, int32_t* d_tri, float* d_t, float* d_u,
+                               float* d_v, int32_t blocks) {
+  if (!c || !d_rays || !d_tri || !d_t || !d_u || !d_v) return fail(DSPT_ERR_ARG, "NULL argument");
+  if (c->scenes.empty() || !c->scenes[0]) return fail(DSPT_ERR_STATE, "trace before dspt_upload_scene");
+  if (n < 0) return fail(DSPT_ERR_ARG, "negative ray count");
+  if (blocks <= 0) blocks = kNoAllocBlocks;
+  if (blocks > kNoAllocBlocks)
+    return fail(DSPT_ERR_ARG, "blocks " + std::to_string(blocks) + " exceeds the no-alloc scratch (" +
+                                  std::to_string(kNoAllocBlocks) + " blocks of " + std::to_string(kTraceBlock) +
+                                  " lanes, " + std::to_string(kNoAllocScratchInts * 4 >> 20) + " MiB)");
+  if (!c->scratch) return fail(DSPT_ERR_STATE, "no-alloc scratch address unknown");
+  if (n == 0) return 0;
+  HIPCHK(hipSetDevice(c->device));
+  JobRes tmp;
+  tmp.scene = 0;
+  SceneView S = view_of(c, &tmp);
+  blocks = (int)std::min<int64_t>(blocks, grid_for(n, kTraceBlock));
+  // the spill is written before it is read (stack discipline): nothing to clear
+  k_trace_rays<<<blocks, kTraceBlock, 0, c->stream>>>(S, (const float4*)d_rays, n, 0, d_tri, d_t, d_u, d_v, nullptr,
+                                                      c->scratch, c->cnt);
+  HIPCHK(hipGetLastError());
+  return 0;
 }
 
 int dspt_trace_occluded(dspt_ctx* c, const float* rays, int64_t n, int32_t* occluded) {

@@ -50,6 +50,7 @@ SIGNATURES = {
     "dspt_reset_counters": (_c_int, [_vp]),
     "dspt_trace_closest": (_c_int, [_vp, _fp, _c_i64, _i32p, _fp, _fp, _fp]),
     "dspt_trace_occluded": (_c_int, [_vp, _fp, _c_i64, _i32p]),
+    "dspt_trace_closest_noalloc": (_c_int, [_vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _c_i32]),
     "dspt_build_bvh": (_c_int, [_fp, _c_i64, _u32p, _c_i64, _c_i32, _fp, _i64p, _i32p, _i64p]),
     "dspt_memory_usage": (_c_int, [_vp, _i64p]),
     "dspt_set_kernel_timing": (_c_int, [_vp, _c_int]),

@@ -183,6 +183,21 @@ int dspt_job_postprocess(dspt_ctx* ctx, int32_t job_id, int32_t tonemap_mode, fl
 int dspt_trace_closest(dspt_ctx* ctx, const float* rays, int64_t n, int32_t* tri, float* t, float* u, float* v);
 int dspt_trace_occluded(dspt_ctx* ctx, const float* rays, int64_t n, int32_t* occluded);
 
+/* Allocation-free closest-hit query. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * results as dspt_trace_closest, but every buffer is caller-owned DEVICE
+ * memory: d_rays f32[8n] ({org.xyz, tfar, dir.xyz, pad}), outputs i32[n] /
+ * f32[n]. The call enqueues one kernel on the context's stream and returns;
+ * the inputs must be ready before the call and dspt_synchronize() waits for
+ * the outputs. It allocates nothing: the traversal-stack spill lives in a
+ * fixed per-device scratch array of 24 MiB (512 blocks of 256 lanes), held
+ * until the process ends. blocks = grid size (<= 0: 512); more than 512 is
+ * rejected with DSPT_ERR_ARG. Calls that overlap on one GPU (from several
+ * contexts or threads) share that scratch and race. */
+int dspt_trace_closest_noalloc(dspt_ctx* ctx, const float* d_rays, int64_t n, int32_t* d_tri, float* d_t, float* d_u,
+                               float* d_v, int32_t blocks);
+
 /* Engine-side BVH builder emitting the TinyBVH bvh_gpu layout consumed by
  * SH/bvh.glsl:1-6 (replaces the WASM call of RT:880-882 /
  * tinybvh_builder.ts:142-215). positions f32[3*V], indices u32[3*T].
