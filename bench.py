@@ -9,9 +9,21 @@ IBL with a sun disc, rendered at 1920x1080, 64 spp, max depth 12, webgl-full
 profile. One step = one complete 64-spp render of that frame (a job).
 
 Prints one JSON line: Mpaths/s (value), Mrays/s, p50/p99 job latency, the
-roofline of the dominant kernel (closest-hit traversal, HBM-bound, from HIP
-events on the engine stream) and the CPU baseline (the oracle on the host
-cores over a bounded sample of the same workload). Two more objects:
+roofline of the dominant kernel and the CPU baseline (the oracle on the host
+cores over a bounded sample of the same workload).
+
+The timed steps run in the engine's shipping mode (bounce loop in captured
+HIP graphs, no per-launch events). A second pass of the same steps with
+per-launch HIP events on the engine stream gives each kernel's average
+launch duration. The roofline of the closest-hit traversal (k_trace_ext,
+HBM-bound) divides its measured memory traffic per launch by that
+duration: the traffic comes from two child runs of this script under
+rocprofv3 (--pmc FETCH_SIZE, then --pmc WRITE_SIZE), each scaled by what the
+same counter reads for a plain copy kernel of known size in the same run.
+The byte model of the traversal (64 B per node visit, 64 B per triangle
+test) is reported beside it as bytes served (caches included).
+
+More objects:
   batched  the uneven 32-scene mix (10k-2M triangles, 256^2-1024^2,
            16-256 spp, 1024x512 IBL each) rendered through the shared path
            pool vs one job at a time in frame-synchronous passes of a full
@@ -26,10 +38,15 @@ cores over a bounded sample of the same workload). Two more objects:
            and as one long job beside small-job traffic.
 """
 import argparse
+import csv
+import glob
 import json
 import math
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -38,12 +55,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dspbr-pt_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# Memory traffic of one k_trace_ext launch on the default workload (1080p,
-# 64 spp, full-material scene, 64 Mi-path pool), from rocprofv3 FETCH_SIZE +
-# WRITE_SIZE passes scaled by a plain-copy calibration
-# (profiles/k_trace_ext_counters_*.md).
-TRACE_EXT_TRAFFIC_BYTES = 14.81e9
-TRACE_EXT_TRAFFIC_POOL = 64 << 20  # the pool size that figure was measured at
+COPY_CALIB_BYTES = 1 << 30  # the plain copy that calibrates the memory counters
+PMC_KERNELS = ("k_trace_ext", "k_shade", "k_trace_shadow")
 DEFAULT_MAX_PATHS = 64 << 20
 METRIC = "Mpaths/s + Mrays/s at 1080p; p50/p99 job latency, batched vs plain, 1 GPU"
 
@@ -83,7 +96,7 @@ def cpu_baseline(wl, block, budget_s=20.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ref as O
     s = wl.scene
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     osc = O.OracleScene(s.vertexBuffer, s.triangleIndexBuffer, s.bvhNodes, s.bvhTriangleIndices, s.getFlatMaterialBuffer(),
                         wl.ibl, 1 if wl.profile == "webgl-full" else 0)
     W, H = wl.width, wl.height
@@ -105,7 +118,20 @@ def cpu_baseline(wl, block, budget_s=20.0):
             break
     return {"value": paths / spent / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
             "sample": f"oracle FP32 restatement over centred full-width row bands (doubling from 8 rows) then "
-                      f"{frames_full} whole frames of the same {W}x{H} image, {paths} paths in {spent:.1f}s"}
+                      f"{frames_full} whole frames of the same {W}x{H} image, {paths} paths in {spent:.1f}s, "
+                      f"{threads} threads = every core this process may use (affinity "
+                      f"{len(os.sched_getaffinity(0))}, os.cpu_count {os.cpu_count()}, "
+                      f"OMP_NUM_THREADS {os.environ.get('OMP_NUM_THREADS')})"}
+
+
+def host_threads():
+    """The host cores this process may use: the CPU share the machine grants
+    it (OMP_NUM_THREADS where the machine sets it, else the affinity mask)."""
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        return min(int(omp), aff)
+    return aff
 
 
 def batched_vs_plain(args):
@@ -150,6 +176,7 @@ def batched_vs_plain(args):
         eng.synchronize()
         el = time.perf_counter() - t_start
         run.kms = {k: round(v["total_ms"], 1) for k, v in eng.kernel_stats().items()}
+        run.cnt = eng.counters()
         eng.set_kernel_timing(False)
         lat = np.array(sorted(done_at.values()))
         return el, lat
@@ -169,6 +196,22 @@ def batched_vs_plain(args):
                     live.discard(j)
         run.ids = ids
 
+    def single(done_at, t_start):
+        # the caller's plain path: one job at a time through the single-job
+        # API (each job alone in the streaming pool, as dspt_render renders)
+        ids = []
+        eng.set_schedule(SCHED_STREAM, 0, best["chunk_iters"], 2.0)
+        for s in specs:
+            t0 = time.perf_counter()
+            j = b.create_job(s)
+            eng.jobs_run([j])
+            eng.synchronize()
+            done_at[j] = time.perf_counter() - t_start
+            single.alone[j] = (s.paths, time.perf_counter() - t0)
+            ids.append(j)
+        run.ids = ids
+    single.alone = {}
+
     def plain(done_at, t_start, pass_frames=0):
         ids = []
         eng.set_schedule(SCHED_PASS, pass_frames)
@@ -182,7 +225,7 @@ def batched_vs_plain(args):
         run.ids = ids
 
     el_b, lat_b = run(batched)
-    ids_b, kms_b = run.ids, run.kms
+    ids_b, kms_b, cnt_b = run.ids, run.kms, run.cnt
     acc_b = eng.job_read_accum(ids_b[0])
     for j in ids_b:
         eng.job_destroy(j)
@@ -195,6 +238,12 @@ def batched_vs_plain(args):
     ids_p, kms_p = run.ids, run.kms
     acc_p = eng.job_read_accum(ids_p[0])
     for j in ids_p:
+        eng.job_destroy(j)
+    el_s, lat_s = run(single)
+    kms_s = run.kms
+    acc_s = eng.job_read_accum(run.ids[0])
+    alone = [single.alone[j] for j in run.ids]
+    for j in run.ids:
         eng.job_destroy(j)
     # the reference's loop structure: one job at a time, one frame per pass
     el_f, lat_f = run(plain, pass_frames=1)
@@ -219,13 +268,30 @@ def batched_vs_plain(args):
         "kernel_ms": {"batched": kms_b, "round_robin": kms_r, "plain": kms_p},
         "plain_mode": "one job at a time, frame-synchronous passes of up to a full pool",
         "frame_loop_mode": "one job at a time, one frame per pass (the reference's render loop)",
-        "identical_accumulation": bool(np.array_equal(acc_b, acc_p) and np.array_equal(acc_b, acc_f)),
+        "identical_accumulation": bool(np.array_equal(acc_b, acc_p) and np.array_equal(acc_b, acc_f)
+                                       and np.array_equal(acc_b, acc_s)),
+        "single_job_api": {"mpaths_per_s": round(total / el_s / 1e6, 2),
+                           "latency_ms": {"p50": q(lat_s, 50), "p99": q(lat_s, 99)}, "kernel_ms": kms_s,
+                           "mode": "one job at a time, each alone in the streaming pool (the dspt_render path)",
+                           "jobs_below_one_pool": sum(1 for p_, _ in alone if p_ < best["max_paths"]),
+                           "work_share_below_one_pool": round(sum(p_ for p_, _ in alone if p_ < best["max_paths"])
+                                                              / total, 4),
+                           "time_share_below_one_pool": round(sum(t_ for p_, t_ in alone if p_ < best["max_paths"])
+                                                              / sum(t_ for _, t_ in alone), 4)},
+        "speedup_vs_single_job_api": round(el_s / el_b, 3),
+        "work_per_path": {"rays": round((cnt_b["rays_primary"] + cnt_b["rays_extension"] + cnt_b["rays_shadow"])
+                                        / total, 3),
+                          "node_visits_per_ray": round(cnt_b["node_visits"] / max(1, cnt_b["rays_primary"]
+                                                       + cnt_b["rays_extension"] + cnt_b["rays_shadow"]), 2),
+                          "tri_tests_per_ray": round(cnt_b["tri_tests"] / max(1, cnt_b["rays_primary"]
+                                                     + cnt_b["rays_extension"] + cnt_b["rays_shadow"]), 2)},
         "autotuned": best,
         "frame_loop": {"mpaths_per_s": round(total / el_f / 1e6, 2),
                        "latency_ms": {"p50": q(lat_f, 50), "p99": q(lat_f, 99)},
                        "speedup_batched": round(el_f / el_b, 3), "kernel_ms": kms_f},
     }
-    log(f"[bench] batched {res['batched_mpaths_per_s']} vs plain {res['plain_mpaths_per_s']} Mpaths/s")
+    log(f"[bench] batched {res['batched_mpaths_per_s']} vs plain {res['plain_mpaths_per_s']} vs single-job API "
+        f"{res['single_job_api']['mpaths_per_s']} Mpaths/s")
     return res
 
 
@@ -389,17 +455,104 @@ def bvh_builds(wl, spp):
     return res
 
 
-def calibrate_copy(mib):
-    """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
-    per dispatch give the scale between the counters and real HBM bytes."""
-    import torch
-    n = mib << 20
-    a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(5):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"calibration": "copy", "bytes_read": n, "bytes_written": n, "dispatches": 5}), flush=True)
+def pmc_child(args):
+    """Runs under rocprofv3 --pmc: the plain copy that calibrates the counter,
+    then one render of the workload with the parent's launch configuration."""
+    from dspt import Engine, PROFILE_FULL, PROFILE_LEAN
+    from dspt._lib import selftest_copy
+    selftest_copy(COPY_CALIB_BYTES, 3)
+    wl = make_workload(args.workload)
+    spp = args.spp or wl.spp
+    eng = Engine(PROFILE_FULL if wl.profile == "webgl-full" else PROFILE_LEAN)
+    s = wl.scene
+    eng.upload_scene(s.vertexBuffer, s.triangleIndexBuffer, s.bvhNodes, s.bvhTriangleIndices, s.getFlatMaterialBuffer())
+    img = wl.ibl["image"]
+    total = eng.upload_ibl(img["data"], img["width"], img["height"])
+    eng.resize(wl.width, wl.height)
+    eng.set_frame(frame_block(wl, total))
+    eng.set_launch_config(max_paths=args.max_paths, shade_blocks=args.shade_blocks)
+    eng.set_kernel_timing(True)  # eager launches: one profiled dispatch per kernel launch
+    eng.render(1, spp)
+    eng.synchronize()
+    eng.close()
+    log("[bench] pmc child: rendered one frame")
+
+
+def rocprofv3_path():
+    p = shutil.which("rocprofv3")
+    if p:
+        return p
+    p = "/opt/rocm/bin/rocprofv3"
+    return p if os.path.exists(p) else None
+
+
+def _counter_means(outdir, counter):
+    """Mean value per dispatch of `counter` for each kernel of interest and
+    for the calibration copy, from rocprofv3's counter_collection.csv."""
+    files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError(f"no counter_collection.csv under {outdir}")
+    per = {}
+    for f in files:
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                name = row.get("Kernel_Name", "")
+                if row.get("Counter_Name", counter) != counter:
+                    continue
+                val = float(row.get("Counter_Value", row.get(counter, "nan")))
+                key = next((k for k in PMC_KERNELS + ("k_copy_calib",) if k in name), None)
+                if key is None:
+                    continue
+                disp = row.get("Dispatch_Id") or row.get("Correlation_Id") or str(len(per.get(key, {})))
+                per.setdefault(key, {})
+                per[key][disp] = per[key].get(disp, 0.0) + val  # sum of per-instance rows of one dispatch
+    return {k: (sum(v.values()) / len(v), len(v)) for k, v in per.items()}
+
+
+def measure_traffic(args):
+    """Memory traffic per launch of the bouncing kernels, in bytes, from two
+    child runs under rocprofv3 (FETCH_SIZE, then WRITE_SIZE; one counter per
+    run, each within one pass of the hardware), scaled by the same counters
+    of the plain copy: bytes = counter x (copy bytes / copy counter)."""
+    prof = rocprofv3_path()
+    if not prof:
+        return None, "rocprofv3 not found"
+    means = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out = tempfile.mkdtemp(prefix=f"dspt_pmc_{counter.lower()}_")
+        cmd = [prof, "--pmc", counter, "-d", out, "-o", "pmc", "--output-format", "csv", "--",
+               sys.executable, os.path.abspath(__file__), "--pmc-child", "--workload", args.workload,
+               "--max-paths", str(args.max_paths), "--shade-blocks", str(args.shade_blocks)]
+        if args.spp:
+            cmd += ["--spp", str(args.spp)]
+        log(f"[bench] counters: {' '.join(cmd[:3])} ... (child run)")
+        t0 = time.time()
+        try:
+            r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, timeout=300, cwd=ROOT)
+        except subprocess.TimeoutExpired:
+            return None, f"rocprofv3 {counter} run timed out"
+        if r.returncode != 0 and not glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True):
+            return None, f"rocprofv3 {counter} run failed ({r.returncode})"
+        try:
+            means[counter] = _counter_means(out, counter)
+        except (RuntimeError, ValueError) as e:
+            return None, f"{counter}: {e}"
+        finally:
+            shutil.rmtree(out, ignore_errors=True)
+        log(f"[bench] counters {counter} in {time.time() - t0:.1f}s: "
+            + ", ".join(f"{k} {v[0]:.0f} KiB x{v[1]}" for k, v in means[counter].items()))
+    f, w = means["FETCH_SIZE"], means["WRITE_SIZE"]
+    if "k_copy_calib" not in f or "k_copy_calib" not in w:
+        return None, "calibration copy not seen by the profiler"
+    kf = COPY_CALIB_BYTES / (f["k_copy_calib"][0] * 1024.0)
+    kw = COPY_CALIB_BYTES / (w["k_copy_calib"][0] * 1024.0)
+    res = {"fetch_scale": round(kf, 4), "write_scale": round(kw, 4), "copy_bytes": COPY_CALIB_BYTES, "kernels": {}}
+    for k in PMC_KERNELS:
+        if k in f and k in w:
+            rd, wr = f[k][0] * 1024.0 * kf, w[k][0] * 1024.0 * kw
+            res["kernels"][k] = {"read_bytes": round(rd), "written_bytes": round(wr), "traffic_bytes": round(rd + wr),
+                                 "dispatches": f[k][1]}
+    return res, None
 
 
 def main():
@@ -425,17 +578,22 @@ def main():
     ap.add_argument("--inst-spp", type=int, default=256)
     ap.add_argument("--inst-max-paths", type=int, default=16 << 20, help="path pool of the instanced leg")
     ap.add_argument("--inst-traffic-rate", type=float, default=10.0, help="small jobs per second beside the long job")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="skip the two rocprofv3 child runs that measure the traversal's memory traffic")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine is measured on one GPU only; --gpus must be 1 "
                                    "(multi-GPU sharding is design-only)"}))
         sys.exit(2)
 
-    if args.calibrate_copy:
-        calibrate_copy(args.calibrate_copy)
+    if args.pmc_child:
+        pmc_child(args)
         return
+    # counters first, in child processes, before this process touches the GPU
+    traffic, traffic_err = (None, "skipped (--no-pmc)") if args.no_pmc else measure_traffic(args)
+    if traffic_err:
+        log(f"[bench] memory counters unavailable: {traffic_err}")
 
     from dspt import Engine, PROFILE_FULL, PROFILE_LEAN
     wl = make_workload(args.workload)
@@ -457,8 +615,8 @@ def main():
         eng.synchronize()
         log(f"[bench] warmup {i}: {time.time() - t0:.2f}s")
 
+    # timed steps: the shipping launch mode (captured bounce graphs, no events)
     eng.reset_counters()
-    eng.set_kernel_timing(True)
     lat = []
     eng.synchronize()
     t_start = time.perf_counter()
@@ -471,21 +629,53 @@ def main():
     eng.synchronize()
     elapsed = time.perf_counter() - t_start
     c = eng.counters()
-    ks = eng.kernel_stats()
-    eng.set_kernel_timing(False)
 
-    default_workload = args.workload == "full_materials" and spp == 64
+    # timing pass: the same steps again with HIP events on the engine stream
+    # around every launch (eager launches), for per-kernel durations
+    eng.reset_counters()
+    eng.set_kernel_timing(True)
+    eng.synchronize()
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        eng.render(1, spp)
+        eng.synchronize()
+    timing_elapsed = time.perf_counter() - t1
+    ks = eng.kernel_stats()
+    c_timing = eng.counters()
+    eng.set_kernel_timing(False)
+    log(f"[bench] timing pass: {timing_elapsed / args.steps:.3f}s per step")
+
     npaths = wl.width * wl.height * spp * args.steps
     rays = c["rays_primary"] + c["rays_extension"] + c["rays_shadow"]
     mpaths = npaths / elapsed / 1e6
     mrays = rays / elapsed / 1e6
-    # dominant kernel: closest-hit traversal; algorithmic bytes = 64 B per node visit + 64 B per triangle test
+    # dominant kernel: closest-hit traversal (HBM-bound)
     te = ks["trace_ext"]
     avg_ms = te["total_ms"] / max(1, te["launches"])
-    alg_bytes = 64.0 * te["node_visits"] + 64.0 * te["tri_tests"]
-    bytes_per_launch = alg_bytes / max(1, te["launches"])
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    served = (64.0 * te["node_visits"] + 64.0 * te["tri_tests"]) / max(1, te["launches"])
     kernel_ms = {k: round(v["total_ms"], 2) for k, v in ks.items()}
+    tr = traffic["kernels"].get("k_trace_ext") if traffic else None
+    hbm = tr["traffic_bytes"] if tr else None
+    achieved = hbm / (avg_ms * 1e-3) / 1e9 if (hbm and avg_ms > 0) else None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": hbm,
+                "kernel": "k_trace_ext", "avg_launch_ms": round(avg_ms, 4), "launches": int(te["launches"]),
+                "traffic_source": ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE child runs of this script, "
+                                   f"scaled by a {COPY_CALIB_BYTES >> 20} MiB copy" if tr else traffic_err),
+                "bytes_served_per_launch": round(served),
+                "served_gbs": round(served / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 else None,
+                "served_model": "64 B per node visit + 64 B per triangle test (caches included)"}
+    if traffic:
+        per = {}
+        for k, cls in (("k_trace_ext", "trace_ext"), ("k_shade", "shade"), ("k_trace_shadow", "trace_shadow")):
+            t = traffic["kernels"].get(k)
+            st = ks[cls]
+            ms = st["total_ms"] / max(1, st["launches"])
+            if t and ms > 0:
+                per[k] = dict(t, avg_launch_ms=round(ms, 4), gbs=round(t["traffic_bytes"] / (ms * 1e-3) / 1e9, 1),
+                              frac=round(t["traffic_bytes"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+        roofline["kernels"] = per
+        roofline["calibration"] = {k: traffic[k] for k in ("fetch_scale", "write_scale", "copy_bytes")}
 
     out = {
         "metric": METRIC,
@@ -501,18 +691,19 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{wl.name} {wl.width}x{wl.height} {spp}spp depth{wl.max_bounces} "
-                               f"{s.num_triangles} tris {wl.profile}", "mode": "single job, streaming path pool"},
+                               f"{s.num_triangles} tris {wl.profile}",
+                   "mode": "single job, streaming path pool, bounce loop in HIP graphs"},
         "mrays_per_s": round(mrays, 2),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 1),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 1)},
         "counters": c,
+        "work_per_path": {"rays": round(rays / npaths, 3), "node_visits_per_ray": round(c["node_visits"] / max(1, rays), 2),
+                          "tri_tests_per_ray": round(c["tri_tests"] / max(1, rays), 2)},
+        "timing_pass": {"ms_per_step": round(timing_elapsed / args.steps * 1e3, 2),
+                        "identical_counters": c_timing == c,
+                        "mode": "same steps, eager launches with HIP events around each"},
         "kernel_ms": kernel_ms,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": (TRACE_EXT_TRAFFIC_BYTES
-                                 if default_workload and args.max_paths == TRACE_EXT_TRAFFIC_POOL else None),
-                     "kernel": "k_trace_ext", "avg_launch_ms": round(avg_ms, 4),
-                     "alg_bytes_per_launch": round(bytes_per_launch)},
+        "roofline": roofline,
     }
     # display chain (ACES + gamma + FXAA) on the rendered frame; algorithmic
     # bytes per launch = 16 B accumulation read + 4 B canvas written per pixel

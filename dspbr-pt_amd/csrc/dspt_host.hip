@@ -69,24 +69,37 @@ constexpr int kMaxJobs = 4096;
 constexpr int kMaxTasks = 8192;
 constexpr int64_t kStallChunks = 4096;
 
+// Device-memory limit of a context (dspt_set_memory_limit): every Allocs of
+// the context counts against it while it holds memory.
+struct MemLimit {
+  int64_t cap = 0;  // 0 = no limit
+  int64_t used = 0;
+};
+
 struct Allocs {
   std::vector<void*> ptrs;
   size_t bytes = 0;
+  MemLimit* lim = nullptr;
   template <class T>
   int get(T** out, size_t n) {
     void* p = nullptr;
     size_t sz = std::max<size_t>(n, 1) * sizeof(T);
+    if (lim && lim->cap > 0 && lim->used + (int64_t)sz > lim->cap)
+      return fail(DSPT_ERR_MEMORY, "device memory limit: " + std::to_string(sz) + " B more would exceed " +
+                                       std::to_string(lim->cap) + " B (" + std::to_string(lim->used) + " B in use)");
     hipError_t e = hipMalloc(&p, sz);
     if (e != hipSuccess)
       return fail(DSPT_ERR_MEMORY, std::string("hipMalloc(") + std::to_string(sz) + "): " + hipGetErrorString(e));
     ptrs.push_back(p);
     bytes += sz;
+    if (lim) lim->used += (int64_t)sz;
     *out = (T*)p;
     return 0;
   }
   void release() {
     for (void* p : ptrs) (void)hipFree(p);
     ptrs.clear();
+    if (lim) lim->used -= (int64_t)bytes;
     bytes = 0;
   }
 };
@@ -197,6 +210,7 @@ struct dspt_ctx {
   DevJob* d_jobs = nullptr;
   DevJob* h_jobs = nullptr;  // pinned staging
   bool jobs_dirty = true;
+  MemLimit lim;  // device-memory limit over every Allocs below
   // path pool
   Allocs state_mem;
   int64_t cap = 0;
@@ -796,6 +810,7 @@ int make_scene_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint
   }
   HIPCHK(hipSetDevice(c->device));
   SceneRes* s = new SceneRes();
+  s->mem.lim = s->tex_mem.lim = &c->lim;
   Allocs& A = s->mem;
   int r = 0;
   float4* big_nodes = nullptr;
@@ -873,6 +888,7 @@ int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tr
     if (int r = validate_nodes(nodes, N)) return r;
   HIPCHK(hipSetDevice(c->device));
   SceneRes* s = new SceneRes();
+  s->mem.lim = s->tex_mem.lim = &c->lim;
   Allocs& A = s->mem;
   int r = 0;
   int32_t* d_bvh = nullptr;
@@ -1069,22 +1085,27 @@ int make_scene_instanced(dspt_ctx* c, const float* vertices, int64_t V, const ui
 int make_ibl(dspt_ctx* c, const float* rgba, int32_t w, int32_t h, IblRes** out) {
   if (!rgba || w <= 0 || h <= 0) return fail(DSPT_ERR_ARG, "invalid IBL image");
   HIPCHK(hipSetDevice(c->device));
-  IblRes* b = new IblRes();
+  // owns the new IBL until it is handed out; every early return frees it
+  struct Owner {
+    IblRes* b;
+    ~Owner() {
+      if (b) { b->mem.release(); delete b; }
+    }
+  } own{new IblRes()};
+  IblRes* b = own.b;
+  b->mem.lim = &c->lim;
   size_t n = (size_t)w * h;
   Allocs& A = b->mem;
   int r = 0;
   if ((r = A.get(&b->env, n)) || (r = A.get(&b->pdf, n)) || (r = A.get(&b->cdf, n)) || (r = A.get(&b->ypdf, (size_t)h)) ||
-      (r = A.get(&b->ycdf, (size_t)h)) || (r = A.get(&b->guide, (size_t)h * (w + 1))) || (r = A.get(&b->yguide, (size_t)h + 1))) {
-    A.release();
-    delete b;
+      (r = A.get(&b->ycdf, (size_t)h)) || (r = A.get(&b->guide, (size_t)h * (w + 1))) || (r = A.get(&b->yguide, (size_t)h + 1)))
     return r;
-  }
+  ScopedAllocs tmp;  // the build's temporaries, freed on every return
+  tmp.lim = &c->lim;
   float* lum = nullptr;
   float* rowsum = nullptr;
   double* total = nullptr;
-  HIPCHK(hipMalloc(&lum, n * 4));
-  HIPCHK(hipMalloc(&rowsum, (size_t)h * 4));
-  HIPCHK(hipMalloc(&total, 8));
+  if ((r = tmp.get(&lum, n)) || (r = tmp.get(&rowsum, (size_t)h)) || (r = tmp.get(&total, 1))) return r;
   HIPCHK(hipMemcpyAsync(b->env, rgba, n * 16, hipMemcpyHostToDevice, c->stream));
   k_ibl_luminance<<<grid_for((int64_t)n, 256), 256, 0, c->stream>>>(b->env, (int64_t)n, lum);
   k_ibl_rows<<<grid_for(h, 64), 64, 0, c->stream>>>(lum, w, h, b->pdf, b->cdf, rowsum);
@@ -1094,10 +1115,8 @@ int make_ibl(dspt_ctx* c, const float* rgba, int32_t w, int32_t h, IblRes** out)
   double tot = 0;
   HIPCHK(hipMemcpyAsync(&tot, total, 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  (void)hipFree(lum);
-  (void)hipFree(rowsum);
-  (void)hipFree(total);
   b->w = w; b->h = h; b->total = tot;
+  own.b = nullptr;
   *out = b;
   return 0;
 }
@@ -1141,6 +1160,14 @@ int create_job(dspt_ctx* c, int scene, int ibl, const FrameParams& F, int W, int
   int64_t need = job_bytes(npix, record, num_frames, ring);
   if (admit && c->budget > 0 && memory_in_use(c) + need > c->budget)
     return fail(DSPT_ERR_MEMORY, "admission: job needs " + std::to_string(need) + " B over the memory budget");
+  if (admit) {  // the device's free HBM gates admission whatever the budget
+    constexpr int64_t kAdmitReserve = 256ll << 20;
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipSetDevice(c->device));
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && (int64_t)free_b < need + kAdmitReserve)
+      return fail(DSPT_ERR_MEMORY, "admission: job needs " + std::to_string(need) + " B, the device has " +
+                                       std::to_string(free_b) + " B free");
+  }
   int slot_id = want_id;
   if (slot_id < 0) {
     slot_id = (int)c->jobs.size();
@@ -1149,6 +1176,7 @@ int create_job(dspt_ctx* c, int scene, int ibl, const FrameParams& F, int W, int
   }
   if (slot_id >= kMaxJobs) return fail(DSPT_ERR_MEMORY, "too many jobs");
   JobRes* j = new JobRes();
+  j->mem.lim = &c->lim;
   int r = 0;
   if ((r = j->mem.get(&j->accum, (size_t)npix)) || (r = j->mem.get(&j->ring, (size_t)(npix * ring))) ||
       (r = j->mem.get(&j->done, (size_t)ring))) {
@@ -1229,6 +1257,7 @@ int dspt_create(int profile, int device, dspt_ctx** out) {
   HIPCHK(hipSetDevice(device));
   dspt_ctx* c = new dspt_ctx();
   c->device = device;
+  c->state_mem.lim = &c->lim;
   c->profile = profile;
   if (const char* mp = std::getenv("DSPT_MAX_PATHS")) c->max_paths = std::max<int64_t>(65536, std::atoll(mp));
   if (const char* g = std::getenv("DSPT_BOUNCE_GRAPHS")) c->use_graphs = std::atoi(g) != 0;
@@ -1749,6 +1778,12 @@ int dspt_jobs_run(dspt_ctx* c, const int32_t* job_ids, int32_t n) {
   return run_until_complete(c, listed_jobs(c, job_ids, n));
 }
 
+int dspt_set_memory_limit(dspt_ctx* c, int64_t bytes) {
+  if (!c || bytes < 0) return fail(DSPT_ERR_ARG, "invalid limit");
+  c->lim.cap = bytes;
+  return 0;
+}
+
 int dspt_set_memory_budget(dspt_ctx* c, int64_t bytes) {
   if (!c || bytes < 0) return fail(DSPT_ERR_ARG, "invalid budget");
   c->budget = bytes;
@@ -1885,20 +1920,18 @@ static int trace_common(dspt_ctx* c, const float* rays, int64_t n, int any, int3
   JobRes tmp;
   tmp.scene = 0;
   SceneView S = view_of(c, &tmp);
+  ScopedAllocs A;  // freed on every return
+  A.lim = &c->lim;
   float4* d_rays = nullptr;
   int32_t *d_tri = nullptr, *d_inst = nullptr;
   float *d_t = nullptr, *d_u = nullptr, *d_v = nullptr;
   int* d_spill = nullptr;
   int blocks = (int)std::min<int64_t>(2048, grid_for(n, kTraceBlock));
-  HIPCHK(hipMalloc(&d_rays, n * 32));
-  HIPCHK(hipMalloc(&d_tri, n * 4));
-  if (inst) HIPCHK(hipMalloc(&d_inst, n * 4));
-  if (t) {
-    HIPCHK(hipMalloc(&d_t, n * 4));
-    HIPCHK(hipMalloc(&d_u, n * 4));
-    HIPCHK(hipMalloc(&d_v, n * 4));
-  }
-  HIPCHK(hipMalloc(&d_spill, (size_t)blocks * kTraceBlock * (kStackSize - kLdsStack) * 4));
+  int r = 0;
+  if ((r = A.get(&d_rays, (size_t)n * 2)) || (r = A.get(&d_tri, (size_t)n))) return r;
+  if (inst && (r = A.get(&d_inst, (size_t)n))) return r;
+  if (t && ((r = A.get(&d_t, (size_t)n)) || (r = A.get(&d_u, (size_t)n)) || (r = A.get(&d_v, (size_t)n)))) return r;
+  if ((r = A.get(&d_spill, (size_t)blocks * kTraceBlock * (kStackSize - kLdsStack)))) return r;
   HIPCHK(hipMemcpyAsync(d_rays, rays, n * 32, hipMemcpyHostToDevice, c->stream));
   k_trace_rays<<<blocks, kTraceBlock, 0, c->stream>>>(S, d_rays, n, any, d_tri, d_t, d_u, d_v, d_inst, d_spill, c->cnt);
   HIPCHK(hipGetLastError());
@@ -1910,9 +1943,6 @@ static int trace_common(dspt_ctx* c, const float* rays, int64_t n, int any, int3
     HIPCHK(hipMemcpyAsync(v, d_v, n * 4, hipMemcpyDeviceToHost, c->stream));
   }
   HIPCHK(hipStreamSynchronize(c->stream));
-  (void)hipFree(d_rays); (void)hipFree(d_tri); (void)hipFree(d_spill);
-  if (d_inst) (void)hipFree(d_inst);
-  if (d_t) { (void)hipFree(d_t); (void)hipFree(d_u); (void)hipFree(d_v); }
   return 0;
 }
 
@@ -2006,6 +2036,24 @@ int dspt_selftest_pin_math(int device, int32_t fn, const float* x, const float* 
   k_pin_math<<<grid_for(n, 256), 256>>>(fn, dx, dy, dout, n);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int dspt_selftest_copy(int device, int64_t bytes, int32_t reps) {
+  if (bytes <= 0 || bytes % 16 || reps <= 0) return fail(DSPT_ERR_ARG, "bytes must be a positive multiple of 16, reps > 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(DSPT_ERR_DEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(DSPT_ERR_ARG, "device index out of range");
+  HIPCHK(hipSetDevice(device));
+  ScopedAllocs A;
+  float4 *src = nullptr, *dst = nullptr;
+  const int64_t n = bytes / 16;
+  int r;
+  if ((r = A.get(&src, (size_t)n)) || (r = A.get(&dst, (size_t)n))) return r;
+  HIPCHK(hipMemset(src, 0, (size_t)bytes));
+  for (int i = 0; i < reps; i++) k_copy_calib<<<4096, 256>>>(src, dst, n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
   return 0;
 }
 

@@ -934,3 +934,9 @@ __global__ __launch_bounds__(256) void k_pin_math(int fn, const float* __restric
   }
   out[i] = r;
 }
+
+// Plain device copy (16-byte loads and stores, grid-stride): the calibration
+// kernel that relates the memory counters to bytes actually moved.
+__global__ __launch_bounds__(256) void k_copy_calib(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}

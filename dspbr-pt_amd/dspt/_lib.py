@@ -79,12 +79,14 @@ SIGNATURES = {
     "dspt_jobs_step": (_c_int, [_vp, _i32p, _c_i32, _i32p]),
     "dspt_jobs_run": (_c_int, [_vp, _i32p, _c_i32]),
     "dspt_set_memory_budget": (_c_int, [_vp, _c_i64]),
+    "dspt_set_memory_limit": (_c_int, [_vp, _c_i64]),
     "dspt_set_launch_config": (_c_int, [_vp, _c_i64, _c_i32, _c_i32]),
     "dspt_get_launch_config": (_c_int, [_vp, _i64p, _i32p, _i32p]),
     "dspt_set_schedule": (_c_int, [_vp, _c_i32, _c_i32, _c_i32, ctypes.c_float]),
     "dspt_get_schedule": (_c_int, [_vp, _i32p, _i32p, _i32p, _fp]),
     "dspt_set_policy": (_c_int, [_vp, _c_i32]),
     "dspt_selftest_pin_math": (_c_int, [_c_int, _c_i32, _fp, _fp, _fp, _c_i64]),
+    "dspt_selftest_copy": (_c_int, [_c_int, _c_i64, _c_i32]),
     "dspt_scene_set_textures": (_c_int, [_vp, _c_i32, _fp, _c_i32, _i32p, _c_i32, ctypes.POINTER(ctypes.c_uint8), _c_i64]),
 }
 SCHED_STREAM = 0
@@ -167,6 +169,11 @@ def selftest_pin_math(fn, x, y=None, device=0):
     out = np.empty_like(x)
     check(lib().dspt_selftest_pin_math(device, fn, ptr(x), ptr(yy), ptr(out), x.size))
     return out
+
+
+def selftest_copy(nbytes, reps=5, device=0):
+    """reps launches of a plain device copy of nbytes (counter calibration)."""
+    check(lib().dspt_selftest_copy(device, int(nbytes), int(reps)))
 
 
 class Engine:
@@ -479,6 +486,9 @@ class Engine:
         rf = ctypes.c_float(0)
         check(self._L.dspt_get_schedule(self.h, ctypes.byref(m), ctypes.byref(pf), ctypes.byref(ci), ctypes.byref(rf)))
         return {"mode": m.value, "pass_frames": pf.value, "chunk_iters": ci.value, "ring_factor": rf.value}
+
+    def set_memory_limit(self, nbytes):
+        check(self._L.dspt_set_memory_limit(self.h, int(nbytes)))
 
     def set_memory_budget(self, nbytes):
         check(self._L.dspt_set_memory_budget(self.h, int(nbytes)))

@@ -282,7 +282,9 @@ int dspt_scene_set_textures(dspt_ctx* ctx, int32_t scene_id, const float* texinf
 
 /* A job renders frames 1..num_frames of (scene, IBL or -1, frame block, W x H)
  * into its own accumulation buffer. Admission: fails with DSPT_ERR_MEMORY when
- * the job's buffers would exceed the memory budget (dspt_set_memory_budget). */
+ * the job's buffers would exceed the memory budget (dspt_set_memory_budget)
+ * or, budget or not, the device's free HBM (hipMemGetInfo) less a 256 MiB
+ * reserve. */
 int dspt_job_create(dspt_ctx* ctx, int32_t scene_id, int32_t ibl_id, const float block[DSPT_FRAME_FLOATS], int32_t width,
                     int32_t height, int32_t num_frames, int32_t record_samples, int32_t* job_id);
 int dspt_job_destroy(dspt_ctx* ctx, int32_t job_id);
@@ -300,6 +302,11 @@ int dspt_jobs_step(dspt_ctx* ctx, const int32_t* job_ids, int32_t n, int32_t* ac
 /* Runs chunks until the listed jobs have folded all their frames. */
 int dspt_jobs_run(dspt_ctx* ctx, const int32_t* job_ids, int32_t n);
 int dspt_set_memory_budget(dspt_ctx* ctx, int64_t bytes);
+/* Hard limit on the device memory the context allocates for scenes, IBLs,
+ * textures, jobs, the path pool and per-call temporaries (0 = none, the
+ * default). A call that would pass it fails with DSPT_ERR_MEMORY and frees
+ * what it had allocated, so dspt_memory_usage() is unchanged. */
+int dspt_set_memory_limit(dspt_ctx* ctx, int64_t bytes);
 
 /* Launch configuration: path-pool size (paths in flight; default 16 Mi,
  * about 5 GB of path state; larger pools amortise each launch's drain over
@@ -341,6 +348,12 @@ int dspt_set_policy(dspt_ctx* ctx, int32_t policy);
  * 1 cos, 2 exp, 3 log, 4 pow(x, y), 5 atan2(x, y) (x = the y-coordinate),
  * 6 acos, 7 / 8 the sine / cosine of psincos. y is read for fn 4 and 5. */
 int dspt_selftest_pin_math(int device, int32_t fn, const float* x, const float* y, float* out, int64_t n);
+
+/* Measurement aid (no counterpart in the reference): `reps` launches of a
+ * plain device copy of `bytes` bytes (k_copy_calib), whose memory counters
+ * (FETCH_SIZE, WRITE_SIZE) per dispatch calibrate the counters against bytes
+ * actually read and written. */
+int dspt_selftest_copy(int device, int64_t bytes, int32_t reps);
 
 #ifdef __cplusplus
 }
