@@ -34,6 +34,11 @@ function packFrameBlock(r, camera, frameCount) {
   return block;
 }
 
+// Paths per native render call once the loop has ramped up: about one path
+// pool, so stopRendering(), resetAccumulation() and camera moves take effect
+// within a few milliseconds of GPU work.
+const PATHS_PER_CALL = 1 << 24;
+
 class PathtracingRenderer {
   constructor(parameters = {}) {
     this.materialProfile = parameters.materialProfile || 'webgl-lean';
@@ -43,11 +48,21 @@ class PathtracingRenderer {
     this._maxBounces = 8; this._useIBL = true; this._showBackground = true; this._iblRotation = 0;
     this._backgroundColor = [0, 0, 0, 1]; this._rayEps = 0.0001; this._clampThreshold = 3.0;
     this._exposure = 1.0; this._tonemapping = 'None'; this._enableGamma = true; this._enableFxaa = false;
-    this._pixelRatio = 1.0; this._tileRes = 4;
-    this._frameCount = 1; this._isRendering = false;
+    this._pixelRatio = 1.0; this._tileRes = 4; this._debugMode = 'None'; this._forceIBLEval = false;
+    // renderer.ts:181,192
+    this.debugModes = ['None', 'Albedo', 'Metalness', 'Roughness', 'Normals', 'Tangents', 'Bitangents', 'Transparency',
+                       'UV0', 'Clearcoat', 'IBL PDF', 'IBL CDF', 'Specular', 'SpecularTint', 'Translucency'];
+    this.tonemappingModes = TONEMAPPING_MODES.slice();
+    this._frameCount = 1; this._isRendering = false; this.schedulerMode = 'stopped';
+    this._gen = 0;               // bumped by render() and stopRendering(): a loop of an older generation exits
+    this._inflight = null;       // promise of the native call on the GPU, if any
+    this._resetPending = false;  // accumulation restarts at frame 1 before the next call
+    this._pending = [];          // native updates requested while a call was in flight
     this._iblSize = [0, 0]; this._iblTotalSum = 0; this._sceneCounts = [0, 0, 0];
     this._scene = null;
-    this.diagnostics = { backend: 'hip-gfx950', sampleCount: 0, mode: 'stopped', profiling: {} };
+    this.diagnostics = { backend: 'hip-gfx950', sampleCount: 0, mode: 'stopped', renderResolution: '',
+                         displayResolution: '', tileResolution: 2, framePending: false, profiling: {},
+                         shaderPrograms: [] };
     this.displayRes = [0, 0]; this.renderRes = [0, 0];
     this.resize(parameters.width || 1, parameters.height || 1);
   }
@@ -55,6 +70,7 @@ class PathtracingRenderer {
   get maxBounces() { return this._maxBounces; } set maxBounces(v) { this._maxBounces = v; this.resetAccumulation(); }
   get useIBL() { return this._useIBL; } set useIBL(v) { this._useIBL = v; this.resetAccumulation(); }
   get showBackground() { return this._showBackground; } set showBackground(v) { this._showBackground = v; this.resetAccumulation(); }
+  get forceIBLEval() { return this._forceIBLEval; } set forceIBLEval(v) { this._forceIBLEval = v; this.resetAccumulation(); }
   get iblRotation() { return this._iblRotation / Math.PI * 180.0; } set iblRotation(v) { this._iblRotation = v / 180.0 * Math.PI; this.resetAccumulation(); }
   get backgroundColor() { return this._backgroundColor; } set backgroundColor(v) { this._backgroundColor = v; this.resetAccumulation(); }
   get rayEps() { return this._rayEps; } set rayEps(v) { this._rayEps = v; this.resetAccumulation(); }
@@ -65,10 +81,21 @@ class PathtracingRenderer {
   get enableFxaa() { return this._enableFxaa; } set enableFxaa(v) { this._enableFxaa = v; }
   get pixelRatio() { return this._pixelRatio; } set pixelRatio(v) { this._pixelRatio = v; this.resize(this.displayRes[0], this.displayRes[1]); }
   get tileRes() { return this._tileRes; } set tileRes(v) { this._tileRes = Math.max(2, Math.floor(v) || 2); }
+  // renderer.ts:182-190. The debug integrators (AOV views) are not part of
+  // this engine: any mode but 'None' is accepted here and refused by render().
+  get debugMode() { return this._debugMode; } set debugMode(v) { this._debugMode = v; this.resetAccumulation(); }
+
+  // resolves once no native call is in flight (readAccumulation, readDisplay
+  // and saveImage need an idle engine)
+  async idle() {
+    while (this._inflight) await this._inflight.catch(() => {});
+    this._applyPending();
+  }
 
   // renderer.ts:914-958; the BVH is built by the engine when the scene carries no prebuilt one
   async setScene(scene, progress) {
     this.stopRendering();
+    await this.idle();
     if (progress) progress({ phase: 'Building BVH' });
     let nodes = scene.bvhNodes, idx = scene.bvhTriangleIndices;
     if (!nodes) {
@@ -90,47 +117,161 @@ class PathtracingRenderer {
     this.resetAccumulation();
   }
 
-  // renderer.ts:840-865
+  // renderer.ts:840-865 (applied before the next frame when a call is in flight)
   setIBL(texture) {
-    const img = texture.image;
-    this._iblTotalSum = native.uploadIbl(this._h, img.data, img.width, img.height);
-    this._iblSize = [img.width, img.height];
+    this._defer(() => {
+      const img = texture.image;
+      this._iblTotalSum = native.uploadIbl(this._h, img.data, img.width, img.height);
+      this._iblSize = [img.width, img.height];
+    });
     this.resetAccumulation();
   }
 
+  // renderer.ts:365-372
   resize(width, height) {
     this.displayRes = [width, height];
     this.renderRes = [Math.ceil(width * this._pixelRatio), Math.ceil(height * this._pixelRatio)];
-    native.resize(this._h, this.renderRes[0], this.renderRes[1]);
+    const [w, h] = this.renderRes;
+    this._defer(() => native.resize(this._h, w, h));
     this.resetAccumulation();
   }
 
-  resetAccumulation() { this._frameCount = 1; this.diagnostics.sampleCount = 0; }
-  stopRendering() { this._isRendering = false; this.diagnostics.mode = 'stopped'; }
-  setInteractionMode(flag) { this.diagnostics.mode = flag ? 'interactive' : 'settling'; this.resetAccumulation(); }
+  // renderer.ts:327-336: the next frame restarts the accumulation at frame 1
+  // (with the camera and settings of that moment)
+  resetAccumulation() {
+    this._frameCount = 1;
+    this._resetPending = true;
+    if (this._isRendering && this.schedulerMode === 'accumulating') this.schedulerMode = 'settling';
+    this._updateDiagnostics();
+  }
 
-  // renderer.ts:512-565: frames 1 .. numSamples-1 are rendered (the loop stops
-  // once the frame counter reaches numSamples). Resolves when finished.
-  async render(camera, numSamples, tileFinishedCB, frameFinishedCB, renderingFinishedCB, framesPerCall) {
-    if (numSamples < 0) throw new Error('continuous rendering needs a sample limit in this engine');
+  // renderer.ts:338-352: no frame is submitted after this call; frames of a
+  // call already on the GPU finish into the accumulation (and are counted)
+  stopRendering() {
+    this._gen++;
+    this._isRendering = false;
+    this.schedulerMode = 'stopped';
+    this._updateDiagnostics();
+  }
+
+  // renderer.ts:354-363
+  setInteractionMode(flag) {
+    if (flag) this.schedulerMode = 'interactive';
+    else if (this.schedulerMode !== 'stopped') this.schedulerMode = 'settling';
     this.resetAccumulation();
-    this._isRendering = true;
-    this.diagnostics.mode = 'accumulating';
-    native.setFrame(this._h, packFrameBlock(this, camera, 1));
-    const step = framesPerCall || Math.max(1, numSamples - 1);
-    while (this._isRendering && this._frameCount < numSamples) {
-      const n = Math.min(step, numSamples - this._frameCount);
-      await native.render(this._h, this._frameCount, n);
-      this._frameCount += n;
-      this.diagnostics.sampleCount = this._frameCount - 1;
-      if (tileFinishedCB) tileFinishedCB();
-      if (frameFinishedCB) frameFinishedCB(this._frameCount);
-    }
+    this._updateDiagnostics();
+  }
+
+  // renderer.ts:512-565: render(camera, numSamples, tileFinishedCB,
+  // frameFinishedCB, renderingFinishedCB) stops a running render, then renders
+  // frames 1, 2, ... until the frame counter reaches numSamples (frames
+  // 1 .. numSamples-1), or until stopRendering() when numSamples is -1. The
+  // camera object is read again for every call, so a caller that moves it and
+  // calls resetAccumulation() (as the viewport does) restarts with the new
+  // view. Frames are submitted in calls of growing size (1, 2, 4, ... frames,
+  // up to about one path pool); per frame, tileFinishedCB runs once per tile
+  // of the active tile order and frameFinishedCB(frameCount) once. Returns a
+  // promise that settles when the loop ends; errors stop the loop and reject it.
+  render(camera, numSamples, tileFinishedCB, frameFinishedCB, renderingFinishedCB, framesPerCall) {
     this.stopRendering();
-    if (renderingFinishedCB) renderingFinishedCB();
+    if (this._debugMode !== 'None')
+      throw new Error(`debugMode '${this._debugMode}': the debug views are not rendered by this engine`);
+    this._camera = camera;
+    this._numSamples = numSamples;
+    this._cb = { tile: tileFinishedCB, frame: frameFinishedCB, done: renderingFinishedCB };
+    this._framesPerCall = framesPerCall || 0;
+    this.schedulerMode = 'settling';
+    this._isRendering = true;
+    this.resetAccumulation();
+    const gen = ++this._gen;
+    const p = this._loop(gen);
+    p.catch(() => {});  // the caller may ignore the promise, as the reference's render() returns nothing
+    return p;
+  }
+
+  async _loop(gen) {
+    let ramp = 1;
+    try {
+      for (;;) {
+        while (this._inflight) await this._inflight.catch(() => {});  // a call of an earlier loop
+        if (gen !== this._gen || !this._isRendering) return;
+        this._applyPending();
+        if (this._resetPending) { this._resetPending = false; this._frameCount = 1; ramp = 1; }
+        if (this._numSamples !== -1 && this._frameCount >= this._numSamples) {
+          this.stopRendering();
+          if (this._cb.done) this._cb.done();
+          return;
+        }
+        const first = this._frameCount;
+        const cap = Math.max(1, Math.floor(PATHS_PER_CALL / (this.renderRes[0] * this.renderRes[1])));
+        let n = this._framesPerCall || Math.min(ramp, cap);
+        if (this._numSamples !== -1) n = Math.min(n, this._numSamples - first);
+        native.setFrame(this._h, packFrameBlock(this, this._camera, first));
+        const call = native.render(this._h, first, n);
+        this._inflight = call;
+        this.diagnostics.framePending = true;
+        try {
+          await call;
+        } finally {
+          if (this._inflight === call) this._inflight = null;
+          this.diagnostics.framePending = false;
+        }
+        if (this._resetPending) continue;  // reset while on the GPU: those frames are dropped
+        const live = gen === this._gen && this._isRendering;
+        for (let f = 0; f < n; f++) {
+          this._frameCount++;
+          if (!live) continue;  // stopped meanwhile: counted, no callbacks
+          const tiles = this._activeTileRes() ** 2;
+          for (let t = 0; t < tiles; t++) if (this._cb.tile) this._cb.tile();
+          if (this._cb.frame) this._cb.frame(this._frameCount);
+          if (this.schedulerMode === 'settling') this.schedulerMode = 'accumulating';
+        }
+        this._updateDiagnostics();
+        if (!live) return;
+        ramp = Math.min(ramp * 2, cap);
+      }
+    } catch (e) {
+      if (gen === this._gen) this.stopRendering();
+      throw e;
+    }
+  }
+
+  // renderer.ts:567-571
+  _activeTileRes() {
+    if (this.schedulerMode === 'interactive') return 1;
+    if (this.schedulerMode === 'settling') return 2;
+    return Math.max(2, this._tileRes || 2);
+  }
+
+  // renderer.ts:972-981
+  _updateDiagnostics() {
+    const d = this.diagnostics;
+    d.sampleCount = Math.max(0, this._frameCount - 1);
+    d.mode = this.schedulerMode;
+    d.renderResolution = `${this.renderRes[0]}x${this.renderRes[1]}`;
+    d.displayResolution = `${this.displayRes[0]}x${this.displayRes[1]}`;
+    d.tileResolution = this._activeTileRes();
+    d.framePending = !!this._inflight;
+  }
+
+  // runs fn now when no call is in flight, else before the next call
+  _defer(fn) {
+    this._pending.push(fn);
+    if (!this._inflight) this._applyPending();
+    else this._inflight.catch(() => {}).then(() => { if (!this._inflight) this._applyPending(); });
+  }
+
+  _applyPending() {
+    while (this._pending.length) this._pending.shift()();
+  }
+
+  _checkIdle() {
+    if (this._inflight) throw new Error('a render call is on the GPU: await renderer.idle() first');
+    this._applyPending();
   }
 
   readAccumulation() {
+    this._checkIdle();
     const out = new Float32Array(this.renderRes[0] * this.renderRes[1] * 4);
     native.readAccum(this._h, out);
     return out;
@@ -138,6 +279,7 @@ class PathtracingRenderer {
 
   // renderer.ts:192 lists the operators; an unknown name gives indexOf() == -1: no operator
   readDisplay() {
+    this._checkIdle();
     const mode = TONEMAPPING_MODES.indexOf(this._tonemapping);
     return native.postprocess(this._h, -1, mode, this._exposure, !!this._enableGamma, !!this._enableFxaa,
                               this.displayRes[0], this.displayRes[1]);

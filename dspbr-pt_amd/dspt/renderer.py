@@ -8,8 +8,10 @@ PathtracingRenderer   packages/lib/renderer.ts:116-1109 (the object the
 PathtracingViewport   packages/lib/pathtracing_viewport.ts:100-316 (public
                       embedding API: setScene/setIBL/setLookAtCamera/start/
                       stop/on/off/setSettings).
-Post-processing (tonemap, FXAA, display) and the debug AOV integrator are not
-part of the engine; the accumulation buffer is returned in linear radiance.
+readAccumulation() returns the linear accumulation; readDisplay() runs the
+display chain (tonemap, gamma, FXAA, display resample) on the GPU. The debug
+AOV integrator is not part of the engine: any debugMode but 'None' is
+refused by render().
 """
 import math
 import time
@@ -48,7 +50,9 @@ def pack_frame_block(camera, width, height, *, frame_count=1, max_bounces=8, use
 class PathtracingRenderer:
     """MI355X replacement of the reference PathtracingRenderer (renderer.ts:116)."""
 
-    debugModes = ["None"]
+    # renderer.ts:181
+    debugModes = ["None", "Albedo", "Metalness", "Roughness", "Normals", "Tangents", "Bitangents", "Transparency", "UV0",
+                  "Clearcoat", "IBL PDF", "IBL CDF", "Specular", "SpecularTint", "Translucency"]
     tonemappingModes = ["None", "Reinhard", "Cineon", "AcesFilm"]
 
     def __init__(self, materialProfile="webgl-lean", device=0, width=1, height=1):
@@ -73,6 +77,8 @@ class PathtracingRenderer:
         self._debugMode = "None"
         self._frameCount = 1
         self._isRendering = False
+        self._reset_pending = False
+        self.schedulerMode = "stopped"
         self._scene = None
         self._bvh_counts = (0, 0)
         self._ibl = None
@@ -137,9 +143,8 @@ class PathtracingRenderer:
 
     @debugMode.setter
     def debugMode(self, v):
-        if v != "None":
-            raise NotImplementedError("debug AOV integrator is not part of the engine")
         self._debugMode = v
+        self.resetAccumulation()
 
     # ---- scene / IBL / resolution
     def setScene(self, scene: PathtracingSceneData, progress=None):
@@ -199,16 +204,44 @@ class PathtracingRenderer:
 
     # ---- render loop
     def resetAccumulation(self):
+        """renderer.ts:327-336: the next frame restarts at frame 1 (with the
+        camera and settings of that moment)."""
         self._frameCount = 1
-        self.diagnostics["sampleCount"] = 0
+        self._reset_pending = True
+        if self._isRendering and self.schedulerMode == "accumulating":
+            self.schedulerMode = "settling"
+        self._update_diagnostics()
 
     def stopRendering(self):
+        """renderer.ts:338-352: no frame is submitted after this call."""
         self._isRendering = False
-        self.diagnostics["mode"] = "stopped"
+        self.schedulerMode = "stopped"
+        self._update_diagnostics()
 
     def setInteractionMode(self, flag):
-        self.diagnostics["mode"] = "interactive" if flag else ("settling" if self._isRendering else "stopped")
+        """renderer.ts:354-363."""
+        if flag:
+            self.schedulerMode = "interactive"
+        elif self.schedulerMode != "stopped":
+            self.schedulerMode = "settling"
         self.resetAccumulation()
+
+    def _active_tile_res(self):
+        # renderer.ts:567-571
+        if self.schedulerMode == "interactive":
+            return 1
+        if self.schedulerMode == "settling":
+            return 2
+        return max(2, self._tileRes or 2)
+
+    def _update_diagnostics(self):
+        # renderer.ts:972-981
+        d = self.diagnostics
+        d["sampleCount"] = max(0, self._frameCount - 1)
+        d["mode"] = self.schedulerMode
+        d["renderResolution"] = f"{self.renderRes[0]}x{self.renderRes[1]}"
+        d["displayResolution"] = f"{self.displayRes[0]}x{self.displayRes[1]}"
+        d["tileResolution"] = self._active_tile_res()
 
     def frame_block(self, camera, frame_count=None):
         light = self._scene.lights[0] if (self._scene is not None and self._scene.lights) else None
@@ -222,34 +255,62 @@ class PathtracingRenderer:
             scene_counts=(self._scene.num_triangles if self._scene else 0, self._bvh_counts[0], self._bvh_counts[1], 0),
             light=light)
 
+    PATHS_PER_CALL = 1 << 24
+
     def render(self, camera, num_samples, tileFinishedCB=None, frameFinishedCB=None, renderingFinishedCB=None,
                frames_per_call=None):
-        """renderer.ts:512-565: resets accumulation and renders frames while
-        frameCount < num_samples, i.e. frames 1 .. num_samples-1 (the
-        reference's loop stops once the counter reaches num_samples).
-        num_samples == -1 is not supported (no event loop here)."""
-        if num_samples < 0:
-            raise ValueError("continuous rendering (num_samples = -1) needs an event loop; pass a sample count")
-        self.resetAccumulation()
-        self._isRendering = True
-        self.diagnostics["mode"] = "accumulating"
-        self._engine.set_frame(self.frame_block(camera, 1))
-        total = max(0, num_samples - 1)
-        step = total if frames_per_call is None else max(1, frames_per_call)
-        while self._isRendering and self._frameCount < num_samples:
-            n = min(step, num_samples - self._frameCount)
-            self._engine.render(self._frameCount, n)
-            self._frameCount += n
-            self.diagnostics["sampleCount"] = self._frameCount - 1
-            if tileFinishedCB:
-                tileFinishedCB()
-            if frameFinishedCB:
-                self._engine.synchronize()
-                frameFinishedCB(self._frameCount)
-        self._engine.synchronize()
+        """renderer.ts:512-565: stops a running render, resets the
+        accumulation and renders frames 1, 2, ... until the frame counter
+        reaches num_samples (frames 1 .. num_samples-1), or, for
+        num_samples == -1, until stopRendering() is called (from a callback
+        or another thread). A bounded render goes to the engine in one call
+        (or frames_per_call at a time); an unbounded one in calls of growing
+        size (1, 2, 4, ... up to about one path pool). The camera is read
+        again for every call, and resetAccumulation() restarts at frame 1. Per
+        frame, tileFinishedCB runs once per tile of the active tile order and
+        frameFinishedCB(frameCount) once. Frames of a call that was on the GPU
+        when stopRendering() came are counted without callbacks."""
         self.stopRendering()
-        if renderingFinishedCB:
-            renderingFinishedCB()
+        if self._debugMode != "None":
+            raise NotImplementedError(f"debugMode {self._debugMode!r}: the debug views are not rendered by this engine")
+        self.schedulerMode = "settling"
+        self._isRendering = True
+        self.resetAccumulation()
+        ramp = 1
+        while self._isRendering:
+            if self._reset_pending:
+                self._reset_pending, self._frameCount, ramp = False, 1, 1
+            if num_samples != -1 and self._frameCount >= num_samples:
+                self.stopRendering()
+                if renderingFinishedCB:
+                    renderingFinishedCB()
+                return
+            first = self._frameCount
+            cap = max(1, self.PATHS_PER_CALL // (self.renderRes[0] * self.renderRes[1]))
+            if frames_per_call:
+                n = max(1, frames_per_call)
+            else:  # a bounded render goes to the engine in one call; an unbounded one ramps up
+                n = min(ramp, cap) if num_samples == -1 else num_samples - first
+            if num_samples != -1:
+                n = min(n, num_samples - first)
+            self._engine.set_frame(self.frame_block(camera, first))
+            self._engine.render(first, n)
+            self._engine.synchronize()
+            for _ in range(n):
+                if self._reset_pending:
+                    break  # frames of this call after a reset are dropped
+                self._frameCount += 1
+                if not self._isRendering:
+                    continue  # stopped from a callback: counted, no callbacks
+                for _t in range(self._active_tile_res() ** 2):
+                    if tileFinishedCB:
+                        tileFinishedCB()
+                if frameFinishedCB:
+                    frameFinishedCB(self._frameCount)
+                if self.schedulerMode == "settling":
+                    self.schedulerMode = "accumulating"
+            self._update_diagnostics()
+            ramp = min(ramp * 2, cap)
 
     def readAccumulation(self):
         """Linear RGBA f32 accumulation buffer, shape (H, W, 4), row 0 = bottom."""

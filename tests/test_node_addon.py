@@ -147,3 +147,57 @@ def test_node_render_server_matches_python_jobs(addon_built, tmp_path, kind):
         ref = eng.job_read_accum(j)
         got = np.fromfile(tmp_path / f"accum{i}.f32", np.float32).reshape(q["height"], q["width"], 4)
         np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.gpu
+def test_node_viewport_call_sequence(addon_built, tmp_path):
+    """The viewport's calls through the Node drop-in (tests/node_viewport.js):
+    start() with no limit renders until stop() and no frame callback follows
+    stop(); a camera move + resetAccumulation() restarts at frame 1 with the
+    new view; start() during a render stops it; a sample limit ends the loop
+    with renderingFinishedCB; interaction mode gives one tile per frame; debug
+    views are refused. Every accumulation equals the Python engine's render of
+    the same frames with the same camera, bit for bit."""
+    from dspt import Engine, build_bvh, pack_frame_block, scenes
+    from dspt.scene import ViewportCamera
+    wl = scenes.sphere_field(n_spheres=16)
+    s = wl.scene
+    W, H = 64, 48
+    img = wl.ibl["image"]
+    np.asarray(s.vertexBuffer, np.float32).tofile(tmp_path / "verts.f32")
+    np.asarray(s.triangleIndexBuffer, np.uint32).tofile(tmp_path / "tris.u32")
+    np.asarray(s.bvhPositionBuffer, np.float32).tofile(tmp_path / "bvhpos.f32")
+    s.getFlatMaterialBuffer().astype(np.float32).tofile(tmp_path / "mats.f32")
+    np.asarray(img["data"], np.float32).tofile(tmp_path / "ibl.f32")
+    cam2 = ViewportCamera(position=(4.0, 2.5, 9.0), target=(0.0, 0.5, 0.0), fov=wl.camera.fov)
+    cams = [wl.camera, cam2]
+    meta = {"profile": "webgl-lean", "width": W, "height": H, "maxBounces": 6, "iblWidth": img["width"],
+            "iblHeight": img["height"], "framesBeforeStop": 12, "sampleLimit": 5,
+            "cameras": [{"fov": c.fov, "near": c.near, "matrixWorld": [float(x) for x in c.matrixWorld],
+                         "position": dict(zip("xyz", map(float, c.position)))} for c in cams]}
+    (tmp_path / "meta.json").write_text(json.dumps(meta))
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "node_viewport.js"), str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    print(info)
+    un, mv, lim = info["unbounded"], info["moved"], info["limited"]
+    assert un["sampleCount"] >= 12 and un["mode"] == "stopped" and un["finished"] == []
+    assert un["frameEvents"] == list(range(2, len(un["frameEvents"]) + 2))
+    assert mv["sampleCount"] >= 4
+    assert lim["sampleCount"] == 4 and lim["finished"] == [4] and lim["frameEvents"] == [2, 3, 4, 5]
+    assert info["interactive"]["frames"] == 5 and info["interactive"]["tiles"] == 5
+    assert "debug" in info["debug"] and info["lists"]["debugModes"] == 15
+    assert info["lists"]["tonemappingModes"] == ["None", "Reinhard", "Cineon", "AcesFilm"]
+    nodes, bidx = build_bvh(s.bvhPositionBuffer, s.bvhIndexBuffer)
+    eng = Engine(0)
+    eng.upload_scene(s.vertexBuffer, s.triangleIndexBuffer, nodes, bidx, s.getFlatMaterialBuffer())
+    tot = eng.upload_ibl(img["data"], img["width"], img["height"])
+    eng.resize(W, H)
+    for name, cam, n in (("unbounded", cams[0], un["sampleCount"]), ("moved", cams[1], mv["sampleCount"]),
+                         ("limited", cams[1], lim["sampleCount"])):
+        eng.set_frame(pack_frame_block(cam, W, H, max_bounces=6, ibl_size=(img["width"], img["height"]),
+                                       ibl_total_sum=tot, scene_counts=(s.num_triangles, len(nodes) // 16, len(bidx), 0)))
+        eng.render(1, n)
+        got = np.fromfile(tmp_path / f"{name}.f32", np.float32).reshape(H, W, 4)
+        np.testing.assert_array_equal(eng.read_accum().view(np.uint32), got.view(np.uint32), err_msg=name)
