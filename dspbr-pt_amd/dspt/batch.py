@@ -150,9 +150,12 @@ class Server:
     def __init__(self, batch):
         self.b = batch
 
-    def run(self, specs, clock=time.perf_counter, on_progress=None, keep=False):
+    def run(self, specs, clock=time.perf_counter, on_progress=None, keep=False, on_finish=None):
         """specs: JobSpec list with .arrival in seconds from start. Returns per-job
-        records {arrival, admitted, first_frame, done, frames, paths}."""
+        records {arrival, admitted, first_frame, done, frames, paths}.
+        on_progress(job_id, frames_done, frames_total) after every step;
+        on_finish(request_index, job_id) when a request's last frame is
+        folded, before its job is destroyed (read the result there)."""
         eng = self.b.eng
         eng.set_schedule(_lib.SCHED_STREAM)
         pending = sorted(range(len(specs)), key=lambda i: specs[i].arrival)
@@ -194,6 +197,8 @@ class Server:
                     on_progress(jid, d, tot)
                 if d >= tot:
                     rec[i]["done"] = now
+                    if on_finish:
+                        on_finish(i, jid)
                     if not keep:
                         eng.job_destroy(jid)
                     del live[jid]

@@ -550,19 +550,38 @@ class InstancedScene:
     def instanced_triangles(self):
         return int(np.asarray(self.mesh_count, np.int64)[self.inst_mesh].sum())
 
+    def world_boxes(self):
+        """f64[I, 2, 3]: world box of each instance (its mesh's object box,
+        corners transformed)."""
+        p = self.scene.vertexBuffer.reshape(-1, VERTEX_STRIDE)[:, 0:3].astype(np.float64)
+        out = np.zeros((len(self.inst_mesh), 2, 3))
+        mb = []
+        for m in range(len(self.mesh_vfirst)):
+            q = p[self.mesh_vfirst[m]:self.mesh_vfirst[m] + self.mesh_vcount[m]]
+            lo, hi = q.min(0), q.max(0)
+            mb.append(np.array([[x, y, z] for x in (lo[0], hi[0]) for y in (lo[1], hi[1]) for z in (lo[2], hi[2])]))
+        for i, m in enumerate(self.inst_mesh):
+            M = self.o2w[i].astype(np.float64)
+            c = mb[m] @ M[:, :3].T + M[:, 3]
+            out[i, 0], out[i, 1] = c.min(0), c.max(0)
+        return out
+
     def engine_args(self):
         s = self.scene
         return (s.vertexBuffer, s.triangleIndexBuffer, np.asarray(self.mesh_first, np.int64),
                 np.asarray(self.mesh_count, np.int64), self.o2w, self.inst_mesh, s.getFlatMaterialBuffer())
 
-    def flatten(self):
-        """(PathtracingSceneData in world space, inst_of_tri i32[T'], mesh_tri_of_tri i32[T'])."""
+    def flatten(self, instances=None):
+        """(PathtracingSceneData in world space, inst_of_tri i32[T'], mesh_tri_of_tri i32[T']).
+        instances: ids of the instances to bake (default all), in that order."""
         s = self.scene
         vb = s.vertexBuffer.reshape(-1, VERTEX_STRIDE)
         tri = s.triangleIndexBuffer.reshape(-1, 3).astype(np.int64)
         out_v, out_t, inst_of, mtri_of = [], [], [], []
         nv = 0
-        for i, m in enumerate(self.inst_mesh):
+        ids = range(len(self.inst_mesh)) if instances is None else [int(i) for i in instances]
+        for i in ids:
+            m = self.inst_mesh[i]
             M = self.o2w[i].astype(np.float64)
             R, tr = M[:, :3], M[:, 3]
             Nm = np.linalg.inv(R).T
