@@ -287,6 +287,23 @@ void oracle_postprocess(const float* accum, int W, int H, int mode, float exposu
     }
 }
 
+// Camera ray of pixel (px, py) with jitter (r0, r1), SH/renderer.frag:129-142:
+// out = {org.xyz, dir.xyz}.
+int oracle_camera_ray(const Frame* f, int W, int H, int px, int py, float r0, float r1, float* out) {
+  Ray<float> r = camera_ray<float>(*f, W, H, px, py, r0, r1);
+  out[0] = r.org.x; out[1] = r.org.y; out[2] = r.org.z;
+  out[3] = r.dir.x; out[4] = r.dir.y; out[5] = r.dir.z;
+  return 0;
+}
+
+// The analytic GGX albedo fits of the energy compensation (KF:5-91), in
+// double: out = {approxAvgAlbedoGGX(alpha), approxDirAlbedoGGX(alpha, cosT)}.
+int oracle_ggx_albedo_fits(double alpha, double cosT, double* out) {
+  out[0] = approxAvgAlbedoGGX<double>(alpha);
+  out[1] = approxDirAlbedoGGX<double>(alpha, cosT);
+  return 0;
+}
+
 // The pinned float transcendentals (pin_math.hpp) over n arguments:
 // fn 0 sin, 1 cos, 2 exp, 3 log, 4 pow(x, y), 5 atan2(x = y-argument, y = x-argument), 6 acos,
 // 7 sin and 8 cos through psincos.

@@ -47,6 +47,9 @@ def lib():
         L = ctypes.CDLL(ORACLE_LIB)
         L.oracle_ibl_cdf.restype = ctypes.c_double
         L.oracle_pin_math.argtypes = [ctypes.c_int, _f, _f, _f, ctypes.c_int64]
+        L.oracle_camera_ray.argtypes = [_f, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                        ctypes.c_float, _f]
+        L.oracle_ggx_albedo_fits.argtypes = [ctypes.c_double, ctypes.c_double, _d]
         L.oracle_ibl_cdf.argtypes = [_f, ctypes.c_int, ctypes.c_int, _f, _f, _f, _f]
         L.oracle_rng_seq.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _u32, _u32, _f, _u32]
         L.oracle_rng_step.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _u32]
@@ -251,3 +254,18 @@ def pin_math(name, x, y=None):
                               out.ctypes.data_as(_f), x.size)
     assert r == 0
     return out
+
+
+def camera_ray(block, W, H, px, py, r0=0.0, r1=0.0):
+    """(origin, direction) of the oracle's camera ray for pixel (px, py) with jitter (r0, r1)."""
+    out = np.zeros(6, np.float32)
+    b = np.ascontiguousarray(block, np.float32)
+    lib().oracle_camera_ray(b.ctypes.data_as(_f), W, H, px, py, r0, r1, out.ctypes.data_as(_f))
+    return out[:3], out[3:]
+
+
+def ggx_albedo_fits(alpha, cos_t):
+    """(approxAvgAlbedoGGX(alpha), approxDirAlbedoGGX(alpha, cos_t)) of the oracle, in double."""
+    out = np.zeros(2, np.float64)
+    lib().oracle_ggx_albedo_fits(float(alpha), float(cos_t), out.ctypes.data_as(_d))
+    return out[0], out[1]

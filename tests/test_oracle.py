@@ -34,9 +34,15 @@ wrong sign/index, swapped operand) fails:
                                 limit R12 + T^2 R23 / (1 - R12 R23)
   dispersion IORs (KF:1289-98)  Abbe spread n_F - n_C = (n_d - 1) dispersion / 20 and
                                 Cauchy's A + B/l^2 at the Fraunhofer C, d, F lines
-Parity unpinned: a conductor substrate whose channels lie on both sides of the
-film index (each channel's phase then enters one XYZ component, a trait of the
-closed form with no physical counterpart); checked for invariants only.
+  sheen, dielectric transmission (thick and thin, Snell + Schlick split +
+  straight-through), dielectric F0 with specular tint, diffuse transmission,
+  the Kulla-Conty energy compensation, the camera ray and the point light are
+  pinned in test_oracle_pins.py (published models and closed forms computed
+  there independently of the oracle's code); the transcendentals the float
+  build uses are pinned in test_pin_math.py.
+Checked for invariants only: the thin film over a conductor substrate whose
+channels lie on both sides of the film index (each channel's phase then enters
+one XYZ component, a trait of the closed form with no physical counterpart).
 """
 import math
 from fractions import Fraction
