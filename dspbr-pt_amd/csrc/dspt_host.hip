@@ -424,7 +424,7 @@ int ensure_pool(dspt_ctx* c) {
   int r = 0;
   for (PathState* ps : {&a, &b})
     if ((r = A.get(&ps->org, cap)) || (r = A.get(&ps->dir, cap)) || (r = A.get(&ps->pw, cap)) || (r = A.get(&ps->L, cap)) ||
-        (r = A.get(&ps->rng, cap)) || (r = A.get(&ps->job, cap)) || (r = A.get(&ps->bounce, cap)) || (r = A.get(&ps->fp, cap)) ||
+        (r = A.get(&ps->meta, cap)) ||
         (r = A.get(&ps->sig, cap)) || (r = A.get(&ps->prim, cap)))
       return r;
   if ((r = A.get(&a.hit, cap)) || (r = A.get(&a.hinst, cap)) || (r = A.get(&a.sh_org, cap)) || (r = A.get(&a.sh_dir, cap)) || (r = A.get(&a.sh_c, cap)) ||

@@ -23,9 +23,11 @@
 // Occupancy target of the shading kernel (waves per SIMD; 0 = compiler's
 // choice). Left alone the compiler gives the full-profile shading kernel
 // (BSDF mixture + material textures) more than 256 registers, i.e. one wave
-// per SIMD; at 2 it spills a few registers and runs about 1.5x faster.
+// per SIMD. At 3 (168 registers) it spills some of them to scratch and is
+// still faster than at 2 (no spills): the kernel waits on memory, and the
+// extra resident waves hide more of it.
 #ifndef DSPT_SHADE_WAVES
-#define DSPT_SHADE_WAVES 2
+#define DSPT_SHADE_WAVES 3
 #endif
 #if DSPT_SHADE_WAVES > 0
 #define SHADE_BOUNDS __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DSPT_SHADE_WAVES, DSPT_SHADE_WAVES)))
@@ -260,7 +262,8 @@ __global__ void k_ray_keys(const DevJob* __restrict__ jobs, PathState ps, const 
   __syncthreads();
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
     const int p = (int)q;
-    const uint32_t k = ray_key(jobs[ps.job[p]].S, ps.org[p], ps.dir[p]);
+    const float4 o = ps.org[p];
+    const uint32_t k = ray_key(jobs[__float_as_int(o.w)].S, o, ps.dir[p]);
     keys[p] = (uint16_t)k;
     atomicAdd(&h[k], 1u);
   }
@@ -323,7 +326,7 @@ __global__ void k_hit_keys(const DevJob* __restrict__ jobs, PathState ps, const 
     const int tri = __float_as_int(ps.hit[p].x);
     uint32_t k = 0u;
     if (tri >= 0) {
-      k = 1u + jobs[ps.job[p]].S.tri_mat[tri] % (uint32_t)(kOrderBins - 1);
+      k = 1u + jobs[__float_as_int(ps.org[p].w)].S.tri_mat[tri] % (uint32_t)(kOrderBins - 1);
     }
     keys[p] = (uint16_t)k;
     atomicAdd(&h[k], 1u);
@@ -371,14 +374,11 @@ __global__ void k_regen(PathState ps, const DevJob* __restrict__ jobs, const Pas
     const float* m = F.view;
     f3 wd = f3{m[0] * d.x + m[4] * d.y + m[8] * d.z, m[1] * d.x + m[5] * d.y + m[9] * d.z, m[2] * d.x + m[6] * d.y + m[10] * d.z};
     const int p = qb + i;
-    ps.org[p] = make_float4(F.cam[0], F.cam[1], F.cam[2], 0.0f);
-    ps.dir[p] = make_float4(wd.x, wd.y, wd.z, 0.0f);
+    ps.org[p] = make_float4(F.cam[0], F.cam[1], F.cam[2], __int_as_float(T.job));
+    ps.dir[p] = make_float4(wd.x, wd.y, wd.z, __int_as_float(0));
     ps.pw[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     ps.L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    ps.rng[p] = make_uint2(rng.x, rng.y);
-    ps.job[p] = T.job;
-    ps.bounce[p] = 0;
-    ps.fp[p] = make_int2(frame, pix);
+    ps.meta[p] = make_uint4(rng.x, rng.y, (uint32_t)frame, (uint32_t)pix);
     if (J->rec_L) ps.sig[p] = 2166136261u;
     issued++;
   }
@@ -485,7 +485,7 @@ __global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathSt
     if (q >= 0) {
       p = q < nsorted ? perm[q] : q;
       const float4 o = ps.org[p], d = ps.dir[p];
-      w.start(jobs[ps.job[p]].S, st, xyz(o), xyz(d), kTFAR_MAX);
+      w.start(jobs[__float_as_int(o.w)].S, st, xyz(o), xyz(d), kTFAR_MAX);
       active = true;
     }
     if (__ballot(active) == 0) {
@@ -539,12 +539,12 @@ __global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, Pat
     if (q >= 0) {
       p = queue[q];
       const float4 so = ps.sh_org[p];
-      flags = __float_as_int(so.w);
+      flags = __float_as_int(so.w) & 3;
       org = xyz(so);
       contrib = sp3(0.0f);
       phase = (flags & 1) ? 0 : 1;
       const float4 sd = phase == 0 ? ps.sh_dir[p] : ps.sh_pdir[p];
-      w.start(jobs[ps.job[p]].S, st, org, xyz(sd), sd.w);
+      w.start(jobs[__float_as_int(so.w) >> 2].S, st, org, xyz(sd), sd.w);
       active = true;
     }
     if (__ballot(active) == 0) {
@@ -632,7 +632,9 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
   unsigned n_paths = 0, n_hits = 0, n_miss = 0, n_nee = 0, n_ext = 0;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < count; q += (int64_t)gridDim.x * blockDim.x) {
     const int p = q < nsorted ? perm[q] : (int)q;
-    const int jid = ps.job[p];
+    const float4 o4 = ps.org[p], d4 = ps.dir[p];
+    const uint4 meta = ps.meta[p];
+    const int jid = __float_as_int(o4.w);
     const DevJob* J = jobs + jid;
     const bool rec = J->rec_L != nullptr;
     uint32_t sig = rec ? ps.sig[p] : 0u;
@@ -642,10 +644,9 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
       const SceneView& S = J->S;
       const FrameParams& F = J->F;
       const int maxB = (int)F.max_bounces;
-      const int bounce = ps.bounce[p];
+      const int bounce = __float_as_int(d4.w);
       float4 hit = ps.hit[p];
       int tri = __float_as_int(hit.x);
-      float4 o4 = ps.org[p], d4 = ps.dir[p];
       f3 org = xyz(o4), dir = xyz(d4);
       float4 pw4 = ps.pw[p];
       f3 pw = xyz(pw4);
@@ -674,8 +675,7 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
         out = finish(L, F.clamp_threshold, true);
         break;
       }
-      uint2 rs = ps.rng[p];
-      Rng rng{rs.x, rs.y};
+      Rng rng{meta.x, meta.y};
       // Russian roulette (renderer.frag:119-123)
       pw = pw * (1.0f / (1.0f - 0.1f));
       if (bounce > 2 && rng.uniform() <= 0.1f) {
@@ -697,18 +697,18 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
       // BSDF bounce (renderer.frag:94-116)
       f3 wo, bw = sp3(1.0f);
       float pdf = 1.0f;
-      bsdf::Closure cl;
-      bsdf::Frame3 fr;
-      bool have_closure = false;
+      // the closure is built once per hit, before the sample (the reference
+      // builds it for the sample and again for NEE; both builds are the same
+      // pure function of the material), so the glTF inputs are dead before
+      // the BSDF code and its registers
+      const bsdf::Closure cl = bsdf::build_closure(sh.in, sh.tanA, S.full_profile != 0);
+      const bsdf::Frame3 fr = bsdf::aniso_frame(sh.n, cl.tanA);
       if (rng.uniform() > sh.cutout) {
         wo = -wi;
         if (rec) sig_mix(sig, 2u);
       } else {
         float lx = rng.uniform(), ly = rng.uniform(), bz = rng.uniform();
         float comp = rng.uniform();
-        cl = bsdf::build_closure(sh.in, sh.tanA, S.full_profile != 0);
-        fr = bsdf::aniso_frame(sh.n, cl.tanA);
-        have_closure = true;
         bsdf::Sample bs = bsdf::sample(cl, fr, sh.n, wi, comp, lx, ly, bz);
         pdf = bs.pdf;
         bw = bs.w;
@@ -738,11 +738,6 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
         f3 d = ibl_sample(S, F, n0, n1, ipdf);
         float cosNL = sat(dot3(d, nn));
         if (cosNL > kEPS_COS && ipdf > kEPS_PDF) {
-          if (!have_closure) {
-            cl = bsdf::build_closure(sh.in, sh.tanA, S.full_profile != 0);
-            fr = bsdf::aniso_frame(sh.n, cl.tanA);
-            have_closure = true;
-          }
           f3 e = bsdf::eval(cl, fr, sh.n, wi, d) / gmax(fabsf(dot3(sh.n, d)), kEPS);
           float bp = bsdf::pdf(cl, fr, sh.n, wi, d);
           f3 Li = ibl_eval(S, F, d) * e * cosNL / ipdf;
@@ -765,11 +760,6 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
         pdir.x = dn.x; pdir.y = dn.y; pdir.z = dn.z;
         flags |= 2;
         if (cosNL > kEPS_COS) {
-          if (!have_closure) {
-            cl = bsdf::build_closure(sh.in, sh.tanA, S.full_profile != 0);
-            fr = bsdf::aniso_frame(sh.n, cl.tanA);
-            have_closure = true;
-          }
           f3 e = bsdf::eval(cl, fr, sh.n, wi, ld) / gmax(fabsf(dot3(sh.n, ld)), kEPS);
           c_pt = e * (f3{F.light_emission[0], F.light_emission[1], F.light_emission[2]} / dist2) * cosNL;
         }
@@ -782,27 +772,24 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
       if (flags) {
         const int k = atomicAdd(count_sh, 1);
         shq[k] = o;
-        ps.sh_org[o] = make_float4(pos.x, pos.y, pos.z, __int_as_float(flags));
+        ps.sh_org[o] = make_float4(pos.x, pos.y, pos.z, __int_as_float(flags | (jid << 2)));
         ps.sh_dir[o] = make_float4(ldir.x, ldir.y, ldir.z, kTFAR_MAX);
         ps.sh_c[o] = make_float4(c_ibl.x, c_ibl.y, c_ibl.z, 0.0f);
         ps.sh_pw[o] = make_float4(pw_nee.x, pw_nee.y, pw_nee.z, 0.0f);
         if (flags & 2) { ps.sh_pdir[o] = pdir; ps.sh_pc[o] = make_float4(c_pt.x, c_pt.y, c_pt.z, 0.0f); }
       }
-      nx.org[o] = make_float4(pos.x, pos.y, pos.z, 0.0f);
-      nx.dir[o] = make_float4(wo.x, wo.y, wo.z, 0.0f);
+      nx.org[o] = make_float4(pos.x, pos.y, pos.z, __int_as_float(jid));
+      nx.dir[o] = make_float4(wo.x, wo.y, wo.z, __int_as_float(bounce + 1));
       nx.pw[o] = make_float4(pw.x, pw.y, pw.z, last_pdf);
       nx.L[o] = make_float4(L.x, L.y, L.z, L4.w);
-      nx.rng[o] = make_uint2(rng.x, rng.y);
-      nx.job[o] = jid;
-      nx.bounce[o] = bounce + 1;
-      nx.fp[o] = ps.fp[p];
+      nx.meta[o] = make_uint4(rng.x, rng.y, meta.z, meta.w);
       if (rec) { nx.sig[o] = sig; nx.prim[o] = ps.prim[p]; }
       term = false;
     } while (false);
     if (term) {
       // finished sample -> the job's frame ring (+ records)
       n_paths++;
-      const int2 fp = ps.fp[p];
+      const int2 fp = make_int2((int)meta.z, (int)meta.w);
       const int rs = fp.x % J->ring_frames;
       J->ring[(int64_t)rs * J->npix + fp.y] = out;
       if (rec) {

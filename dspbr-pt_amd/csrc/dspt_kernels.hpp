@@ -52,24 +52,23 @@ struct StreamSched {
 };
 
 // Structure-of-arrays path state, capacity = pool size, indexed by
-// position. Two instances alternate per bounce (the per-path fields are
+// position. Every per-path field the shading kernel reads is a 16-byte
+// record (the job and bounce ride in the ray records' w), so a path read
+// through the shading order touches six cache lines. Two instances alternate per bounce (the per-path fields are
 // double-buffered); hit and the shadow-ray fields are shared: hit is written
 // and read at the current position, the shadow fields at the position the
 // path continues at.
 struct PathState {
-  float4* org;    // extension ray origin
-  float4* dir;    // extension ray direction
+  float4* org;    // extension ray origin xyz, job index of the path (int bits) in w
+  float4* dir;    // extension ray direction xyz, bounce index (int bits, 0 = camera ray) in w
   float4* hit;    // (tri id bits, t, u, v) of the last extension ray
   int32_t* hinst; // instance of that hit (two-level scenes; -1 otherwise)
   float4* pw;     // path weight xyz, last BSDF pdf in w
   float4* L;      // radiance xyz (+alpha)
-  uint2* rng;     // generator state
-  int32_t* job;   // job index of the path
-  int32_t* bounce;  // bounce index of the current extension ray (0 = camera ray)
-  int2* fp;       // (frame, pixel) of the path
+  uint4* meta;    // generator state (x, y), frame, pixel: one 16-byte record
   uint32_t* sig;  // path signature (recording jobs only)
   int32_t* prim;  // primary hit triangle (recording jobs only)
-  float4* sh_org;   // shadow origin xyz, flags bits in w (1 = IBL ray, 2 = point light ray)
+  float4* sh_org;   // shadow origin xyz; w (int bits) = flags (1 = IBL ray, 2 = point light ray) | job << 2
   float4* sh_dir;   // IBL shadow direction, tmax
   float4* sh_c;     // IBL NEE radiance before path weight
   float4* sh_pw;    // path weight at the time of NEE
