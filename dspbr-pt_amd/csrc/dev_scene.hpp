@@ -361,9 +361,11 @@ DI bool trav_step(const Geo& G, RayState& r, LaneStack& st, unsigned& nv, unsign
 // to trav_leaf, which tests its triangles before the ray pops again, so the
 // per-ray sequence of node visits, triangle tests and steps is exactly that
 // of trav_step. Returns 0 = continue, 1 = leaf pending, 2 = ray finished.
+// The caller never steps a finished ray: a ray starts unfinished, and the
+// wavefront loops retire it as soon as a step or a leaf test finishes it
+// (ray_done after every leaf), so the step opens without that test.
 template <bool FAST = false>
 DI int trav_inner(const Geo& G, RayState& r, LaneStack& st, unsigned& nv) {
-  if (ray_done(r)) return 2;
   r.step++;
   int ni;
   if (r.has_next) {
