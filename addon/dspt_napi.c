@@ -171,7 +171,7 @@ static napi_value js_set_frame(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
-enum { OP_RENDER = 0, OP_STEP = 1, OP_RUN = 2 };
+enum { OP_RENDER = 0, OP_STEP = 1, OP_RUN = 2, OP_REGION = 3 };
 
 typedef struct {
   napi_async_work work;
@@ -179,6 +179,7 @@ typedef struct {
   napi_ref handle_ref;
   Handle* h;
   int op, first, count, rc, active;
+  int x0, y0, rw, rh; /* OP_REGION: the tile */
   int32_t* ids;
   int nids;
   char err[512];
@@ -190,6 +191,9 @@ static void async_execute(napi_env env, void* data) {
   dspt_ctx* c = j->h->c;
   if (j->op == OP_RENDER) {
     j->rc = dspt_render(c, j->first, j->count);
+    if (!j->rc) j->rc = dspt_synchronize(c);
+  } else if (j->op == OP_REGION) {
+    j->rc = dspt_render_region(c, j->first, j->count, j->x0, j->y0, j->rw, j->rh);
     if (!j->rc) j->rc = dspt_synchronize(c);
   } else if (j->op == OP_STEP) {
     j->rc = dspt_jobs_step(c, j->ids, j->nids, &j->active);
@@ -247,6 +251,26 @@ static napi_value js_render(napi_env env, napi_callback_info info) {
   j->first = (int)get_i64(env, argv[1]);
   j->count = (int)get_i64(env, argv[2]);
   return queue_call(env, argv[0], j, "dspt_render");
+}
+
+/* renderRegion(h, firstFrame, numFrames, x0, y0, width, height) -> Promise<void>:
+ * one scissored tile pass (rows from the bottom) */
+static napi_value js_render_region(napi_env env, napi_callback_info info) {
+  size_t argc = 7;
+  napi_value argv[7];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 7) { napi_throw_type_error(env, NULL, "renderRegion(h, first, count, x0, y0, width, height)"); return NULL; }
+  if (!get_ctx(env, argv[0])) return NULL;
+  AsyncCall* j = (AsyncCall*)calloc(1, sizeof(AsyncCall));
+  j->h = get_handle(env, argv[0]);
+  j->op = OP_REGION;
+  j->first = (int)get_i64(env, argv[1]);
+  j->count = (int)get_i64(env, argv[2]);
+  j->x0 = (int)get_i64(env, argv[3]);
+  j->y0 = (int)get_i64(env, argv[4]);
+  j->rw = (int)get_i64(env, argv[5]);
+  j->rh = (int)get_i64(env, argv[6]);
+  return queue_call(env, argv[0], j, "dspt_render_region");
 }
 
 /* jobsStep(h, Int32Array ids | null) -> Promise<number of unfinished jobs>;
@@ -472,7 +496,7 @@ static napi_value js_job_read_accum(napi_env env, napi_callback_info info) {
  * setMemoryBudget(h, bytes), setLaunchConfig(h, maxPaths, traceBlocks,
  * shadeBlocks), setSchedule(h, mode, passFrames, chunkIters, ringFactor),
  * setPolicy(h, policy) */
-enum { S_JOB_DESTROY, S_SCENE_DESTROY, S_IBL_DESTROY, S_BUDGET, S_LAUNCH, S_SCHEDULE, S_POLICY };
+enum { S_JOB_DESTROY, S_SCENE_DESTROY, S_IBL_DESTROY, S_BUDGET, S_LAUNCH, S_SCHEDULE, S_POLICY, S_CLEAR_ACCUM };
 static napi_value setter(napi_env env, napi_callback_info info, int what) {
   size_t argc = 5;
   napi_value argv[5];
@@ -492,11 +516,13 @@ static napi_value setter(napi_env env, napi_callback_info info, int what) {
     case S_LAUNCH: rc = dspt_set_launch_config(c, a[0], (int32_t)a[1], (int32_t)a[2]); break;
     case S_SCHEDULE: rc = dspt_set_schedule(c, (int32_t)a[0], (int32_t)a[1], (int32_t)a[2], (float)rf); break;
     case S_POLICY: rc = dspt_set_policy(c, (int32_t)a[0]); break;
+    case S_CLEAR_ACCUM: rc = dspt_clear_accum(c); break;
   }
   if (rc) return throw_dspt(env, rc);
   return NULL;
 }
 static napi_value js_job_destroy(napi_env e, napi_callback_info i) { return setter(e, i, S_JOB_DESTROY); }
+static napi_value js_clear_accum(napi_env e, napi_callback_info i) { return setter(e, i, S_CLEAR_ACCUM); }
 static napi_value js_scene_destroy(napi_env e, napi_callback_info i) { return setter(e, i, S_SCENE_DESTROY); }
 static napi_value js_ibl_destroy(napi_env e, napi_callback_info i) { return setter(e, i, S_IBL_DESTROY); }
 static napi_value js_set_budget(napi_env e, napi_callback_info i) { return setter(e, i, S_BUDGET); }
@@ -623,6 +649,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"resize", NULL, js_resize, NULL, NULL, NULL, napi_enumerable, NULL},
       {"setFrame", NULL, js_set_frame, NULL, NULL, NULL, napi_enumerable, NULL},
       {"render", NULL, js_render, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"renderRegion", NULL, js_render_region, NULL, NULL, NULL, napi_enumerable, NULL},
+      {"clearAccum", NULL, js_clear_accum, NULL, NULL, NULL, napi_enumerable, NULL},
       {"readAccum", NULL, js_read_accum, NULL, NULL, NULL, napi_enumerable, NULL},
       {"postprocess", NULL, js_postprocess, NULL, NULL, NULL, napi_enumerable, NULL},
       {"counters", NULL, js_counters, NULL, NULL, NULL, napi_enumerable, NULL},

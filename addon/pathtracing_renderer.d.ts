@@ -23,10 +23,14 @@ export interface SceneBuffers {
   instances?: { meshFirst: Int32Array; meshCount: Int32Array; objectToWorld: Float32Array; mesh: Int32Array };
 }
 export declare class PathtracingRenderer {
-  constructor(parameters?: { materialProfile?: 'webgl-lean' | 'webgl-full'; device?: number; width?: number; height?: number });
+  constructor(parameters?: { materialProfile?: 'webgl-lean' | 'webgl-full'; device?: number; width?: number; height?: number;
+                             tileSchedule?: 'frames' | 'tiles' });
   maxBounces: number; useIBL: boolean; showBackground: boolean; iblRotation: number;
   backgroundColor: number[]; rayEps: number; clampThreshold: number; exposure: number;
   tonemapping: string; enableGamma: boolean; enableFxaa: boolean; pixelRatio: number; tileRes: number;
+  // 'frames' (default): whole frames per native call; 'tiles': the reference's scissored tile passes
+  tileSchedule: 'frames' | 'tiles';
+  readonly debugModes: string[]; readonly tonemappingModes: string[]; debugMode: string;
   readonly diagnostics: { backend: string; sampleCount: number; mode: string; profiling: Record<string, number> };
   setScene(scene: SceneBuffers, progress?: (e: { phase: string }) => void): Promise<void>;
   setIBL(texture: { image: { data: Float32Array; width: number; height: number } }): void;
@@ -43,6 +47,9 @@ export declare class PathtracingRenderer {
   counters(): Float64Array;
 }
 export declare function packFrameBlock(r: PathtracingRenderer, camera: PathtracingCamera, frameCount: number): Float32Array;
+// renderer.ts:573-593 tile order and renderer.ts:632-643 scissor rectangle [x0, y0, width, height] (clipped; null if empty)
+export declare function tileOrder(tileRes: number): number[];
+export declare function tileRect(tile: number, tileRes: number, renderRes: [number, number]): [number, number, number, number] | null;
 
 export interface RenderRequest {
   scene: number; ibl: number; camera: PathtracingCamera; width: number; height: number; spp: number;

@@ -49,6 +49,13 @@ class PathtracingRenderer {
     this._backgroundColor = [0, 0, 0, 1]; this._rayEps = 0.0001; this._clampThreshold = 3.0;
     this._exposure = 1.0; this._tonemapping = 'None'; this._enableGamma = true; this._enableFxaa = false;
     this._pixelRatio = 1.0; this._tileRes = 4; this._debugMode = 'None'; this._forceIBLEval = false;
+    // 'frames': whole frames per native call (a 1080p frame takes milliseconds
+    // here, shorter than one of the reference's tiles); 'tiles': the
+    // reference's scissored tile passes, one native call per tile
+    this._tileSchedule = parameters.tileSchedule || 'frames';
+    if (this._tileSchedule !== 'frames' && this._tileSchedule !== 'tiles')
+      throw new Error(`unknown tile schedule ${this._tileSchedule}`);
+    this._tileIndex = 0;
     // renderer.ts:181,192
     this.debugModes = ['None', 'Albedo', 'Metalness', 'Roughness', 'Normals', 'Tangents', 'Bitangents', 'Transparency',
                        'UV0', 'Clearcoat', 'IBL PDF', 'IBL CDF', 'Specular', 'SpecularTint', 'Translucency'];
@@ -81,6 +88,11 @@ class PathtracingRenderer {
   get enableFxaa() { return this._enableFxaa; } set enableFxaa(v) { this._enableFxaa = v; }
   get pixelRatio() { return this._pixelRatio; } set pixelRatio(v) { this._pixelRatio = v; this.resize(this.displayRes[0], this.displayRes[1]); }
   get tileRes() { return this._tileRes; } set tileRes(v) { this._tileRes = Math.max(2, Math.floor(v) || 2); }
+  get tileSchedule() { return this._tileSchedule; }
+  set tileSchedule(v) {
+    if (v !== 'frames' && v !== 'tiles') throw new Error(`unknown tile schedule ${v}`);
+    this._tileSchedule = v; this.resetAccumulation();
+  }
   // renderer.ts:182-190. The debug integrators (AOV views) are not part of
   // this engine: any mode but 'None' is accepted here and refused by render().
   get debugMode() { return this._debugMode; } set debugMode(v) { this._debugMode = v; this.resetAccumulation(); }
@@ -140,6 +152,7 @@ class PathtracingRenderer {
   // (with the camera and settings of that moment)
   resetAccumulation() {
     this._frameCount = 1;
+    this._tileIndex = 0;
     this._resetPending = true;
     if (this._isRendering && this.schedulerMode === 'accumulating') this.schedulerMode = 'settling';
     this._updateDiagnostics();
@@ -168,10 +181,15 @@ class PathtracingRenderer {
   // 1 .. numSamples-1), or until stopRendering() when numSamples is -1. The
   // camera object is read again for every call, so a caller that moves it and
   // calls resetAccumulation() (as the viewport does) restarts with the new
-  // view. Frames are submitted in calls of growing size (1, 2, 4, ... frames,
-  // up to about one path pool); per frame, tileFinishedCB runs once per tile
-  // of the active tile order and frameFinishedCB(frameCount) once. Returns a
-  // promise that settles when the loop ends; errors stop the loop and reject it.
+  // view. With tileSchedule 'frames', frames are submitted in calls of
+  // growing size (1, 2, 4, ... frames, up to about one path pool); per frame,
+  // tileFinishedCB runs once per tile of the active tile order and
+  // frameFinishedCB(frameCount) once. With 'tiles', every tile of the active
+  // tile order (renderer.ts:567-593) is its own native call over the
+  // scissor rectangle of renderer.ts:632-643, followed by tileFinishedCB,
+  // and frameFinishedCB follows the frame's last tile, as in the reference.
+  // Returns a promise that settles when the loop ends; errors stop the loop
+  // and reject it.
   render(camera, numSamples, tileFinishedCB, frameFinishedCB, renderingFinishedCB, framesPerCall) {
     this.stopRendering();
     if (this._debugMode !== 'None')
@@ -196,11 +214,19 @@ class PathtracingRenderer {
         while (this._inflight) await this._inflight.catch(() => {});  // a call of an earlier loop
         if (gen !== this._gen || !this._isRendering) return;
         this._applyPending();
-        if (this._resetPending) { this._resetPending = false; this._frameCount = 1; ramp = 1; }
+        const tiles = this._tileSchedule === 'tiles';
+        if (this._resetPending) {
+          this._resetPending = false; this._frameCount = 1; this._tileIndex = 0; ramp = 1;
+          if (tiles) native.clearAccum(this._h);  // a tile pass never clears: the reset does (renderer.ts:604-621)
+        }
         if (this._numSamples !== -1 && this._frameCount >= this._numSamples) {
           this.stopRendering();
           if (this._cb.done) this._cb.done();
           return;
+        }
+        if (tiles) {
+          await this._tilePass(gen);
+          continue;
         }
         const first = this._frameCount;
         const cap = Math.max(1, Math.floor(PATHS_PER_CALL / (this.renderRes[0] * this.renderRes[1])));
@@ -234,6 +260,38 @@ class PathtracingRenderer {
       if (gen === this._gen) this.stopRendering();
       throw e;
     }
+  }
+
+  // One tile of the active tile order at the current frame (renderer.ts:595-714)
+  async _tilePass(gen) {
+    const tr = this._activeTileRes();
+    const order = tileOrder(tr);
+    const tile = order[this._tileIndex % order.length];
+    const rect = tileRect(tile, tr, this.renderRes);
+    const frame = this._frameCount;
+    if (rect) {
+      native.setFrame(this._h, packFrameBlock(this, this._camera, frame));
+      const call = native.renderRegion(this._h, frame, 1, rect[0], rect[1], rect[2], rect[3]);
+      this._inflight = call;
+      this.diagnostics.framePending = true;
+      try {
+        await call;
+      } finally {
+        if (this._inflight === call) this._inflight = null;
+        this.diagnostics.framePending = false;
+      }
+    }
+    if (this._resetPending) return;  // reset while on the GPU: the next pass restarts
+    const live = gen === this._gen && this._isRendering;
+    this._tileIndex++;
+    if (live && this._cb.tile) this._cb.tile();
+    if (this._tileIndex >= tr * tr) {
+      this._tileIndex = 0;
+      this._frameCount++;
+      if (live && this._cb.frame) this._cb.frame(this._frameCount);
+      if (this.schedulerMode === 'settling') this.schedulerMode = 'accumulating';
+    }
+    this._updateDiagnostics();
   }
 
   // renderer.ts:567-571
@@ -293,6 +351,38 @@ class PathtracingRenderer {
 }
 
 const TONEMAPPING_MODES = ['None', 'Reinhard', 'Cineon', 'AcesFilm'];
+
+// renderer.ts:573-593: the tiles of a tileRes x tileRes grid, coarse strides
+// first (every stride-th tile of every stride-th row, stride halving down to 1)
+const TILE_ORDERS = new Map();
+function tileOrder(tileRes) {
+  if (TILE_ORDERS.has(tileRes)) return TILE_ORDERS.get(tileRes);
+  const tiles = [];
+  const used = new Set();
+  for (let stride = tileRes; stride >= 1; stride = Math.floor(stride / 2)) {
+    for (let y = 0; y < tileRes; y += stride)
+      for (let x = 0; x < tileRes; x += stride) {
+        const t = y * tileRes + x;
+        if (!used.has(t)) { used.add(t); tiles.push(t); }
+      }
+    if (stride === 1) break;
+  }
+  TILE_ORDERS.set(tileRes, tiles);
+  return tiles;
+}
+
+// renderer.ts:632-643: the scissor rectangle of a tile in GL window
+// coordinates (row 0 at the bottom; tile rows counted from the top), clipped
+// to the render target as the scissor test clips it: [x0, y0, width, height],
+// or null when nothing of it is inside
+function tileRect(tile, tileRes, renderRes) {
+  const tilex = tile % tileRes, tiley = Math.floor(tile / tileRes);
+  const x0 = Math.ceil(tilex * renderRes[0] / tileRes);
+  const y0 = Math.ceil((tileRes - tiley - 1) * renderRes[1] / tileRes);
+  const x1 = Math.min(x0 + Math.ceil(renderRes[0] / tileRes), renderRes[0]);
+  const y1 = Math.min(y0 + Math.ceil(renderRes[1] / tileRes), renderRes[1]);
+  return x1 > x0 && y1 > y0 ? [x0, y0, x1 - x0, y1 - y0] : null;
+}
 
 // PNG of an RGBA8 canvas given bottom row first (GL order): rows are written top first
 const CRC_TABLE = (() => {
@@ -462,4 +552,4 @@ class RenderServer {
   memoryUsage() { return native.memoryUsage(this._h); }
 }
 
-module.exports = { PathtracingRenderer, RenderServer, packFrameBlock, native };
+module.exports = { PathtracingRenderer, RenderServer, packFrameBlock, tileOrder, tileRect, native };

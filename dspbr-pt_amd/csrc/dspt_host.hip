@@ -162,7 +162,11 @@ struct JobRes {
   int32_t* rec_prim = nullptr;
   int rec_first = 1, rec_count = 0;
   bool alive = false, cancel = false;
+  // render region (rw == 0: the whole image), set per call by the
+  // single-image API; its pixels are the work items of each frame
+  int rx0 = 0, ry0 = 0, rw = 0, rh = 0;
   int64_t npix() const { return (int64_t)W * H; }
+  int64_t items() const { return rw > 0 ? (int64_t)rw * rh : npix(); }
   bool complete() const { return acc_known >= frame_end; }
   bool in_flight() const { return next_sched > acc_known; }
 };
@@ -385,6 +389,8 @@ int sync_job_table(dspt_ctx* c) {
       d.S = view_of(c, j);
       d.F = j->F;
       d.W = j->W; d.H = j->H; d.npix = (int32_t)j->npix();
+      d.rx0 = j->rw > 0 ? j->rx0 : 0; d.ry0 = j->rw > 0 ? j->ry0 : 0;
+      d.rw = j->rw > 0 ? j->rw : j->W; d.rn = (int32_t)j->items();
       d.accum = j->accum;
       d.ring = j->ring; d.done = j->done; d.ring_frames = j->ring_frames; d.frame_end = j->frame_end;
       d.rec_L = j->record ? j->rec_L : nullptr;
@@ -456,7 +462,7 @@ bool eligible(const dspt_ctx* c, const JobRes* j) {
 
 bool append_frames(dspt_ctx* c, int id, int nf) {
   JobRes* j = c->jobs[id];
-  const int64_t npix = j->npix();
+  const int64_t npix = j->items();
   if (!c->tasks.empty()) {
     PassTask& b = c->tasks.back();
     if (b.job == id && b.first_frame + b.nframes == j->next_sched && b.offset + (int64_t)b.nframes * b.npix == c->sched_limit) {
@@ -486,7 +492,7 @@ void schedule(dspt_ctx* c, const std::vector<int>& ids) {
       if (!job_ok(c, id)) continue;
       JobRes* j = c->jobs[id];
       if (!eligible(c, j)) continue;
-      int64_t npix = j->npix();
+      int64_t npix = j->items();
       int64_t nf = std::min<int64_t>(j->frame_end - j->next_sched, j->acc_known + j->ring_frames - j->next_sched);
       if (c->pass_frames > 0) nf = std::min<int64_t>(nf, c->pass_frames);
       nf = std::min<int64_t>(nf, room / npix);
@@ -526,12 +532,12 @@ void schedule(dspt_ctx* c, const std::vector<int>& ids) {
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
       const JobRes* x = c->jobs[a];
       const JobRes* y = c->jobs[b];
-      return (int64_t)(x->frame_end - x->next_sched) * x->npix() < (int64_t)(y->frame_end - y->next_sched) * y->npix();
+      return (int64_t)(x->frame_end - x->next_sched) * x->items() < (int64_t)(y->frame_end - y->next_sched) * y->items();
     });
   for (int id : order) {
     JobRes* j = c->jobs[id];
     while (c->sched_limit < target && eligible(c, j)) {
-      int64_t want = (target - c->sched_limit + j->npix() - 1) / j->npix();
+      int64_t want = (target - c->sched_limit + j->items() - 1) / j->items();
       int64_t nf = std::min<int64_t>({want, (int64_t)(j->frame_end - j->next_sched),
                                       (int64_t)(j->acc_known + j->ring_frames - j->next_sched)});
       if (nf <= 0 || !append_frames(c, id, (int)nf)) return;
@@ -581,7 +587,7 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
     JobRes* j = c->jobs[i];
     if (j && j->alive && j->in_flight()) {
       hj[nj++] = (int32_t)i;
-      max_npix = std::max(max_npix, j->npix());
+      max_npix = std::max(max_npix, j->items());
     }
   }
   if (nt > 0) HIPCHK(hipMemcpyAsync(c->d_tasks, ht, (size_t)nt * sizeof(PassTask), hipMemcpyHostToDevice, c->stream));
@@ -1593,13 +1599,20 @@ int dspt_set_record_samples(dspt_ctx* c, int enable) {
   return 0;
 }
 
-int dspt_render(dspt_ctx* c, int32_t first_frame, int32_t num_frames) {
+namespace {
+
+// Frames first .. first+num_frames-1 of the implicit job over the region
+// [x0, x0+w) x [y0, y0+h) (w == 0: the whole image, which restarts the
+// accumulation at frame 1).
+int render_impl(dspt_ctx* c, int32_t first_frame, int32_t num_frames, int32_t x0, int32_t y0, int32_t w, int32_t h) {
   if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
   if (c->scenes.empty() || !c->scenes[0]) return fail(DSPT_ERR_STATE, "render before dspt_upload_scene");
   if (!c->have_frame) return fail(DSPT_ERR_STATE, "render before dspt_set_frame");
   if (c->W <= 0) return fail(DSPT_ERR_STATE, "render before dspt_resize");
   if (first_frame < 1 || num_frames < 0) return fail(DSPT_ERR_ARG, "frames are 1-based and num_frames >= 0");
   if ((int64_t)first_frame + num_frames >= (1LL << 30)) return fail(DSPT_ERR_ARG, "frame index out of range");
+  if (w != 0 && (x0 < 0 || y0 < 0 || w < 1 || h < 1 || (int64_t)x0 + w > c->W || (int64_t)y0 + h > c->H))
+    return fail(DSPT_ERR_ARG, "region outside the image");
   HIPCHK(hipSetDevice(c->device));
   int ibl = (!c->ibls.empty() && c->ibls[0]) ? 0 : -1;
   if (int r = check_job_frame(c, c->F, 0, ibl)) return r;
@@ -1626,6 +1639,11 @@ int dspt_render(dspt_ctx* c, int32_t first_frame, int32_t num_frames) {
   JobRes* j = c->jobs[id];
   j->F = c->F;
   j->ibl = ibl;
+  const bool whole = w == 0 || (x0 == 0 && y0 == 0 && w == c->W && h == c->H);
+  j->rx0 = whole ? 0 : x0;
+  j->ry0 = whole ? 0 : y0;
+  j->rw = whole ? 0 : w;
+  j->rh = whole ? 0 : h;
   j->rec_first = first_frame;
   j->frame_begin = first_frame;
   j->frame_end = first_frame + num_frames;
@@ -1635,11 +1653,36 @@ int dspt_render(dspt_ctx* c, int32_t first_frame, int32_t num_frames) {
   j->epoch = c->chunk_no;
   c->jobs_dirty = true;
   k_fill_i32<<<1, 64, 0, c->stream>>>(c->d_prog + id, 1, first_frame);
-  if (first_frame == 1)
+  if (first_frame == 1 && w == 0)
     k_fill_f4<<<grid_for(j->npix(), 256), 256, 0, c->stream>>>(j->accum, j->npix(), make_float4(0, 0, 0, 0));
   HIPCHK(hipGetLastError());
   if (int r = run_until_complete(c, {id})) return r;
   c->last_frames = num_frames;
+  return 0;
+}
+
+}  // namespace
+
+int dspt_render(dspt_ctx* c, int32_t first_frame, int32_t num_frames) {
+  return render_impl(c, first_frame, num_frames, 0, 0, 0, 0);
+}
+
+int dspt_render_region(dspt_ctx* c, int32_t first_frame, int32_t num_frames, int32_t x0, int32_t y0, int32_t width,
+                       int32_t height) {
+  if (width <= 0 || height <= 0) return fail(DSPT_ERR_ARG, "region width and height must be positive");
+  return render_impl(c, first_frame, num_frames, x0, y0, width, height);
+}
+
+int dspt_clear_accum(dspt_ctx* c) {
+  if (!c) return fail(DSPT_ERR_ARG, "ctx is NULL");
+  if (c->legacy_job < 0 || !job_ok(c, c->legacy_job)) return 0;  // nothing accumulated yet
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (int r = settle(c)) return r;
+  JobRes* j = c->jobs[c->legacy_job];
+  k_fill_f4<<<grid_for(j->npix(), 256), 256, 0, c->stream>>>(j->accum, j->npix(), make_float4(0, 0, 0, 0));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
 

@@ -357,9 +357,10 @@ __global__ void k_regen(PathState ps, const DevJob* __restrict__ jobs, const Pas
     const DevJob* J = jobs + T.job;
     int64_t local = item - T.offset;
     int fslot = (int)(local / T.npix);
-    int pix = (int)(local - (int64_t)fslot * T.npix);
+    int rpix = (int)(local - (int64_t)fslot * T.npix);  // pixel of the render region
     int W = J->W, H = J->H;
-    int px = pix % W, py = pix / W;
+    int px = J->rx0 + rpix % J->rw, py = J->ry0 + rpix / J->rw;
+    int pix = py * W + px;
     int frame = T.first_frame + fslot;
     const FrameParams& F = J->F;
     Rng rng = rng_seed(frame * (int)F.max_bounces, px, py);
@@ -809,20 +810,22 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
 }
 
 // Running mean (renderer.frag:153-156) over every frame that is complete,
-// in frame order, starting at the job's next frame to fold (grid.y = job).
+// in frame order, starting at the job's next frame to fold (grid.y = job),
+// for the pixels of the job's render region (one thread each).
 __global__ void k_accum_ready(const DevJob* __restrict__ jobs, const int32_t* __restrict__ job_list,
                               const int32_t* __restrict__ prog) {
   const int jid = job_list[blockIdx.y];
   const DevJob* J = jobs + jid;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int npix = J->npix;
-  if (i >= npix) return;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;  // pixel of the render region
+  const int npix = J->npix, rn = J->rn;
+  if (r >= rn) return;
   const int R = J->ring_frames, end = J->frame_end;
   int f = prog[jid];
-  if (f >= end || J->done[f % R] != npix) return;
+  if (f >= end || J->done[f % R] != rn) return;
+  const int i = (J->ry0 + r / J->rw) * J->W + J->rx0 + r % J->rw;
   float4 a = J->accum[i];
   // at most R frames: the ring slot after them still holds the first one
-  for (int k = 0; k < R && f < end && J->done[f % R] == npix; k++) {
+  for (int k = 0; k < R && f < end && J->done[f % R] == rn; k++) {
     float4 c = J->ring[(int64_t)(f % R) * npix + i];
     float N = float(f);
     a.x = (a.x * (N - 1.0f) + c.x) / N;
@@ -840,9 +843,9 @@ __global__ void k_advance(const DevJob* __restrict__ jobs, const int32_t* __rest
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const int jid = job_list[k];
     const DevJob* J = jobs + jid;
-    const int R = J->ring_frames, end = J->frame_end, npix = J->npix;
+    const int R = J->ring_frames, end = J->frame_end, rn = J->rn;
     int f = prog[jid];
-    for (int k = 0; k < R && f < end && J->done[f % R] == npix; k++) {
+    for (int k = 0; k < R && f < end && J->done[f % R] == rn; k++) {
       J->done[f % R] = 0;
       f++;
     }

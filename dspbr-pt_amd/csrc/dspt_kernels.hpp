@@ -10,6 +10,12 @@ struct DevJob {
   SceneView S;
   FrameParams F;
   int32_t W, H, npix, rec_first;  // rec_first: first frame held by the record buffers
+  // render region: the pixels [rx0, rx0 + rw) x [ry0, ry0 + rn / rw) of the
+  // image (rows from the bottom); the whole image unless a caller renders a
+  // tile (the reference's scissored tile passes, renderer.ts:632-643). The
+  // region's pixels are the work items of a frame, its count completes the
+  // frame and the fold covers them only.
+  int32_t rx0, ry0, rw, rn;
   float4* accum;
   float4* ring;                   // [frame % ring_frames][pixel] finished samples
   int32_t* done;                  // [frame % ring_frames] finished paths of that frame

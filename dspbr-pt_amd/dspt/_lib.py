@@ -42,6 +42,8 @@ SIGNATURES = {
     "dspt_resize": (_c_int, [_vp, _c_i32, _c_i32]),
     "dspt_set_frame": (_c_int, [_vp, _fp]),
     "dspt_render": (_c_int, [_vp, _c_i32, _c_i32]),
+    "dspt_render_region": (_c_int, [_vp, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32]),
+    "dspt_clear_accum": (_c_int, [_vp]),
     "dspt_synchronize": (_c_int, [_vp]),
     "dspt_read_accum": (_c_int, [_vp, _fp]),
     "dspt_set_record_samples": (_c_int, [_vp, _c_int]),
@@ -234,6 +236,14 @@ class Engine:
 
     def render(self, first_frame, num_frames):
         check(self._L.dspt_render(self.h, int(first_frame), int(num_frames)))
+
+    def render_region(self, first_frame, num_frames, x0, y0, width, height):
+        """Frames of the pixels [x0, x0+width) x [y0, y0+height) only (row 0 = bottom)."""
+        check(self._L.dspt_render_region(self.h, int(first_frame), int(num_frames), int(x0), int(y0), int(width),
+                                         int(height)))
+
+    def clear_accum(self):
+        check(self._L.dspt_clear_accum(self.h))
 
     def synchronize(self):
         check(self._L.dspt_synchronize(self.h))

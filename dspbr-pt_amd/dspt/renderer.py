@@ -47,6 +47,39 @@ def pack_frame_block(camera, width, height, *, frame_count=1, max_bounces=8, use
     return block
 
 
+def tile_order(tile_res):
+    """renderer.ts:573-593: the tiles of a tile_res x tile_res grid, coarse
+    strides first (every stride-th tile of every stride-th row, stride
+    halving down to 1)."""
+    tiles, used = [], set()
+    stride = tile_res
+    while stride >= 1:
+        for y in range(0, tile_res, stride):
+            for x in range(0, tile_res, stride):
+                t = y * tile_res + x
+                if t not in used:
+                    used.add(t)
+                    tiles.append(t)
+        if stride == 1:
+            break
+        stride //= 2
+    return tiles
+
+
+def tile_rect(tile, tile_res, render_res):
+    """renderer.ts:632-643: the scissor rectangle of a tile in GL window
+    coordinates (row 0 at the bottom; tile rows counted from the top),
+    clipped to the render target as the scissor test clips it:
+    (x0, y0, width, height), or None when nothing of it is inside."""
+    W, H = render_res
+    tx, ty = tile % tile_res, tile // tile_res
+    x0 = -((-tx * W) // tile_res)  # ceil of the JS double quotient (exact for integers)
+    y0 = -((-(tile_res - ty - 1) * H) // tile_res)
+    x1 = min(x0 - ((-W) // tile_res), W)
+    y1 = min(y0 - ((-H) // tile_res), H)
+    return (x0, y0, x1 - x0, y1 - y0) if x1 > x0 and y1 > y0 else None
+
+
 class PathtracingRenderer:
     """MI355X replacement of the reference PathtracingRenderer (renderer.ts:116)."""
 
@@ -55,9 +88,15 @@ class PathtracingRenderer:
                   "Clearcoat", "IBL PDF", "IBL CDF", "Specular", "SpecularTint", "Translucency"]
     tonemappingModes = ["None", "Reinhard", "Cineon", "AcesFilm"]
 
-    def __init__(self, materialProfile="webgl-lean", device=0, width=1, height=1):
+    def __init__(self, materialProfile="webgl-lean", device=0, width=1, height=1, tileSchedule="frames"):
         if materialProfile not in ("webgl-lean", "webgl-full"):
             raise ValueError(f"unknown material profile {materialProfile!r}")
+        if tileSchedule not in ("frames", "tiles"):
+            raise ValueError(f"unknown tile schedule {tileSchedule!r}")
+        # 'frames': whole frames per engine call; 'tiles': the reference's
+        # scissored tile passes (renderer.ts:567-643), one engine call per tile
+        self._tileSchedule = tileSchedule
+        self._tileIndex = 0
         self.materialProfile = materialProfile
         self._engine = _lib.Engine(_lib.PROFILE_FULL if materialProfile == "webgl-full" else _lib.PROFILE_LEAN, device)
         # settings with the reference defaults (renderer.ts:172-306)
@@ -138,6 +177,17 @@ class PathtracingRenderer:
         self._tileRes = max(2, int(v) or 2)
 
     @property
+    def tileSchedule(self):
+        return self._tileSchedule
+
+    @tileSchedule.setter
+    def tileSchedule(self, v):
+        if v not in ("frames", "tiles"):
+            raise ValueError(f"unknown tile schedule {v!r}")
+        self._tileSchedule = v
+        self.resetAccumulation()
+
+    @property
     def debugMode(self):
         return self._debugMode
 
@@ -207,6 +257,7 @@ class PathtracingRenderer:
         """renderer.ts:327-336: the next frame restarts at frame 1 (with the
         camera and settings of that moment)."""
         self._frameCount = 1
+        self._tileIndex = 0
         self._reset_pending = True
         if self._isRendering and self.schedulerMode == "accumulating":
             self.schedulerMode = "settling"
@@ -269,7 +320,11 @@ class PathtracingRenderer:
         again for every call, and resetAccumulation() restarts at frame 1. Per
         frame, tileFinishedCB runs once per tile of the active tile order and
         frameFinishedCB(frameCount) once. Frames of a call that was on the GPU
-        when stopRendering() came are counted without callbacks."""
+        when stopRendering() came are counted without callbacks. With
+        tileSchedule 'tiles' every tile of the active tile order is its own
+        engine call over its scissor rectangle, followed by tileFinishedCB,
+        and frameFinishedCB follows the frame's last tile, as in the
+        reference (renderer.ts:595-714)."""
         self.stopRendering()
         if self._debugMode != "None":
             raise NotImplementedError(f"debugMode {self._debugMode!r}: the debug views are not rendered by this engine")
@@ -278,13 +333,19 @@ class PathtracingRenderer:
         self.resetAccumulation()
         ramp = 1
         while self._isRendering:
+            tiles = self._tileSchedule == "tiles"
             if self._reset_pending:
-                self._reset_pending, self._frameCount, ramp = False, 1, 1
+                self._reset_pending, self._frameCount, self._tileIndex, ramp = False, 1, 0, 1
+                if tiles:
+                    self._engine.clear_accum()  # a tile pass never clears: the reset does (renderer.ts:604-621)
             if num_samples != -1 and self._frameCount >= num_samples:
                 self.stopRendering()
                 if renderingFinishedCB:
                     renderingFinishedCB()
                 return
+            if tiles:
+                self._tile_pass(camera, tileFinishedCB, frameFinishedCB)
+                continue
             first = self._frameCount
             cap = max(1, self.PATHS_PER_CALL // (self.renderRes[0] * self.renderRes[1]))
             if frames_per_call:
@@ -311,6 +372,30 @@ class PathtracingRenderer:
                     self.schedulerMode = "accumulating"
             self._update_diagnostics()
             ramp = min(ramp * 2, cap)
+
+    def _tile_pass(self, camera, tileFinishedCB, frameFinishedCB):
+        """One tile of the active tile order at the current frame (renderer.ts:595-714)."""
+        tr = self._active_tile_res()
+        order = tile_order(tr)
+        rect = tile_rect(order[self._tileIndex % len(order)], tr, self.renderRes)
+        frame = self._frameCount
+        if rect is not None:
+            self._engine.set_frame(self.frame_block(camera, frame))
+            self._engine.render_region(frame, 1, *rect)
+            self._engine.synchronize()
+        if self._reset_pending:
+            return  # reset from another thread meanwhile: the next pass restarts
+        self._tileIndex += 1
+        if self._isRendering and tileFinishedCB:
+            tileFinishedCB()
+        if self._tileIndex >= tr * tr:
+            self._tileIndex = 0
+            self._frameCount += 1
+            if self._isRendering and frameFinishedCB:
+                frameFinishedCB(self._frameCount)
+            if self.schedulerMode == "settling":
+                self.schedulerMode = "accumulating"
+        self._update_diagnostics()
 
     def readAccumulation(self):
         """Linear RGBA f32 accumulation buffer, shape (H, W, 4), row 0 = bottom."""

@@ -97,6 +97,21 @@ int dspt_set_frame(dspt_ctx* ctx, const float block[DSPT_FRAME_FLOATS]);
 int dspt_render(dspt_ctx* ctx, int32_t first_frame, int32_t num_frames);
 int dspt_synchronize(dspt_ctx* ctx);
 
+/* One scissored tile pass of RT.renderFrame (RT:632-643): frames first_frame
+ * .. first_frame+num_frames-1 of the pixels [x0, x0+width) x [y0, y0+height)
+ * only (GL window coordinates: row 0 is the bottom row, as gl.scissor), each
+ * folded into those pixels' running mean with u_frame_count = its frame
+ * (renderer.frag:153-156); every other pixel keeps its value. A tile pass
+ * never clears, first_frame 1 included: as in the reference, the clear
+ * belongs to the reset (dspt_clear_accum). Overlapping calls for one frame
+ * fold that frame again into the overlap, as overlapping scissor rectangles
+ * do in the reference. */
+int dspt_render_region(dspt_ctx* ctx, int32_t first_frame, int32_t num_frames, int32_t x0, int32_t y0, int32_t width,
+                       int32_t height);
+/* Clears the accumulation to zero: the clear of both render targets when
+ * RT.renderFrame starts after RT.resetAccumulation (RT:327-336, 604-621). */
+int dspt_clear_accum(dspt_ctx* ctx);
+
 /* Accumulation readback: f32[4*W*H]. No counterpart in the reference lib
  * (the app reads the canvas, packages/app/demo_viewer.ts:246-251). */
 int dspt_read_accum(dspt_ctx* ctx, float* rgba);

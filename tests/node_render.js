@@ -13,7 +13,9 @@ function load(dir, name, T) {
 (async () => {
   const dir = process.argv[2];
   const meta = JSON.parse(fs.readFileSync(path.join(dir, 'meta.json'), 'utf8'));
-  const r = new PathtracingRenderer({ materialProfile: meta.profile, width: meta.width, height: meta.height });
+  const r = new PathtracingRenderer({ materialProfile: meta.profile, width: meta.width, height: meta.height,
+                                      tileSchedule: meta.tileSchedule || 'frames' });
+  if (meta.tileRes) r.tileRes = meta.tileRes;
   r.maxBounces = meta.maxBounces;
   const mats = load(dir, 'mats.f32', Float32Array);
   const scene = {
@@ -27,12 +29,14 @@ function load(dir, name, T) {
   await r.setScene(scene);
   r.setIBL({ image: { data: load(dir, 'ibl.f32', Float32Array), width: meta.iblWidth, height: meta.iblHeight } });
   const camera = { fov: meta.fov, near: meta.near, matrixWorld: { elements: meta.matrixWorld }, position: meta.position };
-  let frames = 0;
-  await r.render(camera, meta.numSamples, null, (n) => { frames = n; }, null, meta.framesPerCall);
+  let frames = 0, tiles = 0;
+  const tilesPerFrame = [];
+  await r.render(camera, meta.numSamples, () => { tiles++; }, (n) => { frames = n; tilesPerFrame.push(tiles); tiles = 0; },
+                 null, meta.framesPerCall);
   fs.writeFileSync(path.join(dir, 'accum.f32'), Buffer.from(r.readAccumulation().buffer));
   // the display chain with the reference's settings names
   r.tonemapping = 'AcesFilm'; r.enableFxaa = true;
   fs.writeFileSync(path.join(dir, 'display.u8'), Buffer.from(r.readDisplay().buffer));
   r.saveImage(path.join(dir, 'display.png'));
-  console.log(JSON.stringify({ frames, sampleCount: r.diagnostics.sampleCount, bvh: r.diagnostics.bvh }));
+  console.log(JSON.stringify({ frames, tilesPerFrame, sampleCount: r.diagnostics.sampleCount, bvh: r.diagnostics.bvh }));
 })().catch((e) => { console.error(e.message); process.exit(1); });

@@ -201,3 +201,38 @@ def test_node_viewport_call_sequence(addon_built, tmp_path):
         eng.render(1, n)
         got = np.fromfile(tmp_path / f"{name}.f32", np.float32).reshape(H, W, 4)
         np.testing.assert_array_equal(eng.read_accum().view(np.uint32), got.view(np.uint32), err_msg=name)
+
+
+def test_tile_order_and_rects(addon_built):
+    """The shim's tile order (renderer.ts:573-593) and scissor rectangles
+    (renderer.ts:632-643) equal the Python mirror's; the order is a
+    permutation of the grid, coarse strides first; the rectangles cover the
+    render target, overlapping only where ceil rounding makes them."""
+    from dspt.renderer import tile_order, tile_rect
+    cases = [(1, (7, 5)), (2, (64, 48)), (3, (33, 7)), (4, (1001, 601)), (4, (1, 1)), (5, (13, 1)), (8, (1920, 1080))]
+    r = _node("const {tileOrder, tileRect} = require('./addon/pathtracing_renderer.js');"
+              "const cases = JSON.parse(process.argv[1]);"
+              "console.log(JSON.stringify(cases.map(([tr, res]) => ({order: tileOrder(tr),"
+              " rects: tileOrder(tr).map((t) => tileRect(t, tr, res))}))));", json.dumps(cases))
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert tile_order(4) == [0, 2, 8, 10, 1, 3, 4, 5, 6, 7, 9, 11, 12, 13, 14, 15]
+    assert tile_order(2) == [0, 1, 2, 3] and tile_order(1) == [0]
+    for (tr, res), o in zip(cases, out):
+        order = tile_order(tr)
+        assert o["order"] == order and sorted(order) == list(range(tr * tr))
+        rects = [tile_rect(t, tr, res) for t in order]
+        assert [None if x is None else list(x) for x in rects] == o["rects"]
+        W, H = res
+        cover = np.zeros((H, W), np.int32)
+        for rc in rects:
+            if rc is not None:
+                x, y, w, h = rc
+                assert 0 <= x and 0 <= y and x + w <= W and y + h <= H
+                cover[y:y + h, x:x + w] += 1
+        assert cover.min() >= 1
+        if W % tr == 0 and H % tr == 0:
+            assert cover.max() == 1
+    # ceil rounding: 1001 / 4 columns overlap at column 751 (tiles 2 and 3)
+    assert tile_rect(2, 4, (1001, 601))[0] + tile_rect(2, 4, (1001, 601))[2] == 752
+    assert tile_rect(3, 4, (1001, 601))[0] == 751
