@@ -144,6 +144,10 @@ __device__ double cdf_row(const float* __restrict__ f, float* __restrict__ pdf, 
   for (int i = 0; i < num; i++) {
     float p = (float)((double)f[i] / sum);
     pdf[i] = p;
+    // the reference adds two f32 values in double and rounds the sum on the
+    // Float32Array store; a 53-bit sum of 24-bit operands rounded to 24 bits
+    // equals the correctly rounded f32 sum (double rounding is innocuous for
+    // + when 53 >= 2*24 + 2, Figueroa 1995), so the f32 add is that result
     acc = (i == 0) ? p : acc + p;
     cdf[i] = acc;
   }
