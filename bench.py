@@ -21,7 +21,9 @@ duration: the traffic comes from two child runs of this script under
 rocprofv3 (--pmc FETCH_SIZE, then --pmc WRITE_SIZE), each scaled by what the
 same counter reads for a plain copy kernel of known size in the same run.
 The byte model of the traversal (64 B per node visit, 64 B per triangle
-test) is reported beside it as bytes served (caches included).
+test) is reported beside it as bytes served (caches included). A third child
+run reads each bouncing kernel's L2 hit rate (TCC_HIT/TCC_MISS), VALU busy
+and lanes active (VALUBusy, VALUUtilization).
 
 More objects:
   batched  the uneven 32-scene mix (10k-2M triangles, 256^2-1024^2,
@@ -509,38 +511,48 @@ def _counter_means(outdir, counter):
     return {k: (sum(v.values()) / len(v), len(v)) for k, v in per.items()}
 
 
+def _pmc_run(args, counters):
+    """One child run of this script under rocprofv3 --pmc with `counters`
+    (all within one pass of the hardware) -> {counter: {kernel: (mean per
+    dispatch, dispatches)}}, or an error string."""
+    prof = rocprofv3_path()
+    if not prof:
+        return None, "rocprofv3 not found"
+    out = tempfile.mkdtemp(prefix="dspt_pmc_")
+    cmd = [prof, "--pmc", *counters, "-d", out, "-o", "pmc", "--output-format", "csv", "--",
+           sys.executable, os.path.abspath(__file__), "--pmc-child", "--workload", args.workload,
+           "--max-paths", str(args.max_paths), "--shade-blocks", str(args.shade_blocks)]
+    if args.spp:
+        cmd += ["--spp", str(args.spp)]
+    log(f"[bench] counters {' '.join(counters)}: child run under rocprofv3")
+    t0 = time.time()
+    try:
+        try:
+            r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, timeout=300, cwd=ROOT)
+        except subprocess.TimeoutExpired:
+            return None, f"rocprofv3 {' '.join(counters)} run timed out"
+        if r.returncode != 0 and not glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True):
+            return None, f"rocprofv3 {' '.join(counters)} run failed ({r.returncode})"
+        means = {c: _counter_means(out, c) for c in counters}
+    except (RuntimeError, ValueError) as e:
+        return None, str(e)
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+    log(f"[bench] counters {' '.join(counters)} in {time.time() - t0:.1f}s")
+    return means, None
+
+
 def measure_traffic(args):
     """Memory traffic per launch of the bouncing kernels, in bytes, from two
     child runs under rocprofv3 (FETCH_SIZE, then WRITE_SIZE; one counter per
     run, each within one pass of the hardware), scaled by the same counters
     of the plain copy: bytes = counter x (copy bytes / copy counter)."""
-    prof = rocprofv3_path()
-    if not prof:
-        return None, "rocprofv3 not found"
     means = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        out = tempfile.mkdtemp(prefix=f"dspt_pmc_{counter.lower()}_")
-        cmd = [prof, "--pmc", counter, "-d", out, "-o", "pmc", "--output-format", "csv", "--",
-               sys.executable, os.path.abspath(__file__), "--pmc-child", "--workload", args.workload,
-               "--max-paths", str(args.max_paths), "--shade-blocks", str(args.shade_blocks)]
-        if args.spp:
-            cmd += ["--spp", str(args.spp)]
-        log(f"[bench] counters: {' '.join(cmd[:3])} ... (child run)")
-        t0 = time.time()
-        try:
-            r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, timeout=300, cwd=ROOT)
-        except subprocess.TimeoutExpired:
-            return None, f"rocprofv3 {counter} run timed out"
-        if r.returncode != 0 and not glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True):
-            return None, f"rocprofv3 {counter} run failed ({r.returncode})"
-        try:
-            means[counter] = _counter_means(out, counter)
-        except (RuntimeError, ValueError) as e:
-            return None, f"{counter}: {e}"
-        finally:
-            shutil.rmtree(out, ignore_errors=True)
-        log(f"[bench] counters {counter} in {time.time() - t0:.1f}s: "
-            + ", ".join(f"{k} {v[0]:.0f} KiB x{v[1]}" for k, v in means[counter].items()))
+        m, err = _pmc_run(args, [counter])
+        if err:
+            return None, err
+        means[counter] = m[counter]
     f, w = means["FETCH_SIZE"], means["WRITE_SIZE"]
     if "k_copy_calib" not in f or "k_copy_calib" not in w:
         return None, "calibration copy not seen by the profiler"
@@ -552,6 +564,24 @@ def measure_traffic(args):
             rd, wr = f[k][0] * 1024.0 * kf, w[k][0] * 1024.0 * kw
             res["kernels"][k] = {"read_bytes": round(rd), "written_bytes": round(wr), "traffic_bytes": round(rd + wr),
                                  "dispatches": f[k][1]}
+    return res, None
+
+
+def measure_units(args):
+    """L2 hit rate and VALU busy / lanes active of the bouncing kernels (one
+    more child run): what bounds the traversal against the L2 and the shading
+    against the VALU."""
+    m, err = _pmc_run(args, ["TCC_HIT_sum", "TCC_MISS_sum", "VALUBusy", "VALUUtilization"])
+    if err:
+        return None, err
+    res = {}
+    for k in PMC_KERNELS:
+        try:
+            hit, miss = m["TCC_HIT_sum"][k][0], m["TCC_MISS_sum"][k][0]
+            res[k] = {"l2_hit_rate": round(hit / max(1.0, hit + miss), 4), "valu_busy_pct": round(m["VALUBusy"][k][0], 1),
+                      "lanes_active_pct": round(m["VALUUtilization"][k][0], 1)}
+        except KeyError:
+            continue
     return res, None
 
 
@@ -594,6 +624,9 @@ def main():
     traffic, traffic_err = (None, "skipped (--no-pmc)") if args.no_pmc else measure_traffic(args)
     if traffic_err:
         log(f"[bench] memory counters unavailable: {traffic_err}")
+    units, units_err = (None, "skipped (--no-pmc)") if args.no_pmc else measure_units(args)
+    if units_err:
+        log(f"[bench] cache and VALU counters unavailable: {units_err}")
 
     from dspt import Engine, PROFILE_FULL, PROFILE_LEAN
     wl = make_workload(args.workload)
@@ -676,6 +709,11 @@ def main():
                               frac=round(t["traffic_bytes"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
         roofline["kernels"] = per
         roofline["calibration"] = {k: traffic[k] for k in ("fetch_scale", "write_scale", "copy_bytes")}
+    if units:
+        for k, v in units.items():
+            roofline.setdefault("kernels", {}).setdefault(k, {}).update(v)
+        if "k_trace_ext" in units:
+            roofline["l2_hit_rate"] = units["k_trace_ext"]["l2_hit_rate"]
 
     out = {
         "metric": METRIC,
