@@ -28,9 +28,11 @@ and lanes active (VALUBusy, VALUUtilization).
 More objects:
   batched  the uneven 32-scene mix (10k-2M triangles, 256^2-1024^2,
            16-256 spp, 1024x512 IBL each) rendered through the shared path
-           pool vs one job at a time in frame-synchronous passes of a full
-           pool and vs one job at a time with one frame per pass (the
-           reference's render loop): Mpaths/s and p50/p99 time-to-N-spp;
+           pool (least remaining work first; FIFO and round robin beside
+           it) vs one job at a time through the single-job path, in
+           frame-synchronous passes of a full pool, and with one frame per
+           pass (the reference's render loop): Mpaths/s and p50/p99
+           time-to-N-spp;
   serving  Poisson arrival streams over the full-material scene with
            resolution/spp mixes, admission by a device-memory budget,
            p50/p99 time-to-N-spp and time-to-first-frame against Mpaths/s;
@@ -140,7 +142,7 @@ def batched_vs_plain(args):
     """Config-3 style comparison; every job is rendered by both schedules and
     one job's accumulation is compared bit for bit."""
     from dspt import (Engine, ORDER_BUCKETS, ORDER_NONE, ORDER_SHADE, PROFILE_FULL, SCHED_PASS, SCHED_STREAM, POLICY_FIFO,
-                      POLICY_RR, build_bvh, scenes)
+                      POLICY_RR, POLICY_SRPT, build_bvh, scenes)
     from dspt.batch import BatchRenderer, JobSpec
     t0 = time.time()
     mix = scenes.batch_mix(seed=3, n_jobs=args.batch_jobs)
@@ -226,13 +228,19 @@ def batched_vs_plain(args):
         eng.set_schedule(SCHED_STREAM)
         run.ids = ids
 
-    el_b, lat_b = run(batched)
+    # batched headline: all jobs in the pool, least remaining work first
+    el_b, lat_b = run(batched, policy=POLICY_SRPT)
     ids_b, kms_b, cnt_b = run.ids, run.kms, run.cnt
     acc_b = eng.job_read_accum(ids_b[0])
     for j in ids_b:
         eng.job_destroy(j)
     el_r, lat_r = run(batched, policy=POLICY_RR)
     kms_r = run.kms
+    for j in run.ids:
+        eng.job_destroy(j)
+    el_q, lat_q = run(batched, policy=POLICY_FIFO)
+    acc_q = eng.job_read_accum(run.ids[0])
+    kms_q = run.kms
     for j in run.ids:
         eng.job_destroy(j)
     eng.set_policy(POLICY_RR)
@@ -264,14 +272,17 @@ def batched_vs_plain(args):
         "speedup": round(el_p / el_b, 3),
         "batched_latency_ms": {"p50": q(lat_b, 50), "p99": q(lat_b, 99)},
         "plain_latency_ms": {"p50": q(lat_p, 50), "p99": q(lat_p, 99)},
-        "batched_policy": "fifo",
+        "batched_policy": "srpt",
         "round_robin": {"mpaths_per_s": round(total / el_r / 1e6, 2),
                         "latency_ms": {"p50": q(lat_r, 50), "p99": q(lat_r, 99)}},
-        "kernel_ms": {"batched": kms_b, "round_robin": kms_r, "plain": kms_p},
+        "fifo": {"mpaths_per_s": round(total / el_q / 1e6, 2),
+                 "latency_ms": {"p50": q(lat_q, 50), "p99": q(lat_q, 99)},
+                 "mode": "all jobs in the pool, whole runs of frames in submission order (FIFO)"},
+        "kernel_ms": {"batched": kms_b, "round_robin": kms_r, "fifo": kms_q, "plain": kms_p},
         "plain_mode": "one job at a time, frame-synchronous passes of up to a full pool",
         "frame_loop_mode": "one job at a time, one frame per pass (the reference's render loop)",
         "identical_accumulation": bool(np.array_equal(acc_b, acc_p) and np.array_equal(acc_b, acc_f)
-                                       and np.array_equal(acc_b, acc_s)),
+                                       and np.array_equal(acc_b, acc_s) and np.array_equal(acc_b, acc_q)),
         "single_job_api": {"mpaths_per_s": round(total / el_s / 1e6, 2),
                            "latency_ms": {"p50": q(lat_s, 50), "p99": q(lat_s, 99)}, "kernel_ms": kms_s,
                            "mode": "one job at a time, each alone in the streaming pool (the dspt_render path)",
