@@ -30,6 +30,7 @@ export declare class PathtracingRenderer {
   tonemapping: string; enableGamma: boolean; enableFxaa: boolean; pixelRatio: number; tileRes: number;
   // 'frames' (default): whole frames per native call; 'tiles': the reference's scissored tile passes
   tileSchedule: 'frames' | 'tiles';
+  // debugMode other than 'None' renders the debug program (u_debug_mode = debugModes.indexOf(debugMode))
   readonly debugModes: string[]; readonly tonemappingModes: string[]; debugMode: string;
   readonly diagnostics: { backend: string; sampleCount: number; mode: string; profiling: Record<string, number> };
   setScene(scene: SceneBuffers, progress?: (e: { phase: string }) => void): Promise<void>;

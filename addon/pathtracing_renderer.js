@@ -23,7 +23,7 @@ function packFrameBlock(r, camera, frameCount) {
   v.push(camera.position.x, camera.position.y, camera.position.z, filmHeight);
   v.push(1.0 / r.renderRes[0], 1.0 / r.renderRes[1]);
   v.push(r._iblSize[0], r._iblSize[1]);
-  v.push(frameCount, 0, 0, r._useIBL ? 1 : 0);
+  v.push(frameCount, r.debugModes.indexOf(r._debugMode), 0, r._useIBL ? 1 : 0);  // u_debug_mode, renderer.ts:406
   v.push(r._iblRotation, r._showBackground ? 1 : 0, r._maxBounces, camera.near);
   v.push(r._iblTotalSum, r._rayEps, r._clampThreshold, 0);
   v.push(r._sceneCounts[0], r._sceneCounts[1], r._sceneCounts[2], 0);
@@ -93,8 +93,9 @@ class PathtracingRenderer {
     if (v !== 'frames' && v !== 'tiles') throw new Error(`unknown tile schedule ${v}`);
     this._tileSchedule = v; this.resetAccumulation();
   }
-  // renderer.ts:182-190. The debug integrators (AOV views) are not part of
-  // this engine: any mode but 'None' is accepted here and refused by render().
+  // renderer.ts:182-190. Any mode but 'None' renders the debug program (one
+  // attribute of the first hit per sample); a name not in debugModes packs
+  // -1, which shows the hit as black and the background as in 'Albedo'.
   get debugMode() { return this._debugMode; } set debugMode(v) { this._debugMode = v; this.resetAccumulation(); }
 
   // resolves once no native call is in flight (readAccumulation, readDisplay
@@ -192,8 +193,6 @@ class PathtracingRenderer {
   // and reject it.
   render(camera, numSamples, tileFinishedCB, frameFinishedCB, renderingFinishedCB, framesPerCall) {
     this.stopRendering();
-    if (this._debugMode !== 'None')
-      throw new Error(`debugMode '${this._debugMode}': the debug views are not rendered by this engine`);
     this._camera = camera;
     this._numSamples = numSamples;
     this._cb = { tile: tileFinishedCB, frame: frameFinishedCB, done: renderingFinishedCB };

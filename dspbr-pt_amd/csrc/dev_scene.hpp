@@ -627,6 +627,22 @@ DI f3 env_lookup(const SceneView& S, float2 uv) {
   f3 bot = t01 * (1.0f - a) + t11 * a;
   return top * (1.0f - b) + bot * b;
 }
+// texture() of a one-channel R32F importance table (renderer.ts:858-859:
+// the pdf and cdf textures, LINEAR + REPEAT by gl_utils.ts:203-216): the red
+// channel of the bilinear fetch, with the arithmetic of env_lookup.
+DI float table_lookup(const float* __restrict__ t, int w, int h, float2 uv) {
+  float x = uv.x * float(w) - 0.5f, y = uv.y * float(h) - 0.5f;
+  float fx = floorf(x), fy = floorf(y);
+  float a = x - fx, b = y - fy;
+  int ix = (int)fx, iy = (int)fy;
+  int x0 = ix % w; if (x0 < 0) x0 += w;
+  int x1 = (ix + 1) % w; if (x1 < 0) x1 += w;
+  int y0 = iy % h; if (y0 < 0) y0 += h;
+  int y1 = (iy + 1) % h; if (y1 < 0) y1 += h;
+  float top = t[(size_t)y0 * w + x0] * (1.0f - a) + t[(size_t)y0 * w + x1] * a;
+  float bot = t[(size_t)y1 * w + x0] * (1.0f - a) + t[(size_t)y1 * w + x1] * a;
+  return top * (1.0f - b) + bot * b;
+}
 // evaluateMaterialTextureValue (scene_data_webgl2.ts:325-340):
 // texture(array, vec3(uv * scale + offset, layer)) with GL LINEAR + REPEAT on
 // RGBA8 UNORM (texel / 255, bilinear in f32, as the environment lookup). The

@@ -260,10 +260,11 @@ struct dspt_ctx {
   char* post_buf = nullptr;
   size_t post_cap = 0;
   // the bounce loop of a chunk captured as a HIP graph, one per (state
-  // buffer holding the live paths at chunk start, two-level kernels);
-  // rebuilt when the pool, the grids or the chunk length change
+  // buffer holding the live paths at chunk start, two-level kernels, debug
+  // shading instance); rebuilt when the pool, the grids or the chunk length
+  // change
   bool use_graphs = true;
-  hipGraphExec_t bounce_graph[2][2] = {};
+  hipGraphExec_t bounce_graph[2][2][2] = {};
   std::array<int64_t, 6> graph_key{};  // (pool, pool address, grids, chunk length, ray order) of the graphs
   // ray order of the extension queue (allocated with the pool)
   int ray_order = DSPT_ORDER_SHADE;
@@ -290,12 +291,13 @@ int read_counters(dspt_ctx* c, unsigned long long* h) {
 }
 
 void drop_graphs(dspt_ctx* c) {
-  for (auto& row : c->bounce_graph)
-    for (auto& g : row)
-      if (g) {
-        (void)hipGraphExecDestroy(g);
-        g = nullptr;
-      }
+  for (auto& plane : c->bounce_graph)
+    for (auto& row : plane)
+      for (auto& g : row)
+        if (g) {
+          (void)hipGraphExecDestroy(g);
+          g = nullptr;
+        }
 }
 
 #include "bvh_gpu.hip"
@@ -602,6 +604,10 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
   bool inst = false;
   for (const JobRes* j : c->jobs)
     if (j && j->alive && c->scenes[j->scene] && c->scenes[j->scene]->top >= 0) inst = true;
+  // the shading kernel's debug instance only while a job renders a debug view
+  bool dbg = false;
+  for (const JobRes* j : c->jobs)
+    if (j && j->alive && j->F.debug_mode != 0.0f) dbg = true;
   const bool order = c->ray_order == DSPT_ORDER_BUCKETS || c->ray_order == DSPT_ORDER_SHADE;
   const bool shade_order = c->ray_order == DSPT_ORDER_SHADE;
   const int32_t* perm = order ? c->order_perm : nullptr;
@@ -638,12 +644,14 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
                                                              c->shade_perm);
         });
       timed(c, DSPT_K_SHADE, [&] {
-        if (shade_order)
-          k_shade<<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1], c->shq,
-                                                        &ctl->sh_count, c->cnt, c->shade_perm, &ctl->qcount[a]);
+        const int32_t* sperm = shade_order ? c->shade_perm : perm;
+        const int32_t* nsorted = shade_order ? &ctl->qcount[a] : &ctl->q_base;
+        if (dbg)
+          k_shade<true><<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1],
+                                                              c->shq, &ctl->sh_count, c->cnt, sperm, nsorted);
         else
-          k_shade<<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1], c->shq,
-                                                        &ctl->sh_count, c->cnt, perm, &ctl->q_base);
+          k_shade<false><<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1],
+                                                               c->shq, &ctl->sh_count, c->cnt, sperm, nsorted);
       });
       timed(c, DSPT_K_TRACE_SHADOW, [&] {
         if (inst)
@@ -658,7 +666,7 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
   const int q0 = c->qsel;
   if (c->use_graphs && !c->timing) {
     // every launch argument of the loop is fixed by (pool, grids, chunk
-    // length, q0, inst): the work sizes live in device counters
+    // length, q0, inst, dbg): the work sizes live in device counters
     const std::array<int64_t, 6> key{P, (int64_t)(uintptr_t)c->ps[0].org, trace_blocks, shade_blocks, c->chunk_iters,
                                      c->ray_order};
     if (key != c->graph_key) {
@@ -666,7 +674,7 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
       drop_graphs(c);
       c->graph_key = key;
     }
-    hipGraphExec_t& ge = c->bounce_graph[q0][inst ? 1 : 0];
+    hipGraphExec_t& ge = c->bounce_graph[q0][inst ? 1 : 0][dbg ? 1 : 0];
     if (!ge) {
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));

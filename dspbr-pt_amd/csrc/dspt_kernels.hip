@@ -627,6 +627,52 @@ DD void atomic_inc_grouped(int32_t* addr) {
   }
 }
 
+// The debug program (integrator/debug.glsl:1-83), which the renderer runs
+// whenever debugMode is not 'None' (renderer.ts:1020), u_debug_mode being
+// the mode's index in debugModes (renderer.ts:181, 406; -1 for an unknown
+// name): the camera ray's first hit shows one attribute of its render state;
+// a miss shows the environment, or its pdf (x 10) or cdf table, when the
+// background comes from the IBL, and vec4(0) otherwise. Out of line, in the
+// shading kernel's DEBUG instance only.
+__device__ __noinline__ float4 debug_sample(const SceneView& S, const FrameParams& F, f3 org, f3 dir, int tri, float4 hit,
+                                            int inst) {
+  const int mode = (int)F.debug_mode;
+  if (tri < 0) {
+    if (!(F.bg_from_ibl > 0.0f)) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float th;
+    const float2 uv = dir_uv(rotate_phi(dir, F.ibl_rotation, false), th);
+    const bool have = S.has_ibl && S.ew > 0 && S.eh > 0;
+    if (mode == 10) {  // vec4(texture(pdf).xyz, 1.0) * 10.0
+      const float p = have ? table_lookup(S.pdf, S.ew, S.eh, uv) : 0.0f;
+      return make_float4(p * 10.0f, 0.0f * 10.0f, 0.0f * 10.0f, 1.0f * 10.0f);
+    }
+    if (mode == 11) return make_float4(have ? table_lookup(S.cdf, S.ew, S.eh, uv) : 0.0f, 0.0f, 0.0f, 1.0f);
+    const f3 e = have ? env_lookup(S, uv) : sp3(0.0f);
+    return make_float4(e.x, e.y, e.z, 1.0f);
+  }
+  const Hitpoint hp = fetch_hitpoint(S, org, dir, tri, hit.y, hit.z, hit.w, inst);
+  const Shading sh = configure(S, hp, -dir);
+  const bsdf::GltfInputs& m = sh.in;
+  f3 c = sp3(0.0f);
+  switch (mode) {
+    case 1: c = m.baseColor; break;                          // D_ALBEDO
+    case 2: c = sp3(m.metallic); break;                      // D_METAL
+    case 3: c = sp3(m.roughness); break;                     // D_ROUGHNESS
+    case 4: c = sh.n; break;                                 // D_NORMAL
+    case 5: c = sh.tanA; break;                              // D_TANGENT (after the anisotropy rotation)
+    case 6: c = cross3(sh.n, sh.tanA) * hp.tw; break;        // D_BITANGENT: calculateBasis(c.n, c.t).b
+    case 7: c = sp3(m.transmission); break;                  // D_TRANSPARENCY
+    case 8: c = f3{hp.uv0.x, hp.uv0.y, 0.0f}; break;         // D_UV0
+    case 9: c = sp3(m.clearcoat); break;                     // D_CLEARCOAT
+    case 12: c = sp3(m.specular); break;                     // D_SPECULAR
+    case 13: c = m.specularColor; break;                     // D_SPECULAR_TINT
+    case 14: c = m.diffuseTransmission * m.diffuseTransmissionColor; break;  // D_TRANSLUCENCY
+    default: break;
+  }
+  return make_float4(c.x, c.y, c.z, 1.0f);
+}
+
+template <bool DEBUG>
 __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState ps, PathState nx,
                                    const int32_t* __restrict__ count_in, int32_t* __restrict__ count_out,
                                    int32_t* __restrict__ shq, int32_t* __restrict__ count_sh,
@@ -659,6 +705,11 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
       float4 L4 = ps.L[p];
       f3 L = xyz(L4);
 
+      if (DEBUG && F.debug_mode != 0.0f) {  // the debug program: one hit, no bounce
+        if (rec) { sig_mix(sig, tri < 0 ? 0xFFFFFFFFu : (uint32_t)tri); ps.prim[p] = tri; }
+        out = debug_sample(S, F, org, dir, tri, hit, (tri >= 0 && S.top >= 0) ? ps.hinst[p] : -1);
+        break;
+      }
       if (tri < 0) {
         if (rec) sig_mix(sig, 0xFFFFFFFFu);
         if (bounce == 0) {
@@ -812,6 +863,10 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
   stat_add(cnt, 8, n_nee);
   stat_add(cnt, 2, n_ext);
 }
+template __global__ void k_shade<false>(const DevJob*, PathState, PathState, const int32_t*, int32_t*, int32_t*, int32_t*,
+                                        unsigned long long*, const int32_t*, const int32_t*);
+template __global__ void k_shade<true>(const DevJob*, PathState, PathState, const int32_t*, int32_t*, int32_t*, int32_t*,
+                                       unsigned long long*, const int32_t*, const int32_t*);
 
 // Running mean (renderer.frag:153-156) over every frame that is complete,
 // in frame order, starting at the job's next frame to fold (grid.y = job),

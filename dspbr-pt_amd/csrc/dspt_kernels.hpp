@@ -133,6 +133,9 @@ __global__ void k_trace_ext(const DevJob* jobs, PathState ps, const int32_t* cou
 template <bool INST>
 __global__ void k_trace_shadow(const DevJob* jobs, PathState ps, const int32_t* queue, const int32_t* count_ptr,
                                int32_t* next_ptr, int* spill, unsigned long long* cnt);
+// DEBUG: the instance that also runs the debug program for jobs whose frame
+// has a debug mode; the host launches it only while such a job is live.
+template <bool DEBUG>
 __global__ void k_shade(const DevJob* jobs, PathState cur, PathState nxt, const int32_t* count_in, int32_t* count_out,
                         int32_t* shq, int32_t* count_sh, unsigned long long* cnt, const int32_t* perm,
                         const int32_t* nsorted_ptr);

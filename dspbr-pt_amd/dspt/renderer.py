@@ -9,9 +9,10 @@ PathtracingViewport   packages/lib/pathtracing_viewport.ts:100-316 (public
                       embedding API: setScene/setIBL/setLookAtCamera/start/
                       stop/on/off/setSettings).
 readAccumulation() returns the linear accumulation; readDisplay() runs the
-display chain (tonemap, gamma, FXAA, display resample) on the GPU. The debug
-AOV integrator is not part of the engine: any debugMode but 'None' is
-refused by render().
+display chain (tonemap, gamma, FXAA, display resample) on the GPU. Any
+debugMode but 'None' renders the debug program (integrator/debug.glsl): one
+attribute of each sample's first hit, as u_debug_mode = the mode's index in
+debugModes (-1 for a name not in the list), renderer.ts:406.
 """
 import math
 import time
@@ -304,7 +305,11 @@ class PathtracingRenderer:
             ibl_size=self._ibl or (0, 0), ray_eps=self._rayEps, clamp_threshold=self._clampThreshold,
             ibl_total_sum=self._ibl_total,
             scene_counts=(self._scene.num_triangles if self._scene else 0, self._bvh_counts[0], self._bvh_counts[1], 0),
-            light=light)
+            light=light, debug_mode=self._debug_index())
+
+    def _debug_index(self):
+        """debugModes.indexOf(debugMode) (renderer.ts:406): -1 for a name not in the list."""
+        return self.debugModes.index(self._debugMode) if self._debugMode in self.debugModes else -1
 
     PATHS_PER_CALL = 1 << 24
 
@@ -326,8 +331,6 @@ class PathtracingRenderer:
         and frameFinishedCB follows the frame's last tile, as in the
         reference (renderer.ts:595-714)."""
         self.stopRendering()
-        if self._debugMode != "None":
-            raise NotImplementedError(f"debugMode {self._debugMode!r}: the debug views are not rendered by this engine")
         self.schedulerMode = "settling"
         self._isRendering = True
         self.resetAccumulation()

@@ -273,6 +273,66 @@ template <class R> Ray<R> camera_ray(const Frame& f, int W, int H, int px, int p
   return make_ray(wd, V3<R>(R(f.cam[0]), R(f.cam[1]), R(f.cam[2])), K<R>::TFAR_MAX);
 }
 
+// SH/integrator/debug.glsl:1-83, the trace() of the debug program, which the
+// renderer selects whenever debugMode is not 'None' (renderer.ts:1020) with
+// u_debug_mode = the mode's index in debugModes (renderer.ts:181, 406; -1
+// for a name not in the list): the camera ray's first hit shows one
+// attribute of its render state; a miss shows the environment, or its
+// pdf (x 10) or cdf table, when the background comes from the IBL, and
+// vec4(0) otherwise.
+template <class R> PathOut<R> debug_trace(const Scene& s, const Frame& f, const Ray<R>& ray, Counters* cnt, Sig& sig) {
+  PathOut<R> out;
+  const int mode = int(f.debug_mode);
+  Hit<R> hit;
+  out.primary_tri = -1;
+  out.alpha = R(0);
+  out.L = V3<R>(R(0));
+  if (!intersect_nearest(s, ray, hit, cnt)) {
+    sig.mix(0xFFFFFFFFu);
+    if (f.bg_from_ibl > 0.0f) {
+      R pdf;
+      const V2<R> uv = dir_to_uv(rotate_dir_phi(ray.dir, R(f.ibl_rotation), false), pdf);
+      const bool have = s.env && s.ew > 0 && s.eh > 0;
+      if (mode == 10) {  // vec4(texture(pdf).xyz, 1.0) * 10.0
+        const R p = have ? table_bilinear(s.pdf, s.ew, s.eh, uv) : R(0);
+        out.L = V3<R>(p * R(10), R(0) * R(10), R(0) * R(10));
+        out.alpha = R(1) * R(10);
+      } else if (mode == 11) {
+        out.L = V3<R>(have ? table_bilinear(s.cdf, s.ew, s.eh, uv) : R(0), R(0), R(0));
+        out.alpha = R(1);
+      } else {
+        out.L = have ? env_bilinear(s, uv) : V3<R>(R(0));
+        out.alpha = R(1);
+      }
+    }
+    out.sig = sig.h;
+    return out;
+  }
+  out.primary_tri = hit.tri;
+  sig.mix((uint32_t)hit.tri);
+  RState<R> rs;
+  fill_render_state(s, ray, hit, rs);
+  const MClosure<R>& c = rs.closure;
+  const PbrMat<R>& m = c.material;
+  V3<R> con(R(0));
+  if (mode == 1) con = m.baseColor;                       // D_ALBEDO
+  if (mode == 2) con = V3<R>(m.metallic);                 // D_METAL
+  if (mode == 3) con = V3<R>(m.roughness);                // D_ROUGHNESS
+  if (mode == 4) con = c.n;                               // D_NORMAL
+  if (mode == 5) con = c.t;                               // D_TANGENT (after the anisotropy rotation)
+  if (mode == 6) con = cross(c.n, c.t) * c.tw;            // D_BITANGENT: calculateBasis(c.n, c.t).b
+  if (mode == 7) con = V3<R>(m.transmission);             // D_TRANSPARENCY
+  if (mode == 14) con = m.diffuseTransmission * m.diffuseTransmissionColor;  // D_TRANSLUCENCY
+  if (mode == 8) con = V3<R>(rs.uv0x, rs.uv0y, R(0));     // D_UV0
+  if (mode == 9) con = V3<R>(m.clearcoat);                // D_CLEARCOAT
+  if (mode == 12) con = V3<R>(m.specular);                // D_SPECULAR
+  if (mode == 13) con = m.specularColor;                  // D_SPECULAR_TINT
+  out.L = con;
+  out.alpha = R(1);
+  out.sig = sig.h;
+  return out;
+}
+
 // SH/renderer.frag:146-157 + SH/integrator/misptdl.glsl:34-93
 template <class R>
 PathOut<R> trace_path(const Scene& s, const Frame& f, int W, int H, int px, int py, Counters* cnt) {
@@ -281,6 +341,7 @@ PathOut<R> trace_path(const Scene& s, const Frame& f, int W, int H, int px, int 
   Rng rng = rng_init(int(f.frame_count) * int(f.max_bounces), px, py);
   float r0 = rng_float(rng), r1 = rng_float(rng);
   Ray<R> ray = camera_ray<R>(f, W, H, px, py, r0, r1);
+  if (f.debug_mode != 0.0f) return debug_trace(s, f, ray, cnt, sig);
   Hit<R> hit;
   V3<R> pw(R(1));
   V3<R> L(R(0));

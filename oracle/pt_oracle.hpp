@@ -425,6 +425,22 @@ template <class R> inline V3<R> env_bilinear(const Scene& s, V2<R> uv) {
   return top * (R(1) - b) + bot * b;
 }
 
+// texture() of a one-channel R32F importance table (the pdf and cdf textures
+// of renderer.ts:858-859, created with LINEAR filtering and REPEAT wrap by
+// gl_utils.ts:203-216): the red channel of the bilinear fetch.
+template <class R> inline R table_bilinear(const float* t, int w, int h, V2<R> uv) {
+  R x = uv.x * R(w) - R(0.5), y = uv.y * R(h) - R(0.5);
+  R fx = std::floor(x), fy = std::floor(y);
+  R a = x - fx, b = y - fy;
+  long ix = (long)fx, iy = (long)fy;
+  auto wrap = [](long i, int n) { long m = i % n; return (int)(m < 0 ? m + n : m); };
+  int x0 = wrap(ix, w), x1 = wrap(ix + 1, w), y0 = wrap(iy, h), y1 = wrap(iy + 1, h);
+  auto tex = [&](int xx, int yy) { return R(t[(size_t)yy * w + xx]); };
+  R top = tex(x0, y0) * (R(1) - a) + tex(x1, y0) * a;
+  R bot = tex(x0, y1) * (R(1) - a) + tex(x1, y1) * a;
+  return top * (R(1) - b) + bot * b;
+}
+
 // SH/lighting.glsl:58-65, mat3 column-major times vector
 template <class R> inline V3<R> rotate_dir_phi(V3<R> d, R rot, bool inverse) {
   R a = inverse ? -rot : rot;

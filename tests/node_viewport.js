@@ -112,9 +112,14 @@ async function waitFor(pred, ms = 20000) {
   out.interactive = { tiles, frames, sampleCount: r.diagnostics.sampleCount, mode: r.diagnostics.mode };
   vp.setInteractionMode(false);
 
-  // 5. debug views are refused; 'None' renders again
-  r.debugMode = 'Albedo';
-  try { vp.start(); out.debug = 'rendered'; } catch (e) { out.debug = e.message; }
+  // 5. debug views: the Normals view, then a name not in debugModes (packs -1)
+  for (const [mode, name] of [['Normals', 'debug_normals'], ['Wireframe', 'debug_unknown']]) {
+    vp.events.frameRendered = []; vp.events.renderFinished = [];
+    r.debugMode = mode;
+    await vp.start(meta.sampleLimit);
+    await r.idle();
+    save(name);
+  }
   r.debugMode = 'None';
   out.lists = { debugModes: r.debugModes.length, tonemappingModes: r.tonemappingModes };
   console.log(JSON.stringify(out));

@@ -155,9 +155,11 @@ def test_node_viewport_call_sequence(addon_built, tmp_path):
     start() with no limit renders until stop() and no frame callback follows
     stop(); a camera move + resetAccumulation() restarts at frame 1 with the
     new view; start() during a render stops it; a sample limit ends the loop
-    with renderingFinishedCB; interaction mode gives one tile per frame; debug
-    views are refused. Every accumulation equals the Python engine's render of
-    the same frames with the same camera, bit for bit."""
+    with renderingFinishedCB; interaction mode gives one tile per frame; a
+    debugMode renders the debug program with u_debug_mode = its index in
+    debugModes, -1 for an unknown name. Every accumulation equals the Python
+    engine's render of the same frames with the same camera and mode, bit for
+    bit."""
     from dspt import Engine, build_bvh, pack_frame_block, scenes
     from dspt.scene import ViewportCamera
     wl = scenes.sphere_field(n_spheres=16)
@@ -187,17 +189,21 @@ def test_node_viewport_call_sequence(addon_built, tmp_path):
     assert mv["sampleCount"] >= 4
     assert lim["sampleCount"] == 4 and lim["finished"] == [4] and lim["frameEvents"] == [2, 3, 4, 5]
     assert info["interactive"]["frames"] == 5 and info["interactive"]["tiles"] == 5
-    assert "debug" in info["debug"] and info["lists"]["debugModes"] == 15
+    for name in ("debug_normals", "debug_unknown"):
+        assert info[name]["sampleCount"] == 4 and info[name]["finished"] == [4], name
+    assert info["lists"]["debugModes"] == 15
     assert info["lists"]["tonemappingModes"] == ["None", "Reinhard", "Cineon", "AcesFilm"]
     nodes, bidx = build_bvh(s.bvhPositionBuffer, s.bvhIndexBuffer)
     eng = Engine(0)
     eng.upload_scene(s.vertexBuffer, s.triangleIndexBuffer, nodes, bidx, s.getFlatMaterialBuffer())
     tot = eng.upload_ibl(img["data"], img["width"], img["height"])
     eng.resize(W, H)
-    for name, cam, n in (("unbounded", cams[0], un["sampleCount"]), ("moved", cams[1], mv["sampleCount"]),
-                         ("limited", cams[1], lim["sampleCount"])):
+    for name, cam, n, dbg in (("unbounded", cams[0], un["sampleCount"], 0), ("moved", cams[1], mv["sampleCount"], 0),
+                              ("limited", cams[1], lim["sampleCount"], 0), ("debug_normals", cams[1], 4, 4),
+                              ("debug_unknown", cams[1], 4, -1)):
         eng.set_frame(pack_frame_block(cam, W, H, max_bounces=6, ibl_size=(img["width"], img["height"]),
-                                       ibl_total_sum=tot, scene_counts=(s.num_triangles, len(nodes) // 16, len(bidx), 0)))
+                                       ibl_total_sum=tot, scene_counts=(s.num_triangles, len(nodes) // 16, len(bidx), 0),
+                                       debug_mode=dbg))
         eng.render(1, n)
         got = np.fromfile(tmp_path / f"{name}.f32", np.float32).reshape(H, W, 4)
         np.testing.assert_array_equal(eng.read_accum().view(np.uint32), got.view(np.uint32), err_msg=name)

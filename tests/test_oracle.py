@@ -37,7 +37,7 @@ wrong sign/index, swapped operand) fails:
   sheen, dielectric transmission (thick and thin, Snell + Schlick split +
   straight-through), dielectric F0 with specular tint, diffuse transmission,
   the Kulla-Conty energy compensation, the camera ray and the point light are
-  pinned in test_oracle_pins.py (published models and closed forms computed
+  pinned in test_oracle_pins.py, the debug program in test_debug_views.py (published models and closed forms computed
   there independently of the oracle's code); the transcendentals the float
   build uses are pinned in test_pin_math.py.
 Checked for invariants only: the thin film over a conductor substrate whose
