@@ -55,7 +55,7 @@ export declare function tileRect(tile: number, tileRes: number, renderRes: [numb
 export interface RenderRequest {
   scene: number; ibl: number; camera: PathtracingCamera; width: number; height: number; spp: number;
   maxBounces?: number; useIBL?: boolean; showBackground?: boolean; backgroundColor?: number[];
-  iblRotation?: number; rayEps?: number; clampThreshold?: number;
+  iblRotation?: number; rayEps?: number; clampThreshold?: number; debugMode?: string;
   onProgress?: (framesDone: number, framesTotal: number) => void;
 }
 export interface RenderResult { accumulation: Float32Array; width: number; height: number; frames: number; latencyMs: number; }

@@ -13,6 +13,10 @@ const native = require(path.join(__dirname, 'build', 'dspt.node'));
 
 const FRAME_FLOATS = 52;
 
+// renderer.ts:181
+const DEBUG_MODES = ['None', 'Albedo', 'Metalness', 'Roughness', 'Normals', 'Tangents', 'Bitangents', 'Transparency',
+                     'UV0', 'Clearcoat', 'IBL PDF', 'IBL CDF', 'Specular', 'SpecularTint', 'Translucency'];
+
 function packFrameBlock(r, camera, frameCount) {
   // renderer.ts:376-438: JS numbers rounded into a Float32Array in key order
   const filmHeight = Math.tan(camera.fov * 0.5 * Math.PI / 180.0) * camera.near;
@@ -23,7 +27,7 @@ function packFrameBlock(r, camera, frameCount) {
   v.push(camera.position.x, camera.position.y, camera.position.z, filmHeight);
   v.push(1.0 / r.renderRes[0], 1.0 / r.renderRes[1]);
   v.push(r._iblSize[0], r._iblSize[1]);
-  v.push(frameCount, r.debugModes.indexOf(r._debugMode), 0, r._useIBL ? 1 : 0);  // u_debug_mode, renderer.ts:406
+  v.push(frameCount, DEBUG_MODES.indexOf(r._debugMode === undefined ? 'None' : r._debugMode), 0, r._useIBL ? 1 : 0);  // u_debug_mode, renderer.ts:406
   v.push(r._iblRotation, r._showBackground ? 1 : 0, r._maxBounces, camera.near);
   v.push(r._iblTotalSum, r._rayEps, r._clampThreshold, 0);
   v.push(r._sceneCounts[0], r._sceneCounts[1], r._sceneCounts[2], 0);
@@ -57,8 +61,7 @@ class PathtracingRenderer {
       throw new Error(`unknown tile schedule ${this._tileSchedule}`);
     this._tileIndex = 0;
     // renderer.ts:181,192
-    this.debugModes = ['None', 'Albedo', 'Metalness', 'Roughness', 'Normals', 'Tangents', 'Bitangents', 'Transparency',
-                       'UV0', 'Clearcoat', 'IBL PDF', 'IBL CDF', 'Specular', 'SpecularTint', 'Translucency'];
+    this.debugModes = DEBUG_MODES.slice();
     this.tonemappingModes = TONEMAPPING_MODES.slice();
     this._frameCount = 1; this._isRendering = false; this.schedulerMode = 'stopped';
     this._gen = 0;               // bumped by render() and stopRendering(): a loop of an older generation exits
@@ -509,7 +512,7 @@ class RenderServer {
       _maxBounces: req.maxBounces || 8, _iblTotalSum: ib.totalSum,
       _rayEps: req.rayEps !== undefined ? req.rayEps : 0.0001,
       _clampThreshold: req.clampThreshold !== undefined ? req.clampThreshold : 3.0,
-      _sceneCounts: sc.counts, _scene: { lights: sc.lights },
+      _sceneCounts: sc.counts, _scene: { lights: sc.lights }, _debugMode: req.debugMode,
     };
     return packFrameBlock(settings, req.camera, 1);
   }

@@ -80,3 +80,32 @@ def test_python_renderer_debug_mode(wl):
         np.testing.assert_array_equal(out[mode].view(np.uint32), eng.read_accum().view(np.uint32), err_msg=mode)
     eng.close()
     assert not np.array_equal(out["Tangents"], out["None"])
+
+
+def test_debug_and_path_jobs_share_the_pool(wl):
+    """A debug job and path-tracing jobs in one batched pool: the shading
+    kernel's debug instance runs them together, and every job's
+    accumulation, samples, signatures and primary hits equal the same job run
+    alone (the path job's alone runs the plain instance)."""
+    from dspt import Engine
+    from dspt.batch import BatchRenderer, JobSpec
+    eng = Engine(0)
+    b = BatchRenderer(eng)
+    b.add_scene(0, wl.scene)
+    b.add_ibl(0, wl.ibl)
+    specs = [JobSpec(0, 0, wl.camera, W, H, 3, max_bounces=wl.max_bounces, record=True, debug_mode=4),
+             JobSpec(0, 0, wl.camera, W, H, 3, max_bounces=wl.max_bounces, record=True),
+             JobSpec(0, 0, wl.camera, 40, 24, 2, max_bounces=wl.max_bounces, record=True, debug_mode=10,
+                     ibl_rotation_rad=1.3)]
+    alone = []
+    for s in specs:
+        j = b.create_job(s)
+        eng.jobs_run([j])
+        alone.append(j)
+    mixed = [b.create_job(s) for s in specs]
+    b.run_batched(mixed)
+    for a, m in zip(alone, mixed):
+        for x, y in zip((eng.job_read_accum(a), *eng.job_read_samples(a)),
+                        (eng.job_read_accum(m), *eng.job_read_samples(m))):
+            np.testing.assert_array_equal(x, y)
+    eng.close()
