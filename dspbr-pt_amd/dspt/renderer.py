@@ -115,6 +115,7 @@ class PathtracingRenderer:
         self._tileRes = 4
         self._clampThreshold = 3.0
         self._debugMode = "None"
+        self._forceIBLEval = False
         self._frameCount = 1
         self._isRendering = False
         self._reset_pending = False
@@ -149,6 +150,8 @@ class PathtracingRenderer:
     rayEps = _setting("rayEps", conv=float)
     enableFxaa = _setting("enableFxaa", reset=False)
     clampThreshold = _setting("clampThreshold", conv=float)
+    # stored and resets the accumulation; no shader reads it (renderer.ts:229-236)
+    forceIBLEval = _setting("forceIBLEval", conv=bool)
     del _setting
 
     @property

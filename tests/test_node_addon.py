@@ -242,3 +242,31 @@ def test_tile_order_and_rects(addon_built):
     # ceil rounding: 1001 / 4 columns overlap at column 751 (tiles 2 and 3)
     assert tile_rect(2, 4, (1001, 601))[0] + tile_rect(2, 4, (1001, 601))[2] == 752
     assert tile_rect(3, 4, (1001, 601))[0] == 751
+
+
+# The reference renderer's public members (renderer.ts:160-306, 327-365,
+# 512, 840, 914): settings properties, mode lists, methods.
+REFERENCE_RENDERER_MEMBERS = [
+    "exposure", "debugModes", "debugMode", "tonemappingModes", "tonemapping", "maxBounces", "useIBL",
+    "showBackground", "forceIBLEval", "enableGamma", "iblRotation", "pixelRatio", "backgroundColor", "rayEps",
+    "enableFxaa", "tileRes", "clampThreshold", "resetAccumulation", "stopRendering", "setInteractionMode", "resize",
+    "render", "setIBL", "setScene",
+]
+
+
+def test_renderer_surface_matches_reference(addon_built):
+    """Both drop-ins expose every public member of the reference renderer,
+    checked without a GPU: the Python class's attributes, and the Node
+    class's prototype plus the fields its constructor assigns."""
+    from dspt.renderer import PathtracingRenderer
+    missing = [m for m in REFERENCE_RENDERER_MEMBERS if not hasattr(PathtracingRenderer, m)]
+    assert not missing, missing
+    assert PathtracingRenderer.debugModes[10:12] == ["IBL PDF", "IBL CDF"] and len(PathtracingRenderer.debugModes) == 15
+    r = _node("const {PathtracingRenderer} = require('./addon/pathtracing_renderer.js');"
+              "const names = JSON.parse(process.argv[1]);"
+              "const proto = PathtracingRenderer.prototype;"
+              "const src = PathtracingRenderer.toString();"
+              "console.log(JSON.stringify(names.filter((n) => !(n in proto) && !src.includes('this.' + n + ' '))));",
+              json.dumps(REFERENCE_RENDERER_MEMBERS))
+    assert r.returncode == 0, r.stderr
+    assert json.loads(r.stdout.strip().splitlines()[-1]) == []
