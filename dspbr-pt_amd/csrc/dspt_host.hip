@@ -200,12 +200,6 @@ struct dspt_ctx {
   int device = 0;
   int profile = 0;
   hipStream_t stream = nullptr;
-  // second stream of the bounce loop: each bounce's shadow trace runs there,
-  // beside the next bounce's refill, ordering and extension trace (joined
-  // before the next shading pass and at the end of a chunk)
-  hipStream_t stream_sh = nullptr;
-  bool overlap_shadow = true;
-  std::vector<hipEvent_t> ev_bounce;  // [2 * bounce]: shading done, shadow trace done
   std::vector<SceneRes*> scenes;
   std::vector<IblRes*> ibls;
   std::vector<JobRes*> jobs;
@@ -227,7 +221,6 @@ struct dspt_ctx {
   PathState ps[2] = {};  // alternate per bounce; hit and shadow fields shared
   int32_t* shq = nullptr;
   int* spill = nullptr;
-  int* spill_sh = nullptr;  // the shadow trace's own stack spill (it may run beside the extension trace)
   int trace_blocks = 0;
   int trace_blocks_max = 2048;
   int shade_blocks_max = 16384;  // blocks of 256 threads, 64 per CU (8192 and 32768 measured slower)
@@ -448,9 +441,7 @@ int ensure_pool(dspt_ctx* c) {
   b.hit = a.hit; b.hinst = a.hinst; b.sh_org = a.sh_org; b.sh_dir = a.sh_dir; b.sh_c = a.sh_c; b.sh_pw = a.sh_pw; b.sh_pdir = a.sh_pdir;
   b.sh_pc = a.sh_pc;
   c->trace_blocks = (int)std::min<int64_t>(c->trace_blocks_max, grid_for(cap, kTraceBlock));
-  if ((r = A.get(&c->spill, (size_t)c->trace_blocks * kTraceBlock * (kStackSize - kLdsStack))) ||
-      (r = A.get(&c->spill_sh, (size_t)c->trace_blocks * kTraceBlock * (kStackSize - kLdsStack))))
-    return r;
+  if ((r = A.get(&c->spill, (size_t)c->trace_blocks * kTraceBlock * (kStackSize - kLdsStack)))) return r;
   if ((r = A.get(&c->order_keys, cap)) || (r = A.get(&c->order_perm, cap)) || (r = A.get(&c->order_bins, kOrderBins)) ||
       (r = A.get(&c->shade_perm, cap)) || (r = A.get(&c->shade_bins, kOrderBins)))
     return r;
@@ -621,22 +612,6 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
   const bool shade_order = c->ray_order == DSPT_ORDER_SHADE;
   const int32_t* perm = order ? c->order_perm : nullptr;
   const int order_blocks = (int)std::min<int64_t>(DSPT_ORDER_BLOCKS, grid_for(P, DSPT_ORDER_THREADS));
-  // the shadow trace of bounce k runs on the second stream beside bounce
-  // k + 1's refill, ordering and extension trace, which neither read nor
-  // write what it does (it adds to the radiance of the continuing paths;
-  // the refill writes new paths behind them); bounce k + 1's shading waits
-  // for it. Not while launches are timed one by one.
-  const bool ovl = c->overlap_shadow && !c->timing;
-  if (ovl && (int)c->ev_bounce.size() < 2 * c->chunk_iters) {
-    const size_t have = c->ev_bounce.size();
-    c->ev_bounce.resize((size_t)2 * c->chunk_iters, nullptr);
-    for (size_t i = have; i < c->ev_bounce.size(); i++)
-      HIPCHK(hipEventCreateWithFlags(&c->ev_bounce[i], hipEventDisableTiming));
-  }
-  hipError_t ovl_err = hipSuccess;
-  auto ovl_chk = [&](hipError_t e) {
-    if (e != hipSuccess && ovl_err == hipSuccess) ovl_err = e;
-  };
   // chunk_iters bounces; the state buffers alternate, starting at q0
   auto bounces = [&](int q0) {
     for (int k = 0; k < c->chunk_iters; k++) {
@@ -668,34 +643,25 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
           k_bin_scatter<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->order_keys, &ctl->qcount[a], c->shade_bins,
                                                              c->shade_perm);
         });
-      if (ovl && k > 0) ovl_chk(hipStreamWaitEvent(c->stream, c->ev_bounce[2 * (k - 1) + 1], 0));
       timed(c, DSPT_K_SHADE, [&] {
         const int32_t* sperm = shade_order ? c->shade_perm : perm;
         const int32_t* nsorted = shade_order ? &ctl->qcount[a] : &ctl->q_base;
         if (dbg)
           k_shade<true><<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1],
-                                                              c->shq, &ctl->sh_count[a], c->cnt, sperm, nsorted);
+                                                              c->shq, &ctl->sh_count, c->cnt, sperm, nsorted);
         else
           k_shade<false><<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1],
-                                                               c->shq, &ctl->sh_count[a], c->cnt, sperm, nsorted);
+                                                               c->shq, &ctl->sh_count, c->cnt, sperm, nsorted);
       });
-      hipStream_t ss = c->stream;
-      if (ovl) {
-        ovl_chk(hipEventRecord(c->ev_bounce[2 * k], c->stream));
-        ovl_chk(hipStreamWaitEvent(c->stream_sh, c->ev_bounce[2 * k], 0));
-        ss = c->stream_sh;
-      }
       timed(c, DSPT_K_TRACE_SHADOW, [&] {
         if (inst)
-          k_trace_shadow<true><<<trace_blocks, kTraceBlock, 0, ss>>>(c->d_jobs, nxt, c->shq, &ctl->sh_count[a],
-                                                                    &ctl->sh_next[a], c->spill_sh, c->cnt);
+          k_trace_shadow<true><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, nxt, c->shq, &ctl->sh_count,
+                                                                           &ctl->sh_next, c->spill, c->cnt);
         else
-          k_trace_shadow<false><<<trace_blocks, kTraceBlock, 0, ss>>>(c->d_jobs, nxt, c->shq, &ctl->sh_count[a],
-                                                                     &ctl->sh_next[a], c->spill_sh, c->cnt);
+          k_trace_shadow<false><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, nxt, c->shq, &ctl->sh_count,
+                                                                            &ctl->sh_next, c->spill, c->cnt);
       });
-      if (ovl) ovl_chk(hipEventRecord(c->ev_bounce[2 * k + 1], c->stream_sh));
     }
-    if (ovl) ovl_chk(hipStreamWaitEvent(c->stream, c->ev_bounce[2 * (c->chunk_iters - 1) + 1], 0));
   };
   const int q0 = c->qsel;
   if (c->use_graphs && !c->timing) {
@@ -713,8 +679,7 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       bounces(q0);
-      hipError_t ce = hipStreamEndCapture(c->stream, &g);
-      if (ce == hipSuccess && ovl_err != hipSuccess) ce = ovl_err;
+      const hipError_t ce = hipStreamEndCapture(c->stream, &g);
       if (ce != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
         return fail(DSPT_ERR_DEVICE, std::string("bounce-loop capture: ") + hipGetErrorString(ce));
@@ -729,7 +694,6 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
     HIPCHK(hipGraphLaunch(ge, c->stream));
   } else {
     bounces(q0);
-    HIPCHK(ovl_err);
   }
   c->qsel = (q0 + c->chunk_iters) & 1;
   if (nj > 0) {
@@ -1315,10 +1279,7 @@ int dspt_create(int profile, int device, dspt_ctx** out) {
     const int m = std::atoi(o);
     c->ray_order = (m == DSPT_ORDER_BUCKETS || m == DSPT_ORDER_SHADE) ? m : DSPT_ORDER_NONE;
   }
-  if (const char* o = std::getenv("DSPT_OVERLAP_SHADOW")) c->overlap_shadow = std::atoi(o) != 0;
-  if (const char* ci = std::getenv("DSPT_CHUNK_ITERS")) c->chunk_iters = std::max(1, std::min(64, std::atoi(ci)));
-  bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
-            hipStreamCreateWithFlags(&c->stream_sh, hipStreamNonBlocking) == hipSuccess;
+  bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
   void* p = nullptr;
   auto dev = [&](size_t bytes) -> void* {
     void* q = nullptr;
@@ -1370,13 +1331,11 @@ int dspt_destroy(dspt_ctx* c) {
   c->state_mem.release();
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->ev_chunk) if (e) (void)hipEventDestroy(e);
-  for (hipEvent_t e : c->ev_bounce) if (e) (void)hipEventDestroy(e);
   for (void* d : {(void*)c->cnt, (void*)c->d_jobs, (void*)c->d_tasks, (void*)c->d_ctl, (void*)c->d_sched, (void*)c->d_prog,
                   (void*)c->d_joblist, (void*)c->post_buf})
     if (d) (void)hipFree(d);
   for (void* h : {(void*)c->h_jobs, (void*)c->h_rb, (void*)c->h_sched, (void*)c->h_tasks, (void*)c->h_joblist})
     if (h) (void)hipHostFree(h);
-  if (c->stream_sh) (void)hipStreamDestroy(c->stream_sh);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;

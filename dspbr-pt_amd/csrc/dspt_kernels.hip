@@ -218,9 +218,9 @@ __global__ void k_plan(StreamCtl* __restrict__ ctl, const StreamSched* __restric
   ctl->next_item += n;
   ctl->qcount[a] = Q + n;
   ctl->qcount[a ^ 1] = 0;
-  ctl->sh_count[a] = 0;
-  ctl->sh_next[a] = 0;
+  ctl->sh_count = 0;
   ctl->ext_next = 0;
+  ctl->sh_next = 0;
 }
 
 // ---- ray order of the extension queue -----------------------------------

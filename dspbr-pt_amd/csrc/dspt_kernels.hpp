@@ -44,12 +44,9 @@ struct StreamCtl {
   int32_t regen_n;     // items issued by the current refill
   int32_t q_base;      // position where the refill appends
   int32_t qcount[2];   // live paths in state buffer 0 / 1
-  // shadow-ray queue of the bounce that started from state buffer 0 / 1 and
-  // the next entry a traversal wave takes: a bounce's shadow trace may still
-  // run while the next bounce starts (its plan resets only its own pair)
-  int32_t sh_count[2];
-  int32_t sh_next[2];
+  int32_t sh_count;    // shadow-ray queue
   int32_t ext_next;    // next position a traversal wave takes
+  int32_t sh_next;     // next shadow-queue entry a traversal wave takes
   int32_t pad;
 };
 

@@ -151,10 +151,7 @@ int dspt_get_kernel_stats(dspt_ctx* ctx, double* out);
  * replayed; the work sizes live in device counters, so every replay is the
  * same launch sequence. On by default (the environment variable
  * DSPT_BOUNCE_GRAPHS=0 turns it off at dspt_create); kernel timing launches
- * eagerly. Each bounce's shadow trace runs on a second stream beside the
- * next bounce's refill and closest-hit trace (DSPT_OVERLAP_SHADOW=0 at
- * dspt_create keeps it in line; kernel timing always does). Results are
- * identical either way. No counterpart in the
+ * eagerly. Results are identical either way. No counterpart in the
  * reference, whose loop is one WebGL draw per tile and frame
  * (renderer.ts:595-714). */
 int dspt_set_bounce_graphs(dspt_ctx* ctx, int enable);

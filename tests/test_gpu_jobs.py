@@ -198,20 +198,17 @@ def test_bounce_graph_replays_equal_eager_launches(mix):
         _same(a, g)
 
 
-def test_ray_order_keeps_results(mix, monkeypatch):
+def test_ray_order_keeps_results(mix):
     """Bucketing the extension queue by (octant, origin cell) changes which
     lane walks which ray, not any path's result: accumulations, samples and
     the work counters equal those of the compaction order, with and without
-    the captured graphs, one- and two-level, with each bounce's shadow trace
-    on the second stream beside the next bounce (the default) or in line."""
+    the captured graphs, one- and two-level."""
     from dspt import Engine, ORDER_BUCKETS, ORDER_NONE, ORDER_SHADE, PROFILE_FULL, SCHED_STREAM, scenes
     from dspt.batch import JobSpec
     inst = scenes.instanced_scene(seed=9, n_meshes=3, n_instances=40, tri_range=(300, 1500), extent=5.0, height=2.0,
                                   width=40, height_px=32, spp=5, max_bounces=6, ibl_size=(128, 64))
     runs = {}
-    for order, graphs, overlap in ((ORDER_NONE, True, False), (ORDER_BUCKETS, True, True), (ORDER_BUCKETS, False, True),
-                                   (ORDER_SHADE, True, True), (ORDER_SHADE, False, False)):
-        monkeypatch.setenv("DSPT_OVERLAP_SHADOW", "1" if overlap else "0")  # read when the engine is created
+    for order, graphs in ((ORDER_NONE, True), (ORDER_BUCKETS, True), (ORDER_BUCKETS, False), (ORDER_SHADE, True)):
         eng = Engine(PROFILE_FULL)
         eng.set_ray_order(order)
         eng.set_bounce_graphs(graphs)
@@ -226,11 +223,11 @@ def test_ray_order_keeps_results(mix, monkeypatch):
         ids = [b.create_job(s) for s in specs]
         eng.jobs_run(ids)
         c = eng.counters()
-        runs[(order, graphs, overlap)] = ([_result(eng, j) for j in ids], {k: c[k] for k in ("paths", "rays_extension",
+        runs[(order, graphs)] = ([_result(eng, j) for j in ids], {k: c[k] for k in ("paths", "rays_extension",
                                                                                    "rays_shadow", "node_visits",
                                                                                    "tri_tests", "shade_hits")})
         eng.close()
-    ref, ref_cnt = runs[(ORDER_NONE, True, False)]
+    ref, ref_cnt = runs[(ORDER_NONE, True)]
     for key, (res, cnt) in runs.items():
         assert cnt == ref_cnt, key
         for a, g in zip(ref, res):
