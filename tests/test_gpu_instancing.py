@@ -140,3 +140,34 @@ def test_plain_job_unchanged_beside_instanced_job(inst_wl):
     for x, y in zip(eng.job_read_samples(j1), a_s):
         np.testing.assert_array_equal(x, y)
     assert eng.job_status(j2) == (wl.spp, wl.spp)
+
+
+@pytest.mark.parametrize("mode,index,tol", [("Albedo", 1, 1e-5), ("Normals", 4, 1e-4), ("Bitangents", 6, 1e-3),
+                                            ("UV0", 8, 1e-4), ("IBL PDF", 10, 0.0)])
+def test_instanced_debug_view_matches_flattened(inst_wl, mode, index, tol):
+    """The debug program on a two-level scene reads the hit's attributes
+    through the instance (object-space vertices, normals moved to world
+    space): it equals the flattened scene's view, which the one-level path
+    pins to the oracle, to rounding, wherever both rays hit the same
+    triangle (>= 99.9 % of pixels); misses are identical."""
+    from dspt import Engine, PROFILE_FULL
+    from dspt.batch import BatchRenderer, JobSpec
+    wl, flat, _, _ = inst_wl
+    eng = Engine(PROFILE_FULL)
+    b = BatchRenderer(eng)
+    b.add_scene("inst", wl.scene)
+    b.add_scene("flat", flat)
+    b.add_ibl("inst", wl.ibl)
+    b.add_ibl("flat", wl.ibl)
+    jobs = [b.create_job(JobSpec(k, k, wl.camera, wl.width, wl.height, 2, max_bounces=wl.max_bounces, record=True,
+                                 debug_mode=index)) for k in ("inst", "flat")]
+    eng.jobs_run(jobs)
+    (si, _, pi), (sf, _, pf) = (eng.job_read_samples(j) for j in jobs)
+    eng.close()
+    miss = (pi < 0) & (pf < 0)
+    hit = (pi >= 0) & (pf >= 0)
+    assert miss.mean() > 0.05 and hit.mean() > 0.3
+    assert (miss | hit).mean() >= 0.999
+    np.testing.assert_array_equal(si[miss], sf[miss])
+    close = np.all(np.abs(si[hit] - sf[hit]) <= np.maximum(tol * np.abs(sf[hit]), tol), axis=-1)
+    assert close.mean() >= 0.999, f"{mode}: only {close.mean():.5f} of hit samples agree"
