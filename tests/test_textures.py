@@ -138,3 +138,40 @@ def test_textured_render_matches_oracle():
     with pytest.raises(DsptError) as e:
         eng2.render(1, 1)
     assert e.value.code == -3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [1, 2, 3, 4, 5, 6, 8, 9, 13])
+def test_textured_debug_views_match_oracle(mode):
+    """Debug views of textured materials (debug.glsl:1-83 over the render
+    state of material.glsl:5-110): albedo, metalness and roughness read
+    through their textures, normals / tangents / bitangents after the normal
+    map, uv0, clearcoat and specular tint, sample for sample against the
+    oracle (the bar of test_gpu_parity.py) and the accumulation bit for bit."""
+    from dspt import Engine, PROFILE_FULL, pack_frame_block, scenes
+    from test_gpu_parity import _check_samples
+    wl = scenes.textured_spheres(n_spheres=22)
+    W, H, F = 64, 48, 2
+    eng = Engine(PROFILE_FULL)
+    tot = _upload(eng, wl)
+    s = wl.scene
+    dims, texels = s.getTextureArrays()
+    eng.scene_set_textures(0, s.getFlatTextureInfoBuffer(), dims, texels)
+    img = wl.ibl["image"]
+    blk = pack_frame_block(wl.camera, W, H, max_bounces=wl.max_bounces, ibl_size=(img["width"], img["height"]),
+                           ibl_total_sum=tot, scene_counts=(s.num_triangles, len(s.bvhNodes) // 16,
+                                                             len(s.bvhTriangleIndices), 0), debug_mode=mode)
+    eng.resize(W, H)
+    eng.set_frame(blk)
+    eng.set_record_samples(True)
+    eng.render(1, F)
+    acc = eng.read_accum()
+    g_s, g_sig, g_prim = eng.read_samples(F)
+    eng.close()
+    osc = O.OracleScene(s.vertexBuffer, s.triangleIndexBuffer, s.bvhNodes, s.bvhTriangleIndices, s.getFlatMaterialBuffer(),
+                        wl.ibl, 1)
+    osc.set_textures(s.getFlatTextureInfoBuffer(), dims, texels)
+    o = osc.render(blk, W, H, 1, F, threads=8)
+    _check_samples(g_s, g_sig, g_prim, o)
+    np.testing.assert_array_equal(acc.view(np.uint32), o["accum"].view(np.uint32))
+    assert (g_prim >= 0).mean() > 0.2
