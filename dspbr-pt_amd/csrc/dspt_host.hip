@@ -592,6 +592,9 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
       max_npix = std::max(max_npix, j->items());
     }
   }
+  // the jobs with paths in the pool are those listed (progress is read back
+  // late, so a listed job may already be done); one of them: its index
+  c->h_sched[s].solo = nj == 1 ? hj[0] : -1;
   if (nt > 0) HIPCHK(hipMemcpyAsync(c->d_tasks, ht, (size_t)nt * sizeof(PassTask), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_sched, c->h_sched + s, sizeof(StreamSched), hipMemcpyHostToDevice, c->stream));
   if (nj > 0) HIPCHK(hipMemcpyAsync(c->d_joblist, hj, (size_t)nj * 4, hipMemcpyHostToDevice, c->stream));
@@ -638,7 +641,8 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
       });
       if (shade_order)
         timed(c, DSPT_K_ORDER, [&] {
-          k_hit_keys<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], c->order_keys, c->shade_bins);
+          k_hit_keys<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], c->order_keys,
+                                                                         c->shade_bins, c->d_sched);
           k_bin_scan<<<1, 1024, 0, c->stream>>>(c->shade_bins);
           k_bin_scatter<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->order_keys, &ctl->qcount[a], c->shade_bins,
                                                              c->shade_perm);

@@ -318,19 +318,26 @@ __global__ void k_bin_scatter(const uint16_t* __restrict__ keys, const int32_t* 
 // Shading order (mode DSPT_ORDER_SHADE): the whole queue [0, qcount) keyed
 // by the material of each path's hit (misses in bucket 0), so that a wave
 // shades one material's lobes together.
+// sch->solo >= 0: every path in the pool belongs to that job, so the job
+// index is not read per path (half the bytes this pass reads); the triangle
+// range check keeps the lookup inside that job's table whatever the hit.
 __global__ void k_hit_keys(const DevJob* __restrict__ jobs, PathState ps, const int32_t* __restrict__ n_ptr,
-                           uint16_t* __restrict__ keys, uint32_t* __restrict__ bins) {
+                           uint16_t* __restrict__ keys, uint32_t* __restrict__ bins, const StreamSched* __restrict__ sch) {
   __shared__ uint32_t h[kOrderBins];
   const int n = *n_ptr;
+  const int solo = sch->solo;
   if ((int64_t)blockIdx.x * blockDim.x >= n) return;
   for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x) h[i] = 0u;
+  const uint32_t* __restrict__ solo_mat = solo >= 0 ? jobs[solo].S.tri_mat : nullptr;
+  const int32_t solo_t = solo >= 0 ? jobs[solo].S.T : 0;
   __syncthreads();
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
     const int p = (int)q;
     const int tri = __float_as_int(ps.hit[p].x);
     uint32_t k = 0u;
     if (tri >= 0) {
-      k = 1u + jobs[__float_as_int(ps.org[p].w)].S.tri_mat[tri] % (uint32_t)(kOrderBins - 1);
+      if (solo_mat) k = tri < solo_t ? 1u + solo_mat[tri] % (uint32_t)(kOrderBins - 1) : 0u;
+      else k = 1u + jobs[__float_as_int(ps.org[p].w)].S.tri_mat[tri] % (uint32_t)(kOrderBins - 1);
     }
     keys[p] = (uint16_t)k;
     atomicAdd(&h[k], 1u);

@@ -54,7 +54,7 @@ struct StreamCtl {
 struct StreamSched {
   int64_t limit;   // items [next_item, limit) may be issued
   int32_t ntasks;  // entries in the task table
-  int32_t pad;
+  int32_t solo;    // the one job with paths in flight, or -1
 };
 
 // Structure-of-arrays path state, capacity = pool size, indexed by
@@ -126,7 +126,8 @@ static_assert(DSPT_ORDER_THREADS % 64 == 0 && DSPT_ORDER_THREADS <= 1024, "order
 __global__ void k_ray_keys(const DevJob* jobs, PathState ps, const StreamCtl* ctl, uint16_t* keys, uint32_t* bins);
 __global__ void k_bin_scan(uint32_t* bins);
 __global__ void k_bin_scatter(const uint16_t* keys, const int32_t* n_ptr, uint32_t* bins, int32_t* perm);
-__global__ void k_hit_keys(const DevJob* jobs, PathState ps, const int32_t* n_ptr, uint16_t* keys, uint32_t* bins);
+__global__ void k_hit_keys(const DevJob* jobs, PathState ps, const int32_t* n_ptr, uint16_t* keys, uint32_t* bins,
+                           const StreamSched* sch);
 template <bool INST>
 __global__ void k_trace_ext(const DevJob* jobs, PathState ps, const int32_t* count_ptr, int32_t* next_ptr, int* spill,
                             unsigned long long* cnt, const int32_t* perm, const int32_t* nsorted_ptr);
