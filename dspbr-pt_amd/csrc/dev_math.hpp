@@ -38,6 +38,48 @@ DI float4 ldg(const float4* p) {
 #endif
 }
 
+// Streaming (non-temporal) loads and stores of 16-byte path-state records:
+// the shading pass reads each path's state once per bounce and writes the
+// continuing path's next state once, so these lines are marked for early
+// eviction and the L2 keeps the vertex, material and IBL lines the pass
+// reuses. Host builds access memory directly.
+DI float4 ld_stream(const float4* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const v4f x = __builtin_nontemporal_load((const __attribute__((address_space(1))) v4f*)p);
+  return make_float4(x.x, x.y, x.z, x.w);
+#else
+  return *p;
+#endif
+}
+DI uint4 ld_stream(const uint4* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef unsigned v4u __attribute__((ext_vector_type(4)));
+  const v4u x = __builtin_nontemporal_load((const __attribute__((address_space(1))) v4u*)p);
+  return make_uint4(x.x, x.y, x.z, x.w);
+#else
+  return *p;
+#endif
+}
+DI void st_stream(float4* p, float4 v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const v4f x = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(x, (__attribute__((address_space(1))) v4f*)p);
+#else
+  *p = v;
+#endif
+}
+DI void st_stream(uint4* p, uint4 v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef unsigned v4u __attribute__((ext_vector_type(4)));
+  const v4u x = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(x, (__attribute__((address_space(1))) v4u*)p);
+#else
+  *p = v;
+#endif
+}
+
 // issued(): an empty asm that consumes the loaded values, placed after all
 // loads of one record. Without it the compiler sinks loads that only one
 // side of a later branch uses into that branch (e.g. a node's child boxes

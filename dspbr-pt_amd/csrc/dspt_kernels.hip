@@ -690,8 +690,8 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
   unsigned n_paths = 0, n_hits = 0, n_miss = 0, n_nee = 0, n_ext = 0;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < count; q += (int64_t)gridDim.x * blockDim.x) {
     const int p = q < nsorted ? perm[q] : (int)q;
-    const float4 o4 = ps.org[p], d4 = ps.dir[p];
-    const uint4 meta = ps.meta[p];
+    const float4 o4 = ld_stream(ps.org + p), d4 = ld_stream(ps.dir + p);
+    const uint4 meta = ld_stream(ps.meta + p);
     const int jid = __float_as_int(o4.w);
     const DevJob* J = jobs + jid;
     const bool rec = J->rec_L != nullptr;
@@ -703,13 +703,13 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
       const FrameParams& F = J->F;
       const int maxB = (int)F.max_bounces;
       const int bounce = __float_as_int(d4.w);
-      float4 hit = ps.hit[p];
+      float4 hit = ld_stream(ps.hit + p);
       int tri = __float_as_int(hit.x);
       f3 org = xyz(o4), dir = xyz(d4);
-      float4 pw4 = ps.pw[p];
+      float4 pw4 = ld_stream(ps.pw + p);
       f3 pw = xyz(pw4);
       float last_pdf = pw4.w;
-      float4 L4 = ps.L[p];
+      float4 L4 = ld_stream(ps.L + p);
       f3 L = xyz(L4);
 
       if (DEBUG && F.debug_mode != 0.0f) {  // the debug program: one hit, no bounce
@@ -841,11 +841,11 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
         ps.sh_pw[o] = make_float4(pw_nee.x, pw_nee.y, pw_nee.z, 0.0f);
         if (flags & 2) { ps.sh_pdir[o] = pdir; ps.sh_pc[o] = make_float4(c_pt.x, c_pt.y, c_pt.z, 0.0f); }
       }
-      nx.org[o] = make_float4(pos.x, pos.y, pos.z, __int_as_float(jid));
-      nx.dir[o] = make_float4(wo.x, wo.y, wo.z, __int_as_float(bounce + 1));
-      nx.pw[o] = make_float4(pw.x, pw.y, pw.z, last_pdf);
-      nx.L[o] = make_float4(L.x, L.y, L.z, L4.w);
-      nx.meta[o] = make_uint4(rng.x, rng.y, meta.z, meta.w);
+      st_stream(nx.org + o, make_float4(pos.x, pos.y, pos.z, __int_as_float(jid)));
+      st_stream(nx.dir + o, make_float4(wo.x, wo.y, wo.z, __int_as_float(bounce + 1)));
+      st_stream(nx.pw + o, make_float4(pw.x, pw.y, pw.z, last_pdf));
+      st_stream(nx.L + o, make_float4(L.x, L.y, L.z, L4.w));
+      st_stream(nx.meta + o, make_uint4(rng.x, rng.y, meta.z, meta.w));
       if (rec) { nx.sig[o] = sig; nx.prim[o] = ps.prim[p]; }
       term = false;
     } while (false);
