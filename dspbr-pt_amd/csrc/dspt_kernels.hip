@@ -487,7 +487,7 @@ __global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathSt
   int p = 0;
   Walker<INST> w;
   auto finish_ray = [&]() {
-    ps.hit[p] = make_float4(__int_as_float(w.r.h.tri), w.r.h.t, w.r.h.u, w.r.h.v);
+    st_stream(ps.hit + p, make_float4(__int_as_float(w.r.h.tri), w.r.h.t, w.r.h.u, w.r.h.v));  // read once, by the next passes
     if (INST) ps.hinst[p] = w.inst();
     nr++;
     active = false;
