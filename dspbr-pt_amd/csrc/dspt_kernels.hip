@@ -370,7 +370,20 @@ __global__ void k_regen(PathState ps, const DevJob* __restrict__ jobs, const Pas
     int fslot = (int)(local / T.npix);
     int rpix = (int)(local - (int64_t)fslot * T.npix);  // pixel of the render region
     int W = J->W, H = J->H;
-    int px = J->rx0 + rpix % J->rw, py = J->ry0 + rpix / J->rw;
+    // region pixels in 8 x 8 tiles (bands of 8 rows, tiles left to right;
+    // the last band and tile of a row are cut to the region), so the 64
+    // camera rays of a traversal wave start as a square of pixels; each
+    // sample is still seeded by its own pixel, so only the order changes
+    int px, py;
+    {
+      const int rw = J->rw, rh = J->rn / J->rw;
+      const int band = rpix / (8 * rw), idx = rpix - band * 8 * rw;
+      const int bh = min(8, rh - band * 8);
+      const int t = idx / (8 * bh), j = idx - t * 8 * bh;
+      const int tw = min(8, rw - t * 8);
+      px = J->rx0 + t * 8 + j % tw;
+      py = J->ry0 + band * 8 + j / tw;
+    }
     int pix = py * W + px;
     int frame = T.first_frame + fslot;
     const FrameParams& F = J->F;
