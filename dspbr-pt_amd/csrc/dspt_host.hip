@@ -629,7 +629,8 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
         timed(c, DSPT_K_ORDER, [&] {
           k_ray_keys<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->d_jobs, cur, ctl, c->order_keys, c->order_bins);
           k_bin_scan<<<1, 1024, 0, c->stream>>>(c->order_bins);
-          k_bin_scatter<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->order_keys, &ctl->q_base, c->order_bins, c->order_perm);
+          k_bin_scatter<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->order_keys, &ctl->q_base, c->order_bins, c->order_perm,
+                                                                            nullptr, nullptr);
         });
       timed(c, DSPT_K_TRACE_EXT, [&] {
         if (inst)
@@ -645,7 +646,7 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
                                                                          c->shade_bins, c->d_sched);
           k_bin_scan<<<1, 1024, 0, c->stream>>>(c->shade_bins);
           k_bin_scatter<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->order_keys, &ctl->qcount[a], c->shade_bins,
-                                                             c->shade_perm);
+                                                             c->shade_perm, perm, &ctl->q_base);  // buckets in ray order
         });
       timed(c, DSPT_K_SHADE, [&] {
         const int32_t* sperm = shade_order ? c->shade_perm : perm;

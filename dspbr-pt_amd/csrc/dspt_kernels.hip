@@ -297,22 +297,30 @@ __global__ void k_bin_scan(uint32_t* __restrict__ bins) {
 
 // perm[start of bucket + rank] = position, for [0, *n_ptr): each block takes
 // a contiguous range, counts it per bucket in LDS, reserves its slots with
-// one global add per non-empty bucket, then places its positions.
+// one global add per non-empty bucket, then places its positions. With src,
+// the range walks the positions in the order src gives for entries below
+// *ns_ptr (the ray order), so each bucket holds its paths in about that
+// order (block by block).
 __global__ void k_bin_scatter(const uint16_t* __restrict__ keys, const int32_t* __restrict__ n_ptr,
-                              uint32_t* __restrict__ bins, int32_t* __restrict__ perm) {
+                              uint32_t* __restrict__ bins, int32_t* __restrict__ perm, const int32_t* __restrict__ src,
+                              const int32_t* __restrict__ ns_ptr) {
   __shared__ uint32_t h[kOrderBins];
   const int n = *n_ptr;
+  const int ns = src ? *ns_ptr : 0;
   const int per = (n + gridDim.x - 1) / gridDim.x;
   const int lo = (int)min((int64_t)blockIdx.x * per, (int64_t)n), hi = min(lo + per, n);
   if (lo >= hi) return;  // block-uniform: no work for this block
   for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x) h[i] = 0u;
   __syncthreads();
-  for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) atomicAdd(&h[keys[q]], 1u);
+  for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) atomicAdd(&h[keys[q < ns ? src[q] : q]], 1u);
   __syncthreads();
   for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x)
     if (h[i]) h[i] = atomicAdd(&bins[i], h[i]);  // count -> this block's first slot in the bucket
   __syncthreads();
-  for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) perm[atomicAdd(&h[keys[q]], 1u)] = q;
+  for (int q = lo + threadIdx.x; q < hi; q += blockDim.x) {
+    const int p = q < ns ? src[q] : q;
+    perm[atomicAdd(&h[keys[p]], 1u)] = p;
+  }
 }
 
 // Shading order (mode DSPT_ORDER_SHADE): the whole queue [0, qcount) keyed

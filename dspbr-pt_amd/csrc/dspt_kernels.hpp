@@ -125,7 +125,8 @@ constexpr int kOrderBins = 8192;
 static_assert(DSPT_ORDER_THREADS % 64 == 0 && DSPT_ORDER_THREADS <= 1024, "order passes: whole waves, at most 1024 threads");
 __global__ void k_ray_keys(const DevJob* jobs, PathState ps, const StreamCtl* ctl, uint16_t* keys, uint32_t* bins);
 __global__ void k_bin_scan(uint32_t* bins);
-__global__ void k_bin_scatter(const uint16_t* keys, const int32_t* n_ptr, uint32_t* bins, int32_t* perm);
+__global__ void k_bin_scatter(const uint16_t* keys, const int32_t* n_ptr, uint32_t* bins, int32_t* perm, const int32_t* src,
+                              const int32_t* ns_ptr);
 __global__ void k_hit_keys(const DevJob* jobs, PathState ps, const int32_t* n_ptr, uint16_t* keys, uint32_t* bins,
                            const StreamSched* sch);
 template <bool INST>
