@@ -419,25 +419,31 @@ __global__ void k_regen(PathState ps, const DevJob* __restrict__ jobs, const Pas
 }
 
 // Lanes of a traversal wave take new rays from the queue whenever at least
-// kRefill of them are idle (one atomic per wave), so a wave is not held by
-// its slowest ray; every ray still runs the same step sequence.
-#ifndef DSPT_REFILL
-#define DSPT_REFILL 48
+// that many of them are idle (one atomic per wave), so a wave is not held by
+// its slowest ray; every ray still runs the same step sequence. Per kernel:
+// closest hit (EXT) and shadow rays (SH).
+#ifndef DSPT_REFILL_EXT
+#define DSPT_REFILL_EXT 48
 #endif
-constexpr int kRefill = DSPT_REFILL;
+#ifndef DSPT_REFILL_SH
+#define DSPT_REFILL_SH 48
+#endif
 // Inner-node steps a wave takes before it tests the leaves its lanes reached.
-#ifndef DSPT_INNER_RUN
-#define DSPT_INNER_RUN 4
+#ifndef DSPT_INNER_RUN_EXT
+#define DSPT_INNER_RUN_EXT 4
 #endif
-constexpr int kInnerRun = DSPT_INNER_RUN;
+#ifndef DSPT_INNER_RUN_SH
+#define DSPT_INNER_RUN_SH 4
+#endif
 
 // Wave-uniform refill: idle lanes get consecutive queue entries. Returns the
 // queue index for this lane (or -1) and sets `exhausted` once the queue is
 // used up.
+template <int MIN_IDLE>
 DD int refill(bool active, int count, int32_t* next_ptr, bool& exhausted) {
   const unsigned long long idle = __ballot(!active);
   const int n_idle = __popcll(idle);
-  if (exhausted || n_idle < kRefill) return -1;
+  if (exhausted || n_idle < MIN_IDLE) return -1;
   const int lane = __lane_id();
   const int leader = __ffsll((long long)idle) - 1;
   int base = 0;
@@ -514,7 +520,7 @@ __global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathSt
     active = false;
   };
   for (;;) {
-    const int q = refill(active, count, next_ptr, exhausted);
+    const int q = refill<DSPT_REFILL_EXT>(active, count, next_ptr, exhausted);
     if (q >= 0) {
       p = q < nsorted ? perm[q] : q;
       const float4 o = ps.org[p], d = ps.dir[p];
@@ -526,11 +532,11 @@ __global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathSt
       continue;
     }
     // inner nodes until every busy lane has reached its next leaf (or at
-    // most kInnerRun steps), then the leaves together; the slab test runs on
+    // most DSPT_INNER_RUN_EXT steps), then the leaves together; the slab test runs on
     // hardware min/max when every busy lane allows it (a uniform branch)
     const bool wfast = !INST && __ballot(active && !w.fast) == 0;
     bool leaf = false;
-    for (int k = 0; k < kInnerRun; k++) {
+    for (int k = 0; k < DSPT_INNER_RUN_EXT; k++) {
       const bool inner = active && !leaf;
       if (__ballot(inner) == 0) break;
       if (inner) {
@@ -568,7 +574,7 @@ __global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, Pat
   Walker<INST> w;
   f3 org = sp3(0.0f), contrib = sp3(0.0f);
   for (;;) {
-    const int q = refill(active, count, next_ptr, exhausted);
+    const int q = refill<DSPT_REFILL_SH>(active, count, next_ptr, exhausted);
     if (q >= 0) {
       p = queue[q];
       const float4 so = ps.sh_org[p];
@@ -586,7 +592,7 @@ __global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, Pat
     }
     // the exact slab test only: the fast one measured no gain here and cost a spill
     bool leaf = false, fin = false;
-    for (int k = 0; k < kInnerRun; k++) {
+    for (int k = 0; k < DSPT_INNER_RUN_SH; k++) {
       const bool inner = active && !leaf && !fin;
       if (__ballot(inner) == 0) break;
       if (inner) {
