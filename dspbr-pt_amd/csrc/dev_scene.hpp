@@ -61,7 +61,10 @@ struct SceneView {
   // 1 when every inner node's child boxes are finite (checked at upload):
   // then a ray with finite origin and finite, non-zero 1/dir can take the
   // slab test on hardware min/max (slab_fast)
-  int32_t fast_slab, pad_fs;
+  int32_t fast_slab;
+  // ray order: bits of the origin-cell key per axis (x | y << 4 | z << 8),
+  // 12 in all, set at upload from the root box (scene_key_bits)
+  int32_t key_bits;
   const uint32_t* __restrict__ tri_mat;  // material index of each triangle (its first vertex's pos.w)
   // the three vertex records of each triangle (5 float4 each), copied at
   // upload so a hit reaches its vertices without the index load; NULL: go

@@ -128,6 +128,7 @@ struct SceneRes {
   // two-level scenes: root of the top level and first instance record in `nodes`
   int32_t top = -1, inst_base = -1, I = 0;
   int32_t fast_slab = 0;  // every inner node's child boxes are finite
+  int32_t key_bits = 0;   // ray-order grid bits per axis (SceneView::key_bits)
   int32_t* d_flag = nullptr;
   uint32_t* tri_mat = nullptr;  // material of each triangle (resident scenes)
   float4* tri_verts = nullptr;  // vertex records per triangle corner (resident scenes)
@@ -371,6 +372,7 @@ SceneView view_of(dspt_ctx* c, const JobRes* j) {
   S.num_tex = s->num_tex; S.num_texarr = s->num_texarr;
   S.top = s->top; S.inst_base = s->inst_base;
   S.fast_slab = s->fast_slab;
+  S.key_bits = s->key_bits;
   const IblRes* ib = (j->ibl >= 0 && j->ibl < (int)c->ibls.size()) ? c->ibls[j->ibl] : nullptr;
   if (ib) {
     S.env = ib->env; S.pdf = ib->pdf; S.cdf = ib->cdf; S.ypdf = ib->ypdf; S.ycdf = ib->ycdf;
@@ -811,6 +813,26 @@ int validate_geometry(const float* vertices, int64_t V, const uint32_t* tri_idx,
   return 0;
 }
 
+// Bits of the ray-order grid per axis from the scene's root box (the node
+// at `root`, read back once): each of the 12 bits halves the longest cell
+// side (ties: x, then y), so the cells of a flat scene spread over its
+// extent. Packed as x | y << 4 | z << 8 for ray_key.
+int scene_key_bits(dspt_ctx* c, SceneRes* s, int64_t root) {
+  float nd[16];
+  HIPCHK(hipMemcpy(nd, s->nodes + (size_t)root * 4, sizeof(nd), hipMemcpyDeviceToHost));
+  float ex = std::max(nd[4], nd[12]) - std::min(nd[0], nd[8]);
+  float ey = std::max(nd[5], nd[13]) - std::min(nd[1], nd[9]);
+  float ez = std::max(nd[6], nd[14]) - std::min(nd[2], nd[10]);
+  int bx = 0, by = 0, bz = 0;
+  for (int i = 0; i < 12; i++) {
+    if (ex >= ey && ex >= ez) { bx++; ex *= 0.5f; }
+    else if (ey >= ez) { by++; ey *= 0.5f; }
+    else { bz++; ez *= 0.5f; }
+  }
+  s->key_bits = bx | (by << 4) | (bz << 8);
+  return 0;
+}
+
 // Scene whose BVH is built on the GPU (gpu_bvh_build) from the uploaded
 // vertex stream; no host-side node array. With mats == NULL (K = 0) only the
 // geometry is checked and uploaded (dspt_build_bvh_gpu).
@@ -884,11 +906,15 @@ int make_scene_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint
   (void)hipEventElapsedTime(&ms, e0, e1);
   c->ev_pool.push_back(e0);
   c->ev_pool.push_back(e1);
+  e0 = e1 = nullptr;
   if (build_ms) *build_ms = ms;
   (void)hipFree(big_nodes);
   (void)hipFree(d_bvh);
+  big_nodes = nullptr;
+  d_bvh = nullptr;
   s->N = (int32_t)N; s->M = (int32_t)T; s->T = (int32_t)T; s->K = K; s->V = V;
   s->max_tex_id = max_tex;
+  if (mats && (r = scene_key_bits(c, s, 0))) return bail(r);
   *out = s;
   return 0;
 }
@@ -945,8 +971,10 @@ int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tr
   if ((r = chk(hipGetLastError(), "scene kernels")) || (r = chk(hipStreamSynchronize(c->stream), "scene upload"))) return r;
   if ((r = chk(hipMemcpy(&s->fast_slab, s->d_flag, 4, hipMemcpyDeviceToHost), "scene flag"))) return r;
   (void)hipFree(d_bvh);
+  d_bvh = nullptr;
   s->N = (int32_t)N; s->M = (int32_t)M; s->T = (int32_t)T; s->K = K; s->V = V;
   s->max_tex_id = max_tex;
+  if ((r = scene_key_bits(c, s, 0))) return bail(r);
   *out = s;
   return 0;
 }
@@ -1097,6 +1125,11 @@ int make_scene_instanced(dspt_ctx* c, const float* vertices, int64_t V, const ui
   s->top = (int32_t)top;
   s->inst_base = (int32_t)inst_base;
   s->I = num_inst;
+  if (int r = scene_key_bits(c, s, top)) {
+    s->mem.release();
+    delete s;
+    return r;
+  }
   *out = s;
   return 0;
 }

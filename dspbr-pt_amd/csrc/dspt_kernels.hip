@@ -225,15 +225,14 @@ __global__ void k_plan(StreamCtl* __restrict__ ctl, const StreamSched* __restric
 
 // ---- ray order of the extension queue -----------------------------------
 // Bucket key of a continuing path: the Morton code of its origin's cell in a
-// 16x16x16 grid over the root box of its job's scene (12 bits), then the sign
-// of dir.y (up or down). Rays of one bucket start close together, walk the
-// same part of the tree and mostly hit the same objects, which keeps both the
+// grid of 2^12 cells over the root box of its job's scene, then the sign of
+// dir.y (up or down). The grid's 12 bits go to the axes by halving the
+// longest cell side each time (scene_key_bits), so a flat terrain gets its
+// cells in x and z. Rays of one bucket start close together, walk the same
+// part of the tree and mostly hit the same objects, which keeps both the
 // traversal's node fetches and the shading's material branches coherent.
-// (Measured alternatives: direction octant above an 8^3 grid, 8^3 with the
-// octant below, 16^3 alone, 16^3 with two direction bits.)
-DI uint32_t spread3(uint32_t v) {  // bits b2 b1 b0 -> b2 0 0 b1 0 0 b0
-  return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4);
-}
+// (Measured alternatives: 16 cells per axis; direction octant above an 8^3
+// grid, 8^3 with the octant below, 16^3 alone, 16^3 with two direction bits.)
 DI uint32_t ray_key(const SceneView& S, float4 o, float4 d) {
   f3 lo = sp3(0.0f), hi = sp3(0.0f);
   if (S.N > 0) {
@@ -242,15 +241,21 @@ DI uint32_t ray_key(const SceneView& S, float4 o, float4 d) {
     lo = f3{fminf(a.x, c.x), fminf(a.y, c.y), fminf(a.z, c.z)};
     hi = f3{fmaxf(b.x, e.x), fmaxf(b.y, e.y), fmaxf(b.z, e.z)};
   }
-  auto cell = [](float x, float l, float h) -> uint32_t {
+  const int bx = S.key_bits & 15, by = (S.key_bits >> 4) & 15, bz = (S.key_bits >> 8) & 15;
+  auto cell = [](float x, float l, float h, int b) -> uint32_t {
     const float w = h - l;
-    if (!(w > 0.0f)) return 0u;
-    const float t = (x - l) * (16.0f / w);
-    return t >= 15.0f ? 15u : (t > 0.0f ? (uint32_t)t : 0u);  // NaN -> 0
+    if (!(w > 0.0f) || b == 0) return 0u;
+    const uint32_t nmax = (1u << b) - 1u;
+    const float t = (x - l) * ((float)(1u << b) / w);
+    return t >= (float)nmax ? nmax : (t > 0.0f ? (uint32_t)t : 0u);  // NaN -> 0
   };
-  const uint32_t cx = cell(o.x, lo.x, hi.x), cy = cell(o.y, lo.y, hi.y), cz = cell(o.z, lo.z, hi.z);
-  const uint32_t m = spread3(cx & 7u) | (spread3(cy & 7u) << 1) | (spread3(cz & 7u) << 2) |
-                     (((cx >> 3) | ((cy >> 3) << 1) | ((cz >> 3) << 2)) << 9);
+  const uint32_t cx = cell(o.x, lo.x, hi.x, bx), cy = cell(o.y, lo.y, hi.y, by), cz = cell(o.z, lo.z, hi.z, bz);
+  uint32_t m = 0u;
+  for (int lvl = 11; lvl >= 0; lvl--) {  // interleaved from each axis's top bit down
+    if (lvl < bx) m = (m << 1) | ((cx >> lvl) & 1u);
+    if (lvl < by) m = (m << 1) | ((cy >> lvl) & 1u);
+    if (lvl < bz) m = (m << 1) | ((cz >> lvl) & 1u);
+  }
   return (m << 1) | (d.y < 0.0f ? 1u : 0u);
 }
 static_assert(kOrderBins == 8192, "the key has 13 bits");
