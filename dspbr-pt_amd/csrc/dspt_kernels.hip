@@ -351,7 +351,10 @@ __global__ void k_hit_keys(const DevJob* __restrict__ jobs, PathState ps, const 
     uint32_t k = 0u;
     if (tri >= 0) {
       if (solo_mat) k = tri < solo_t ? 1u + solo_mat[tri] % (uint32_t)(kOrderBins - 1) : 0u;
-      else k = 1u + jobs[__float_as_int(ps.org[p].w)].S.tri_mat[tri] % (uint32_t)(kOrderBins - 1);
+      else {  // several jobs: their material indices are offset by job, so jobs do not share buckets
+        const int jid = __float_as_int(ps.org[p].w);
+        k = 1u + ((uint32_t)jid * 509u + jobs[jid].S.tri_mat[tri]) % (uint32_t)(kOrderBins - 1);
+      }
     }
     keys[p] = (uint16_t)k;
     atomicAdd(&h[k], 1u);
