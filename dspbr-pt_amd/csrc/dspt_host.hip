@@ -597,6 +597,9 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
   // the jobs with paths in the pool are those listed (progress is read back
   // late, so a listed job may already be done); one of them: its index
   c->h_sched[s].solo = nj == 1 ? hj[0] : -1;
+  int jbits = 0;  // ray-order key bits for the job index: enough for the jobs in flight, at most 5
+  while (jbits < 5 && (1 << jbits) < nj) jbits++;
+  c->h_sched[s].jbits = jbits;
   if (nt > 0) HIPCHK(hipMemcpyAsync(c->d_tasks, ht, (size_t)nt * sizeof(PassTask), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_sched, c->h_sched + s, sizeof(StreamSched), hipMemcpyHostToDevice, c->stream));
   if (nj > 0) HIPCHK(hipMemcpyAsync(c->d_joblist, hj, (size_t)nj * 4, hipMemcpyHostToDevice, c->stream));
@@ -629,7 +632,8 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
       });
       if (order)
         timed(c, DSPT_K_ORDER, [&] {
-          k_ray_keys<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->d_jobs, cur, ctl, c->order_keys, c->order_bins);
+          k_ray_keys<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->d_jobs, cur, ctl, c->order_keys, c->order_bins,
+                                                                         c->d_sched);
           k_bin_scan<<<1, 1024, 0, c->stream>>>(c->order_bins);
           k_bin_scatter<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->order_keys, &ctl->q_base, c->order_bins, c->order_perm,
                                                                             nullptr, nullptr);

@@ -262,18 +262,23 @@ DI uint32_t ray_key(const SceneView& S, float4 o, float4 d) {
 static_assert(kOrderBins == 8192, "the key has 13 bits");
 
 // Keys of [0, q_base) and the bucket sizes (block-local counts in LDS, one
-// global add per non-empty bucket and block).
+// global add per non-empty bucket and block). With several jobs in the pool
+// the job index takes the key's top sch->jbits bits (the cell's low bits
+// give way), so a bucket holds one scene's rays.
 __global__ void k_ray_keys(const DevJob* __restrict__ jobs, PathState ps, const StreamCtl* __restrict__ ctl,
-                           uint16_t* __restrict__ keys, uint32_t* __restrict__ bins) {
+                           uint16_t* __restrict__ keys, uint32_t* __restrict__ bins, const StreamSched* __restrict__ sch) {
   __shared__ uint32_t h[kOrderBins];
   const int n = ctl->q_base;
+  const int jb = sch->jbits;
   if ((int64_t)blockIdx.x * blockDim.x >= n) return;  // small queues leave most blocks idle
   for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x) h[i] = 0u;
   __syncthreads();
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
     const int p = (int)q;
     const float4 o = ps.org[p];
-    const uint32_t k = ray_key(jobs[__float_as_int(o.w)].S, o, ps.dir[p]);
+    const int jid = __float_as_int(o.w);
+    uint32_t k = ray_key(jobs[jid].S, o, ps.dir[p]);
+    if (jb > 0) k = (((uint32_t)jid & ((1u << jb) - 1u)) << (13 - jb)) | (k >> jb);
     keys[p] = (uint16_t)k;
     atomicAdd(&h[k], 1u);
   }

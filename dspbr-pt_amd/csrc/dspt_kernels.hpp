@@ -55,6 +55,8 @@ struct StreamSched {
   int64_t limit;   // items [next_item, limit) may be issued
   int32_t ntasks;  // entries in the task table
   int32_t solo;    // the one job with paths in flight, or -1
+  int32_t jbits;   // top bits of the ray-order key given to the job index (0: one job)
+  int32_t pad;
 };
 
 // Structure-of-arrays path state, capacity = pool size, indexed by
@@ -123,7 +125,8 @@ constexpr int kOrderBins = 8192;
 #define DSPT_ORDER_BLOCKS 512
 #endif
 static_assert(DSPT_ORDER_THREADS % 64 == 0 && DSPT_ORDER_THREADS <= 1024, "order passes: whole waves, at most 1024 threads");
-__global__ void k_ray_keys(const DevJob* jobs, PathState ps, const StreamCtl* ctl, uint16_t* keys, uint32_t* bins);
+__global__ void k_ray_keys(const DevJob* jobs, PathState ps, const StreamCtl* ctl, uint16_t* keys, uint32_t* bins,
+                           const StreamSched* sch);
 __global__ void k_bin_scan(uint32_t* bins);
 __global__ void k_bin_scatter(const uint16_t* keys, const int32_t* n_ptr, uint32_t* bins, int32_t* perm, const int32_t* src,
                               const int32_t* ns_ptr);
