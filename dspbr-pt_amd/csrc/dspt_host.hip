@@ -395,6 +395,7 @@ int sync_job_table(dspt_ctx* c) {
       d.W = j->W; d.H = j->H; d.npix = (int32_t)j->npix();
       d.rx0 = j->rw > 0 ? j->rx0 : 0; d.ry0 = j->rw > 0 ? j->ry0 : 0;
       d.rw = j->rw > 0 ? j->rw : j->W; d.rn = (int32_t)j->items();
+      d.scene = j->scene;
       d.accum = j->accum;
       d.ring = j->ring; d.done = j->done; d.ring_frames = j->ring_frames; d.frame_end = j->frame_end;
       d.rec_L = j->record ? j->rec_L : nullptr;
@@ -597,8 +598,16 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
   // the jobs with paths in the pool are those listed (progress is read back
   // late, so a listed job may already be done); one of them: its index
   c->h_sched[s].solo = nj == 1 ? hj[0] : -1;
-  int jbits = 0;  // ray-order key bits for the job index: enough for the jobs in flight, at most 5
-  while (jbits < 5 && (1 << jbits) < nj) jbits++;
+  int nscenes = 0;  // distinct scenes among them (counted up to 32)
+  int32_t sc[32];
+  for (int a = 0; a < nj && nscenes < 32; a++) {
+    const int32_t id = c->jobs[hj[a]]->scene;
+    bool seen = false;
+    for (int b = 0; b < nscenes && !seen; b++) seen = sc[b] == id;
+    if (!seen) sc[nscenes++] = id;
+  }
+  int jbits = 0;  // ray-order key bits for the scene id: enough for those scenes, at most 5
+  while (jbits < 5 && (1 << jbits) < nscenes) jbits++;
   c->h_sched[s].jbits = jbits;
   if (nt > 0) HIPCHK(hipMemcpyAsync(c->d_tasks, ht, (size_t)nt * sizeof(PassTask), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_sched, c->h_sched + s, sizeof(StreamSched), hipMemcpyHostToDevice, c->stream));

@@ -262,8 +262,8 @@ DI uint32_t ray_key(const SceneView& S, float4 o, float4 d) {
 static_assert(kOrderBins == 8192, "the key has 13 bits");
 
 // Keys of [0, q_base) and the bucket sizes (block-local counts in LDS, one
-// global add per non-empty bucket and block). With several jobs in the pool
-// the job index takes the key's top sch->jbits bits (the cell's low bits
+// global add per non-empty bucket and block). With several scenes in the
+// pool the scene id takes the key's top sch->jbits bits (the cell's low bits
 // give way), so a bucket holds one scene's rays.
 __global__ void k_ray_keys(const DevJob* __restrict__ jobs, PathState ps, const StreamCtl* __restrict__ ctl,
                            uint16_t* __restrict__ keys, uint32_t* __restrict__ bins, const StreamSched* __restrict__ sch) {
@@ -276,9 +276,9 @@ __global__ void k_ray_keys(const DevJob* __restrict__ jobs, PathState ps, const 
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
     const int p = (int)q;
     const float4 o = ps.org[p];
-    const int jid = __float_as_int(o.w);
-    uint32_t k = ray_key(jobs[jid].S, o, ps.dir[p]);
-    if (jb > 0) k = (((uint32_t)jid & ((1u << jb) - 1u)) << (13 - jb)) | (k >> jb);
+    const DevJob* J = jobs + __float_as_int(o.w);
+    uint32_t k = ray_key(J->S, o, ps.dir[p]);
+    if (jb > 0) k = (((uint32_t)J->scene & ((1u << jb) - 1u)) << (13 - jb)) | (k >> jb);
     keys[p] = (uint16_t)k;
     atomicAdd(&h[k], 1u);
   }
@@ -356,9 +356,9 @@ __global__ void k_hit_keys(const DevJob* __restrict__ jobs, PathState ps, const 
     uint32_t k = 0u;
     if (tri >= 0) {
       if (solo_mat) k = tri < solo_t ? 1u + solo_mat[tri] % (uint32_t)(kOrderBins - 1) : 0u;
-      else {  // several jobs: their material indices are offset by job, so jobs do not share buckets
-        const int jid = __float_as_int(ps.org[p].w);
-        k = 1u + ((uint32_t)jid * 509u + jobs[jid].S.tri_mat[tri]) % (uint32_t)(kOrderBins - 1);
+      else {  // several jobs: material indices offset by scene, so scenes do not share buckets
+        const DevJob* J = jobs + __float_as_int(ps.org[p].w);
+        k = 1u + ((uint32_t)J->scene * 509u + J->S.tri_mat[tri]) % (uint32_t)(kOrderBins - 1);
       }
     }
     keys[p] = (uint16_t)k;

@@ -16,6 +16,7 @@ struct DevJob {
   // region's pixels are the work items of a frame, its count completes the
   // frame and the fold covers them only.
   int32_t rx0, ry0, rw, rn;
+  int32_t scene, pad;             // scene id (jobs of one scene share a tree and materials)
   float4* accum;
   float4* ring;                   // [frame % ring_frames][pixel] finished samples
   int32_t* done;                  // [frame % ring_frames] finished paths of that frame
@@ -55,7 +56,7 @@ struct StreamSched {
   int64_t limit;   // items [next_item, limit) may be issued
   int32_t ntasks;  // entries in the task table
   int32_t solo;    // the one job with paths in flight, or -1
-  int32_t jbits;   // top bits of the ray-order key given to the job index (0: one job)
+  int32_t jbits;   // top bits of the ray-order key given to the scene id (0: one scene)
   int32_t pad;
 };
 
