@@ -244,6 +244,10 @@ struct dspt_ctx {
   bool have_prev = false;
   int qsel = 0;
   int64_t sched_limit = 0, next_known = 0, limit_known = 0, alive_known = 0;
+  // items below this were offered to an enqueued chunk: the tasks holding
+  // them may be partly issued, so they are never extended (the item order
+  // within a task depends on its frame count)
+  int64_t exposed_limit = 0;
   std::deque<PassTask> tasks;
   int mode = DSPT_SCHED_STREAM;
   int policy = DSPT_POLICY_RR;
@@ -454,6 +458,7 @@ int ensure_pool(dspt_ctx* c) {
   c->cap = cap;
   c->qsel = 0;
   c->sched_limit = c->next_known = c->limit_known = c->alive_known = 0;
+  c->exposed_limit = 0;
   c->limit_at[0] = c->limit_at[1] = 0;
   c->tasks.clear();
   c->have_prev = false;
@@ -470,7 +475,8 @@ bool append_frames(dspt_ctx* c, int id, int nf) {
   const int64_t npix = j->items();
   if (!c->tasks.empty()) {
     PassTask& b = c->tasks.back();
-    if (b.job == id && b.first_frame + b.nframes == j->next_sched && b.offset + (int64_t)b.nframes * b.npix == c->sched_limit) {
+    if (b.job == id && b.first_frame + b.nframes == j->next_sched && b.offset + (int64_t)b.nframes * b.npix == c->sched_limit &&
+        b.offset >= c->exposed_limit) {
       b.nframes += nf;
       c->sched_limit += (int64_t)nf * npix;
       j->next_sched += nf;
@@ -582,6 +588,7 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
   PassTask* ht = c->h_tasks + (size_t)s * kMaxTasks;
   for (int i = 0; i < nt; i++) ht[i] = c->tasks[i];
   c->h_sched[s].limit = c->sched_limit;
+  c->exposed_limit = c->sched_limit;
   c->qsel_at[s] = (c->qsel + c->chunk_iters) & 1;
   c->h_sched[s].ntasks = nt;
   c->limit_at[s] = c->sched_limit;
