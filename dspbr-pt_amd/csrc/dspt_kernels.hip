@@ -19,6 +19,7 @@
 #include "dspt_kernels.hpp"
 #include "dev_shade.hpp"
 #include "dev_post.hpp"
+#include "dev_order.hpp"
 
 // Occupancy target of the shading kernel (waves per SIMD; 0 = compiler's
 // choice). Left alone the compiler gives the full-profile shading kernel
@@ -369,40 +370,6 @@ __global__ void k_hit_keys(const DevJob* __restrict__ jobs, PathState ps, const 
     if (h[i]) atomicAdd(&bins[i], h[i]);
 }
 
-// Order of a task's items over (frame, region pixel): the task's first
-// frame on its own, pixel by pixel (so a job's first frame completes as
-// soon as it can), then the other frames in groups of kFrameGroup, each
-// group walked chunk by chunk of 64 pixels (one traversal wave, an 8 x 8
-// tile of the tiled pixel order) and, within a chunk, frame by frame: the
-// waves that follow each other trace the same pixels with other jitter and
-// find the same nodes in the caches. Every (frame, pixel) occurs once.
-constexpr int kFrameGroup = 16;
-DI void item_frame_pixel(int64_t local, int npix, int nframes, int& fslot, int& rpix) {
-  if (local < npix || nframes <= 1) {
-    fslot = (int)(local / npix);
-    rpix = (int)(local - (int64_t)fslot * npix);
-    return;
-  }
-  local -= npix;
-  const int64_t gsz = (int64_t)kFrameGroup * npix;
-  const int grp = (int)(local / gsz);
-  const int gframes = min(kFrameGroup, nframes - 1 - grp * kFrameGroup);
-  const int64_t l = local - (int64_t)grp * gsz;
-  const int C = npix / 64, R = npix - C * 64;
-  int f;
-  if (l < (int64_t)C * 64 * gframes) {
-    const int c = (int)(l / (64 * gframes));
-    const int rem = (int)(l - (int64_t)c * 64 * gframes);
-    f = rem / 64;
-    rpix = c * 64 + rem % 64;
-  } else {  // the last, partial chunk (R > 0 here)
-    const int l2 = (int)(l - (int64_t)C * 64 * gframes);
-    f = l2 / R;
-    rpix = C * 64 + l2 % R;
-  }
-  fslot = 1 + grp * kFrameGroup + f;
-}
-
 // Camera rays for the items of this refill, renderer.frag:129-142 +
 // rng_init (rng.glsl:23-29). Item -> task by binary search over the task
 // table (offsets ascending).
@@ -425,20 +392,10 @@ __global__ void k_regen(PathState ps, const DevJob* __restrict__ jobs, const Pas
     int fslot, rpix;  // frame of the task, pixel of the render region
     item_frame_pixel(item - T.offset, T.npix, T.nframes, fslot, rpix);
     int W = J->W, H = J->H;
-    // region pixels in 8 x 8 tiles (bands of 8 rows, tiles left to right;
-    // the last band and tile of a row are cut to the region), so the 64
-    // camera rays of a traversal wave start as a square of pixels; each
-    // sample is still seeded by its own pixel, so only the order changes
     int px, py;
-    {
-      const int rw = J->rw, rh = J->rn / J->rw;
-      const int band = rpix / (8 * rw), idx = rpix - band * 8 * rw;
-      const int bh = min(8, rh - band * 8);
-      const int t = idx / (8 * bh), j = idx - t * 8 * bh;
-      const int tw = min(8, rw - t * 8);
-      px = J->rx0 + t * 8 + j % tw;
-      py = J->ry0 + band * 8 + j / tw;
-    }
+    region_tile_pixel(rpix, J->rw, J->rn / J->rw, px, py);
+    px += J->rx0;
+    py += J->ry0;
     int pix = py * W + px;
     int frame = T.first_frame + fslot;
     const FrameParams& F = J->F;

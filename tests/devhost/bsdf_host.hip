@@ -53,3 +53,18 @@ extern "C" int devhost_bsdf_sample(const float* mat, int profile, const float* g
   }
   return 0;
 }
+
+// The refill's item order (dev_order.hpp) over a whole task: item i of a
+// task of nframes frames over a rw x rh region -> frame slot and pixel
+// offset (dx, dy).
+#include "../../dspbr-pt_amd/csrc/dev_order.hpp"
+extern "C" int devhost_task_items(int rw, int rh, int nframes, int* fslot, int* dx, int* dy) {
+  const int npix = rw * rh;
+  for (int64_t i = 0; i < (int64_t)npix * nframes; i++) {
+    int f, r, x, y;
+    item_frame_pixel(i, npix, nframes, f, r);
+    region_tile_pixel(r, rw, rh, x, y);
+    fslot[i] = f; dx[i] = x; dy[i] = y;
+  }
+  return 0;
+}
