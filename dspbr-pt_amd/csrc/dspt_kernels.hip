@@ -574,13 +574,13 @@ __global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, Pat
   st.spill = spill + gtid * (kStackSize - kLdsStack);
   unsigned nv = 0, nt = 0, nr = 0;
   bool active = false, exhausted = false;
-  int p = 0, flags = 0, phase = 0;
+  int p = 0, flags = 0, phase = 0;  // p: the lane's shadow queue slot
   Walker<INST> w;
   f3 org = sp3(0.0f), contrib = sp3(0.0f);
   for (;;) {
     const int q = refill<DSPT_REFILL_SH>(active, count, next_ptr, exhausted);
     if (q >= 0) {
-      p = queue[q];
+      p = q;
       const float4 so = ps.sh_org[p];
       flags = __float_as_int(so.w) & 3;
       org = xyz(so);
@@ -618,10 +618,11 @@ __global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, Pat
         const float4 pd = ps.sh_pdir[p];
         w.restart(st, org, xyz(pd), pd.w);
       } else {
-        const float4 L = ps.L[p];
+        const int path = queue[p];
+        const float4 L = ps.L[path];
         const f3 wt = xyz(ps.sh_pw[p]);
         const f3 nl = xyz(L) + contrib * wt;
-        ps.L[p] = make_float4(nl.x, nl.y, nl.z, L.w);
+        ps.L[path] = make_float4(nl.x, nl.y, nl.z, L.w);
         active = false;
       }
     }
@@ -865,12 +866,14 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
       n_ext++;
       if (flags) {
         const int k = atomicAdd(count_sh, 1);
+        // the shadow records at the dense queue slot k (full-line stores,
+        // read in order by the shadow trace); shq[k] names the path
         shq[k] = o;
-        ps.sh_org[o] = make_float4(pos.x, pos.y, pos.z, __int_as_float(flags | (jid << 2)));
-        ps.sh_dir[o] = make_float4(ldir.x, ldir.y, ldir.z, kTFAR_MAX);
-        ps.sh_c[o] = make_float4(c_ibl.x, c_ibl.y, c_ibl.z, 0.0f);
-        ps.sh_pw[o] = make_float4(pw_nee.x, pw_nee.y, pw_nee.z, 0.0f);
-        if (flags & 2) { ps.sh_pdir[o] = pdir; ps.sh_pc[o] = make_float4(c_pt.x, c_pt.y, c_pt.z, 0.0f); }
+        ps.sh_org[k] = make_float4(pos.x, pos.y, pos.z, __int_as_float(flags | (jid << 2)));
+        ps.sh_dir[k] = make_float4(ldir.x, ldir.y, ldir.z, kTFAR_MAX);
+        ps.sh_c[k] = make_float4(c_ibl.x, c_ibl.y, c_ibl.z, 0.0f);
+        ps.sh_pw[k] = make_float4(pw_nee.x, pw_nee.y, pw_nee.z, 0.0f);
+        if (flags & 2) { ps.sh_pdir[k] = pdir; ps.sh_pc[k] = make_float4(c_pt.x, c_pt.y, c_pt.z, 0.0f); }
       }
       st_stream(nx.org + o, make_float4(pos.x, pos.y, pos.z, __int_as_float(jid)));
       st_stream(nx.dir + o, make_float4(wo.x, wo.y, wo.z, __int_as_float(bounce + 1)));

@@ -77,6 +77,7 @@ struct PathState {
   uint4* meta;    // generator state (x, y), frame, pixel: one 16-byte record
   uint32_t* sig;  // path signature (recording jobs only)
   int32_t* prim;  // primary hit triangle (recording jobs only)
+  // shadow records, indexed by shadow queue slot (the queue holds the path)
   float4* sh_org;   // shadow origin xyz; w (int bits) = flags (1 = IBL ray, 2 = point light ray) | job << 2
   float4* sh_dir;   // IBL shadow direction, tmax
   float4* sh_c;     // IBL NEE radiance before path weight
