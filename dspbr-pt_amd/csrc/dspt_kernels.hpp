@@ -63,10 +63,10 @@ struct StreamSched {
 // Structure-of-arrays path state, capacity = pool size, indexed by
 // position. Every per-path field the shading kernel reads is a 16-byte
 // record (the job and bounce ride in the ray records' w), so a path read
-// through the shading order touches six cache lines. Two instances alternate per bounce (the per-path fields are
-// double-buffered); hit and the shadow-ray fields are shared: hit is written
-// and read at the current position, the shadow fields at the position the
-// path continues at.
+// through the shading order touches six cache lines. Two instances alternate
+// per bounce (the per-path fields are double-buffered); hit and the
+// shadow-ray fields are shared: hit is written and read at the current
+// position, the shadow fields at the path's slot in the shadow-ray queue.
 struct PathState {
   float4* org;    // extension ray origin xyz, job index of the path (int bits) in w
   float4* dir;    // extension ray direction xyz, bounce index (int bits, 0 = camera ray) in w
