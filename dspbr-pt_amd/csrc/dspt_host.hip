@@ -797,6 +797,9 @@ int drain_job(dspt_ctx* c, int id) {
   return 0;
 }
 
+#ifndef DSPT_PACK_INNER_NODES
+#define DSPT_PACK_INNER_NODES 1
+#endif
 int validate_nodes(const float* nodes, int64_t n) {
   for (int64_t i = 0; i < n; i++) {
     const float* d = nodes + i * 16;
@@ -941,6 +944,40 @@ int make_scene_gpu_bvh(dspt_ctx* c, const float* vertices, int64_t V, const uint
 
 // encode_limit: nodes below it get child codes (k_encode_children); -1 = all
 // N (a two-level scene passes the first instance-record slot).
+// The engine's copy of a one-level node array with the inner nodes first,
+// in their given order, and the leaf nodes behind them. The traversal never
+// fetches a well-formed leaf (its parent holds its box and the child field
+// becomes a leaf code), so in a depth-first array about half of every
+// 128-byte line of fetched nodes was a leaf nobody reads; packed, a line
+// holds two inner nodes, and a node is followed by its first inner child.
+// A relabelling of node ids: child fields are rewritten as the traversal
+// reads them ((int) of the float; out-of-range values unchanged), the root
+// stays at 0, leaves keep their fields, so visit order, steps and counters
+// are those of the given array, malformed trees included. Skipped for a
+// leaf root and for arrays whose ids a float cannot hold exactly.
+static bool pack_inner_nodes(const float* nodes, int64_t N, std::vector<float>& out) {
+  if (N > (1 << 24)) return false;
+  auto is_leaf = [&](int64_t n) { return (int)nodes[n * 16 + 11] > 0; };  // (int) count > 0, as the kernels test it
+  if (is_leaf(0)) return false;
+  std::vector<int32_t> id((size_t)N);
+  int32_t k = 0;
+  for (int64_t n = 0; n < N; n++)
+    if (!is_leaf(n)) id[n] = k++;
+  for (int64_t n = 0; n < N; n++)
+    if (is_leaf(n)) id[n] = k++;
+  out.resize((size_t)N * 16);
+  for (int64_t n = 0; n < N; n++) {
+    float* o = &out[(size_t)id[n] * 16];
+    std::memcpy(o, nodes + n * 16, 64);
+    if (is_leaf(n)) continue;
+    for (int f : {3, 7}) {
+      const int child = (int)o[f];
+      if (child >= 0 && child < N) o[f] = (float)id[child];
+    }
+  }
+  return true;
+}
+
 int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tri_idx, int64_t T, const float* nodes, int64_t N,
                const int32_t* bvh_idx, int64_t M, const float* mats, int32_t K, SceneRes** out, bool own_nodes = false,
                int64_t encode_limit = -1) {
@@ -973,7 +1010,16 @@ int make_scene(dspt_ctx* c, const float* vertices, int64_t V, const uint32_t* tr
     d_bvh = nullptr;
     return bail(fail(DSPT_ERR_MEMORY, "bvh index staging"));
   }
-  if ((r = chk(hipMemcpyAsync(s->nodes, nodes, (size_t)N * 64, hipMemcpyHostToDevice, c->stream), "node upload")) ||
+  {
+    // the packed copy is uploaded and waited for here, so no copy reads it after it is freed
+    std::vector<float> packed;
+    const bool pack = encode_limit < 0 && DSPT_PACK_INNER_NODES && pack_inner_nodes(nodes, N, packed);
+    if ((r = chk(hipMemcpyAsync(s->nodes, pack ? packed.data() : nodes, (size_t)N * 64, hipMemcpyHostToDevice, c->stream),
+                 "node upload")))
+      return r;
+    if (pack && (r = chk(hipStreamSynchronize(c->stream), "node upload"))) return r;
+  }
+  if (
       (r = chk(hipMemcpyAsync(s->verts, vertices, (size_t)V * 80, hipMemcpyHostToDevice, c->stream), "vertex upload")) ||
       (r = chk(hipMemcpyAsync(s->tris, tri_idx, (size_t)T * 12, hipMemcpyHostToDevice, c->stream), "index upload")) ||
       (r = chk(hipMemcpyAsync(s->mats, mats, (size_t)K * 272, hipMemcpyHostToDevice, c->stream), "material upload")) ||
