@@ -161,6 +161,12 @@ def test_malformed_trees_match_oracle():
         "lone_inner": ([inner(5, 9)], np.arange(T, dtype=np.int32)),
         # a leaf child reached both as a leaf code (well-formed) and a self-cycle deeper down
         "mixed": ([inner(1, 2), inner(3, 1), leaf(1, 2), leaf(0, 1)], np.arange(T, dtype=np.int32)),
+        # inner nodes after leaves, so the engine's copy (inner nodes packed
+        # first) relabels every node: a leaf read by index (range starts
+        # before the array), an unreferenced leaf, child fields 2.7 and 4.9
+        # (read as 2 and 4) and -0.5 (read as 0: a cycle back to the root)
+        "permuted": ([inner(1, 2.7), leaf(0, 2), inner(3, 5), leaf(-3, 5), leaf(2, 3), inner(4.9, -0.5)],
+                     np.arange(T, dtype=np.int32)),
     }
     for name, (nl, bidx) in cases.items():
         nodes = np.concatenate(nl).astype(np.float32)
