@@ -118,22 +118,34 @@ DI f3 xf_tvec(const float4* m, f3 v) {
             m[0].z * v.x + m[1].z * v.y + m[2].z * v.z};
 }
 
+#ifndef DSPT_SPILL_WAIT
+#define DSPT_SPILL_WAIT 1
+#endif
 // Per-lane traversal stack: the first kLdsStack entries live in LDS
 // (lane-interleaved, conflict-free), the rest in a global spill area that
 // is private to the lane.
 struct LaneStack {
   int* lds;        // &smem[threadIdx.x], stride kTraceBlock
   int* spill;      // per-lane global area of kStackSize - kLdsStack ints
+  bool wait_spill = true;  // a constant per kernel (see get)
   DI void put(int i, int v) {
     if (i < kLdsStack) lds[i * kTraceBlock] = v;
     else spill[i - kLdsStack] = v;
   }
   // The spill read is a non-temporal global load so the compiler cannot
   // merge the two reads into one generic (FLAT) load of a selected pointer.
+  // It is waited for inside its own branch: otherwise the popped value may
+  // come from vector memory, and the common LDS pop would wait for every
+  // outstanding vector memory operation (vmcnt counts stores too, e.g. the
+  // hit records of rays that just finished) before the next node fetch.
+  // The two-level shadow trace, whose walks spill more often, measured
+  // slower with it and clears wait_spill.
   DI int get(int i) const {
     if (i < kLdsStack) return lds[i * kTraceBlock];
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_nontemporal_load((const __attribute__((address_space(1))) int*)(spill + (i - kLdsStack)));
+    const int v = __builtin_nontemporal_load((const __attribute__((address_space(1))) int*)(spill + (i - kLdsStack)));
+    if (DSPT_SPILL_WAIT && wait_spill) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt and lgkmcnt not waited
+    return v;
 #else
     return spill[i - kLdsStack];
 #endif

@@ -569,6 +569,7 @@ __global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, Pat
   __shared__ int smem[kLdsStack * kTraceBlock];
   const int count = *count_ptr;
   LaneStack st;
+  st.wait_spill = !INST;
   st.lds = smem + threadIdx.x;
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   st.spill = spill + gtid * (kStackSize - kLdsStack);
