@@ -459,6 +459,19 @@ DD int refill(bool active, int count, int32_t* next_ptr, bool& exhausted) {
   return q < count ? q : -1;
 }
 
+// The loads of a ray start (its record and its scene's node and triangle
+// arrays and sizes) complete at the start: otherwise the compiler treats
+// them as possibly in flight at every traversal step and waits on vmcnt(0)
+// there, which also waits for the wave's outstanding stores before the
+// next node fetch issues.
+#ifndef DSPT_START_WAIT_ON
+#define DSPT_START_WAIT_ON 1
+#endif
+#define DSPT_START_WAIT()                                      \
+  do {                                                         \
+    if (DSPT_START_WAIT_ON) __builtin_amdgcn_s_waitcnt(0x0F70); \
+  } while (0)
+
 // Per-lane ray walker: one-level scenes, or (INST) scenes whose jobs may be
 // two-level (a per-lane choice by the scene's top-level BVH).
 template <bool INST> struct Walker;
@@ -530,6 +543,7 @@ __global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathSt
       const float4 o = ps.org[p], d = ps.dir[p];
       w.start(jobs[__float_as_int(o.w)].S, st, xyz(o), xyz(d), kTFAR_MAX);
       active = true;
+      DSPT_START_WAIT();
     }
     if (__ballot(active) == 0) {
       if (exhausted) break;
@@ -590,6 +604,7 @@ __global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, Pat
       const float4 sd = phase == 0 ? ps.sh_dir[p] : ps.sh_pdir[p];
       w.start(jobs[__float_as_int(so.w) >> 2].S, st, org, xyz(sd), sd.w);
       active = true;
+      if (!INST) DSPT_START_WAIT();
     }
     if (__ballot(active) == 0) {
       if (exhausted) break;
