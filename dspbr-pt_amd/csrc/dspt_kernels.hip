@@ -610,7 +610,8 @@ __global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, Pat
       if (exhausted) break;
       continue;
     }
-    // the exact slab test only: the fast one measured no gain here and cost a spill
+    // the exact slab test only: the fast one measured no gain here (with and
+    // without a register spill)
     bool leaf = false, fin = false;
     for (int k = 0; k < DSPT_INNER_RUN_SH; k++) {
       const bool inner = active && !leaf && !fin;
