@@ -561,7 +561,7 @@ void consume_readback(dspt_ctx* c, int s, int64_t chunk) {
   if (c->cap <= 0) return;  // the pool was released after this chunk
   const Readback& rb = c->h_rb[s];
   c->next_known = rb.ctl.next_item;
-  c->alive_known = rb.ctl.qcount[c->qsel_at[s]];
+  c->alive_known = rb.ctl.q[c->qsel_at[s]].paths;
   c->limit_known = c->limit_at[s];
   const int n = std::min<int>((int)c->jobs.size(), kMaxJobs);
   for (int i = 0; i < n; i++) {
@@ -656,36 +656,36 @@ int run_chunk(dspt_ctx* c, const std::vector<int>& ids, bool schedule_new) {
         });
       timed(c, DSPT_K_TRACE_EXT, [&] {
         if (inst)
-          k_trace_ext<true><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], &ctl->ext_next,
+          k_trace_ext<true><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, cur, &ctl->q[a].paths, &ctl->ext_next,
                                                                         c->spill, c->cnt, perm, &ctl->q_base);
         else
-          k_trace_ext<false><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], &ctl->ext_next,
+          k_trace_ext<false><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, cur, &ctl->q[a].paths, &ctl->ext_next,
                                                                          c->spill, c->cnt, perm, &ctl->q_base);
       });
       if (shade_order)
         timed(c, DSPT_K_ORDER, [&] {
-          k_hit_keys<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->d_jobs, cur, &ctl->qcount[a], c->order_keys,
+          k_hit_keys<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->d_jobs, cur, &ctl->q[a].paths, c->order_keys,
                                                                          c->shade_bins, c->d_sched);
           k_bin_scan<<<1, 1024, 0, c->stream>>>(c->shade_bins);
-          k_bin_scatter<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->order_keys, &ctl->qcount[a], c->shade_bins,
+          k_bin_scatter<<<order_blocks, DSPT_ORDER_THREADS, 0, c->stream>>>(c->order_keys, &ctl->q[a].paths, c->shade_bins,
                                                              c->shade_perm, perm, &ctl->q_base);  // buckets in ray order
         });
       timed(c, DSPT_K_SHADE, [&] {
         const int32_t* sperm = shade_order ? c->shade_perm : perm;
-        const int32_t* nsorted = shade_order ? &ctl->qcount[a] : &ctl->q_base;
+        const int32_t* nsorted = shade_order ? &ctl->q[a].paths : &ctl->q_base;
         if (dbg)
-          k_shade<true><<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1],
-                                                              c->shq, &ctl->sh_count, c->cnt, sperm, nsorted);
+          k_shade<true><<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->q[a].paths, &ctl->q[a ^ 1],
+                                                              c->shq, c->cnt, sperm, nsorted);
         else
-          k_shade<false><<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->qcount[a], &ctl->qcount[a ^ 1],
-                                                               c->shq, &ctl->sh_count, c->cnt, sperm, nsorted);
+          k_shade<false><<<shade_blocks, 256, 0, c->stream>>>(c->d_jobs, cur, nxt, &ctl->q[a].paths, &ctl->q[a ^ 1],
+                                                               c->shq, c->cnt, sperm, nsorted);
       });
       timed(c, DSPT_K_TRACE_SHADOW, [&] {
         if (inst)
-          k_trace_shadow<true><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, nxt, c->shq, &ctl->sh_count,
+          k_trace_shadow<true><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, nxt, c->shq, &ctl->q[a ^ 1].shadow,
                                                                            &ctl->sh_next, c->spill, c->cnt);
         else
-          k_trace_shadow<false><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, nxt, c->shq, &ctl->sh_count,
+          k_trace_shadow<false><<<trace_blocks, kTraceBlock, 0, c->stream>>>(c->d_jobs, nxt, c->shq, &ctl->q[a ^ 1].shadow,
                                                                             &ctl->sh_next, c->spill, c->cnt);
       });
     }

@@ -209,7 +209,7 @@ __global__ void k_plan(StreamCtl* __restrict__ ctl, const StreamSched* __restric
   if (shade_bins)
     for (int i = threadIdx.x; i < kOrderBins; i += blockDim.x) shade_bins[i] = 0u;
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const int Q = ctl->qcount[a];
+  const int Q = ctl->q[a].paths;
   const int F = P - Q;
   int64_t avail = sch->limit - ctl->next_item;
   if (avail < 0) avail = 0;
@@ -218,9 +218,9 @@ __global__ void k_plan(StreamCtl* __restrict__ ctl, const StreamSched* __restric
   ctl->item0 = ctl->next_item;
   ctl->q_base = Q;
   ctl->next_item += n;
-  ctl->qcount[a] = Q + n;
-  ctl->qcount[a ^ 1] = 0;
-  ctl->sh_count = 0;
+  ctl->q[a].paths = Q + n;
+  ctl->q[a ^ 1].paths = 0;
+  ctl->q[a ^ 1].shadow = 0;
   ctl->ext_next = 0;
   ctl->sh_next = 0;
 }
@@ -730,8 +730,8 @@ __device__ __noinline__ float4 debug_sample(const SceneView& S, const FrameParam
 
 template <bool DEBUG>
 __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState ps, PathState nx,
-                                   const int32_t* __restrict__ count_in, int32_t* __restrict__ count_out,
-                                   int32_t* __restrict__ shq, int32_t* __restrict__ count_sh,
+                                   const int32_t* __restrict__ count_in, QueuePair* __restrict__ qout,
+                                   int32_t* __restrict__ shq,
                                    unsigned long long* __restrict__ cnt, const int32_t* __restrict__ perm,
                                    const int32_t* __restrict__ nsorted_ptr) {
   const int count = *count_in;
@@ -878,11 +878,21 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
       }
       const f3 pw_nee = pw;  // path weight at the time of NEE
       pw = pw * bw;
-      // the path continues at position o of the next state buffer
-      const int o = atomicAdd(count_out, 1);
+      // the path continues at position o of the next state buffer and,
+      // with a shadow ray, takes shadow-queue slot k: one 64-bit atomic per
+      // wave claims both runs (lane order within the wave, as before)
+      const unsigned long long act = __ballot(1), shm = __ballot(flags != 0);
+      const int lane = __lane_id(), leader = __ffsll((long long)act) - 1;
+      const unsigned long long below = (1ull << lane) - 1ull;
+      unsigned long long base = 0;
+      if (lane == leader)
+        base = atomicAdd(reinterpret_cast<unsigned long long*>(qout),
+                         (unsigned long long)__popcll(act) | ((unsigned long long)__popcll(shm) << 32));
+      base = __shfl(base, leader);
+      const int o = (int)(uint32_t)base + __popcll(act & below);
       n_ext++;
       if (flags) {
-        const int k = atomicAdd(count_sh, 1);
+        const int k = (int)(base >> 32) + __popcll(shm & below);
         // the shadow records at the dense queue slot k (full-line stores,
         // read in order by the shadow trace); shq[k] names the path
         shq[k] = o;
@@ -921,9 +931,9 @@ __global__ SHADE_BOUNDS void k_shade(const DevJob* __restrict__ jobs, PathState 
   stat_add(cnt, 8, n_nee);
   stat_add(cnt, 2, n_ext);
 }
-template __global__ void k_shade<false>(const DevJob*, PathState, PathState, const int32_t*, int32_t*, int32_t*, int32_t*,
+template __global__ void k_shade<false>(const DevJob*, PathState, PathState, const int32_t*, QueuePair*, int32_t*,
                                         unsigned long long*, const int32_t*, const int32_t*);
-template __global__ void k_shade<true>(const DevJob*, PathState, PathState, const int32_t*, int32_t*, int32_t*, int32_t*,
+template __global__ void k_shade<true>(const DevJob*, PathState, PathState, const int32_t*, QueuePair*, int32_t*,
                                        unsigned long long*, const int32_t*, const int32_t*);
 
 // Running mean (renderer.frag:153-156) over every frame that is complete,

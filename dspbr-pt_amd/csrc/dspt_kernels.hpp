@@ -39,13 +39,20 @@ struct PassTask {
 // occupy positions [0, qcount) of the current state buffer; the paths that
 // continue are compacted into the other buffer, and the next bounce's new
 // paths are appended behind them.
+// Queue counts of one state buffer, side by side so the shading pass claims
+// a path's next-state position and its shadow-queue slot with one 64-bit
+// atomic (paths in the low word, shadow rays in the high word).
+struct alignas(8) QueuePair {
+  int32_t paths;   // live paths in the state buffer
+  int32_t shadow;  // shadow rays queued by the paths written into it
+};
+
 struct StreamCtl {
   int64_t next_item;   // first item not yet issued
   int64_t item0;       // first item of the current refill
   int32_t regen_n;     // items issued by the current refill
   int32_t q_base;      // position where the refill appends
-  int32_t qcount[2];   // live paths in state buffer 0 / 1
-  int32_t sh_count;    // shadow-ray queue
+  QueuePair q[2];      // state buffer 0 / 1
   int32_t ext_next;    // next position a traversal wave takes
   int32_t sh_next;     // next shadow-queue entry a traversal wave takes
   int32_t pad;
@@ -143,9 +150,8 @@ __global__ void k_trace_shadow(const DevJob* jobs, PathState ps, const int32_t* 
 // DEBUG: the instance that also runs the debug program for jobs whose frame
 // has a debug mode; the host launches it only while such a job is live.
 template <bool DEBUG>
-__global__ void k_shade(const DevJob* jobs, PathState cur, PathState nxt, const int32_t* count_in, int32_t* count_out,
-                        int32_t* shq, int32_t* count_sh, unsigned long long* cnt, const int32_t* perm,
-                        const int32_t* nsorted_ptr);
+__global__ void k_shade(const DevJob* jobs, PathState cur, PathState nxt, const int32_t* count_in, QueuePair* qout,
+                        int32_t* shq, unsigned long long* cnt, const int32_t* perm, const int32_t* nsorted_ptr);
 __global__ void k_accum_ready(const DevJob* jobs, const int32_t* job_list, const int32_t* prog);
 __global__ void k_advance(const DevJob* jobs, const int32_t* job_list, int n, int32_t* prog);
 __global__ void k_trace_rays(SceneView S, const float4* rays, int64_t n, int any, int32_t* tri, float* t, float* u, float* v,
