@@ -440,23 +440,47 @@ __global__ void k_regen(PathState ps, const DevJob* __restrict__ jobs, const Pas
 #define DSPT_INNER_RUN_SH 4
 #endif
 
-// Wave-uniform refill: idle lanes get consecutive queue entries. Returns the
-// queue index for this lane (or -1) and sets `exhausted` once the queue is
-// used up.
-template <int MIN_IDLE>
-DD int refill(bool active, int count, int32_t* next_ptr, bool& exhausted) {
+// Wave-uniform refill: idle lanes get consecutive queue entries. With
+// CHUNK > 0 a wave claims at least CHUNK positions per atomic on the shared
+// queue counter and keeps what it did not hand out as a reserve
+// [res_lo, res_hi) for its next refills, so the contended counter sees
+// fewer atomics; every position is still taken once (CHUNK = 0: exactly the
+// idle lanes, no reserve). Returns the queue index for this lane (or -1);
+// `exhausted` is set once the queue is fully claimed, and the wave ends when
+// that holds and its reserve is empty. Measured: chunks of 128 speed up the
+// one-level closest-hit trace (−3 %) but slow the shadow trace and the
+// two-level traces (+3 to +5 %), which keep CHUNK = 0.
+#ifndef DSPT_REFILL_CHUNK
+#define DSPT_REFILL_CHUNK 128
+#endif
+template <int MIN_IDLE, int CHUNK>
+DD int refill(bool active, int count, int32_t* next_ptr, bool& exhausted, int& res_lo, int& res_hi) {
   const unsigned long long idle = __ballot(!active);
   const int n_idle = __popcll(idle);
-  if (exhausted || n_idle < MIN_IDLE) return -1;
+  const int have = res_hi - res_lo;  // wave-uniform
+  if (n_idle < MIN_IDLE || (exhausted && have == 0)) return -1;
   const int lane = __lane_id();
-  const int leader = __ffsll((long long)idle) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(next_ptr, n_idle);
-  base = __shfl(base, leader);
-  if (base + n_idle >= count) exhausted = true;
-  if (active) return -1;
-  const int q = base + __popcll(idle & ((1ull << lane) - 1ull));
-  return q < count ? q : -1;
+  const int rank = __popcll(idle & ((1ull << lane) - 1ull));
+  int q;
+  if (have >= n_idle || exhausted) {  // from the reserve alone
+    const int take = min(have, n_idle);
+    q = rank < take ? res_lo + rank : -1;
+    res_lo += take;
+  } else {  // the reserve first, then a new claim
+    const int leader = __ffsll((long long)idle) - 1;
+    const int want = max(CHUNK, n_idle - have);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(next_ptr, want);
+    base = __builtin_amdgcn_readfirstlane(__shfl(base, leader));
+    if ((int64_t)base + want >= count) exhausted = true;
+    const int got = (int)max((int64_t)0, min((int64_t)base + want, (int64_t)count) - base);
+    const int r2 = rank - have;
+    q = rank < have ? res_lo + rank : (r2 < got ? base + r2 : -1);
+    const int used = min(got, n_idle - have);
+    res_lo = base + used;
+    res_hi = base + got;
+  }
+  return active ? -1 : q;
 }
 
 // The loads of a ray start (its record and its scene's node and triangle
@@ -528,6 +552,7 @@ __global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathSt
   st.spill = spill + gtid * (kStackSize - kLdsStack);
   unsigned nv = 0, nt = 0, nr = 0;
   bool active = false, exhausted = false;
+  int res_lo = 0, res_hi = 0;  // the wave's reserve of claimed queue positions
   int p = 0;
   Walker<INST> w;
   auto finish_ray = [&]() {
@@ -537,7 +562,7 @@ __global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathSt
     active = false;
   };
   for (;;) {
-    const int q = refill<DSPT_REFILL_EXT>(active, count, next_ptr, exhausted);
+    const int q = refill<DSPT_REFILL_EXT, INST ? 0 : DSPT_REFILL_CHUNK>(active, count, next_ptr, exhausted, res_lo, res_hi);
     if (q >= 0) {
       p = q < nsorted ? perm[q] : q;
       const float4 o = ps.org[p], d = ps.dir[p];
@@ -589,11 +614,12 @@ __global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, Pat
   st.spill = spill + gtid * (kStackSize - kLdsStack);
   unsigned nv = 0, nt = 0, nr = 0;
   bool active = false, exhausted = false;
+  int res_lo = 0, res_hi = 0;  // the wave's reserve of claimed queue positions
   int p = 0, flags = 0, phase = 0;  // p: the lane's shadow queue slot
   Walker<INST> w;
   f3 org = sp3(0.0f), contrib = sp3(0.0f);
   for (;;) {
-    const int q = refill<DSPT_REFILL_SH>(active, count, next_ptr, exhausted);
+    const int q = refill<DSPT_REFILL_SH, 0>(active, count, next_ptr, exhausted, res_lo, res_hi);
     if (q >= 0) {
       p = q;
       const float4 so = ps.sh_org[p];
