@@ -449,12 +449,18 @@ __global__ void k_regen(PathState ps, const DevJob* __restrict__ jobs, const Pas
 // `exhausted` is set once the queue is fully claimed, and the wave ends when
 // that holds and its reserve is empty. Measured: chunks of 128 speed up the
 // one-level closest-hit trace (−3 %) but slow the shadow trace and the
-// two-level traces (+3 to +5 %), which keep CHUNK = 0.
+// two-level traces (+3 to +5 %), which keep CHUNK = 0; and a small queue
+// (a one-frame pass of a small job) would sit in the reserves of a few waves
+// while the rest idle, so chunks apply only from four per wave of the grid.
 #ifndef DSPT_REFILL_CHUNK
 #define DSPT_REFILL_CHUNK 128
 #endif
-template <int MIN_IDLE, int CHUNK>
-DD int refill(bool active, int count, int32_t* next_ptr, bool& exhausted, int& res_lo, int& res_hi) {
+DD int refill_chunk(int count, int chunk) {
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+  return (int64_t)count >= waves * chunk * 4 ? chunk : 0;
+}
+template <int MIN_IDLE>
+DD int refill(bool active, int count, int32_t* next_ptr, bool& exhausted, int& res_lo, int& res_hi, int chunk) {
   const unsigned long long idle = __ballot(!active);
   const int n_idle = __popcll(idle);
   const int have = res_hi - res_lo;  // wave-uniform
@@ -468,7 +474,7 @@ DD int refill(bool active, int count, int32_t* next_ptr, bool& exhausted, int& r
     res_lo += take;
   } else {  // the reserve first, then a new claim
     const int leader = __ffsll((long long)idle) - 1;
-    const int want = max(CHUNK, n_idle - have);
+    const int want = max(chunk, n_idle - have);
     int base = 0;
     if (lane == leader) base = atomicAdd(next_ptr, want);
     base = __builtin_amdgcn_readfirstlane(__shfl(base, leader));
@@ -546,6 +552,7 @@ __global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathSt
   __shared__ int smem[kLdsStack * kTraceBlock];
   const int count = *count_ptr;
   const int nsorted = perm ? *nsorted_ptr : 0;  // queue entries [0, nsorted) are read through perm
+  const int chunk = INST ? 0 : refill_chunk(count, DSPT_REFILL_CHUNK);
   LaneStack st;
   st.lds = smem + threadIdx.x;
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -562,7 +569,7 @@ __global__ TRACE_BOUNDS void k_trace_ext(const DevJob* __restrict__ jobs, PathSt
     active = false;
   };
   for (;;) {
-    const int q = refill<DSPT_REFILL_EXT, INST ? 0 : DSPT_REFILL_CHUNK>(active, count, next_ptr, exhausted, res_lo, res_hi);
+    const int q = refill<DSPT_REFILL_EXT>(active, count, next_ptr, exhausted, res_lo, res_hi, chunk);
     if (q >= 0) {
       p = q < nsorted ? perm[q] : q;
       const float4 o = ps.org[p], d = ps.dir[p];
@@ -619,7 +626,7 @@ __global__ TRACE_BOUNDS void k_trace_shadow(const DevJob* __restrict__ jobs, Pat
   Walker<INST> w;
   f3 org = sp3(0.0f), contrib = sp3(0.0f);
   for (;;) {
-    const int q = refill<DSPT_REFILL_SH, 0>(active, count, next_ptr, exhausted, res_lo, res_hi);
+    const int q = refill<DSPT_REFILL_SH>(active, count, next_ptr, exhausted, res_lo, res_hi, 0);
     if (q >= 0) {
       p = q;
       const float4 so = ps.sh_org[p];
